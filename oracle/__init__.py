@@ -1,0 +1,25 @@
+"""The oracle: a CPU (numpy/scipy, FP64) restatement of GalSim's FFT rendering path.
+
+TEST INFRASTRUCTURE ONLY.  Nothing in the ``galsim_amd`` product imports, calls, links or
+executes anything under ``oracle/``; only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` use it, and only as the reference the product is compared
+against (or, for ``cpu_baseline``, as the CPU path that is timed beside the GPU).
+
+Layout:
+
+* ``numerics``  -- Brent solver (include/galsim/Solve.h), natural cubic spline table
+  (src/Table.cpp), goodFFTSize (src/Image.cpp:1141-1152), Ogata Hankel transform
+  (src/math/Hankel.cpp:79-247).
+* ``profiles``  -- leaf profile scalars and k-space fills: Exponential, Gaussian, Moffat,
+  Kolmogorov, Sersic, Box/Pixel (src/SB*.cpp and galsim/*.py).
+* ``draw``      -- the drawImage(method='fft') orchestration: world->image transform, size
+  planning, k-image fill through Transformation/Convolution/Sum, Hermitian wrap, inverse real
+  FFT and crop (galsim/gsobject.py, galsim/transform.py, galsim/convolve.py, galsim/sum.py,
+  src/SBProfile.cpp, src/SBTransform.cpp, src/Image.cpp).
+* ``ref``       -- ctypes loader for ``oracle/_ref/libgsref.so``, the reference's own profile
+  numerics compiled from its sources by ``oracle/Makefile`` (used to pin this restatement).
+
+Every function cites the reference file:line it restates.  Where a check pins a function it is
+named in the test that does it (tests/test_oracle_*.py); functions with no such check say
+"parity unpinned" in their docstring.
+"""

@@ -1,0 +1,742 @@
+"""Oracle leaf profiles: size scalars and k-space fills.
+
+Test infrastructure only (see ``oracle/__init__.py``).
+
+Each leaf exposes ``flux``, ``maxk``, ``stepk`` (in the leaf's own length unit) and two fills
+that restate the reference's ``fillKImage`` overloads:
+
+* ``fill_1d(kx0, dkx, m, ky0, dky, n)`` -- the separable/diagonal fill (no quadrant logic),
+* ``fill_2d(kx0, dkx, dkxy, ky0, dky, dkyx, m, n)`` -- the general-Jacobian fill.
+
+Coordinates are accumulated with ``np.cumsum`` which adds sequentially, i.e. with the same
+rounding as the reference's ``kx += dkx`` loops.
+"""
+import math
+from collections import deque
+
+import numpy as np
+import scipy.special as sps
+
+from . import numerics as nm
+
+
+class GSParams:
+    """galsim/gsparams.py:126-158 defaults."""
+
+    def __init__(self, minimum_fft_size=128, maximum_fft_size=8192, folding_threshold=5.e-3,
+                 stepk_minimum_hlr=5, maxk_threshold=1.e-3, kvalue_accuracy=1.e-5,
+                 xvalue_accuracy=1.e-5, table_spacing=1, realspace_relerr=1.e-4,
+                 realspace_abserr=1.e-6, integration_relerr=1.e-6, integration_abserr=1.e-8,
+                 shoot_accuracy=1.e-5):
+        self.minimum_fft_size = int(minimum_fft_size)
+        self.maximum_fft_size = int(maximum_fft_size)
+        self.folding_threshold = float(folding_threshold)
+        self.stepk_minimum_hlr = float(stepk_minimum_hlr)
+        self.maxk_threshold = float(maxk_threshold)
+        self.kvalue_accuracy = float(kvalue_accuracy)
+        self.xvalue_accuracy = float(xvalue_accuracy)
+        self.table_spacing = int(table_spacing)
+        self.realspace_relerr = float(realspace_relerr)
+        self.realspace_abserr = float(realspace_abserr)
+        self.integration_relerr = float(integration_relerr)
+        self.integration_abserr = float(integration_abserr)
+        self.shoot_accuracy = float(shoot_accuracy)
+
+    def key(self):
+        return tuple(sorted(self.__dict__.items()))
+
+    def __eq__(self, other):
+        return isinstance(other, GSParams) and self.key() == other.key()
+
+    def __hash__(self):
+        return hash(self.key())
+
+    @staticmethod
+    def combine(lst):
+        """galsim/gsparams.py:217-241."""
+        if len(lst) == 1 or all(g == lst[0] for g in lst[1:]):
+            return lst[0]
+        return GSParams(
+            max(g.minimum_fft_size for g in lst), max(g.maximum_fft_size for g in lst),
+            min(g.folding_threshold for g in lst), max(g.stepk_minimum_hlr for g in lst),
+            min(g.maxk_threshold for g in lst), min(g.kvalue_accuracy for g in lst),
+            min(g.xvalue_accuracy for g in lst), min(g.table_spacing for g in lst),
+            min(g.realspace_relerr for g in lst), min(g.realspace_abserr for g in lst),
+            min(g.integration_relerr for g in lst), min(g.integration_abserr for g in lst),
+            min(g.shoot_accuracy for g in lst))
+
+
+DEFAULT_GSPARAMS = GSParams()
+
+
+def _axis(x0, dx, m):
+    """x0, x0+dx, ... accumulated left to right (the reference's ``x += dx``)."""
+    if m <= 0:
+        return np.zeros(0)
+    seq = np.empty(m)
+    seq[0] = x0
+    seq[1:] = dx
+    return np.cumsum(seq)
+
+
+def _grid2d(kx0, dkx, dkxy, ky0, dky, dkyx, m, n):
+    """Row starts accumulate by (dkxy, dky) per row, then (dkx, dkyx) per column."""
+    rx = _axis(kx0, dkxy, n)
+    ry = _axis(ky0, dky, n)
+    kx = np.empty((n, m))
+    ky = np.empty((n, m))
+    kx[:, 0] = rx
+    ky[:, 0] = ry
+    kx[:, 1:] = dkx
+    ky[:, 1:] = dkyx
+    return np.cumsum(kx, axis=1), np.cumsum(ky, axis=1)
+
+
+def fast_pow(x, y):
+    """``fmath::expd(y*log(x))`` (src/SBMoffat.cpp:46-47, src/SBKolmogorov.cpp:48-49,
+    src/SBSersic.cpp:33-34)."""
+    with np.errstate(divide='ignore', invalid='ignore'):
+        return nm.expd(y * np.log(x))
+
+
+def sinc(x):
+    """src/math/Sinc.cpp:29-33."""
+    x = np.asarray(x, dtype=float)
+    with np.errstate(invalid='ignore', divide='ignore'):
+        big = np.sin(np.pi * x) / (np.pi * x)
+    return np.where(np.abs(x) < 1.e-4, 1. - (np.pi * np.pi / 6.) * x * x, big)
+
+
+# ----------------------------------------------------------------------------------------------
+# Exponential -- src/SBExponential.cpp, galsim/exponential.py
+# ----------------------------------------------------------------------------------------------
+def _kvalue_range_1d(m, kmax, ksqmax, kx0, dkx, ky):
+    """GetKValueRange1d (src/SBProfile.cpp:508-531). Returns (i1, i2, kysq)."""
+    if abs(ky) >= kmax:
+        return m, m, 0.
+    kysq = ky * ky
+    dsq = ksqmax - kysq
+    if kx0 * kx0 > dsq or (kx0 + m * dkx) * (kx0 + m * dkx) > dsq:
+        d = math.sqrt(dsq)
+        i1 = int(math.ceil((-kx0 - d) / dkx))
+        i2 = int(math.floor((-kx0 + d) / dkx))
+        if i1 > i2:
+            i1, i2 = i2, i1
+        i2 += 1
+        if i2 <= 0 or i1 >= m:
+            return m, m, kysq
+        return max(i1, 0), min(i2, m), kysq
+    return 0, m, kysq
+
+
+def _kvalue_range_2d(m, kmax, ksqmax, kx0, dkx, ky0, dky):
+    """GetKValueRange2d (src/SBProfile.cpp:533-588), including the ``ky0+m+dky`` test of 544."""
+    if (abs(kx0) > kmax or abs(kx0 + m * dkx) > kmax or
+            abs(ky0) > kmax or abs(ky0 + m + dky) > kmax):
+        a = dkx * dkx + dky * dky
+        b = dkx * kx0 + dky * ky0
+        c = kx0 * kx0 + ky0 * ky0 - ksqmax
+        d = b * b - a * c
+        if d <= 0.:
+            return m, m
+        d = math.sqrt(d)
+        i1 = int(math.ceil((-b - d) / a))
+        i2 = int(math.floor((-b + d) / a))
+        if i1 > i2:
+            i1, i2 = i2, i1
+        i2 += 1
+        if i2 <= 0 or i1 >= m:
+            return m, m
+        return max(i1, 0), min(i2, m)
+    return 0, m
+
+
+class Exponential:
+    """SBExponential (src/SBExponential.cpp:64-92 ctor, 442-510 fills, 513-545 info).
+
+    Pinned by the closed form Exponential(k) == Moffat(beta=1.5).xValue (tests/test_draw.py:859)
+    and stepk*r0 = 0.37436747851, maxk*r0 = 10 (tests/test_exponential.py:100-101).
+    """
+    hlr_factor = 1.6783469900166605
+
+    def __init__(self, half_light_radius=None, scale_radius=None, flux=1., gsparams=None):
+        if half_light_radius is not None:
+            scale_radius = half_light_radius / Exponential.hlr_factor
+        self.r0 = float(scale_radius)
+        self.flux = float(flux)
+        self.gsparams = gsparams or DEFAULT_GSPARAMS
+        kva = self.gsparams.kvalue_accuracy
+        self.ksq_max = kva ** (-1. / 1.5) - 1.
+        self.k_max = math.sqrt(self.ksq_max)
+
+    @property
+    def maxk(self):
+        # galsim/exponential.py:131-132
+        return (self.gsparams.maxk_threshold ** (-1. / 3.)) / self.r0
+
+    @property
+    def stepk(self):
+        # ExponentialInfo ctor, src/SBExponential.cpp:527-544
+        gsp = self.gsparams
+        logx = math.log(gsp.folding_threshold)
+        R = -logx
+        for _ in range(3):
+            R = math.log(1. + R) - logx
+        R = max(R, gsp.stepk_minimum_hlr * 1.6783469900166605)
+        return (math.pi / R) / self.r0
+
+    def fill_1d(self, kx0, dkx, m, ky0, dky, n):
+        r0 = self.r0
+        kx0 *= r0; dkx *= r0; ky0 *= r0; dky *= r0
+        out = np.zeros((n, m))
+        kys = _axis(ky0, dky, n)
+        for j in range(n):
+            i1, i2, kysq = _kvalue_range_1d(m, self.k_max, self.ksq_max, kx0, dkx, kys[j])
+            if i1 == m:
+                continue
+            kx = _axis(kx0 + i1 * dkx, dkx, i2 - i1)
+            ksqp1 = kx * kx + (kysq + 1.)
+            out[j, i1:i2] = self.flux / (ksqp1 * np.sqrt(ksqp1))
+        return out
+
+    def fill_2d(self, kx0, dkx, dkxy, ky0, dky, dkyx, m, n):
+        r0 = self.r0
+        kx0 *= r0; dkx *= r0; dkxy *= r0; ky0 *= r0; dky *= r0; dkyx *= r0
+        out = np.zeros((n, m))
+        rx = _axis(kx0, dkxy, n)
+        ry = _axis(ky0, dky, n)
+        for j in range(n):
+            i1, i2 = _kvalue_range_2d(m, self.k_max, self.ksq_max, rx[j], dkx, ry[j], dkyx)
+            if i1 == m:
+                continue
+            kx = _axis(rx[j] + i1 * dkx, dkx, i2 - i1)
+            ky = _axis(ry[j] + i1 * dkyx, dkyx, i2 - i1)
+            ksqp1 = 1. + kx * kx + ky * ky
+            out[j, i1:i2] = self.flux / (ksqp1 * np.sqrt(ksqp1))
+        return out
+
+
+# ----------------------------------------------------------------------------------------------
+# Gaussian -- src/SBGaussian.cpp, galsim/gaussian.py
+# ----------------------------------------------------------------------------------------------
+class Gaussian:
+    """SBGaussian (src/SBGaussian.cpp:52-79 ctor, 200-290 fills); maxk/stepk galsim/gaussian.py:147-155.
+
+    Pinned end to end by gauss_smallshear_convolve_box.fits (tests/test_convolve.py:220).
+    """
+    hlr_factor = 1.1774100225154747
+    fwhm_factor = 2.3548200450309493
+
+    def __init__(self, sigma=None, half_light_radius=None, fwhm=None, flux=1., gsparams=None):
+        if fwhm is not None:
+            sigma = fwhm / Gaussian.fwhm_factor
+        elif half_light_radius is not None:
+            sigma = half_light_radius / Gaussian.hlr_factor
+        self.sigma = float(sigma)
+        self.flux = float(flux)
+        self.gsparams = gsparams or DEFAULT_GSPARAMS
+        kva = self.gsparams.kvalue_accuracy
+        self.ksq_max = -2. * math.log(kva)
+        self.ksq_min = (kva * 48.) ** (1. / 3.)
+
+    @property
+    def maxk(self):
+        return math.sqrt(-2. * math.log(self.gsparams.maxk_threshold)) / self.sigma
+
+    @property
+    def stepk(self):
+        R = max(math.sqrt(-2. * math.log(self.gsparams.folding_threshold)),
+                self.gsparams.stepk_minimum_hlr * Gaussian.hlr_factor)
+        return math.pi / (R * self.sigma)
+
+    def fill_1d(self, kx0, dkx, m, ky0, dky, n):
+        s = self.sigma
+        kx = _axis(kx0 * s, dkx * s, m)
+        ky = _axis(ky0 * s, dky * s, n)
+        gx = np.exp(-0.5 * kx * kx)
+        gy = np.exp(-0.5 * ky * ky)
+        return (self.flux * gx[None, :]) * gy[:, None]
+
+    def fill_2d(self, kx0, dkx, dkxy, ky0, dky, dkyx, m, n):
+        s = self.sigma
+        kx, ky = _grid2d(kx0 * s, dkx * s, dkxy * s, ky0 * s, dky * s, dkyx * s, m, n)
+        ksq = kx * kx + ky * ky
+        out = self.flux * np.exp(-0.5 * ksq)
+        out = np.where(ksq < self.ksq_min, self.flux * (1. - 0.5 * ksq * (1. - 0.25 * ksq)), out)
+        return np.where(ksq > self.ksq_max, 0., out)
+
+
+# ----------------------------------------------------------------------------------------------
+# Moffat -- src/SBMoffat.cpp, galsim/moffat.py
+# ----------------------------------------------------------------------------------------------
+class Moffat:
+    """SBMoffat, untruncated (src/SBMoffat.cpp:163-244 ctor, 286-345 kV_*, 419-479 fills,
+    497-549 maxK/stepK).  K_nu from scipy.special.kv.
+
+    Pinned by kValue(maxk)/flux == maxk_threshold (tests/test_moffat.py:183-214), the drawKImage
+    size N=1174 for beta=1.5, maxk_threshold=1e-4 (tests/test_draw.py:1279) and the Hankel pair
+    with the Exponential (tests/test_draw.py:859).
+    """
+
+    def __init__(self, beta, scale_radius=None, half_light_radius=None, fwhm=None, trunc=0.,
+                 flux=1., gsparams=None):
+        self.beta = float(beta)
+        self.trunc = float(trunc)
+        if self.trunc != 0.:
+            raise NotImplementedError("truncated Moffat is not restated in the oracle")
+        self.flux = float(flux)
+        self.gsparams = gsparams or DEFAULT_GSPARAMS
+        if half_light_radius is not None:
+            # MoffatCalculateScaleRadiusFromHLR, rm == 0 branch (src/SBMoffat.cpp:116-126)
+            rerd = math.sqrt(math.pow(0.5, 1. / (1. - self.beta)) - 1.)
+            scale_radius = half_light_radius / rerd
+        elif fwhm is not None:
+            # galsim/moffat.py:111
+            scale_radius = fwhm / (2. * math.sqrt(2. ** (1. / self.beta) - 1.))
+        self.rD = float(scale_radius)
+        gsp = self.gsparams
+        beta = self.beta
+        self.maxRrD = math.sqrt(math.pow(gsp.xvalue_accuracy, 1. / (1. - beta)) - 1.)
+        self.maxR = self.maxRrD * self.rD
+        self.fluxFactor = 1.
+        kva = gsp.kvalue_accuracy
+        self.knorm2 = 1.
+        if abs(beta - 1.5) < kva:
+            self.kind = 15
+        elif abs(beta - 2) < kva:
+            self.kind = 2
+        elif abs(beta - 2.5) < kva:
+            self.kind = 25
+        elif abs(beta - 3) < kva:
+            self.kind = 3
+            self.knorm2 /= 2.
+        elif abs(beta - 3.5) < kva:
+            self.kind = 35
+            self.knorm2 /= 3.
+        elif abs(beta - 4) < kva:
+            self.kind = 4
+            self.knorm2 /= 8.
+        else:
+            self.kind = 0
+            self.knorm2 *= 4. / (math.gamma(beta - 1.) * math.pow(2., beta))
+        self.knorm = self.flux * self.knorm2
+        self._maxk = None
+
+    def kV(self, ksq):
+        """kV_* of src/SBMoffat.cpp:286-345 (argument ksq = k^2 rD^2)."""
+        ksq = np.asarray(ksq, dtype=float)
+        k = np.sqrt(ksq)
+        kind = self.kind
+        with np.errstate(invalid='ignore', divide='ignore', over='ignore'):
+            if kind == 15:
+                return nm.expd(-k)
+            if kind == 25:
+                return nm.expd(-k) * (1. + k)
+            if kind == 35:
+                return nm.expd(-k) * (3. + (3. + k) * k)
+            if kind == 2:
+                return np.where(ksq == 0., 1., sps.kv(1, k) * k)
+            if kind == 3:
+                return np.where(ksq == 0., 2., sps.kv(2, k) * ksq)
+            if kind == 4:
+                return np.where(ksq == 0., 8., sps.kv(3, k) * k * ksq)
+            nu = self.beta - 1.
+            return np.where(ksq == 0., self.flux / self.knorm, sps.kv(nu, k) * fast_pow(k, nu))
+
+    @property
+    def half_light_radius(self):
+        return self.rD * math.sqrt(math.pow(1. - 0.5 * self.fluxFactor,
+                                            1. / (1. - self.beta)) - 1.)
+
+    @property
+    def maxk(self):
+        # SBMoffatImpl::maxK (src/SBMoffat.cpp:497-518): Brent on kV2(ksq) - maxk_threshold.
+        if self._maxk is None:
+            thresh = self.gsparams.maxk_threshold
+            func = lambda ksq: float(self.knorm2 * self.kV(ksq)) - thresh
+            s = nm.Solve(func, 0., 100.)
+            s.bracket_upper()
+            self._maxk = math.sqrt(s.root())
+        return self._maxk / self.rD
+
+    @property
+    def stepk(self):
+        # src/SBMoffat.cpp:522-549
+        gsp = self.gsparams
+        if self.beta <= 1.1:
+            return math.pi / self.maxR
+        R = math.pow(gsp.folding_threshold, 0.5 / (1. - self.beta)) * self.rD
+        if R > self.maxR:
+            R = self.maxR
+        R = max(R, gsp.stepk_minimum_hlr * self.half_light_radius)
+        return math.pi / R
+
+    def fill_1d(self, kx0, dkx, m, ky0, dky, n):
+        rD = self.rD
+        kx = _axis(kx0 * rD, dkx * rD, m)
+        ky = _axis(ky0 * rD, dky * rD, n)
+        ksq = kx[None, :] * kx[None, :] + (ky * ky)[:, None]
+        return self.knorm * self.kV(ksq)
+
+    def fill_2d(self, kx0, dkx, dkxy, ky0, dky, dkyx, m, n):
+        rD = self.rD
+        kx, ky = _grid2d(kx0 * rD, dkx * rD, dkxy * rD, ky0 * rD, dky * rD, dkyx * rD, m, n)
+        return self.knorm * self.kV(kx * kx + ky * ky)
+
+
+# ----------------------------------------------------------------------------------------------
+# Kolmogorov -- src/SBKolmogorov.cpp, galsim/kolmogorov.py
+# ----------------------------------------------------------------------------------------------
+K0_FACTOR = 2.992939911888651
+XVAL_ZERO = 0.0876786563672346
+_KOLM_INFO = {}
+
+
+def kolmogorov_info(gsp):
+    """KolmogorovInfo ctor (src/SBKolmogorov.cpp:337-419): (maxk, stepk) in units of k0.
+
+    The x-space radial table is the Hankel transform of exp(-k^(5/3)) on log r from -3 in steps
+    of table_spacing*(xvalue_accuracy/10)^(1/4), stopped once val r^2/1.67 < shoot_accuracy/2pi;
+    stepk follows from the r^(-11/3) tail through the last entry.  Pinned by maxk and stepk at
+    lam_over_r0=1.5 (tests/test_kolmogorov.py:138-139) and against oracle/_ref.
+    """
+    key = gsp.key()
+    if key in _KOLM_INFO:
+        return _KOLM_INFO[key]
+    maxk = math.pow(-math.log(gsp.kvalue_accuracy), 3. / 5.)
+    dlogr = gsp.table_spacing * math.sqrt(math.sqrt(gsp.xvalue_accuracy / 10.))
+    thresh = gsp.shoot_accuracy / (2. * math.pi)
+    kfunc = lambda k: nm.expd(-fast_pow(k, 5. / 3.))
+    logr = -3.
+    r1 = F1 = None
+    while logr < math.log(1.e4):
+        r = math.exp(logr)
+        val = nm.hankel_inf(kfunc, r, gsp.integration_relerr, gsp.integration_abserr)
+        val /= 2. * math.pi
+        r1, F1 = r, val
+        if val * r * r / 1.67 < thresh:
+            break
+        logr += dlogr
+    n = 11. / 3.
+    R = float(fast_pow(2. * math.pi * F1 * float(fast_pow(r1, n)) /
+                       ((n - 2) * gsp.folding_threshold), 1. / (n - 2)))
+    hlr = K0_FACTOR * 0.5548101137
+    R = max(R, gsp.stepk_minimum_hlr * hlr)
+    res = (maxk, math.pi / R)
+    _KOLM_INFO[key] = res
+    return res
+
+
+class Kolmogorov:
+    """SBKolmogorov (src/SBKolmogorov.cpp:83-100 ctor, 185-245 fills, 258-259 kValue);
+    maxk galsim/kolmogorov.py:281-284."""
+    fwhm_factor = 0.9758634299
+    hlr_factor = 0.5548101137
+
+    def __init__(self, lam_over_r0=None, fwhm=None, half_light_radius=None, flux=1.,
+                 gsparams=None):
+        if fwhm is not None:
+            lam_over_r0 = fwhm / Kolmogorov.fwhm_factor
+        elif half_light_radius is not None:
+            lam_over_r0 = half_light_radius / Kolmogorov.hlr_factor
+        self.lam_over_r0 = float(lam_over_r0)
+        self.flux = float(flux)
+        self.gsparams = gsparams or DEFAULT_GSPARAMS
+        self.k0 = K0_FACTOR / self.lam_over_r0
+
+    @property
+    def maxk(self):
+        return (-math.log(self.gsparams.kvalue_accuracy)) ** 0.6 * self.k0
+
+    @property
+    def stepk(self):
+        return kolmogorov_info(self.gsparams)[1] * self.k0
+
+    def _kv(self, ksq):
+        with np.errstate(divide='ignore'):
+            return self.flux * nm.expd(-fast_pow(ksq, 5. / 6.))
+
+    def fill_1d(self, kx0, dkx, m, ky0, dky, n):
+        ik = 1. / self.k0
+        kx = _axis(kx0 * ik, dkx * ik, m)
+        ky = _axis(ky0 * ik, dky * ik, n)
+        return self._kv(kx[None, :] * kx[None, :] + (ky * ky)[:, None])
+
+    def fill_2d(self, kx0, dkx, dkxy, ky0, dky, dkyx, m, n):
+        ik = 1. / self.k0
+        kx, ky = _grid2d(kx0 * ik, dkx * ik, dkxy * ik, ky0 * ik, dky * ik, dkyx * ik, m, n)
+        return self._kv(kx * kx + ky * ky)
+
+
+# ----------------------------------------------------------------------------------------------
+# Sersic -- src/SBSersic.cpp, galsim/sersic.py
+# ----------------------------------------------------------------------------------------------
+def _gamma_p(a, x):
+    return float(sps.gammainc(a, x))
+
+
+class _SersicMissingFlux:
+    """src/SBSersic.cpp:542-556."""
+
+    def __init__(self, n, missing_flux):
+        self.two_n = 2. * n
+        self.target = missing_flux
+        self.g2n = math.gamma(self.two_n)
+
+    def __call__(self, z):
+        return (1. - _gamma_p(self.two_n, z)) * self.g2n - self.target
+
+
+def sersic_calculate_b(n, flux_fraction=1.):
+    """CalculateB (src/SBSersic.cpp:627-654)."""
+    invn = 1. / n
+    gamma2n = math.gamma(2. * n)
+    invnsq = invn * invn
+    b1 = 2. * n - 1. / 3.
+    b2 = b1 + (8. / 405.) * invn + (46. / 25515.) * invnsq + (131. / 1148175.) * invn * invnsq
+    func = _SersicMissingFlux(n, (1. - 0.5 * flux_fraction) * gamma2n)
+    s = nm.Solve(func, b1, b2)
+    s.bracket_lower_with_limit(0.)
+    return s.root()
+
+
+def sersic_hlr(n, flux_fraction=1.):
+    """SersicHLR (src/SBSersic.cpp:665-669): re/r0."""
+    return math.pow(sersic_calculate_b(n, flux_fraction), n)
+
+
+_SERSIC_INFO = {}
+
+
+class SersicInfo:
+    """SersicInfo for an untruncated profile (src/SBSersic.cpp:223-537).
+
+    Pinned against oracle/_ref (the reference's own SersicInfo compiled from its sources):
+    table, ksq_min/ksq_max, high-k coefficients, maxK, stepK and HLR; and by the Sersic
+    maxk/stepk/r0 known answers of tests/test_sersic.py:497-558 (untruncated n=3.1 case).
+    """
+
+    def __init__(self, n, gsp):
+        self.n = float(n)
+        self.gsp = gsp
+        self.invn = 1. / self.n
+        self.inv2n = 0.5 * self.invn
+        self.gamma2n = math.gamma(2. * self.n)
+        self.flux_fraction = 1.
+        self._b = None
+        self._build_ft()
+        self._stepk = None
+
+    @staticmethod
+    def get(n, gsp):
+        key = (float(n), gsp.key())
+        if key not in _SERSIC_INFO:
+            _SERSIC_INFO[key] = SersicInfo(n, gsp)
+        return _SERSIC_INFO[key]
+
+    @property
+    def b(self):
+        if self._b is None:
+            self._b = sersic_calculate_b(self.n, 1.)
+        return self._b
+
+    @property
+    def hlr(self):
+        return math.pow(self.b, self.n)
+
+    def missing_flux_radius(self, frac):
+        """calculateMissingFluxRadius (src/SBSersic.cpp:558-625)."""
+        n = self.n
+        missing_flux = frac * self.gamma2n
+        z1 = -math.log(missing_flux)
+        if n == 0.5:
+            z = z1
+        else:
+            z = 4. * (n + 1.)
+            twonm1 = 2. * n - 1.
+            z2 = z1 + twonm1 * math.log(z) + twonm1 / z + twonm1 * (2. * n - 3.) / (2. * z * z)
+            if z2 > z1 and z2 - z1 < 0.01:
+                z2 = z1 + 0.01
+            elif z2 < z1 and z2 - z1 > -0.01:
+                z2 = z1 - 0.01
+            if z1 < 0.:
+                z1 = self.b
+            s = nm.Solve(_SersicMissingFlux(n, missing_flux), z1, z2)
+            s.bracket_upper()
+            z = s.root()
+        return math.pow(z, n)
+
+    @property
+    def stepk(self):
+        # SersicInfo::stepK (src/SBSersic.cpp:283-297), untruncated
+        if self._stepk is None:
+            R = self.missing_flux_radius(self.gsp.folding_threshold)
+            R = max(R, self.gsp.stepk_minimum_hlr)
+            self._stepk = math.pi / R
+        return self._stepk
+
+    def _build_ft(self):
+        # SersicInfo::buildFT (src/SBSersic.cpp:387-537), untruncated branch
+        gsp = self.gsp
+        n = self.n
+        gamma4n = math.gamma(4. * n)
+        gamma6n = math.gamma(6. * n)
+        gamma8n = math.gamma(8. * n)
+        ff = self.flux_fraction
+        self.kderiv2 = -gamma4n / (4. * self.gamma2n) / ff
+        self.kderiv4 = gamma6n / (64. * self.gamma2n) / ff
+        kderiv6 = gamma8n / (2304. * self.gamma2n) / ff
+        kmin = math.pow(gsp.kvalue_accuracy / kderiv6, 1. / 6.)
+        self.ksq_min = kmin * kmin
+        hankel_norm = ff * n * self.gamma2n
+        dlogk = gsp.table_spacing * math.sqrt(math.sqrt(gsp.kvalue_accuracy / 10.))
+        n_correct = 0
+        n_fit = 10
+        fit_vals = deque()
+        sf = skf = sk = sk2 = 0.
+        self.ksq_max = -1.
+        maxk = kmin
+        invn = self.invn
+        radial = lambda r: nm.expd(-fast_pow(r, invn))
+        found_maxk = False
+        args, vals = [], []
+        highk_a = highk_b = 0.
+        logk = math.log(kmin) - 0.001
+        while logk < math.log(500.):
+            k = nm.expd(logk)
+            ksq = k * k
+            val = nm.hankel_inf(radial, k, gsp.integration_relerr,
+                                gsp.integration_abserr * hankel_norm)
+            val /= hankel_norm
+            f0 = val * ksq
+            args.append(logk)
+            vals.append(f0)
+            if abs(val) > gsp.maxk_threshold:
+                maxk = k
+                n_correct = 0
+            else:
+                found_maxk = True
+                highk_a = (sf * sk2 - sk * skf) / (n_fit * sk2 - sk * sk)
+                highk_b = (n_fit * skf - sk * sf) / (n_fit * sk2 - sk * sk)
+                f0_pred = highk_a + highk_b / k
+                if abs(f0 - f0_pred) / ksq < gsp.kvalue_accuracy:
+                    n_correct += 1
+                else:
+                    n_correct = 0
+                if n_correct >= 5:
+                    self.ksq_max = ksq
+                    break
+            if len(fit_vals) == n_fit:
+                k_back = nm.expd(logk - n_fit * dlogk)
+                f_back = fit_vals.pop()
+                inv_k = 1. / k
+                inv_k_back = 1. / k_back
+                sf += f0 - f_back
+                skf += f0 * inv_k - f_back * inv_k_back
+                sk += inv_k - inv_k_back
+                sk2 += inv_k * inv_k - inv_k_back * inv_k_back
+            else:
+                inv_k = 1. / k
+                sf += f0
+                skf += f0 * inv_k
+                sk += inv_k
+                sk2 += inv_k * inv_k
+            fit_vals.appendleft(f0)
+            logk += dlogk
+        self.table = nm.SplineTable(args, vals)
+        if self.ksq_max <= 0.:
+            self.ksq_max = nm.expd(2. * self.table.arg_max)
+        if found_maxk:
+            maxk *= nm.expd(dlogk)
+        else:
+            highk_a = (sf * sk2 - sk * skf) / (n_fit * sk2 - sk * sk)
+            highk_b = (n_fit * skf - sk * sf) / (n_fit * sk2 - sk * sk)
+            maxk = math.sqrt(highk_a / gsp.maxk_threshold)
+            for _ in range(3):
+                maxk = math.sqrt((highk_a - highk_b / maxk) / gsp.maxk_threshold)
+        self.highk_a = highk_a
+        self.highk_b = highk_b
+        self.maxk = maxk
+
+    def kvalue(self, ksq):
+        """SersicInfo::kValue (src/SBSersic.cpp:363-376)."""
+        ksq = np.asarray(ksq, dtype=float)
+        with np.errstate(divide='ignore', invalid='ignore'):
+            hi = (self.highk_a + self.highk_b / np.sqrt(ksq)) / ksq
+            lo = 1. + ksq * (self.kderiv2 + ksq * self.kderiv4)
+            mid = self.table(0.5 * np.log(ksq)) / ksq
+        return np.where(ksq >= self.ksq_max, hi, np.where(ksq < self.ksq_min, lo, mid))
+
+
+class Sersic:
+    """SBSersic, untruncated (src/SBSersic.cpp:71-90, 237-281); galsim/sersic.py:201-250."""
+
+    def __init__(self, n, half_light_radius=None, scale_radius=None, flux=1., trunc=0.,
+                 gsparams=None):
+        if trunc != 0.:
+            raise NotImplementedError("truncated Sersic is not restated in the oracle")
+        self.n = float(n)
+        self.flux = float(flux)
+        self.gsparams = gsparams or DEFAULT_GSPARAMS
+        if half_light_radius is not None:
+            scale_radius = half_light_radius / sersic_hlr(self.n, 1.)
+        self.r0 = float(scale_radius)
+        self.info = SersicInfo.get(self.n, self.gsparams)
+
+    @property
+    def maxk(self):
+        return self.info.maxk / self.r0
+
+    @property
+    def stepk(self):
+        return self.info.stepk / self.r0
+
+    def fill_1d(self, kx0, dkx, m, ky0, dky, n):
+        r0 = self.r0
+        kx = _axis(kx0 * r0, dkx * r0, m)
+        ky = _axis(ky0 * r0, dky * r0, n)
+        return self.flux * self.info.kvalue(kx[None, :] * kx[None, :] + (ky * ky)[:, None])
+
+    def fill_2d(self, kx0, dkx, dkxy, ky0, dky, dkyx, m, n):
+        r0 = self.r0
+        kx, ky = _grid2d(kx0 * r0, dkx * r0, dkxy * r0, ky0 * r0, dky * r0, dkyx * r0, m, n)
+        return self.flux * self.info.kvalue(kx * kx + ky * ky)
+
+
+# ----------------------------------------------------------------------------------------------
+# Box / Pixel -- src/SBBox.cpp, galsim/box.py
+# ----------------------------------------------------------------------------------------------
+class Box:
+    """SBBox (src/SBBox.cpp:151-224); maxk/stepk galsim/box.py:100-105."""
+
+    def __init__(self, width, height, flux=1., gsparams=None):
+        self.width = float(width)
+        self.height = float(height)
+        self.flux = float(flux)
+        self.gsparams = gsparams or DEFAULT_GSPARAMS
+        self.wo2pi = self.width / (2. * math.pi)
+        self.ho2pi = self.height / (2. * math.pi)
+
+    @property
+    def maxk(self):
+        return 2. / (self.gsparams.maxk_threshold * min(self.width, self.height))
+
+    @property
+    def stepk(self):
+        return math.pi / max(self.width, self.height)
+
+    def fill_1d(self, kx0, dkx, m, ky0, dky, n):
+        sx = sinc(_axis(kx0 * self.wo2pi, dkx * self.wo2pi, m))
+        sy = sinc(_axis(ky0 * self.ho2pi, dky * self.ho2pi, n))
+        return (self.flux * sx[None, :]) * sy[:, None]
+
+    def fill_2d(self, kx0, dkx, dkxy, ky0, dky, dkyx, m, n):
+        kx = _grid2d(kx0 * self.wo2pi, dkx * self.wo2pi, dkxy * self.wo2pi,
+                     0., 0., 0., m, n)[0]
+        ky = _grid2d(0., 0., 0., ky0 * self.ho2pi, dky * self.ho2pi, dkyx * self.ho2pi, m, n)[1]
+        return (self.flux * sinc(kx)) * sinc(ky)
+
+
+def Pixel(scale, flux=1., gsparams=None):
+    """galsim/box.py:136-160."""
+    return Box(scale, scale, flux, gsparams)
