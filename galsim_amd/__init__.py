@@ -1,0 +1,27 @@
+"""galsim_amd: GalSim's FFT rendering path (GSObject.drawImage(method='fft')) on AMD Instinct
+MI355X.  The public names mirror galsim's: profiles, transformations, Convolve/Add, Image,
+GSParams, Shear; drawing runs in hand-written HIP kernels through the C ABI of libgsmi.so
+(include/gsmi.h)."""
+from ._lib import GalSimAMDError, GalSimAMDRangeError
+from .box import Box, Pixel
+from .convolve import Convolution, Convolve
+from .exponential import Exponential
+from .gaussian import Gaussian
+from .gsobject import GSObject
+from .gsparams import GSParams
+from .image import Image, ImageCD, ImageCF, ImageD, ImageF
+from .kolmogorov import Kolmogorov
+from .moffat import Moffat
+from .position import BoundsI, PositionD, PositionI
+from .sersic import DeVaucouleurs, Sersic
+from .shear import Shear
+from .sum import Add, Sum
+from .transform import Transform, Transformation, _Transform
+
+GalSimError = GalSimAMDError
+
+__all__ = ['Add', 'BoundsI', 'Box', 'Convolution', 'Convolve', 'DeVaucouleurs', 'Exponential',
+           'GSObject', 'GSParams', 'GalSimAMDError', 'GalSimAMDRangeError', 'GalSimError',
+           'Gaussian', 'Image', 'ImageCD', 'ImageCF', 'ImageD', 'ImageF', 'Kolmogorov', 'Moffat',
+           'Pixel', 'PositionD', 'PositionI', 'Sersic', 'Shear', 'Sum', 'Transform',
+           'Transformation']

@@ -1,0 +1,113 @@
+"""ctypes binding of libgsmi.so (the HIP kernels and the host profile numerics).
+
+The library is built in-tree by ``galsim_amd.csrc.build``.  There is no fallback: if it is
+missing, every drawing call raises ``GalSimAMDError`` naming the library.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libgsmi.so")
+
+OP_DTYPE = np.dtype([('code', np.int32), ('arg', np.int32), ('a', np.float64),
+                     ('b', np.float64), ('c', np.float64)], align=True)
+LEAF_DTYPE = np.dtype([('type', np.int32), ('separable', np.int32), ('kind', np.int32),
+                       ('table', np.int32), ('ntab', np.int32), ('cliprows', np.int32),
+                       ('jac', np.float64, (4,)), ('scale', np.float64), ('scale2', np.float64),
+                       ('flux', np.float64), ('p', np.float64, (6,))], align=True)
+STAMP_DTYPE = np.dtype([('N', np.int32), ('Nk', np.int32), ('nx', np.int32), ('ny', np.int32),
+                        ('x0', np.int32), ('y0', np.int32), ('op_begin', np.int32),
+                        ('op_count', np.int32), ('leaf_begin', np.int32), ('pad', np.int32),
+                        ('out_offset', np.int64), ('scratch_offset', np.int64),
+                        ('dk', np.float64)], align=True)
+assert OP_DTYPE.itemsize == 32 and LEAF_DTYPE.itemsize == 128 and STAMP_DTYPE.itemsize == 64
+
+OP_LEAF, OP_MUL, OP_ADD, OP_SCALE, OP_PHASE = 1, 2, 3, 4, 5
+LEAF_EXPONENTIAL, LEAF_GAUSSIAN, LEAF_MOFFAT, LEAF_KOLMOGOROV, LEAF_SERSIC, LEAF_BOX = range(1, 7)
+
+ERRORS = {-1: "invalid argument", -2: "parameter out of range", -3: "solve/table failure",
+          -4: "HIP runtime failure", -5: "unsupported FFT size"}
+
+
+class GalSimAMDError(RuntimeError):
+    """Raised for failures inside the native library (mirrors GalSimError)."""
+
+
+class GalSimAMDRangeError(GalSimAMDError, ValueError):
+    """A parameter outside the range the reference accepts (mirrors GalSimRangeError)."""
+
+
+_lib = None
+
+# name -> (restype, argtypes); every symbol declared in include/gsmi.h
+_D = ctypes.c_double
+_I = ctypes.c_int
+_P = ctypes.c_void_p
+_SIGS = {
+    'gsmi_last_error': (ctypes.c_char_p, []),
+    'gsmi_version': (_I, []),
+    'gsmi_good_fft_size': (_I, [_I]),
+    'gsmi_sersic_info': (_I, [_D, _P, _P, _P]),
+    'gsmi_sersic_table': (_I, [_D, _P, _P, _P, _P, _I]),
+    'gsmi_sersic_hlr': (_I, [_D, _D, _P]),
+    'gsmi_kolmogorov_info': (_I, [_P, _P]),
+    'gsmi_moffat_info': (_I, [_D, _D, _D, _P, _P]),
+    'gsmi_moffat_scale_radius_from_hlr': (_I, [_D, _D, _P]),
+    'gsmi_exponential_cliprows': (_I, [_D, _P, _P, _I, _D, _P]),
+    'gsmi_sersic_kvalue': (_I, [_D, _P, _P, _P, _I]),
+    'gsmi_draw_fft_f64': (_I, [_P, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P]),
+    'gsmi_draw_fft_f32': (_I, [_P, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P]),
+    'gsmi_draw_kimage_f64': (_I, [_P, _I, _P, _P, _P, _I, _I, _I, _I, _I, _D, _P, _P]),
+    'gsmi_irfft_f64': (_I, [_P, _I, _P, _P, _P]),
+    'gsmi_wrap_hermx_f64': (_I, [_P, _I, _I, _P, _P]),
+}
+
+
+def lib():
+    """The loaded library; raises GalSimAMDError if it has not been built."""
+    global _lib
+    if _lib is None:
+        # torch bundles its own HIP runtime (SONAME libamdhip64.so.7).  Loading torch first makes
+        # the dynamic loader resolve libgsmi's libamdhip64.so.7 to that same copy, so device
+        # pointers and streams are shared with torch; a second runtime in the process would not
+        # see torch's allocations.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
+        if not os.path.exists(LIB_PATH):
+            raise GalSimAMDError(
+                "galsim_amd native library not found at %s; build it with "
+                "`python -m galsim_amd.csrc.build`" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc, what=""):
+    if rc != 0:
+        msg = lib().gsmi_last_error().decode(errors='replace')
+        text = "%s failed (%s): %s" % (what, ERRORS.get(rc, rc), msg)
+        if rc == -2:
+            raise GalSimAMDRangeError(text)
+        raise GalSimAMDError(text)
+
+
+def ptr(a):
+    """Address of a numpy array or torch tensor (None -> NULL)."""
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data
+    return a.data_ptr()
+
+
+def gsp_array(gsparams):
+    if gsparams is None:
+        return None
+    return np.ascontiguousarray(gsparams._getinitargs_f(), dtype=np.float64)

@@ -1,0 +1,61 @@
+// Host-side profile numerics for galsim_amd: the FP64 scalars and lookup tables that size a
+// stamp and feed the device kernels (stepK/maxK, Sersic Hankel tables, Kolmogorov radial table,
+// Moffat maxK), plus the root finder, spline and Hankel machinery they are built from.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+namespace gsmi {
+
+// GSParams as a flat record, in the order of galsim/gsparams.py:126-131.
+struct GSP {
+    double minimum_fft_size, maximum_fft_size, folding_threshold, stepk_minimum_hlr,
+        maxk_threshold, kvalue_accuracy, xvalue_accuracy, table_spacing, realspace_relerr,
+        realspace_abserr, integration_relerr, integration_abserr, shoot_accuracy;
+    static GSP from(const double* g);
+    bool operator<(const GSP& o) const;
+};
+
+double expd(double x);
+int good_fft_size(int n);
+double gamma_p(double a, double x);
+
+// Natural cubic spline on (args, vals).
+struct Spline {
+    std::vector<double> args, vals, y2;
+    double da = 0.;
+    bool equal_spaced = true;
+    void build();
+    double lookup(double a) const;
+    double operator()(double a) const;   // 0 outside [args.front(), args.back()]
+};
+
+// Order-0 Hankel transform int_0^inf f(r) J0(k r) r dr by the Ogata (2005) quadrature.
+typedef double (*RadialFn)(double r, const void* ctx);
+double hankel_inf(RadialFn f, const void* ctx, double k, double relerr, double abserr);
+
+struct SersicInfo {
+    double n = 0., invn = 0., gamma2n = 0.;
+    double maxk = 0., stepk = 0., hlr = 0., b = 0.;
+    double kderiv2 = 0., kderiv4 = 0., ksq_min = 0., ksq_max = 0., highk_a = 0., highk_b = 0.;
+    Spline ft;    // f(k) k^2 as a function of log k
+};
+const SersicInfo& sersic_info(double n, const GSP& gsp);
+double sersic_hlr(double n, double flux_fraction);
+
+struct KolmogorovInfo { double maxk = 0., stepk = 0.; };
+const KolmogorovInfo& kolmogorov_info(const GSP& gsp);
+
+// Moffat (untruncated): kind code of the k-space formula, normalisations and sizes.
+struct MoffatInfo {
+    int kind = 0;           // 15, 2, 25, 3, 35, 4 or 0 (generic beta)
+    double knorm = 0., knorm2 = 0., maxk = 0., stepk = 0., rD = 0.;
+};
+MoffatInfo moffat_info(double beta, double rD, double flux, const GSP& gsp);
+double moffat_kv(int kind, double beta, double knorm2, double flux_over_knorm, double ksq);
+double moffat_scale_radius_from_hlr(double re, double beta);
+
+double bessel_k(double nu, double x);
+
+}  // namespace gsmi

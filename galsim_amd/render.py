@@ -1,0 +1,279 @@
+"""Host planning and device execution of the FFT drawing path.
+
+``draw_fft`` restates drawFFT_makeKImage's size planning (galsim/gsobject.py:1927-1973), lowers
+the profile tree into the k-space program of include/gsmi.h, packs the stamp record, program,
+leaves, Sersic tables and Exponential clip rows into one host buffer, copies it to the device
+in one transfer, and runs the two kernels of gsmi_draw_fft_f64/f32 on torch's current stream.
+"""
+import math
+import warnings
+
+import numpy as np
+
+from . import _lib
+
+_ALIGN = 256
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise _lib.GalSimAMDError("galsim_amd draws on the GPU; no HIP device is available")
+    return torch
+
+
+class Lowering:
+    """Accumulates the k-space program of one or more stamps."""
+
+    def __init__(self, precision='f64', Nk=None, dk=None):
+        self.precision = precision
+        self.Nk = Nk
+        self.dk = dk
+        self.ops = []
+        self.leaves = []
+        self.tables = []
+        self.tab_off = {}
+        self.ntab = 0
+        self.clip = []
+        self.nclip = 0
+
+    def emit(self, code, arg=0, a=0., b=0., c=0.):
+        self.ops.append((code, arg, a, b, c))
+
+    def add_leaf(self, rec):
+        self.leaves.append(rec.copy())
+        return len(self.leaves) - 1
+
+    def add_table(self, key, args, vals, y2):
+        if key not in self.tab_off:
+            self.tab_off[key] = self.ntab
+            self.tables.append(np.concatenate([args, vals, y2]))
+            self.ntab += 3 * len(args)
+        return self.tab_off[key]
+
+    def exponential_cliprows(self, r0, gsparams, jac):
+        """Host row ranges of the reference's Exponential clipping (FP64 path only)."""
+        if self.precision != 'f64' or self.Nk is None:
+            return -1
+        out = np.zeros(2 * (self.Nk + 1), dtype=np.int32)
+        j = None if jac is None else np.ascontiguousarray(np.asarray(jac, float).reshape(4))
+        _lib.check(_lib.lib().gsmi_exponential_cliprows(
+            float(r0), _lib.ptr(_lib.gsp_array(gsparams)), _lib.ptr(j), int(self.Nk),
+            float(self.dk), _lib.ptr(out)), "Exponential clip rows")
+        off = self.nclip
+        self.clip.append(out)
+        self.nclip += len(out)
+        return off
+
+    def ops_array(self):
+        a = np.zeros(len(self.ops), dtype=_lib.OP_DTYPE)
+        for i, (code, arg, x, y, z) in enumerate(self.ops):
+            a[i] = (code, arg, x, y, z)
+        return a
+
+    def leaves_array(self):
+        a = np.zeros(len(self.leaves), dtype=_lib.LEAF_DTYPE)
+        for i, r in enumerate(self.leaves):
+            a[i] = r
+        return a
+
+
+def check_stack(ops, begin=0, count=None):
+    """Depth of the post-order program; the kernels hold at most 8 values."""
+    depth = best = 0
+    end = len(ops) if count is None else begin + count
+    for code, *_ in ops[begin:end]:
+        if code == _lib.OP_LEAF:
+            depth += 1
+        elif code in (_lib.OP_MUL, _lib.OP_ADD):
+            depth -= 1
+        best = max(best, depth)
+    if depth != 1 or best > 8:
+        raise _lib.GalSimAMDError("profile tree too deep for the device program (depth %d)"
+                                  % best)
+
+
+def plan_sizes(prof, bounds):
+    """drawFFT_makeKImage size planning (galsim/gsobject.py:1927-1973) with image scale 1."""
+    N = prof.getGoodImageSize(1.0)
+    init = bounds._getinitargs()
+    image_N = max(int(np.max(np.abs(init))) * 2, int(np.max(bounds.numpyShape())))
+    N = max(N, image_N)
+    N = _lib.lib().gsmi_good_fft_size(int(N))
+    N = max(N, prof.gsparams.minimum_fft_size)
+    dk = 2. * np.pi / (N * 1.0)
+    maxk = prof.maxk
+    if N * dk / 2 > maxk:
+        Nk = N
+    else:
+        Nk = int(np.ceil(maxk / dk)) * 2
+    if Nk > prof.gsparams.maximum_fft_size:
+        warnings.warn("drawFFT requires a very large FFT (%d)" % Nk)
+    return N, Nk, dk
+
+
+class _Packer:
+    """Packs numpy sections into one byte buffer with 256-byte aligned offsets."""
+
+    def __init__(self):
+        self.parts = []
+        self.size = 0
+
+    def add(self, arr):
+        arr = np.ascontiguousarray(arr)
+        off = self.size
+        self.parts.append((off, arr.view(np.uint8).reshape(-1)))
+        self.size += (arr.nbytes + _ALIGN - 1) // _ALIGN * _ALIGN
+        return off
+
+    def host(self):
+        buf = np.zeros(max(self.size, _ALIGN), dtype=np.uint8)
+        for off, b in self.parts:
+            buf[off:off + len(b)] = b
+        return buf
+
+
+def _device_buffer(torch, host_bytes):
+    t = torch.from_numpy(host_bytes)
+    return t.to('cuda', non_blocking=False)
+
+
+def draw_fft(prof, bounds, precision='f64', return_device=False):
+    """Render ``prof`` (already in image coordinates) onto ``bounds`` (centred on the profile
+    origin, as drawFFT expects).  Returns (pixels, added_flux)."""
+    if precision not in ('f64', 'f32'):
+        raise ValueError("precision must be 'f64' or 'f32'")
+    lib = _lib.lib()
+    N, Nk, dk = plan_sizes(prof, bounds)
+    ctx = Lowering(precision, Nk, dk)
+    prof._lower(ctx, None)
+    check_stack(ctx.ops)
+    ny, nx = bounds.numpyShape()
+    if not (-N // 2 <= bounds.xmin and bounds.xmax <= N // 2 - 1 and
+            -N // 2 <= bounds.ymin and bounds.ymax <= N // 2 - 1):
+        raise _lib.GalSimAMDError("output bounds %r do not fit the N=%d grid" % (bounds, N))
+    stamp = np.zeros(1, dtype=_lib.STAMP_DTYPE)
+    stamp[0] = (N, Nk, nx, ny, bounds.xmin, bounds.ymin, 0, len(ctx.ops), 0, 0, 0, 0, dk)
+    torch = _torch()
+    pk = _Packer()
+    o_stamp = pk.add(stamp)
+    o_ops = pk.add(ctx.ops_array())
+    o_leaves = pk.add(ctx.leaves_array())
+    tables = np.concatenate(ctx.tables) if ctx.tables else np.zeros(1)
+    o_tab = pk.add(tables)
+    o_tab32 = pk.add(tables.astype(np.float32))
+    clip = np.concatenate(ctx.clip) if ctx.clip else np.zeros(2, dtype=np.int32)
+    o_clip = pk.add(clip)
+    dev = _device_buffer(torch, pk.host())
+    base = dev.data_ptr()
+    fdt = torch.float64 if precision == 'f64' else torch.float32
+    scratch = torch.empty(ny * (N // 2 + 1) * 2, dtype=fdt, device='cuda')
+    out = torch.empty((ny, nx), dtype=fdt, device='cuda')
+    stream = torch.cuda.current_stream().cuda_stream
+    if precision == 'f64':
+        rc = lib.gsmi_draw_fft_f64(base + o_stamp, 1, N, ny, base + o_ops, base + o_leaves,
+                                   base + o_tab, base + o_clip, scratch.data_ptr(),
+                                   out.data_ptr(), stream)
+    else:
+        rc = lib.gsmi_draw_fft_f32(base + o_stamp, 1, N, ny, base + o_ops, base + o_leaves,
+                                   base + o_tab, base + o_tab32, scratch.data_ptr(),
+                                   out.data_ptr(), stream)
+    _lib.check(rc, "gsmi_draw_fft_%s" % precision)
+    if return_device:
+        return out, (dev, scratch)
+    host = out.cpu().numpy()
+    return host, float(host.sum(dtype=float))
+
+
+def draw_kimage(prof, bounds, dk):
+    """k-image of ``prof`` on ``bounds`` with spacing dk (complex128)."""
+    lib = _lib.lib()
+    ctx = Lowering('f64')
+    prof._lower(ctx, None)
+    check_stack(ctx.ops)
+    ny, nx = bounds.numpyShape()
+    torch = _torch()
+    pk = _Packer()
+    o_ops = pk.add(ctx.ops_array())
+    o_leaves = pk.add(ctx.leaves_array())
+    tables = np.concatenate(ctx.tables) if ctx.tables else np.zeros(1)
+    o_tab = pk.add(tables)
+    dev = _device_buffer(torch, pk.host())
+    base = dev.data_ptr()
+    out = torch.empty((ny, nx, 2), dtype=torch.float64, device='cuda')
+    rc = lib.gsmi_draw_kimage_f64(base + o_ops, len(ctx.ops), base + o_leaves, base + o_tab,
+                                  None, bounds.xmin, bounds.ymin, nx, ny, 0, float(dk),
+                                  out.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    _lib.check(rc, "gsmi_draw_kimage_f64")
+    h = out.cpu().numpy()
+    return h[..., 0] + 1j * h[..., 1]
+
+
+def kvalue(prof, kx, ky):
+    """k-space value at (kx, ky): the program with the diagonal Jacobian diag(kx, ky) at the
+    grid point (1, 1) with dk = 1."""
+    lib = _lib.lib()
+    ctx = Lowering('f64')
+    prof._lower(ctx, np.array([[float(kx), 0.], [0., float(ky)]]))
+    check_stack(ctx.ops)
+    torch = _torch()
+    pk = _Packer()
+    o_ops = pk.add(ctx.ops_array())
+    o_leaves = pk.add(ctx.leaves_array())
+    tables = np.concatenate(ctx.tables) if ctx.tables else np.zeros(1)
+    o_tab = pk.add(tables)
+    dev = _device_buffer(torch, pk.host())
+    base = dev.data_ptr()
+    out = torch.empty(2, dtype=torch.float64, device='cuda')
+    rc = lib.gsmi_draw_kimage_f64(base + o_ops, len(ctx.ops), base + o_leaves, base + o_tab,
+                                  None, 1, 1, 1, 1, 0, 1.0, out.data_ptr(),
+                                  torch.cuda.current_stream().cuda_stream)
+    _lib.check(rc, "gsmi_draw_kimage_f64")
+    h = out.cpu().numpy()
+    return complex(h[0], h[1])
+
+
+def irfft(kwrap, N):
+    """irfft(shift_in, shift_out) of a wrapped (N, N/2+1) complex k-image (rows ky=-N/2..N/2-1)
+    -> (N, N) real image on (-N/2..N/2-1)^2, FP64, on the GPU."""
+    lib = _lib.lib()
+    torch = _torch()
+    k = torch.from_numpy(np.ascontiguousarray(kwrap, dtype=np.complex128).view(np.float64)).cuda()
+    scratch = torch.empty(N * (N // 2 + 1) * 2, dtype=torch.float64, device='cuda')
+    out = torch.empty((N, N), dtype=torch.float64, device='cuda')
+    _lib.check(lib.gsmi_irfft_f64(k.data_ptr(), int(N), out.data_ptr(), scratch.data_ptr(),
+                                  torch.cuda.current_stream().cuda_stream), "gsmi_irfft_f64")
+    return out.cpu().numpy()
+
+
+def wrap_hermx(karr, Nk, N):
+    """wrapImage(hermx=True) of a (Nk+1, Nk/2+1) k-image onto (N, N/2+1), on the GPU."""
+    lib = _lib.lib()
+    torch = _torch()
+    k = torch.from_numpy(np.ascontiguousarray(karr, dtype=np.complex128).view(np.float64)).cuda()
+    out = torch.empty((N, N // 2 + 1, 2), dtype=torch.float64, device='cuda')
+    _lib.check(lib.gsmi_wrap_hermx_f64(k.data_ptr(), int(Nk), int(N), out.data_ptr(),
+                                       torch.cuda.current_stream().cuda_stream),
+               "gsmi_wrap_hermx_f64")
+    h = out.cpu().numpy()
+    return h[..., 0] + 1j * h[..., 1]
+
+
+def inverse_fft(kimage):
+    """Image.calculate_inverse_fft (galsim/image.py:958-1016)."""
+    from .image import Image
+    from .position import BoundsI
+    b = kimage.bounds
+    if b.xmin != 0:
+        raise _lib.GalSimAMDError("calculate_inverse_fft requires xmin == 0")
+    No2 = max(-b.ymin, b.ymax, b.xmax)
+    N = 2 * No2
+    full = np.zeros((2 * No2 + 1, No2 + 1), dtype=np.complex128)
+    arr = kimage.array
+    full[b.ymin + No2: b.ymax + No2 + 1, 0: b.xmax + 1] = arr
+    kw = wrap_hermx(full, 2 * No2, N)
+    real = irfft(kw, N)
+    dx = 2. * np.pi / (N * kimage.scale)
+    im = Image(real, xmin=-N // 2, ymin=-N // 2)
+    im.scale = dx
+    return im

@@ -1,0 +1,100 @@
+"""Sersic and DeVaucouleurs, untruncated (galsim/sersic.py; src/SBSersic.cpp)."""
+import numpy as np
+
+from . import _lib
+from ._leaf import Leaf
+from .gsparams import GSParams
+
+_INFO = {}
+
+
+def sersic_info(n, gsparams):
+    """(scalars, args, vals, y2) of the FP64 k-space table for index n (host, cached)."""
+    key = (float(n), gsparams)
+    if key not in _INFO:
+        g = _lib.gsp_array(gsparams)
+        out = np.zeros(10)
+        ntab = np.zeros(1, dtype=np.int32)
+        _lib.check(_lib.lib().gsmi_sersic_info(float(n), _lib.ptr(g), _lib.ptr(out),
+                                               _lib.ptr(ntab)), "Sersic info")
+        m = int(ntab[0])
+        args, vals, y2 = np.zeros(m), np.zeros(m), np.zeros(m)
+        _lib.check(_lib.lib().gsmi_sersic_table(float(n), _lib.ptr(g), _lib.ptr(args),
+                                                _lib.ptr(vals), _lib.ptr(y2), m), "Sersic table")
+        _INFO[key] = (out, args, vals, y2)
+    return _INFO[key]
+
+
+class Sersic(Leaf):
+    _leaf_type = _lib.LEAF_SERSIC
+    _is_analytic_x = True
+    _minimum_n = 0.3
+    _maximum_n = 6.2
+
+    def __init__(self, n, half_light_radius=None, scale_radius=None, flux=1., trunc=0.,
+                 flux_untruncated=False, gsparams=None):
+        self._n = float(n)
+        self._flux = float(flux)
+        self._trunc = float(trunc)
+        self._gsparams = GSParams.check(gsparams)
+        if self._n < Sersic._minimum_n or self._n > Sersic._maximum_n:
+            raise _lib.GalSimAMDRangeError("Requested Sersic index %r out of range" % self._n)
+        if self._trunc != 0.:
+            raise NotImplementedError("truncated Sersic profiles are not part of galsim_amd yet")
+        if half_light_radius is not None:
+            if scale_radius is not None:
+                raise ValueError("Only one of scale_radius or half_light_radius may be specified")
+            out = np.zeros(1)
+            _lib.check(_lib.lib().gsmi_sersic_hlr(self._n, 1., _lib.ptr(out)), "SersicHLR")
+            self._hlr = float(half_light_radius)
+            self._r0 = self._hlr / float(out[0])
+        elif scale_radius is not None:
+            self._r0 = float(scale_radius)
+            self._hlr = 0.
+        else:
+            raise ValueError("Either scale_radius or half_light_radius must be specified")
+
+    @property
+    def n(self):
+        return self._n
+
+    @property
+    def scale_radius(self):
+        return self._r0
+
+    @property
+    def half_light_radius(self):
+        if self._hlr == 0.:
+            self._hlr = self._r0 * float(sersic_info(self._n, self.gsparams)[0][2])
+        return self._hlr
+
+    @property
+    def _maxk(self):
+        return float(sersic_info(self._n, self.gsparams)[0][0]) / self._r0
+
+    @property
+    def _stepk(self):
+        return float(sersic_info(self._n, self.gsparams)[0][1]) / self._r0
+
+    def _fill(self, rec, ctx, jac):
+        sc, args, vals, y2 = sersic_info(self._n, self.gsparams)
+        rec['scale'] = self._r0
+        rec['flux'] = self._flux
+        rec['table'] = ctx.add_table((self._n, self.gsparams), args, vals, y2)
+        rec['ntab'] = len(args)
+        rec['p'][:] = [sc[4], sc[5], sc[6], sc[7], sc[8], sc[9]]
+
+    def withFlux(self, flux):
+        return Sersic(self._n, scale_radius=self._r0, flux=flux, gsparams=self.gsparams)
+
+    def __repr__(self):
+        return 'galsim_amd.Sersic(n=%r, scale_radius=%r, flux=%r)' % (self._n, self._r0,
+                                                                      self._flux)
+
+
+class DeVaucouleurs(Sersic):
+    def __init__(self, half_light_radius=None, scale_radius=None, flux=1., trunc=0.,
+                 flux_untruncated=False, gsparams=None):
+        super().__init__(4, half_light_radius=half_light_radius, scale_radius=scale_radius,
+                         flux=flux, trunc=trunc, flux_untruncated=flux_untruncated,
+                         gsparams=gsparams)

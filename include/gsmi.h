@@ -1,0 +1,171 @@
+/*
+ * gsmi -- C ABI of the galsim_amd library (libgsmi.so): GalSim's FFT rendering path,
+ * GSObject.drawImage(method='fft'), on AMD Instinct MI355X (gfx950).
+ *
+ * Plain pointers and sizes only.  Device pointers are HIP device allocations owned by the
+ * caller; every launch takes an explicit hipStream_t (passed as void*; 0 = null stream).
+ * Every function returns 0 on success or a negative GSMI_E* code; gsmi_last_error() returns a
+ * thread-local message for the last failure on the calling thread.  No function needs the
+ * Python GIL; the host table caches are guarded by a mutex.
+ *
+ * Each entry point names the reference interface it replaces (file:line in GalSim 2.8.4).
+ */
+#ifndef GSMI_H
+#define GSMI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GSMI_OK 0
+#define GSMI_E_ARG (-1)       /* invalid argument (shape, size, NaN parameter, ...)      */
+#define GSMI_E_RANGE (-2)     /* parameter out of the reference's allowed range           */
+#define GSMI_E_SOLVE (-3)     /* a root solve or table build failed                       */
+#define GSMI_E_HIP (-4)       /* a HIP runtime call failed                                */
+#define GSMI_E_SIZE (-5)      /* FFT size beyond what the kernels support                 */
+
+/* ---------------------------------------------------------------------------------------
+ * Lowered stamp description.
+ *
+ * A profile tree (Transformation / Convolution / Sum over leaves) is lowered on the host into
+ * a post-order program evaluated at every k-space sample of the stamp's k-image:
+ *   LEAF i    push leaf i evaluated at (kx', ky') = J_i^T (kx, ky)      (SBProfile::drawK)
+ *   MUL       pop b, a; push a*b                                        (Convolution)
+ *   ADD       pop b, a; push a+b                                        (Sum)
+ *   SCALE s   top *= s                                                  (Transformation flux)
+ *   PHASE     top *= fs * exp(-i (ux kx + uy ky))                       (ApplyKImagePhases)
+ * ------------------------------------------------------------------------------------- */
+enum {
+    GSMI_OP_LEAF = 1,
+    GSMI_OP_MUL = 2,
+    GSMI_OP_ADD = 3,
+    GSMI_OP_SCALE = 4,
+    GSMI_OP_PHASE = 5
+};
+
+enum {
+    GSMI_LEAF_EXPONENTIAL = 1,
+    GSMI_LEAF_GAUSSIAN = 2,
+    GSMI_LEAF_MOFFAT = 3,
+    GSMI_LEAF_KOLMOGOROV = 4,
+    GSMI_LEAF_SERSIC = 5,
+    GSMI_LEAF_BOX = 6
+};
+
+typedef struct gsmi_op {
+    int32_t code;
+    int32_t arg;          /* leaf index for LEAF */
+    double a, b, c;       /* SCALE: a; PHASE: ux, uy, fs */
+} gsmi_op;
+
+typedef struct gsmi_leaf {
+    int32_t type;         /* GSMI_LEAF_* */
+    int32_t separable;    /* 1: diagonal/no Jacobian (the reference's quadrant fill) */
+    int32_t kind;         /* Moffat: 15,2,25,3,35,4 or 0; others 0 */
+    int32_t table;        /* Sersic: offset of its table in the table buffer (doubles) */
+    int32_t ntab;         /* Sersic: number of knots */
+    int32_t cliprows;     /* Exponential (FP64): offset of its int2 row ranges, -1 if none */
+    double jac[4];        /* composite Jacobian (dudx, dudy, dvdx, dvdy) of the leaf       */
+    double scale;         /* r0, sigma, rD, 1/k0, r0, or width/2pi                         */
+    double scale2;        /* Box: height/2pi                                                */
+    double flux;          /* flux (Moffat: knorm)                                            */
+    double p[6];          /* type-specific constants (see gsmi_kernels.hip)                  */
+} gsmi_leaf;
+
+typedef struct gsmi_stamp {
+    int32_t N;            /* FFT (wrap) size                                                  */
+    int32_t Nk;           /* k-image size before wrapping (Nk >= N)                           */
+    int32_t nx, ny;       /* output stamp shape                                               */
+    int32_t x0, y0;       /* position of the first output pixel on the N-grid (-N/2 <= x0)    */
+    int32_t op_begin;     /* first op of this stamp's program                                 */
+    int32_t op_count;
+    int32_t leaf_begin;   /* leaf indices in LEAF ops are relative to this                    */
+    int32_t pad;
+    int64_t out_offset;   /* element offset of the stamp in the output buffer                 */
+    int64_t scratch_offset; /* element offset (complex) of the stamp's column-FFT rows        */
+    double dk;            /* k spacing in pixel units: 2 pi / N                               */
+} gsmi_stamp;
+
+const char* gsmi_last_error(void);
+int gsmi_version(void);
+
+/* ---------------------------------------------------------------------------------------
+ * Host-side profile scalars and tables (FP64, computed on the CPU; no device needed).
+ * gsp: 13 doubles in the order of GSParams(...) (galsim/gsparams.py:126-131), NULL = defaults.
+ * ------------------------------------------------------------------------------------- */
+
+/* goodFFTSize (src/Image.cpp:1141-1152; pybind pysrc/Image.cpp:85). */
+int gsmi_good_fft_size(int n);
+
+/* SBSersic info (src/SBSersic.cpp:223-537 SersicInfo; pysrc/SBSersic.cpp SBSersic ctor):
+ * out = {maxK, stepK, HLR, b, ksq_min, ksq_max, highk_a, highk_b, kderiv2, kderiv4} in units
+ * of the scale radius; *ntab = number of table knots.  Untruncated profiles only. */
+int gsmi_sersic_info(double n, const double* gsp, double* out, int* ntab);
+/* The Sersic k-space table: knots (log k), values f(k) k^2 and spline second derivatives. */
+int gsmi_sersic_table(double n, const double* gsp, double* args, double* vals, double* y2,
+                      int cap);
+/* SersicHLR (src/SBSersic.cpp:665-669; pybind pysrc/SBSersic.cpp). */
+int gsmi_sersic_hlr(double n, double flux_fraction, double* out);
+/* KolmogorovInfo (src/SBKolmogorov.cpp:337-419): out = {maxK, stepK} in units of k0. */
+int gsmi_kolmogorov_info(const double* gsp, double* out);
+/* SBMoffat untruncated (src/SBMoffat.cpp:163-244, 497-549): out = {maxK, stepK, knorm,
+ * knorm2, kind}. */
+int gsmi_moffat_info(double beta, double scale_radius, double flux, const double* gsp,
+                     double* out);
+/* MoffatCalculateScaleRadiusFromHLR, untruncated (src/SBMoffat.cpp:103-126). */
+int gsmi_moffat_scale_radius_from_hlr(double hlr, double beta, double* out);
+/* Exponential k-range clipping of the drawFFT k-image (bounds 0..Nk/2, -Nk/2..Nk/2, spacing dk)
+ * for a leaf of scale radius r0 drawn with Jacobian jac (NULL = identity): out[2j], out[2j+1] =
+ * [i1, i2) of row j (ky index j - Nk/2), as SBExponentialImpl::fillKImage computes them through
+ * GetKValueRange1d/2d (src/SBProfile.cpp:508-588, src/SBExponential.cpp:442-510). */
+int gsmi_exponential_cliprows(double r0, const double* gsp, const double* jac, int Nk, double dk,
+                              int32_t* out);
+/* Host evaluation of a Sersic leaf k-value (SersicInfo::kValue, src/SBSersic.cpp:363-376). */
+int gsmi_sersic_kvalue(double n, const double* gsp, const double* ksq, double* out, int count);
+
+/* ---------------------------------------------------------------------------------------
+ * Device rendering.  All buffers are device pointers; stamps/ops/leaves/tables are device
+ * copies of the lowered description; scratch holds sum(ny * (N/2+1)) complex values.
+ * ------------------------------------------------------------------------------------- */
+
+/* drawImage(method='fft') in FP64 for a batch of stamps that share one FFT size N (max_ny =
+ * tallest output stamp; scratch: sum over stamps of ny * (N/2+1) complex128) (the
+ * reference: gsobject.py:1927-2045 drawFFT_makeKImage + _drawKImage + drawFFT_finish, with
+ * SBProfile::drawK src/SBProfile.cpp:383-425, ApplyKImagePhases src/SBTransform.cpp:763-883,
+ * wrapImage src/Image.cpp:577-726 and irfft src/Image.cpp:814-911).  out is float64. */
+int gsmi_draw_fft_f64(const gsmi_stamp* stamps, int nstamps, int N, int max_ny,
+                      const gsmi_op* ops, const gsmi_leaf* leaves, const double* tables,
+                      const int32_t* cliprows, void* scratch, double* out, void* stream);
+
+/* The same in FP32 arithmetic (FP64 kept for the host tables, the Moffat Bessel K and the flux
+ * sums); tables32 is the Sersic table rounded to float; out is float32.  Batched: thousands of
+ * stamps of one FFT size per call.  max_ny = tallest output stamp. */
+int gsmi_draw_fft_f32(const gsmi_stamp* stamps, int nstamps, int N, int max_ny,
+                      const gsmi_op* ops, const gsmi_leaf* leaves, const double* tables,
+                      const float* tables32, void* scratch, float* out, void* stream);
+
+/* Evaluate a stamp program on a k-image with bounds (xmin..xmin+nx-1, ymin..ymin+ny-1) and
+ * spacing dk, writing complex128 values row-major (SBProfile::drawK src/SBProfile.cpp:383-425
+ * through the Python _drawKImage tree, gsobject.py:2464-2583 drawKImage). */
+int gsmi_draw_kimage_f64(const gsmi_op* ops, int nops, const gsmi_leaf* leaves,
+                         const double* tables, const int32_t* cliprows, int xmin, int ymin,
+                         int nx, int ny, int Nk, double dk, double* out, void* stream);
+
+/* irfft(in, out, shift_in=true, shift_out=true) (src/Image.cpp:814-911; pybind
+ * pysrc/Image.cpp:76): in is the wrapped (N/2+1) x N complex128 k-image, rows ky=-N/2..N/2-1
+ * (row-major, x fastest); out is the N x N float64 real image on (-N/2..N/2-1)^2; scratch holds
+ * N * (N/2+1) complex128. */
+int gsmi_irfft_f64(const double* kin, int N, double* out, void* scratch, void* stream);
+
+/* wrapImage(hermx=true, hermy=false) (src/Image.cpp:577-726; pybind pysrc/Image.cpp:80):
+ * folds the (Nk/2+1) x (Nk+1) k-image (rows ky=-Nk/2..Nk/2) onto (N/2+1) x N (rows
+ * ky=-N/2..N/2-1).  Both complex128 row-major. */
+int gsmi_wrap_hermx_f64(const double* kin, int Nk, int N, double* kout, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,157 @@
+"""GPU parity of the HIP drawing path against the oracle (FP64: 1e-10 of the peak pixel;
+FP32: 1e-5 of the peak pixel)."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+import galsim_amd as g
+from galsim_amd import render
+from oracle import draw as od
+from tests.fitsio import read_fits
+
+F64_TOL = 1e-10
+F32_TOL = 1e-5
+
+
+def _both(make):
+    return make(g), make(od)
+
+
+def _peak_rel(a, b):
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+def _draw_pair(make, nx, ny, scale=0.2, precision='f64', **kw):
+    prod, orc = _both(make)
+    im = prod.drawImage(nx=nx, ny=ny, scale=scale, dtype=np.float64, precision=precision, **kw)
+    ref, info = orc.drawImage(nx, ny, scale=scale, return_info=True, **kw)
+    return im, ref, info
+
+
+def _expmoffat(m, hlr=1.5, g1=0.1, g2=-0.05, beta=3, fwhm=0.7):
+    gal = m.Exponential(half_light_radius=hlr, flux=1e4).shear(g1=g1, g2=g2)
+    return m.Convolve([gal, m.Moffat(beta=beta, fwhm=fwhm)])
+
+
+def test_exponential_moffat_stamp0():
+    # Exponential(hlr=1.5", flux=1e4) sheared (0.1, -0.05), offset (0.13, -0.21) px, with
+    # Moffat(beta=3, fwhm=0.7") on a 64x64 stamp at 0.2"/px
+    im, ref, info = _draw_pair(lambda m: _expmoffat(m), 64, 64, offset=(0.13, -0.21))
+    print("N", info['N'], "Nk", info['Nk'], "peak-rel", _peak_rel(im.array, ref))
+    assert _peak_rel(im.array, ref) < F64_TOL
+    assert abs(im.added_flux - info['added_flux']) < 1e-10 * info['added_flux']
+
+
+def test_exponential_moffat_seeded_sweep():
+    rng = np.random.default_rng(1234)
+    worst = 0.
+    for _ in range(12):
+        hlr = rng.uniform(0.8, 2.5)
+        gmag = rng.uniform(0, 0.5)
+        phi = rng.uniform(0, np.pi)
+        g1, g2 = gmag * np.cos(2 * phi), gmag * np.sin(2 * phi)
+        off = tuple(rng.uniform(-0.5, 0.5, 2))
+        im, ref, info = _draw_pair(lambda m: _expmoffat(m, hlr, g1, g2), 64, 64, offset=off)
+        worst = max(worst, _peak_rel(im.array, ref))
+    print("worst peak-rel", worst)
+    assert worst < F64_TOL
+
+
+@pytest.mark.parametrize("name,make,shape", [
+    ("gaussian_sheared", lambda m: m.Gaussian(sigma=0.7, flux=3.).shear(e1=0.3, e2=0.1),
+     (40, 40)),
+    ("kolmogorov_folded", lambda m: m.Kolmogorov(fwhm=0.6, flux=2.), (32, 32)),
+    ("moffat_15", lambda m: m.Moffat(1.5, scale_radius=0.4), (48, 48)),
+    ("moffat_25_shift", lambda m: m.Moffat(2.5, half_light_radius=0.5).shift(0.1, -0.2),
+     (48, 48)),
+    ("moffat_generic", lambda m: m.Moffat(2.7, fwhm=0.9).shear(g1=-0.2, g2=0.15), (48, 48)),
+    ("sersic_15", lambda m: m.Sersic(1.5, half_light_radius=0.6, flux=5.), (48, 48)),
+    ("sersic_4_conv", lambda m: m.Convolve([m.Sersic(4., half_light_radius=0.8),
+                                            m.Kolmogorov(fwhm=0.8)]), (64, 64)),
+    ("bulge_disk", lambda m: m.Convolve([
+        (m.Exponential(half_light_radius=1.0, flux=600.) +
+         m.Sersic(2.5, half_light_radius=0.5, flux=400.)).shear(g1=0.3, g2=-0.2),
+        m.Kolmogorov(fwhm=0.75)]), (64, 64)),
+    ("odd_nonsquare", lambda m: _expmoffat(m, 1.1, 0.2, 0.3), (37, 52)),
+    ("exp_small_clipped", lambda m: m.Exponential(scale_radius=0.02).shear(g1=0.4, g2=0.1),
+     (24, 24)),
+])
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_profiles_parity(name, make, shape, precision):
+    im, ref, info = _draw_pair(make, shape[0], shape[1], precision=precision)
+    err = _peak_rel(im.array, ref)
+    print(name, precision, "N", info['N'], "Nk", info['Nk'], "peak-rel", err)
+    assert err < (F64_TOL if precision == 'f64' else F32_TOL)
+
+
+def test_methods_and_centering():
+    for method in ('fft', 'sb'):
+        for utc in (True, False):
+            make = lambda m: m.Convolve([m.Gaussian(sigma=0.5), m.Exponential(scale_radius=0.3)])
+            prod, orc = _both(make)
+            im = prod.drawImage(nx=30, ny=30, scale=0.25, dtype=np.float64, method=method,
+                                use_true_center=utc)
+            ref = orc.drawImage(30, 30, scale=0.25, method=method, use_true_center=utc)
+            assert _peak_rel(im.array, ref) < F64_TOL
+
+
+def test_auto_size():
+    make = lambda m: m.Convolve([m.Exponential(half_light_radius=0.6), m.Moffat(3.5, fwhm=0.6)])
+    prod, orc = _both(make)
+    im = prod.drawImage(scale=0.2, dtype=np.float64)
+    ref = orc.drawImage(scale=0.2)
+    assert im.array.shape == ref.shape
+    assert _peak_rel(im.array, ref) < F64_TOL
+
+
+def test_golden_gauss_smallshear_convolve_box(golden_dir):
+    # the reference's FFT-path golden (tests/test_convolve.py:220-236), 5 decimals
+    saved = read_fits(os.path.join(golden_dir, "gauss_smallshear_convolve_box.fits"))
+    psf = g.Gaussian(flux=1, sigma=1).shear(e1=0.04, e2=0.0)
+    conv = g.Convolve([psf, g.Pixel(scale=0.2, flux=1.)])
+    img = g.ImageF(saved.shape[1], saved.shape[0], scale=0.2)
+    img.setCenter(0, 0)
+    conv.drawImage(img, scale=0.2, method="sb", use_true_center=False)
+    np.testing.assert_array_almost_equal(img.array, saved, 5)
+
+
+def test_drawkimage_parity():
+    make = lambda m: m.Exponential(scale_radius=13., flux=4.1)
+    prod, orc = _both(make)
+    for scale in (0.15 / 13., 0.6 / 13.):
+        kim = g.ImageCD(45, 45, scale=scale)
+        prod.drawKImage(kim)
+        ref = orc.drawKImage(45, 45, scale)
+        assert np.max(np.abs(kim.array - ref)) < 1e-12 * np.max(np.abs(ref))
+    make2 = lambda m: m.Convolve([m.Sersic(2.2, half_light_radius=0.7).shear(g1=0.2, g2=0.3)
+                                  .shift(0.3, 0.1), m.Kolmogorov(fwhm=0.5)])
+    prod, orc = _both(make2)
+    kim = g.ImageCD(31, 41, scale=0.3)
+    prod.drawKImage(kim)
+    ref = orc.drawKImage(31, 41, 0.3)
+    assert np.max(np.abs(kim.array - ref)) < 1e-12 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("Nk,N", [(64, 64), (96, 64), (130, 96), (256, 128)])
+def test_wrap_and_irfft_kernels(Nk, N):
+    rng = np.random.default_rng(Nk + N)
+    karr = rng.normal(size=(Nk + 1, Nk // 2 + 1)) + 1j * rng.normal(size=(Nk + 1, Nk // 2 + 1))
+    kw = render.wrap_hermx(karr, Nk, N)
+    kw_ref = od.wrap_hermx(karr, Nk, N)
+    np.testing.assert_allclose(kw, kw_ref, rtol=0, atol=1e-12)
+    real = render.irfft(kw_ref, N)
+    real_ref = od.irfft_shifted(kw_ref, N)
+    assert np.max(np.abs(real - real_ref)) < 1e-13 * np.max(np.abs(real_ref))
+
+
+def test_large_fft_sizes():
+    # big Sersic galaxies need N = 1024 and beyond
+    make = lambda m: m.Convolve([m.Sersic(4.5, half_light_radius=4.), m.Kolmogorov(fwhm=0.8)])
+    im, ref, info = _draw_pair(make, 96, 96, precision='f64')
+    print("N", info['N'], "peak-rel", _peak_rel(im.array, ref))
+    assert info['N'] >= 512
+    assert _peak_rel(im.array, ref) < F64_TOL
