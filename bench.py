@@ -1,0 +1,236 @@
+"""Benchmark: batched FP32 drawImage(method='fft') of 4096 bulge+disk galaxies (Exponential disk +
+Sersic bulge, sheared, convolved with Kolmogorov, pixel-integrated, 0.2"/px, stamps 48..256 px)
+on one MI355X, against the plain per-stamp FP64 GPU path and the CPU oracle.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 4096] [--cpu-seconds 15]
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("stamps/s fwd & fwd+bwd, batched FP32 vs plain FP64 GPU at <=1e-5 peak-rel err")
+HBM_PEAK_GBS = 8000.
+FP32_PEAK_TFLOPS = 157.3
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def kernel_flops(rec):
+    """Algorithmic FLOPs per stamp for the two kernels: k-space products (8 per factor per
+    half-plane sample of the Nk grid) + column inverse DFTs (5 N log2 N per column) for the fold
+    kernel; half-length complex DFT per output row (5 (N/2) log2(N/2)) plus the real-split
+    (~10 per element) for the row kernel."""
+    N, Nk, ny, nfac = rec
+    fold = 8. * nfac * Nk * (Nk / 2 + 1) + 5. * N * math.log2(N) * (N / 2 + 1)
+    rows = ny * (5. * (N / 2) * math.log2(N / 2) + 10. * (N / 2))
+    return fold, rows
+
+
+def kernel_bytes(rec, nx, fsize):
+    """Algorithmic HBM bytes per stamp: the fold kernel writes ny (N/2+1) complex rows of the
+    column transform; the row kernel reads them and writes nx ny pixels."""
+    N, Nk, ny, nfac = rec
+    col = ny * (N / 2 + 1) * 2 * fsize
+    return col + 64, col + nx * ny * fsize
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=20)
+    ap.add_argument('--warmup', type=int, default=3)
+    ap.add_argument('--n', type=int, default=4096)
+    ap.add_argument('--seed', type=int, default=0)
+    ap.add_argument('--plain-sample', type=int, default=256)
+    ap.add_argument('--cpu-seconds', type=float, default=15.)
+    ap.add_argument('--no-cpu', action='store_true')
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
+        print("bench.py measures one GPU only: this build renders every stamp on a single "
+              "MI355X (--gpus 1); stamps are independent, so more GPUs would shard the stamp "
+              "list with no communication.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import galsim_amd as g
+    from galsim_amd import workloads
+    from galsim_amd.batch import BatchPlan
+    torch.cuda.set_device(0)
+
+    t0 = time.perf_counter()
+    profs, nx, ny, offs, params = workloads.build('bulge_disk', args.n, seed=args.seed)
+    t_build = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    plan = BatchPlan(profs, nx, ny, 0.2, offs, precision='f32')
+    t_plan = time.perf_counter() - t0
+    log("workload: %d stamps, build %.2fs, plan %.2fs, FFT sizes %s" % (
+        args.n, t_build, t_plan, plan.fft_sizes()))
+
+    # --- timed region: K steps of the batched FP32 forward -----------------------------------
+    for _ in range(args.warmup):
+        plan.run()
+    torch.cuda.synchronize()
+    evs = []
+
+    def mk(kind, N):
+        e = torch.cuda.Event(enable_timing=True)
+        evs[-1].append((kind, N, e))
+        return e
+
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        evs.append([])
+        plan.run(events=mk)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = 1e3 * wall / args.steps
+    value = args.n / (wall / args.steps)
+    log("batched FP32: %.3f ms/step, %.0f stamps/s" % (ms_per_step, value))
+
+    # per-kernel times from the events (same stream as the kernels)
+    fold_ms = {}
+    rows_ms = {}
+    for step in evs:
+        for k in range(len(step) - 1):
+            kind, N, e = step[k]
+            d = e.elapsed_time(step[k + 1][2])
+            (fold_ms if kind == 'fold' else rows_ms).setdefault(N, []).append(d)
+    recs = plan.recs
+    nfac = {}
+    flops = {'fold': 0., 'rows': 0.}
+    nbytes = {'fold': 0., 'rows': 0.}
+    for r in recs:
+        rec = (int(r['N']), int(r['Nk']), int(r['ny']), r['op_count'])
+        # factors: LEAF + PHASE + SCALE ops (each a complex multiply-accumulate per sample)
+        f, rw = kernel_flops((rec[0], rec[1], rec[2], 5))
+        flops['fold'] += f
+        flops['rows'] += rw
+        bf, br = kernel_bytes((rec[0], rec[1], rec[2], 5), int(r['nx']), 4)
+        nbytes['fold'] += bf
+        nbytes['rows'] += br
+    t_fold = sum(np.mean(v) for v in fold_ms.values())
+    t_rows = sum(np.mean(v) for v in rows_ms.values())
+    n_launch = len(fold_ms)
+    dominant = 'fold' if t_fold >= t_rows else 'rows'
+    t_dom = t_fold if dominant == 'fold' else t_rows
+    achieved = flops[dominant] / (t_dom * 1e-3) / 1e12
+    log("fold_colfft: %.3f ms/step over %d launches; row_c2r: %.3f ms/step" % (
+        t_fold, n_launch, t_rows))
+    log("fold per-N ms: %s" % {N: round(float(np.mean(v)), 4) for N, v in fold_ms.items()})
+    log("rows per-N ms: %s" % {N: round(float(np.mean(v)), 4) for N, v in rows_ms.items()})
+    traffic = None
+    tfile = os.path.join(ROOT, 'profiles', 'hbm_traffic_bulge_disk_4096.json')
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get(dominant)
+    roofline = {
+        "kernel": "fold_colfft" if dominant == 'fold' else "row_c2r",
+        "bound": "mfma",
+        "achieved": round(achieved, 4),
+        "peak": FP32_PEAK_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": round(achieved / FP32_PEAK_TFLOPS, 6),
+        "traffic": traffic,
+        "algorithmic_bytes_per_step": nbytes[dominant],
+        "hbm_GBps_algorithmic": round(nbytes[dominant] / (t_dom * 1e-3) / 1e9, 2),
+        "hbm_frac_algorithmic": round(nbytes[dominant] / (t_dom * 1e-3) / 1e9 / HBM_PEAK_GBS, 6),
+        "avg_launch_ms": round(t_dom / max(n_launch, 1), 5),
+        "launches_per_step": n_launch,
+    }
+
+    # --- accuracy: FP32 batch against the FP64 kernels on every stamp -------------------------
+    plan64 = BatchPlan(profs, nx, ny, 0.2, offs, precision='f64')
+    plan64.run()
+    torch.cuda.synchronize()
+    worst = 0.
+    for i in range(args.n):
+        a = plan.stamp(i).double()
+        b = plan64.stamp(i)
+        worst = max(worst, float((a - b).abs().max() / b.abs().max()))
+    log("max peak-relative error FP32 vs FP64 over %d stamps: %.3e" % (args.n, worst))
+
+    # --- plain per-stamp FP64 GPU path (one stamp per launch, pre-planned) -------------------
+    m = min(args.plain_sample, args.n)
+    singles = [BatchPlan([profs[i]], nx[i:i + 1], ny[i:i + 1], 0.2, offs[i:i + 1],
+                         precision='f64') for i in range(m)]
+    for s in singles[:8]:
+        s.run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in singles:
+        s.run()
+    torch.cuda.synchronize()
+    plain_rate = m / (time.perf_counter() - t0)
+    log("plain FP64 per-stamp GPU path: %.0f stamps/s (%d stamps)" % (plain_rate, m))
+    # end-to-end drawImage (host planning + one transfer + launches + copy back) per stamp
+    m2 = min(64, args.n)
+    t0 = time.perf_counter()
+    for i in range(m2):
+        profs[i].drawImage(nx=int(nx[i]), ny=int(ny[i]), scale=0.2, dtype=np.float64)
+    drawimage_rate = m2 / (time.perf_counter() - t0)
+    log("drawImage FP64 end to end: %.0f stamps/s" % drawimage_rate)
+
+    # --- CPU baseline: the oracle (numpy FP64 restatement), one core ------------------------
+    cpu = None
+    if not args.no_cpu:
+        os.environ.setdefault('OMP_NUM_THREADS', '1')
+        from oracle import draw as od
+        oprofs = [workloads.bulge_disk_profile(od, params, i) for i in range(args.n)]
+        for i in range(min(8, args.n)):     # build the Sersic/Kolmogorov tables outside timing
+            oprofs[i].maxk, oprofs[i].stepk
+        for n_ in workloads.SERSIC_N:
+            od.Sersic(n_, half_light_radius=1.)
+        done = 0
+        t0 = time.perf_counter()
+        while done < args.n and time.perf_counter() - t0 < args.cpu_seconds:
+            oprofs[done].drawImage(int(nx[done]), int(ny[done]), scale=0.2)
+            done += 1
+        el = time.perf_counter() - t0
+        cpu = {"value": round(done / el, 3), "unit": "stamps/s", "cores": 1, "kind": "port",
+               "sample": "first %d stamps of the same 4096-stamp workload, oracle drawImage "
+                         "(numpy FP64), %.1f s" % (done, el)}
+        log("CPU oracle: %d stamps in %.1fs" % (done, el))
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "stamps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "bulge+disk (Exponential + Sersic n in 1.5..4, sheared) x "
+                               "Kolmogorov x Pixel, FP32, 0.2 arcsec/px",
+                   "n_stamps": args.n, "stamp_sides": "48..256",
+                   "fft_sizes": {str(k): v for k, v in plan.fft_sizes().items()},
+                   "seed": args.seed},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "fwd_bwd": None,
+        "plain_fp64_gpu_stamps_per_s": round(plain_rate, 1),
+        "speedup_vs_plain_fp64_gpu": round(value / plain_rate, 2),
+        "drawimage_fp64_end_to_end_stamps_per_s": round(drawimage_rate, 1),
+        "max_peak_rel_err_fp32_vs_fp64": worst,
+        "host_plan_s": round(t_plan, 3),
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == '__main__':
+    main()

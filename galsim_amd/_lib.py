@@ -18,10 +18,12 @@ LEAF_DTYPE = np.dtype([('type', np.int32), ('separable', np.int32), ('kind', np.
                        ('flux', np.float64), ('p', np.float64, (6,))], align=True)
 STAMP_DTYPE = np.dtype([('N', np.int32), ('Nk', np.int32), ('nx', np.int32), ('ny', np.int32),
                         ('x0', np.int32), ('y0', np.int32), ('op_begin', np.int32),
-                        ('op_count', np.int32), ('leaf_begin', np.int32), ('pad', np.int32),
-                        ('out_offset', np.int64), ('scratch_offset', np.int64),
+                        ('op_count', np.int32), ('leaf_begin', np.int32),
+                        ('axis_count', np.int32), ('out_offset', np.int64),
+                        ('scratch_offset', np.int64), ('axis_offset', np.int64),
                         ('dk', np.float64)], align=True)
-assert OP_DTYPE.itemsize == 32 and LEAF_DTYPE.itemsize == 128 and STAMP_DTYPE.itemsize == 64
+assert OP_DTYPE.itemsize == 32 and LEAF_DTYPE.itemsize == 128 and STAMP_DTYPE.itemsize == 72
+STACK_DEPTH = {'f64': 6, 'f32': 4}
 
 OP_LEAF, OP_MUL, OP_ADD, OP_SCALE, OP_PHASE = 1, 2, 3, 4, 5
 LEAF_EXPONENTIAL, LEAF_GAUSSIAN, LEAF_MOFFAT, LEAF_KOLMOGOROV, LEAF_SERSIC, LEAF_BOX = range(1, 7)
@@ -58,6 +60,10 @@ _SIGS = {
     'gsmi_sersic_kvalue': (_I, [_D, _P, _P, _P, _I]),
     'gsmi_draw_fft_f64': (_I, [_P, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P]),
     'gsmi_draw_fft_f32': (_I, [_P, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P]),
+    'gsmi_fold_colfft_f64': (_I, [_P, _I, _I, _P, _P, _P, _P, _P, _P]),
+    'gsmi_fold_colfft_f32': (_I, [_P, _I, _I, _P, _P, _P, _P, _P, _P]),
+    'gsmi_row_c2r_f64': (_I, [_P, _I, _I, _I, _P, _P, _P]),
+    'gsmi_row_c2r_f32': (_I, [_P, _I, _I, _I, _P, _P, _P]),
     'gsmi_draw_kimage_f64': (_I, [_P, _I, _P, _P, _P, _I, _I, _I, _I, _I, _D, _P, _P]),
     'gsmi_irfft_f64': (_I, [_P, _I, _P, _P, _P]),
     'gsmi_wrap_hermx_f64': (_I, [_P, _I, _I, _P, _P]),

@@ -27,7 +27,6 @@ namespace gsmi {
 void set_error(const std::string& msg);
 
 constexpr int kThreads = 256;
-constexpr int kStack = 8;
 
 // ---------------------------------------------------------------------------------------------
 // Leaf evaluation
@@ -114,11 +113,24 @@ __device__ inline bool exp_clipped<float>(const gsmi_leaf& L, const Tables<float
     return ksq > float(L.p[0]);
 }
 
+// Moffat kV for the Bessel-function cases, kept out of line so the common path stays small.
+__device__ __noinline__ double moffat_kv_bessel(int kind, double nu, double k, double ksq,
+                                                double at_zero)
+{
+    switch (kind) {
+      case 2: return ksq == 0. ? 1. : bessel_k(1., k) * k;
+      case 3: return ksq == 0. ? 2. : bessel_k(2., k) * ksq;
+      case 4: return ksq == 0. ? 8. : bessel_k(3., k) * k * ksq;
+      default: return ksq == 0. ? at_zero : bessel_k(nu, k) * pow(k, nu);
+    }
+}
+
 // Value of leaf L at k-image sample (i, j) (kx = i dk, ky = j dk).
 template <typename T>
-__device__ inline T eval_leaf(const gsmi_leaf& L, const Tables<T>& tb, T kx, T ky, int i, int j,
-                              int Nk)
+__device__ inline T eval_leaf(const gsmi_leaf* __restrict__ Lp, const Tables<T>& tb, T kx, T ky,
+                              int i, int j, int Nk)
 {
+    const gsmi_leaf& L = *Lp;
     T kxp, kyp;
     if (L.separable) {
         kxp = T(L.jac[0]) * kx;
@@ -155,14 +167,8 @@ __device__ inline T eval_leaf(const gsmi_leaf& L, const Tables<T>& tb, T kx, T k
           case 15: return flux * dexp(-k);
           case 25: return flux * dexp(-k) * (T(1) + k);
           case 35: return flux * dexp(-k) * (T(3) + (T(3) + k) * k);
-          case 2: return ksq == T(0) ? flux : flux * T(bessel_k(1., double(k))) * k;
-          case 3: return ksq == T(0) ? flux * T(2) : flux * T(bessel_k(2., double(k))) * ksq;
-          case 4: return ksq == T(0) ? flux * T(8) : flux * T(bessel_k(3., double(k))) * k * ksq;
-          default: {
-            if (ksq == T(0)) return flux * T(L.p[1]);
-            double nu = L.p[0] - 1.;
-            return flux * T(bessel_k(nu, double(k)) * pow(double(k), nu));
-          }
+          default:
+            return flux * T(moffat_kv_bessel(L.kind, L.p[0] - 1., double(k), double(ksq), L.p[1]));
         }
       }
       case GSMI_LEAF_KOLMOGOROV: {
@@ -192,47 +198,111 @@ __device__ inline T eval_leaf(const gsmi_leaf& L, const Tables<T>& tb, T kx, T k
     return T(0);
 }
 
+// A small stack held in registers: push/pop shift every slot, so all indices are static.
+template <typename T, int S>
+struct RegStack {
+    cplx<T> s[S];
+    __device__ __forceinline__ void push(cplx<T> v)
+    {
+#pragma unroll
+        for (int k = S - 1; k > 0; --k) s[k] = s[k - 1];
+        s[0] = v;
+    }
+    __device__ __forceinline__ cplx<T> pop()
+    {
+        cplx<T> v = s[0];
+#pragma unroll
+        for (int k = 0; k < S - 1; ++k) s[k] = s[k + 1];
+        return v;
+    }
+};
+
+template <typename T>
+struct StackDepth;
+template <>
+struct StackDepth<double> { static constexpr int value = GSMI_STACK_F64; };
+template <>
+struct StackDepth<float> { static constexpr int value = GSMI_STACK_F32; };
+
 // Evaluate the stamp program at k-image sample (i, j).
 template <typename T>
-__device__ inline cplx<T> eval_program(const gsmi_op* ops, int nops, const gsmi_leaf* leaves,
-                                       const Tables<T>& tb, T dk, int i, int j, int Nk)
+__device__ inline cplx<T> eval_program(const gsmi_op* __restrict__ ops, int nops,
+                                       const gsmi_leaf* __restrict__ leaves, const Tables<T>& tb,
+                                       T dk, int i, int j, int Nk)
 {
-    cplx<T> st[kStack];
-    int sp = 0;
+    RegStack<T, StackDepth<T>::value> st;
     const T kx = T(i) * dk;
     const T ky = T(j) * dk;
     for (int o = 0; o < nops; ++o) {
-        const gsmi_op op = ops[o];
-        switch (op.code) {
-          case GSMI_OP_LEAF:
-            st[sp].x = eval_leaf<T>(leaves[op.arg], tb, kx, ky, i, j, Nk);
-            st[sp].y = T(0);
-            ++sp;
+        const gsmi_op* op = ops + o;
+        switch (op->code) {
+          case GSMI_OP_LEAF: {
+            T v = eval_leaf<T>(leaves + op->arg, tb, kx, ky, i, j, Nk);
+            st.push({v, T(0)});
             break;
-          case GSMI_OP_MUL:
-            --sp;
-            st[sp - 1] = cmul(st[sp - 1], st[sp]);
+          }
+          case GSMI_OP_MUL: {
+            cplx<T> b = st.pop();
+            st.s[0] = cmul(st.s[0], b);
             break;
-          case GSMI_OP_ADD:
-            --sp;
-            st[sp - 1] = cadd(st[sp - 1], st[sp]);
+          }
+          case GSMI_OP_ADD: {
+            cplx<T> b = st.pop();
+            st.s[0] = cadd(st.s[0], b);
             break;
+          }
           case GSMI_OP_SCALE:
-            st[sp - 1].x *= T(op.a);
-            st[sp - 1].y *= T(op.a);
+            st.s[0].x *= T(op->a);
+            st.s[0].y *= T(op->a);
             break;
           case GSMI_OP_PHASE: {
-            T arg = T(op.a) * kx + T(op.b) * ky;
+            T arg = T(op->a) * kx + T(op->b) * ky;
             T sn, cs;
             dsincos(arg, &sn, &cs);
-            const T fs = T(op.c);
-            cplx<T> ph = {fs * cs, -fs * sn};
-            st[sp - 1] = cmul(st[sp - 1], ph);
+            const T fs = T(op->c);
+            st.s[0] = cmul(st.s[0], cplx<T>{fs * cs, -fs * sn});
             break;
           }
         }
     }
-    return st[0];
+    return st.s[0];
+}
+
+// Separable factors hoisted out of the program (pixel sinc with a diagonal Jacobian, the phase
+// of a shift, constant scalings): X(kx) for the stored kx = 0..Nk/2 and Y(ky) for ky =
+// -Nk/2..Nk/2.  Constants and flux go into X.
+template <typename T>
+__device__ inline cplx<T> axis_factor(const gsmi_op* __restrict__ ops, int nops,
+                                      const gsmi_leaf* __restrict__ leaves, T k, bool xaxis)
+{
+    cplx<T> f = {T(1), T(0)};
+    for (int o = 0; o < nops; ++o) {
+        const gsmi_op* op = ops + o;
+        switch (op->code) {
+          case GSMI_OP_LEAF: {
+            const gsmi_leaf* L = leaves + op->arg;     // a Box with a diagonal Jacobian
+            const T u = xaxis ? T(L->jac[0]) * k * T(L->scale) : T(L->jac[3]) * k * T(L->scale2);
+            T sc = fabs(u) < T(1e-4) ? T(1) - T(M_PI * M_PI / 6.) * u * u
+                                     : dsinpi(u) / (T(M_PI) * u);
+            if (xaxis) sc *= T(L->flux);
+            f.x *= sc;
+            f.y *= sc;
+            break;
+          }
+          case GSMI_OP_SCALE:
+            if (xaxis) { f.x *= T(op->a); f.y *= T(op->a); }
+            break;
+          case GSMI_OP_PHASE: {
+            T arg = (xaxis ? T(op->a) : T(op->b)) * k;
+            T sn, cs;
+            dsincos(arg, &sn, &cs);
+            const T fs = xaxis ? T(op->c) : T(1);
+            f = cmul(f, cplx<T>{fs * cs, -fs * sn});
+            break;
+          }
+        }
+    }
+    return f;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -334,6 +404,27 @@ __host__ __device__ inline int rows_per_group(int N)
 }
 
 // ---------------------------------------------------------------------------------------------
+// Kernel 0: per-stamp separable axis factors X(kx), Y(ky) into the workspace (grid: nstamps)
+// ---------------------------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+axis_factors(const gsmi_stamp* __restrict__ stamps, const gsmi_op* __restrict__ ops,
+             const gsmi_leaf* __restrict__ leaves, cplx<T>* __restrict__ ws)
+{
+    const gsmi_stamp* st = stamps + blockIdx.x;
+    const int hk = st->Nk / 2;
+    const T dk = T(st->dk);
+    const gsmi_op* aops = ops + st->op_begin + st->op_count;
+    const gsmi_leaf* sleaves = leaves + st->leaf_begin;
+    cplx<T>* out = ws + st->axis_offset;
+    const int total = (hk + 1) + (2 * hk + 1);
+    for (int e = threadIdx.x; e < total; e += blockDim.x) {
+        if (e <= hk) out[e] = axis_factor<T>(aops, st->axis_count, sleaves, T(e) * dk, true);
+        else out[e] = axis_factor<T>(aops, st->axis_count, sleaves, T(e - hk - 1 - hk) * dk, false);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
 // Kernel 1: fused k-space evaluation + product/sinc/phase + Hermitian wrap + column inverse DFT
 // grid: (ceil((N/2+1)/cpg), nstamps); dynamic LDS: 2 * cpg * N complex
 // ---------------------------------------------------------------------------------------------
@@ -345,18 +436,21 @@ fold_colfft(const gsmi_stamp* __restrict__ stamps, const gsmi_op* __restrict__ o
 {
     extern __shared__ unsigned char smem_raw[];
     cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
-    const gsmi_stamp st = stamps[blockIdx.y];
-    const int N = st.N;
-    const int Nk = st.Nk;
+    const gsmi_stamp* st = stamps + blockIdx.y;
+    const int N = st->N;
+    const int Nk = st->Nk;
     const int half = N / 2;
     const int cpg = cols_per_group(N);
     const int c0 = blockIdx.x * cpg;
     const int ncol = min(cpg, half + 1 - c0);
     if (ncol <= 0) return;
-    const T dk = T(st.dk);
-    const gsmi_op* sops = ops + st.op_begin;
-    const gsmi_leaf* sleaves = leaves + st.leaf_begin;
+    const T dk = T(st->dk);
+    const gsmi_op* sops = ops + st->op_begin;
+    const int nops = st->op_count;
+    const gsmi_leaf* sleaves = leaves + st->leaf_begin;
     const int hk = Nk / 2;
+    const cplx<T>* X = scratch + st->axis_offset;
+    const cplx<T>* Y = X + hk + 1 + hk;               // Y[ky] for ky in [-hk, hk]
     cplx<T>* bufa = lds;
     cplx<T>* bufb = lds + cpg * N;
 
@@ -369,12 +463,14 @@ fold_colfft(const gsmi_stamp* __restrict__ stamps, const gsmi_op* __restrict__ o
         const int ky0 = r - N * ((r + hk) / N);
         for (int kx = kx0; kx <= hk; kx += N) {
             for (int ky = ky0; ky <= hk; ky += N) {
-                if (kx >= 0) {
-                    acc = cadd(acc, eval_program<T>(sops, st.op_count, sleaves, tb, dk, kx, ky, Nk));
-                } else {
-                    acc = cadd(acc, cconj(eval_program<T>(sops, st.op_count, sleaves, tb, dk, -kx,
-                                                          -ky, Nk)));
-                }
+                // the stored sample (kx >= 0); its conjugate stands for (kx, ky) when kx < 0
+                const bool neg = kx < 0;
+                const int i = neg ? -kx : kx;
+                const int j = neg ? -ky : ky;
+                cplx<T> v = eval_program<T>(sops, nops, sleaves, tb, dk, i, j, Nk);
+                v = cmul(v, cmul(X[i], Y[j]));
+                acc.x += v.x;
+                acc.y += neg ? -v.y : v.y;
             }
         }
         bufa[cl * N + r] = acc;
@@ -382,12 +478,14 @@ fold_colfft(const gsmi_stamp* __restrict__ stamps, const gsmi_op* __restrict__ o
     __syncthreads();
     cplx<T>* res = fft_lds<T>(bufa, bufb, N, ncol, tw, 1);
     // rows of the output window: position p <-> DFT index p mod N
-    cplx<T>* out = scratch + st.scratch_offset;
+    cplx<T>* out = scratch + st->scratch_offset;
     const int W = half + 1;
-    for (int e = threadIdx.x; e < ncol * st.ny; e += blockDim.x) {
+    const int ny = st->ny;
+    const int y0 = st->y0;
+    for (int e = threadIdx.x; e < ncol * ny; e += blockDim.x) {
         const int row = e / ncol;
         const int cl = e - row * ncol;
-        const int p = st.y0 + row;
+        const int p = y0 + row;
         const int y = (p % N + N) % N;
         out[row * W + c0 + cl] = res[cl * N + y];
     }
@@ -439,15 +537,15 @@ row_c2r(const gsmi_stamp* __restrict__ stamps, const cplx<T>* __restrict__ scrat
 {
     extern __shared__ unsigned char smem_raw[];
     cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
-    const gsmi_stamp st = stamps[blockIdx.y];
-    const int N = st.N;
+    const gsmi_stamp* st = stamps + blockIdx.y;
+    const int N = st->N;
     const int rpg = rows_per_group(N);
     const int r0 = blockIdx.x * rpg;
-    const int nrow = min(rpg, st.ny - r0);
+    const int nrow = min(rpg, st->ny - r0);
     if (nrow <= 0) return;
     const int W = N / 2 + 1;
-    row_c2r_body<T>(lds, rpg, scratch + st.scratch_offset + size_t(r0) * W, nrow, N, st.x0, st.nx,
-                    tw, out + st.out_offset + size_t(r0) * st.nx);
+    row_c2r_body<T>(lds, rpg, scratch + st->scratch_offset + size_t(r0) * W, nrow, N, st->x0,
+                    st->nx, tw, out + st->out_offset + size_t(r0) * st->nx);
 }
 
 // full N x N real image from N column-transformed rows (irfft entry point)
@@ -595,9 +693,8 @@ int hip_check(hipError_t e, const char* what)
 }
 
 template <typename T>
-int draw_fft_impl(const gsmi_stamp* stamps, int nstamps, int N, const gsmi_op* ops,
-                  const gsmi_leaf* leaves, Tables<T> tb, void* scratch, T* out, void* stream,
-                  int max_ny)
+int fold_impl(const gsmi_stamp* stamps, int nstamps, int N, const gsmi_op* ops,
+              const gsmi_leaf* leaves, Tables<T> tb, void* scratch, void* stream)
 {
     if (!fft_size_ok(N, sizeof(cplx<T>))) {
         set_error("unsupported FFT size " + std::to_string(N) +
@@ -611,20 +708,51 @@ int draw_fft_impl(const gsmi_stamp* stamps, int nstamps, int N, const gsmi_op* o
         set_error("twiddle table allocation failed");
         return GSMI_E_HIP;
     }
-    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     const int cpg = cols_per_group(N);
+    hipLaunchKernelGGL(axis_factors<T>, dim3(nstamps), dim3(kThreads), 0,
+                       reinterpret_cast<hipStream_t>(stream), stamps, ops, leaves,
+                       reinterpret_cast<cplx<T>*>(scratch));
+    int rc = hip_check(hipGetLastError(), "axis_factors launch");
+    if (rc) return rc;
     dim3 g1((N / 2 + 1 + cpg - 1) / cpg, nstamps);
     size_t lds1 = 2 * size_t(cpg) * N * sizeof(cplx<T>);
-    hipLaunchKernelGGL(fold_colfft<T>, g1, dim3(kThreads), lds1, s, stamps, ops, leaves, tb, tw,
+    hipLaunchKernelGGL(fold_colfft<T>, g1, dim3(kThreads), lds1,
+                       reinterpret_cast<hipStream_t>(stream), stamps, ops, leaves, tb, tw,
                        reinterpret_cast<cplx<T>*>(scratch));
-    int rc = hip_check(hipGetLastError(), "fold_colfft launch");
-    if (rc) return rc;
+    return hip_check(hipGetLastError(), "fold_colfft launch");
+}
+
+template <typename T>
+int rows_impl(const gsmi_stamp* stamps, int nstamps, int N, int max_ny, const void* scratch,
+              T* out, void* stream)
+{
+    if (!fft_size_ok(N, sizeof(cplx<T>))) {
+        set_error("unsupported FFT size " + std::to_string(N));
+        return GSMI_E_SIZE;
+    }
+    if (nstamps <= 0) return GSMI_OK;
+    const cplx<T>* tw = (sizeof(T) == 8) ? reinterpret_cast<const cplx<T>*>(twd().get(N))
+                                         : reinterpret_cast<const cplx<T>*>(twf().get(N));
+    if (!tw) {
+        set_error("twiddle table allocation failed");
+        return GSMI_E_HIP;
+    }
     const int rpg = rows_per_group(N);
     dim3 g2((max_ny + rpg - 1) / rpg, nstamps);
     size_t lds2 = 2 * size_t(rpg) * (N / 2) * sizeof(cplx<T>);
-    hipLaunchKernelGGL(row_c2r<T>, g2, dim3(kThreads), lds2, s, stamps,
-                       reinterpret_cast<const cplx<T>*>(scratch), tw, out);
+    hipLaunchKernelGGL(row_c2r<T>, g2, dim3(kThreads), lds2, reinterpret_cast<hipStream_t>(stream),
+                       stamps, reinterpret_cast<const cplx<T>*>(scratch), tw, out);
     return hip_check(hipGetLastError(), "row_c2r launch");
+}
+
+template <typename T>
+int draw_fft_impl(const gsmi_stamp* stamps, int nstamps, int N, const gsmi_op* ops,
+                  const gsmi_leaf* leaves, Tables<T> tb, void* scratch, T* out, void* stream,
+                  int max_ny)
+{
+    int rc = fold_impl<T>(stamps, nstamps, N, ops, leaves, tb, scratch, stream);
+    if (rc) return rc;
+    return rows_impl<T>(stamps, nstamps, N, max_ny, scratch, out, stream);
 }
 }  // namespace
 
@@ -721,6 +849,39 @@ int gsmi_irfft_f64(const double* kin, int N, double* out, void* scratch, void* s
     hipLaunchKernelGGL(irfft_rows_f64, dim3((N + rpg - 1) / rpg), dim3(kThreads), lds2, s,
                        reinterpret_cast<const cplx<double>*>(scratch), N, tw, out);
     return hip_check(hipGetLastError(), "irfft rows launch");
+}
+
+
+int gsmi_fold_colfft_f64(const gsmi_stamp* stamps, int nstamps, int N, const gsmi_op* ops,
+                         const gsmi_leaf* leaves, const double* tables, const int32_t* cliprows,
+                         void* scratch, void* stream)
+{
+    if (!stamps || !ops || !leaves || !scratch) { set_error("null buffer"); return GSMI_E_ARG; }
+    Tables<double> tb{tables, cliprows};
+    return fold_impl<double>(stamps, nstamps, N, ops, leaves, tb, scratch, stream);
+}
+
+int gsmi_fold_colfft_f32(const gsmi_stamp* stamps, int nstamps, int N, const gsmi_op* ops,
+                         const gsmi_leaf* leaves, const double* tables, const float* tables32,
+                         void* scratch, void* stream)
+{
+    if (!stamps || !ops || !leaves || !scratch) { set_error("null buffer"); return GSMI_E_ARG; }
+    Tables<float> tb{tables, tables32};
+    return fold_impl<float>(stamps, nstamps, N, ops, leaves, tb, scratch, stream);
+}
+
+int gsmi_row_c2r_f64(const gsmi_stamp* stamps, int nstamps, int N, int max_ny, const void* scratch,
+                     double* out, void* stream)
+{
+    if (!stamps || !scratch || !out || max_ny <= 0) { set_error("bad argument"); return GSMI_E_ARG; }
+    return rows_impl<double>(stamps, nstamps, N, max_ny, scratch, out, stream);
+}
+
+int gsmi_row_c2r_f32(const gsmi_stamp* stamps, int nstamps, int N, int max_ny, const void* scratch,
+                     float* out, void* stream)
+{
+    if (!stamps || !scratch || !out || max_ny <= 0) { set_error("bad argument"); return GSMI_E_ARG; }
+    return rows_impl<float>(stamps, nstamps, N, max_ny, scratch, out, stream);
 }
 
 }

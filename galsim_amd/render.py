@@ -78,8 +78,8 @@ class Lowering:
         return a
 
 
-def check_stack(ops, begin=0, count=None):
-    """Depth of the post-order program; the kernels hold at most 8 values."""
+def check_stack(ops, begin=0, count=None, precision='f64'):
+    """Depth of the post-order program against the kernels' register stack."""
     depth = best = 0
     end = len(ops) if count is None else begin + count
     for code, *_ in ops[begin:end]:
@@ -88,9 +88,32 @@ def check_stack(ops, begin=0, count=None):
         elif code in (_lib.OP_MUL, _lib.OP_ADD):
             depth -= 1
         best = max(best, depth)
-    if depth != 1 or best > 8:
-        raise _lib.GalSimAMDError("profile tree too deep for the device program (depth %d)"
-                                  % best)
+    limit = _lib.STACK_DEPTH[precision]
+    if depth != 1 or best > limit:
+        raise _lib.GalSimAMDError("profile tree too deep for the device program (depth %d > %d)"
+                                  % (best, limit))
+
+
+def hoist_axis_factors(ctx, op0):
+    """Move the trailing separable factors of the program that starts at op0 -- PHASE and SCALE
+    applied to the whole result, and a diagonal-Jacobian Box multiplied into it (the pixel) --
+    behind the program, where the kernels evaluate them once per k column and row.
+    Returns (op_count, axis_count)."""
+    prog = ctx.ops[op0:]
+    axis = []
+    while True:
+        if len(prog) > 1 and prog[-1][0] in (_lib.OP_PHASE, _lib.OP_SCALE):
+            axis.append(prog.pop())
+            continue
+        if (len(prog) > 2 and prog[-1][0] == _lib.OP_MUL and prog[-2][0] == _lib.OP_LEAF):
+            leaf = ctx.leaves[prog[-2][1]]
+            if leaf['type'] == _lib.LEAF_BOX and leaf['separable']:
+                axis.append(prog[-2])
+                del prog[-2:]
+                continue
+        break
+    ctx.ops[op0:] = prog + axis
+    return len(prog), len(axis)
 
 
 def plan_sizes(prof, bounds):
@@ -147,13 +170,15 @@ def draw_fft(prof, bounds, precision='f64', return_device=False):
     N, Nk, dk = plan_sizes(prof, bounds)
     ctx = Lowering(precision, Nk, dk)
     prof._lower(ctx, None)
-    check_stack(ctx.ops)
+    nops, naxis = hoist_axis_factors(ctx, 0)
+    check_stack(ctx.ops, 0, nops, precision)
     ny, nx = bounds.numpyShape()
     if not (-N // 2 <= bounds.xmin and bounds.xmax <= N // 2 - 1 and
             -N // 2 <= bounds.ymin and bounds.ymax <= N // 2 - 1):
         raise _lib.GalSimAMDError("output bounds %r do not fit the N=%d grid" % (bounds, N))
+    rows = ny * (N // 2 + 1)
     stamp = np.zeros(1, dtype=_lib.STAMP_DTYPE)
-    stamp[0] = (N, Nk, nx, ny, bounds.xmin, bounds.ymin, 0, len(ctx.ops), 0, 0, 0, 0, dk)
+    stamp[0] = (N, Nk, nx, ny, bounds.xmin, bounds.ymin, 0, nops, 0, naxis, 0, 0, rows, dk)
     torch = _torch()
     pk = _Packer()
     o_stamp = pk.add(stamp)
@@ -167,7 +192,7 @@ def draw_fft(prof, bounds, precision='f64', return_device=False):
     dev = _device_buffer(torch, pk.host())
     base = dev.data_ptr()
     fdt = torch.float64 if precision == 'f64' else torch.float32
-    scratch = torch.empty(ny * (N // 2 + 1) * 2, dtype=fdt, device='cuda')
+    scratch = torch.empty((rows + 3 * (Nk // 2) + 2) * 2, dtype=fdt, device='cuda')
     out = torch.empty((ny, nx), dtype=fdt, device='cuda')
     stream = torch.cuda.current_stream().cuda_stream
     if precision == 'f64':
@@ -190,7 +215,7 @@ def draw_kimage(prof, bounds, dk):
     lib = _lib.lib()
     ctx = Lowering('f64')
     prof._lower(ctx, None)
-    check_stack(ctx.ops)
+    check_stack(ctx.ops, precision='f64')
     ny, nx = bounds.numpyShape()
     torch = _torch()
     pk = _Packer()
@@ -215,7 +240,7 @@ def kvalue(prof, kx, ky):
     lib = _lib.lib()
     ctx = Lowering('f64')
     prof._lower(ctx, np.array([[float(kx), 0.], [0., float(ky)]]))
-    check_stack(ctx.ops)
+    check_stack(ctx.ops, precision='f64')
     torch = _torch()
     pk = _Packer()
     o_ops = pk.add(ctx.ops_array())

@@ -75,17 +75,23 @@ typedef struct gsmi_leaf {
     double p[6];          /* type-specific constants (see gsmi_kernels.hip)                  */
 } gsmi_leaf;
 
+/* Stack depth of the device program (FP64 / FP32 kernels). */
+#define GSMI_STACK_F64 6
+#define GSMI_STACK_F32 4
+
 typedef struct gsmi_stamp {
     int32_t N;            /* FFT (wrap) size                                                  */
     int32_t Nk;           /* k-image size before wrapping (Nk >= N)                           */
     int32_t nx, ny;       /* output stamp shape                                               */
     int32_t x0, y0;       /* position of the first output pixel on the N-grid (-N/2 <= x0)    */
     int32_t op_begin;     /* first op of this stamp's program                                 */
-    int32_t op_count;
+    int32_t op_count;     /* ops evaluated per k sample                                       */
     int32_t leaf_begin;   /* leaf indices in LEAF ops are relative to this                    */
-    int32_t pad;
+    int32_t axis_count;   /* separable factors stored after the program: Box leaves with a
+                             diagonal Jacobian, PHASE and SCALE applying to the whole result  */
     int64_t out_offset;   /* element offset of the stamp in the output buffer                 */
-    int64_t scratch_offset; /* element offset (complex) of the stamp's column-FFT rows        */
+    int64_t scratch_offset; /* complex offset in scratch of the stamp's ny x (N/2+1) rows     */
+    int64_t axis_offset;  /* complex offset in scratch of its (Nk/2+1) + (Nk+1) axis factors  */
     double dk;            /* k spacing in pixel units: 2 pi / N                               */
 } gsmi_stamp;
 
@@ -132,7 +138,8 @@ int gsmi_sersic_kvalue(double n, const double* gsp, const double* ksq, double* o
  * ------------------------------------------------------------------------------------- */
 
 /* drawImage(method='fft') in FP64 for a batch of stamps that share one FFT size N (max_ny =
- * tallest output stamp; scratch: sum over stamps of ny * (N/2+1) complex128) (the
+ * tallest output stamp; scratch holds each stamp's ny * (N/2+1) rows and its
+ * (Nk/2+1) + (Nk+1) axis factors at the offsets in its record, complex128) (the
  * reference: gsobject.py:1927-2045 drawFFT_makeKImage + _drawKImage + drawFFT_finish, with
  * SBProfile::drawK src/SBProfile.cpp:383-425, ApplyKImagePhases src/SBTransform.cpp:763-883,
  * wrapImage src/Image.cpp:577-726 and irfft src/Image.cpp:814-911).  out is float64. */
@@ -146,6 +153,20 @@ int gsmi_draw_fft_f64(const gsmi_stamp* stamps, int nstamps, int N, int max_ny,
 int gsmi_draw_fft_f32(const gsmi_stamp* stamps, int nstamps, int N, int max_ny,
                       const gsmi_op* ops, const gsmi_leaf* leaves, const double* tables,
                       const float* tables32, void* scratch, float* out, void* stream);
+
+/* The two kernels of gsmi_draw_fft_* as separate launches (same arguments): the fused k-space
+ * evaluation + product/sinc/phase + wrap + column inverse DFT into scratch, and the row real
+ * inverse DFT + crop + 1/N^2 into out.  Calling both in order equals gsmi_draw_fft_*. */
+int gsmi_fold_colfft_f64(const gsmi_stamp* stamps, int nstamps, int N, const gsmi_op* ops,
+                         const gsmi_leaf* leaves, const double* tables, const int32_t* cliprows,
+                         void* scratch, void* stream);
+int gsmi_fold_colfft_f32(const gsmi_stamp* stamps, int nstamps, int N, const gsmi_op* ops,
+                         const gsmi_leaf* leaves, const double* tables, const float* tables32,
+                         void* scratch, void* stream);
+int gsmi_row_c2r_f64(const gsmi_stamp* stamps, int nstamps, int N, int max_ny,
+                     const void* scratch, double* out, void* stream);
+int gsmi_row_c2r_f32(const gsmi_stamp* stamps, int nstamps, int N, int max_ny,
+                     const void* scratch, float* out, void* stream);
 
 /* Evaluate a stamp program on a k-image with bounds (xmin..xmin+nx-1, ymin..ymin+ny-1) and
  * spacing dk, writing complex128 values row-major (SBProfile::drawK src/SBProfile.cpp:383-425

@@ -107,7 +107,24 @@ def eval_program(ctx, i, j, dk, Nk):
     return st[0]
 
 
-def emulate_draw(prof, bounds):
+def eval_axis(ctx, axis_ops, i, j, dk):
+    """The hoisted separable factors X(kx) Y(ky) (axis_factor in the kernel)."""
+    kx = i * dk
+    ky = j * dk
+    f = np.ones(np.broadcast(i, j).shape, dtype=complex)
+    for code, arg, a, b, c in axis_ops:
+        if code == _lib.OP_LEAF:
+            L = ctx.leaves[arg]
+            f = f * (L['flux'] * np.sinc(L['jac'][0] * kx * L['scale']) *
+                     np.sinc(L['jac'][3] * ky * L['scale2']))
+        elif code == _lib.OP_SCALE:
+            f = f * a
+        elif code == _lib.OP_PHASE:
+            f = f * (c * np.exp(-1j * a * kx) * np.exp(-1j * b * ky))
+    return f
+
+
+def emulate_draw(prof, bounds, hoist=False):
     """Plan and lower ``prof`` exactly as render.draw_fft does, evaluate the program on the
     stored k-image and finish with the oracle's wrap/irfft/crop."""
     from oracle import draw as od
@@ -115,7 +132,13 @@ def emulate_draw(prof, bounds):
     ctx = render.Lowering('f64', Nk, dk)
     prof._lower(ctx, None)
     ii, jj = np.meshgrid(np.arange(0, Nk // 2 + 1), np.arange(-Nk // 2, Nk // 2 + 1))
-    karr = eval_program(ctx, ii, jj, dk, Nk)
+    if hoist:
+        nops, naxis = render.hoist_axis_factors(ctx, 0)
+        axis_ops = ctx.ops[nops:]
+        ctx.ops = ctx.ops[:nops]
+        karr = eval_program(ctx, ii, jj, dk, Nk) * eval_axis(ctx, axis_ops, ii, jj, dk)
+    else:
+        karr = eval_program(ctx, ii, jj, dk, Nk)
     kw = od.wrap_hermx(karr, Nk, N)
     real = od.irfft_shifted(kw, N)
     b = bounds

@@ -30,8 +30,9 @@ CASES = [
 ]
 
 
+@pytest.mark.parametrize("hoist", [False, True])
 @pytest.mark.parametrize("name,make,shape,offset", CASES)
-def test_lowered_program_matches_oracle(name, make, shape, offset):
+def test_lowered_program_matches_oracle(name, make, shape, offset, hoist):
     nx, ny = shape
     scale = 0.2
     prod = make(g)
@@ -46,6 +47,6 @@ def test_lowered_program_matches_oracle(name, make, shape, offset):
                    gsparams=prod.gsparams)
     c = im.center
     b = im.bounds.shift(PositionI(-c.x, -c.y))
-    emu = emulate_draw(p, b)
+    emu = emulate_draw(p, b, hoist=hoist)
     err = np.max(np.abs(emu - ref)) / np.max(np.abs(ref))
     assert err < 1e-10, (name, err)
