@@ -55,6 +55,9 @@ def main():
     ap.add_argument('--plain-sample', type=int, default=256)
     ap.add_argument('--cpu-seconds', type=float, default=15.)
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--copy-calibration', type=int, default=0, metavar='MIB',
+                    help='only run a plain device copy of MIB MiB (3 times) and exit; used to '
+                         'scale the FETCH_SIZE/WRITE_SIZE counters')
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
         print("bench.py measures one GPU only: this build renders every stamp on a single "
@@ -63,10 +66,19 @@ def main():
         sys.exit(2)
 
     import torch
+    torch.cuda.set_device(0)
+    if args.copy_calibration:
+        n = args.copy_calibration * (1 << 20) // 4
+        x = torch.ones(n, dtype=torch.float32, device='cuda')
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        log("copied %d MiB three times" % args.copy_calibration)
+        return
     import galsim_amd as g
     from galsim_amd import workloads
     from galsim_amd.batch import BatchPlan
-    torch.cuda.set_device(0)
 
     t0 = time.perf_counter()
     profs, nx, ny, offs, params = workloads.build('bulge_disk', args.n, seed=args.seed)
