@@ -24,6 +24,8 @@ STAMP_DTYPE = np.dtype([('N', np.int32), ('Nk', np.int32), ('nx', np.int32), ('n
                         ('dk', np.float64)], align=True)
 assert OP_DTYPE.itemsize == 32 and LEAF_DTYPE.itemsize == 128 and STAMP_DTYPE.itemsize == 72
 STACK_DEPTH = {'f64': 6, 'f32': 4}
+GRAD_MAXLEAF = 4
+GRAD_FIELDS = 27
 
 OP_LEAF, OP_MUL, OP_ADD, OP_SCALE, OP_PHASE = 1, 2, 3, 4, 5
 LEAF_EXPONENTIAL, LEAF_GAUSSIAN, LEAF_MOFFAT, LEAF_KOLMOGOROV, LEAF_SERSIC, LEAF_BOX = range(1, 7)
@@ -67,6 +69,9 @@ _SIGS = {
     'gsmi_draw_kimage_f64': (_I, [_P, _I, _P, _P, _P, _I, _I, _I, _I, _I, _D, _P, _P]),
     'gsmi_irfft_f64': (_I, [_P, _I, _P, _P, _P]),
     'gsmi_wrap_hermx_f64': (_I, [_P, _I, _I, _P, _P]),
+    'gsmi_adjoint_partials': (_I, [_I, _I, _P]),
+    'gsmi_adjoint_f64': (_I, [_P, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    'gsmi_adjoint_f32': (_I, [_P, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
 }
 
 

@@ -32,12 +32,33 @@ def _image_profile(prof, nx, ny, scale, offset, use_true_center, method):
     return p, b.shift(PositionI(-c.x, -c.y)), flux_scale
 
 
+_RADIAL = (_lib.LEAF_EXPONENTIAL, _lib.LEAF_GAUSSIAN, _lib.LEAF_MOFFAT, _lib.LEAF_KOLMOGOROV,
+           _lib.LEAF_SERSIC)
+
+
+def _prepare_differentiable(ctx, op0, nops, naxis, leaf0):
+    """Make a lowered stamp's fields writable and differentiable: radial leaves of the program
+    take the general-Jacobian path (so every jac entry is live), and the hoisted factors get a
+    whole-result PHASE (ux = uy = 0, fs = 1, an exact no-op) if they have none.  Returns the
+    number of ops appended.  Exponential clip rows (FP64) stay those of the lowered Jacobian."""
+    for code, arg, *_ in ctx.ops[op0:op0 + nops]:
+        if code == _lib.OP_LEAF:
+            rec = ctx.leaves[leaf0 + arg]
+            if rec['type'] in _RADIAL:
+                rec['separable'] = 0
+    axis = ctx.ops[op0 + nops:op0 + nops + naxis]
+    if any(op[0] == _lib.OP_PHASE for op in axis):
+        return 0
+    ctx.emit(_lib.OP_PHASE, 0, 0., 0., 1.)
+    return 1
+
+
 class BatchPlan:
     """A lowered batch, resident on the device.  ``run()`` launches the kernels on torch's
     current stream (no allocation, no host transfer: it can be captured in a HIP graph)."""
 
     def __init__(self, profiles, nx, ny, scale, offsets=None, precision='f32', method='auto',
-                 use_true_center=True):
+                 use_true_center=True, differentiable=False):
         import torch
         if precision not in ('f32', 'f64'):
             raise ValueError("precision must be 'f32' or 'f64'")
@@ -64,6 +85,8 @@ class BatchPlan:
                 if code == _lib.OP_LEAF:
                     ctx.ops[k] = (code, arg - leaf0, a, bb, c)
             render.check_stack(ctx.ops, op0, nops, precision)
+            if differentiable:
+                naxis += _prepare_differentiable(ctx, op0, nops, naxis, leaf0)
             recs[i] = (N, Nk, nx[i], ny[i], b.xmin, b.ymin, op0, nops, leaf0, naxis,
                        0, 0, 0, dk)
             Ns[i] = N
@@ -99,6 +122,10 @@ class BatchPlan:
         clip = np.concatenate(ctx.clip) if ctx.clip else np.zeros(2, dtype=np.int32)
         self._o_clip = pk.add(clip)
         self.recs = recs
+        self.host_ops = ctx.ops_array()
+        self.host_leaves = ctx.leaves_array()
+        self.offsets = {'stamp': self._o_stamp, 'ops': self._o_ops, 'leaves': self._o_leaves,
+                        'tables': self._o_tab, 'tables32': self._o_tab32, 'clip': self._o_clip}
         self.dev = torch.from_numpy(pk.host()).to('cuda')
         fdt = torch.float32 if precision == 'f32' else torch.float64
         self.scratch = torch.empty(2 * max(scratch_need, 1), dtype=fdt, device='cuda')

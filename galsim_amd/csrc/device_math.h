@@ -56,8 +56,9 @@ __device__ inline void temme_gammas(double mu, double* gam1, double* gam2, doubl
     }
 }
 
-// K_nu(x) for nu >= 0, x > 0 (Temme 1975; Steed's algorithm for CF2).
-__device__ inline double bessel_k(double nu, double x)
+// K_nu(x) for nu >= 0, x > 0 (Temme 1975; Steed's algorithm for CF2).  With km1 != nullptr
+// also returns K_{nu-1}(x).
+__device__ inline double bessel_k(double nu, double x, double* km1 = nullptr)
 {
     const double EPS = 1e-16;
     const int MAXIT = 10000;
@@ -124,11 +125,17 @@ __device__ inline double bessel_k(double nu, double x)
         rk1 = rkmu * (mu + x + 0.5 - a1 * h) * xi;
     }
     // upward recurrence K_{m+1} = K_{m-1} + (2m/x) K_m from m = mu+1
-    double kmu = rkmu, kmu1 = rk1;
+    double kmu = rkmu, kmu1 = rk1, kprev = 0.;
     for (int i = 1; i <= nl; ++i) {
         double knext = (mu + i) * xi2 * kmu1 + kmu;
+        kprev = kmu;
         kmu = kmu1;
         kmu1 = knext;
+    }
+    if (km1) {
+        // nu < 1/2: K_{nu-1} = K_{nu+1} - (2 nu / x) K_nu (loses about log10(1/x^(2 nu)) digits
+        // as x -> 0, a digit or two over the k range of a drawn stamp)
+        *km1 = (nl >= 1) ? kprev : kmu1 - mu * xi2 * kmu;
     }
     return kmu;
 }

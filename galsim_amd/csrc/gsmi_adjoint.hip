@@ -1,0 +1,572 @@
+// Adjoint of the FFT drawing path: the gradient of a pixel loss with respect to the fields of the
+// lowered stamp description, given the loss gradient dL/dI at every output pixel.
+//
+// The forward path computes, per stamp (p = pixel position, k-image samples s = (i, j) with
+// kx = i dk, ky = j dk, wrapped cells (c, r) with c = 0..N/2):
+//   I(p) = 1/N^2 sum_{c,r} w_c Re[ W(c, r) e^{+2 pi i (c px + r py) / N} ],  w_c = 1 for c = 0
+//          or N/2 and 2 otherwise,
+//   W(c, r) = sum_{s = (c, r) mod N} F(s) + sum_{s = -(c, r) mod N, i > 0} conj F(s),
+//   F(s) = P(s) X(i) Y(j)  (program value times the hoisted axis factors).
+// So for any field theta,
+//   dL/dtheta = sum_{c,r} Re[ dW(c, r)/dtheta A(c, r) ],  A = w_c Rhat(c, r) / N^2,
+//   Rhat(c, r) = sum_p R(p) e^{+2 pi i (c px + r py) / N},  R = dL/dI,
+// and each aliased sample contributes Re[dF(s)/dtheta A] (or Re[dF(s)/dtheta conj A] where the
+// forward added conj F).  The kernels mirror the forward ones in reverse order:
+//   1. row_r2c: per output row, the real DFT (sign +) of the residual row, placed on the N-grid,
+//      through a half-length complex DFT in LDS; writes the (N/2+1) coefficients to scratch (the
+//      same rows the forward's fold kernel writes).
+//   2. grad_colfft: per group of columns c, the column DFT of those rows in LDS gives Rhat(c, .);
+//      each thread then walks the same aliased samples as the forward, re-evaluates the leaves
+//      with their q-derivatives (no complex grid is stored between forward and backward), gets
+//      dP/dv_l of the post-order program from P(v_l = 1) - P(v_l = 0) (P is affine in every
+//      leaf value), and accumulates the field gradients; a block reduction writes one partial
+//      vector per workgroup.
+//   3. grad_reduce: per stamp, sums the workgroup partials in a fixed order (deterministic).
+//
+// Fields, per stamp (GSMI_GRAD_FIELDS doubles): for each leaf slot l < GSMI_GRAD_MAXLEAF
+// (relative leaf index of the program's LEAF ops) d/d(flux, scale, jac[0..3]) at 6 l .. 6 l + 5,
+// then d/dux, d/duy of a phase applying to the whole result and d/dlog(fs) of a factor applying
+// to the whole result.  Radial leaves only (Exponential, Gaussian, Moffat, Kolmogorov, Sersic):
+// a Box leaf in the program contributes its value, and its slot reads zero.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <string>
+
+#include "../../include/gsmi.h"
+#include "device_program.h"
+#include "launch.h"
+
+namespace gsmi {
+
+namespace {
+
+// Unit-flux leaf value phi and dphi/dq, q = scale^2 |J^T k|^2 (radial leaves), at sample (i, j).
+template <typename T>
+struct LeafD {
+    T phi;
+    T dphi;
+};
+
+// Moffat Bessel kinds in FP64: g(k) and g'(k) / (2 k) for k > 0 (d/dk [k^nu K_nu(k)] =
+// -k^nu K_{nu-1}(k)).
+__device__ __noinline__ LeafD<double> moffat_bessel_d(int kind, double nu, double k, double ksq,
+                                                      double at_zero)
+{
+    if (ksq == 0.)
+        return {(kind == 2) ? 1. : (kind == 3) ? 2. : (kind == 4) ? 8. : at_zero, 0.};
+    double km1;
+    switch (kind) {
+      case 2: {
+        double k1 = bessel_k(1., k, &km1);
+        return {k1 * k, -0.5 * km1};
+      }
+      case 3: {
+        double k2 = bessel_k(2., k, &km1);
+        return {k2 * ksq, -0.5 * k * km1};
+      }
+      case 4: {
+        double k3 = bessel_k(3., k, &km1);
+        return {k3 * k * ksq, -0.5 * ksq * km1};
+      }
+      default: {
+        double kn = bessel_k(nu, k, &km1);
+        double pk = pow(k, nu - 1.);
+        return {kn * pk * k, -0.5 * pk * km1};
+      }
+    }
+}
+
+template <typename T>
+__device__ inline void sersic_spline_d(const gsmi_leaf& L, const Tables<T>& tb, T lk, T* S, T* dS);
+
+template <>
+__device__ inline void sersic_spline_d<double>(const gsmi_leaf& L, const Tables<double>& tb,
+                                               double lk, double* S, double* dS)
+{
+    const int n = L.ntab;
+    const double* args = tb.tab + L.table;
+    const double* vals = args + n;
+    const double* y2 = vals + n;
+    double a0 = args[0];
+    double da = (args[n - 1] - a0) / (n - 1);
+    if (lk < a0 || lk > args[n - 1]) { *S = 0.; *dS = 0.; return; }
+    int i = int(ceil((lk - a0) / da));
+    if (i >= n) i = n - 1;
+    if (i <= 0) i = 1;
+    double h = args[i] - args[i - 1];
+    double aa = args[i] - lk;
+    double bb = h - aa;
+    double cc = (aa + h) * y2[i - 1] + (bb + h) * y2[i];
+    *S = (aa * vals[i - 1] + bb * vals[i] - (1. / 6.) * aa * bb * cc) / h;
+    *dS = (vals[i] - vals[i - 1] -
+           (1. / 6.) * ((aa - bb) * cc + aa * bb * (y2[i] - y2[i - 1]))) / h;
+}
+
+template <>
+__device__ inline void sersic_spline_d<float>(const gsmi_leaf& L, const Tables<float>& tb,
+                                              float lk, float* S, float* dS)
+{
+    const int n = L.ntab;
+    const float* args = tb.tab32 + L.table;
+    const float* vals = args + n;
+    const float* y2 = vals + n;
+    float a0 = args[0];
+    float inv_da = float(n - 1) / (args[n - 1] - a0);
+    if (lk < a0 || lk > args[n - 1]) { *S = 0.f; *dS = 0.f; return; }
+    int i = int(ceilf((lk - a0) * inv_da));
+    i = min(max(i, 1), n - 1);
+    float h = args[i] - args[i - 1];
+    float aa = args[i] - lk;
+    float bb = h - aa;
+    float cc = (aa + h) * y2[i - 1] + (bb + h) * y2[i];
+    *S = (aa * vals[i - 1] + bb * vals[i] - (1.f / 6.f) * aa * bb * cc) / h;
+    *dS = (vals[i] - vals[i - 1] -
+           (1.f / 6.f) * ((aa - bb) * cc + aa * bb * (y2[i] - y2[i - 1]))) / h;
+}
+
+// The forward's eval_leaf at unit flux, with dphi/dq.  The branches and clipping follow
+// eval_leaf exactly, so phi * flux is the forward value.
+template <typename T>
+__device__ inline LeafD<T> eval_leaf_d(const gsmi_leaf* __restrict__ Lp, const Tables<T>& tb,
+                                       T kx, T ky, int i, int j, int Nk)
+{
+    const gsmi_leaf& L = *Lp;
+    T kxp, kyp;
+    if (L.separable) {
+        kxp = T(L.jac[0]) * kx;
+        kyp = T(L.jac[3]) * ky;
+    } else {
+        kxp = T(L.jac[0]) * kx + T(L.jac[2]) * ky;
+        kyp = T(L.jac[1]) * kx + T(L.jac[3]) * ky;
+    }
+    const T s = T(L.scale);
+    const T X = kxp * s;
+    const T Y = kyp * s;
+    const T ksq = X * X + Y * Y;
+    switch (L.type) {
+      case GSMI_LEAF_EXPONENTIAL: {
+        if (exp_clipped<T>(L, tb, i, j, Nk, ksq)) return {T(0), T(0)};
+        T q = T(1) + ksq;
+        T phi = T(1) / (q * dsqrt(q));
+        return {phi, T(-1.5) * phi / q};
+      }
+      case GSMI_LEAF_GAUSSIAN: {
+        if (L.separable) {
+          T phi = dexp(T(-0.5) * X * X) * dexp(T(-0.5) * Y * Y);
+          return {phi, T(-0.5) * phi};
+        }
+        if (ksq > T(L.p[0])) return {T(0), T(0)};
+        if (ksq < T(L.p[1]))
+            return {T(1) - T(0.5) * ksq * (T(1) - T(0.25) * ksq), T(-0.5) + T(0.25) * ksq};
+        T phi = dexp(T(-0.5) * ksq);
+        return {phi, T(-0.5) * phi};
+      }
+      case GSMI_LEAF_MOFFAT: {
+        T k = dsqrt(ksq);
+        T e = dexp(-k);
+        switch (L.kind) {
+          case 15:
+            return {e, ksq == T(0) ? T(0) : T(-0.5) * e / k};
+          case 25:
+            return {e * (T(1) + k), T(-0.5) * e};
+          case 35:
+            return {e * (T(3) + (T(3) + k) * k), T(-0.5) * e * (T(1) + k)};
+          default: {
+            LeafD<double> m = moffat_bessel_d(L.kind, L.p[0] - 1., double(k), double(ksq), L.p[1]);
+            return {T(m.phi), T(m.dphi)};
+          }
+        }
+      }
+      case GSMI_LEAF_KOLMOGOROV: {
+        if (ksq == T(0)) return {T(1), T(0)};
+        T t = dexp(T(5. / 6.) * dlog(ksq));
+        T phi = dexp(-t);
+        return {phi, T(-5. / 6.) * t / ksq * phi};
+      }
+      case GSMI_LEAF_SERSIC: {
+        const T kmax = T(L.p[1]);
+        if (ksq >= kmax) {
+            T a = T(L.p[2]), b = T(L.p[3]);
+            T rk = dsqrt(ksq);
+            T phi = (a + b / rk) / ksq;
+            return {phi, -(a + T(1.5) * b / rk) / (ksq * ksq)};
+        }
+        if (ksq < T(L.p[0]))
+            return {T(1) + ksq * (T(L.p[4]) + ksq * T(L.p[5])),
+                    T(L.p[4]) + T(2) * ksq * T(L.p[5])};
+        T lk = T(0.5) * dlog(ksq);
+        T S, dS;
+        sersic_spline_d<T>(L, tb, lk, &S, &dS);
+        return {S / ksq, (T(0.5) * dS - S) / (ksq * ksq)};
+      }
+      case GSMI_LEAF_BOX: {
+        const T Xb = kxp * T(L.scale);
+        const T Yb = kyp * T(L.scale2);
+        T sx = fabs(Xb) < T(1e-4) ? T(1) - T(M_PI * M_PI / 6.) * Xb * Xb
+                                  : dsinpi(Xb) / (T(M_PI) * Xb);
+        T sy = fabs(Yb) < T(1e-4) ? T(1) - T(M_PI * M_PI / 6.) * Yb * Yb
+                                  : dsinpi(Yb) / (T(M_PI) * Yb);
+        return {sx * sy, T(0)};
+      }
+    }
+    return {T(0), T(0)};
+}
+
+// The post-order program on cached leaf values; leaf slot ovl (if >= 0) replaced by ovv.
+template <typename T, int ML>
+__device__ inline cplx<T> eval_cached(const gsmi_op* __restrict__ ops, int nops, const T (&v)[ML],
+                                      int ovl, T ovv, T kx, T ky)
+{
+    RegStack<T, StackDepth<T>::value> st;
+    for (int o = 0; o < nops; ++o) {
+        const gsmi_op* op = ops + o;
+        switch (op->code) {
+          case GSMI_OP_LEAF: {
+            const int a = op->arg;
+            T x = v[0];
+#pragma unroll
+            for (int l = 1; l < ML; ++l)
+                if (a == l) x = v[l];
+            if (a == ovl) x = ovv;
+            st.push({x, T(0)});
+            break;
+          }
+          case GSMI_OP_MUL: {
+            cplx<T> b = st.pop();
+            st.s[0] = cmul(st.s[0], b);
+            break;
+          }
+          case GSMI_OP_ADD: {
+            cplx<T> b = st.pop();
+            st.s[0] = cadd(st.s[0], b);
+            break;
+          }
+          case GSMI_OP_SCALE:
+            st.s[0].x *= T(op->a);
+            st.s[0].y *= T(op->a);
+            break;
+          case GSMI_OP_PHASE: {
+            T arg = T(op->a) * kx + T(op->b) * ky;
+            T sn, cs;
+            dsincos(arg, &sn, &cs);
+            const T fs = T(op->c);
+            st.s[0] = cmul(st.s[0], cplx<T>{fs * cs, -fs * sn});
+            break;
+          }
+        }
+    }
+    return st.s[0];
+}
+
+__device__ inline double wave_sum(double x)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+    return x;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Kernel A: row DFTs of the residual.  grid (ceil(ny/rpg), nstamps); LDS 2 rpg N/2 complex.
+// H(c) = sum_q R(q) e^{+2 pi i c q / N}, c = 0..N/2, from Z = DFT_{N/2}(R(2m) + i R(2m+1)):
+// H(c) = E(c) + e^{2 pi i c/N} O(c), E = (Z(c) + conj Z(N/2-c)) / 2, O = (Z(c) - conj Z(N/2-c)) / 2i.
+// ---------------------------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+row_r2c(const gsmi_stamp* __restrict__ stamps, const T* __restrict__ resid,
+        const cplx<T>* __restrict__ tw, cplx<T>* __restrict__ scratch)
+{
+    extern __shared__ unsigned char smem_raw[];
+    cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
+    const gsmi_stamp* st = stamps + blockIdx.y;
+    const int N = st->N;
+    const int n2 = N / 2;
+    const int W = n2 + 1;
+    const int rpg = rows_per_group(N);
+    const int r0 = blockIdx.x * rpg;
+    const int nrow = min(rpg, st->ny - r0);
+    if (nrow <= 0) return;
+    const int nx = st->nx;
+    const int x0 = st->x0;
+    const T* in = resid + st->out_offset + size_t(r0) * nx;
+    cplx<T>* bufa = lds;
+    cplx<T>* bufb = lds + rpg * n2;
+    for (int e = threadIdx.x; e < nrow * n2; e += blockDim.x) bufa[e] = {T(0), T(0)};
+    __syncthreads();
+    T* za = reinterpret_cast<T*>(bufa);
+    for (int e = threadIdx.x; e < nrow * nx; e += blockDim.x) {
+        const int rl = e / nx;
+        const int col = e - rl * nx;
+        const int q = ((x0 + col) % N + N) % N;
+        za[2 * (rl * n2 + (q >> 1)) + (q & 1)] = in[size_t(rl) * nx + col];
+    }
+    __syncthreads();
+    const cplx<T>* res = fft_lds<T>(bufa, bufb, n2, nrow, tw, 2);
+    cplx<T>* out = scratch + st->scratch_offset + size_t(r0) * W;
+    for (int e = threadIdx.x; e < nrow * W; e += blockDim.x) {
+        const int rl = e / W;
+        const int c = e - rl * W;
+        const cplx<T> zc = res[rl * n2 + (c == n2 ? 0 : c)];
+        const cplx<T> zm = res[rl * n2 + (c == 0 ? 0 : n2 - c)];
+        const cplx<T> E = {T(0.5) * (zc.x + zm.x), T(0.5) * (zc.y - zm.y)};
+        const cplx<T> d = {zc.x - zm.x, zc.y + zm.y};              // Z(c) - conj Z(N/2-c)
+        const cplx<T> O = {T(0.5) * d.y, T(-0.5) * d.x};          // d / 2i
+        out[size_t(rl) * W + c] = cadd(E, cmul(tw[c], O));
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Kernel B: column DFTs of the row coefficients -> Rhat, then the gradient over every aliased
+// k-image sample of the group's columns.  grid (ceil((N/2+1)/cpg), nstamps);
+// LDS 2 cpg N complex.  ML = leaf slots held in registers (2 or 4).
+// ---------------------------------------------------------------------------------------------
+template <typename T, int ML>
+__global__ void __launch_bounds__(kThreads)
+grad_colfft(const gsmi_stamp* __restrict__ stamps, const gsmi_op* __restrict__ ops,
+            const gsmi_leaf* __restrict__ leaves, Tables<T> tb, const cplx<T>* __restrict__ tw,
+            const cplx<T>* __restrict__ scratch, double* __restrict__ partials)
+{
+    constexpr int NF = 6 * ML + 3;
+    extern __shared__ unsigned char smem_raw[];
+    cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
+    const gsmi_stamp* st = stamps + blockIdx.y;
+    const int N = st->N;
+    const int Nk = st->Nk;
+    const int half = N / 2;
+    const int W = half + 1;
+    const int cpg = cols_per_group(N);
+    const int c0 = blockIdx.x * cpg;
+    const int ncol = max(0, min(cpg, W - c0));
+    double* part = partials + (size_t(blockIdx.y) * gridDim.x + blockIdx.x) * GSMI_GRAD_FIELDS;
+    const T dk = T(st->dk);
+    const gsmi_op* sops = ops + st->op_begin;
+    const int nops = st->op_count;
+    const gsmi_leaf* sleaves = leaves + st->leaf_begin;
+    const int hk = Nk / 2;
+    const cplx<T>* X = scratch + st->axis_offset;
+    const cplx<T>* Y = X + hk + 1 + hk;
+    const cplx<T>* H = scratch + st->scratch_offset;
+    const int ny = st->ny;
+    const int y0 = st->y0;
+    cplx<T>* bufa = lds;
+    cplx<T>* bufb = lds + cpg * N;
+
+    int nl = 0;
+    for (int o = 0; o < nops; ++o)
+        if (sops[o].code == GSMI_OP_LEAF) nl = max(nl, sops[o].arg + 1);
+    const bool ok = nl <= ML;
+
+    for (int e = threadIdx.x; e < ncol * N; e += blockDim.x) bufa[e] = {T(0), T(0)};
+    __syncthreads();
+    for (int e = threadIdx.x; e < ncol * ny; e += blockDim.x) {
+        const int row = e / ncol;
+        const int cl = e - row * ncol;
+        const int y = ((y0 + row) % N + N) % N;
+        bufa[cl * N + y] = H[size_t(row) * W + c0 + cl];
+    }
+    __syncthreads();
+    const cplx<T>* res = fft_lds<T>(bufa, bufb, N, ncol, tw, 1);
+
+    T acc[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) acc[f] = T(0);
+    const T inv_n2 = T(1) / (T(N) * T(N));
+    for (int e = threadIdx.x; ok && e < ncol * N; e += blockDim.x) {
+        const int cl = e / N;
+        const int r = e - cl * N;
+        const int c = c0 + cl;
+        const T wc = (c == 0 || c == half) ? inv_n2 : T(2) * inv_n2;
+        const cplx<T> A = {res[cl * N + r].x * wc, res[cl * N + r].y * wc};
+        const int kx0 = c - N * ((c + hk) / N);
+        const int ky0 = r - N * ((r + hk) / N);
+        for (int kxa = kx0; kxa <= hk; kxa += N) {
+            for (int kya = ky0; kya <= hk; kya += N) {
+                const bool neg = kxa < 0;
+                const int i = neg ? -kxa : kxa;
+                const int j = neg ? -kya : kya;
+                const T kx = T(i) * dk;
+                const T ky = T(j) * dk;
+                const cplx<T> seed = neg ? cconj(A) : A;
+                const cplx<T> Z = cmul(seed, cmul(X[i], Y[j]));
+                T v[ML], phi[ML], dv[ML];
+#pragma unroll
+                for (int l = 0; l < ML; ++l) {
+                    v[l] = phi[l] = dv[l] = T(0);
+                    if (l < nl) {
+                        LeafD<T> d = eval_leaf_d<T>(sleaves + l, tb, kx, ky, i, j, Nk);
+                        const T fl = T(sleaves[l].flux);
+                        phi[l] = d.phi;
+                        v[l] = fl * d.phi;
+                        dv[l] = fl * d.dphi;
+                    }
+                }
+                const cplx<T> P = eval_cached<T, ML>(sops, nops, v, -1, T(0), kx, ky);
+                const cplx<T> FZ = cmul(P, Z);
+                acc[6 * ML] += kx * FZ.y;
+                acc[6 * ML + 1] += ky * FZ.y;
+                acc[6 * ML + 2] += FZ.x;
+#pragma unroll
+                for (int l = 0; l < ML; ++l) {
+                    if (l < nl) {
+                        const cplx<T> p1 = eval_cached<T, ML>(sops, nops, v, l, T(1), kx, ky);
+                        const cplx<T> p0 = eval_cached<T, ML>(sops, nops, v, l, T(0), kx, ky);
+                        const cplx<T> t = cmul(csub(p1, p0), Z);
+                        const gsmi_leaf& L = sleaves[l];
+                        T kxp, kyp;
+                        if (L.separable) {
+                            kxp = T(L.jac[0]) * kx;
+                            kyp = T(L.jac[3]) * ky;
+                        } else {
+                            kxp = T(L.jac[0]) * kx + T(L.jac[2]) * ky;
+                            kyp = T(L.jac[1]) * kx + T(L.jac[3]) * ky;
+                        }
+                        const T s = T(L.scale);
+                        const T g = t.x * dv[l];
+                        const T g2 = T(2) * s * s * g;
+                        acc[6 * l] += t.x * phi[l];
+                        acc[6 * l + 1] += T(2) * s * (kxp * kxp + kyp * kyp) * g;
+                        acc[6 * l + 2] += g2 * kxp * kx;
+                        acc[6 * l + 3] += g2 * kyp * kx;
+                        acc[6 * l + 4] += g2 * kxp * ky;
+                        acc[6 * l + 5] += g2 * kyp * ky;
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();                       // LDS is reused for the reduction
+    double* red = reinterpret_cast<double*>(smem_raw);
+    const int wave = threadIdx.x / 64;
+    const int lane = threadIdx.x % 64;
+    const int nwave = blockDim.x / 64;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+        double x = wave_sum(double(acc[f]));
+        if (lane == 0) red[wave * NF + f] = x;
+    }
+    __syncthreads();
+    for (int f = threadIdx.x; f < GSMI_GRAD_FIELDS; f += blockDim.x) {
+        // output layout: leaf slots at 6 l for l < GSMI_GRAD_MAXLEAF, then the 3 whole-result
+        // fields; slots beyond ML read zero
+        int src = -1;
+        if (f < 6 * ML) src = f;
+        else if (f >= 6 * GSMI_GRAD_MAXLEAF) src = 6 * ML + (f - 6 * GSMI_GRAD_MAXLEAF);
+        double x = 0.;
+        if (!ok) x = NAN;
+        else if (src >= 0)
+            for (int w = 0; w < nwave; ++w) x += red[w * NF + src];
+        part[f] = x;
+    }
+}
+
+// Kernel C: grad[s][f] = sum over the stamp's workgroups of partials, in order.
+__global__ void __launch_bounds__(kThreads)
+grad_reduce(const double* __restrict__ partials, int ngroups, int nstamps,
+            double* __restrict__ grad)
+{
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nstamps * GSMI_GRAD_FIELDS) return;
+    const int s = idx / GSMI_GRAD_FIELDS;
+    const int f = idx - s * GSMI_GRAD_FIELDS;
+    double x = 0.;
+    for (int g = 0; g < ngroups; ++g) x += partials[(size_t(s) * ngroups + g) * GSMI_GRAD_FIELDS + f];
+    grad[idx] = x;
+}
+
+int adjoint_groups(int N) { return (N / 2 + 1 + cols_per_group(N) - 1) / cols_per_group(N); }
+
+template <typename T>
+int adjoint_impl(const gsmi_stamp* stamps, int nstamps, int N, int max_ny, int max_leaves,
+                 const gsmi_op* ops, const gsmi_leaf* leaves, Tables<T> tb, const T* resid,
+                 void* scratch, double* partials, double* grad, void* stream)
+{
+    if (!fft_size_ok(N, sizeof(cplx<T>))) {
+        set_error("unsupported FFT size " + std::to_string(N) +
+                  " (need 2^k or 3*2^k, <= 2048 in FP64, <= 4096 in FP32)");
+        return GSMI_E_SIZE;
+    }
+    if (max_leaves < 1 || max_leaves > GSMI_GRAD_MAXLEAF) {
+        set_error("adjoint: the program must have 1.." + std::to_string(GSMI_GRAD_MAXLEAF) +
+                  " leaves");
+        return GSMI_E_ARG;
+    }
+    if (nstamps <= 0) return GSMI_OK;
+    const cplx<T>* tw = twiddles<T>(N);
+    if (!tw) {
+        set_error("twiddle table allocation failed");
+        return GSMI_E_HIP;
+    }
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    int rc = launch_axis_factors<T>(stamps, nstamps, ops, leaves, scratch, stream);
+    if (rc) return rc;
+    const int rpg = rows_per_group(N);
+    dim3 ga((max_ny + rpg - 1) / rpg, nstamps);
+    size_t ldsa = 2 * size_t(rpg) * (N / 2) * sizeof(cplx<T>);
+    hipLaunchKernelGGL(row_r2c<T>, ga, dim3(kThreads), ldsa, s, stamps, resid, tw,
+                       reinterpret_cast<cplx<T>*>(scratch));
+    rc = hip_check(hipGetLastError(), "row_r2c launch");
+    if (rc) return rc;
+    const int cpg = cols_per_group(N);
+    const int ng = adjoint_groups(N);
+    dim3 gb(ng, nstamps);
+    size_t ldsb = 2 * size_t(cpg) * N * sizeof(cplx<T>);
+    if (max_leaves <= 2)
+        hipLaunchKernelGGL((grad_colfft<T, 2>), gb, dim3(kThreads), ldsb, s, stamps, ops, leaves,
+                           tb, tw, reinterpret_cast<const cplx<T>*>(scratch), partials);
+    else
+        hipLaunchKernelGGL((grad_colfft<T, 4>), gb, dim3(kThreads), ldsb, s, stamps, ops, leaves,
+                           tb, tw, reinterpret_cast<const cplx<T>*>(scratch), partials);
+    rc = hip_check(hipGetLastError(), "grad_colfft launch");
+    if (rc) return rc;
+    const int total = nstamps * GSMI_GRAD_FIELDS;
+    hipLaunchKernelGGL(grad_reduce, dim3((total + kThreads - 1) / kThreads), dim3(kThreads), 0, s,
+                       partials, ng, nstamps, grad);
+    return hip_check(hipGetLastError(), "grad_reduce launch");
+}
+
+}  // namespace
+}  // namespace gsmi
+
+using namespace gsmi;
+
+extern "C" {
+
+int gsmi_adjoint_partials(int N, int nstamps, int64_t* ndoubles)
+{
+    if (!ndoubles || nstamps < 0 || !fft_size_ok(N, sizeof(cplx<float>))) {
+        set_error("gsmi_adjoint_partials: bad size");
+        return GSMI_E_ARG;
+    }
+    *ndoubles = int64_t(adjoint_groups(N)) * nstamps * GSMI_GRAD_FIELDS;
+    return GSMI_OK;
+}
+
+int gsmi_adjoint_f64(const gsmi_stamp* stamps, int nstamps, int N, int max_ny, int max_leaves,
+                     const gsmi_op* ops, const gsmi_leaf* leaves, const double* tables,
+                     const int32_t* cliprows, const double* resid, void* scratch,
+                     double* partials, double* grad, void* stream)
+{
+    if (!stamps || !ops || !leaves || !resid || !scratch || !partials || !grad || max_ny <= 0) {
+        set_error("gsmi_adjoint_f64: null buffer or empty stamp");
+        return GSMI_E_ARG;
+    }
+    Tables<double> tb{tables, cliprows};
+    return adjoint_impl<double>(stamps, nstamps, N, max_ny, max_leaves, ops, leaves, tb, resid,
+                                scratch, partials, grad, stream);
+}
+
+int gsmi_adjoint_f32(const gsmi_stamp* stamps, int nstamps, int N, int max_ny, int max_leaves,
+                     const gsmi_op* ops, const gsmi_leaf* leaves, const double* tables,
+                     const float* tables32, const float* resid, void* scratch, double* partials,
+                     double* grad, void* stream)
+{
+    if (!stamps || !ops || !leaves || !resid || !scratch || !partials || !grad || max_ny <= 0) {
+        set_error("gsmi_adjoint_f32: null buffer or empty stamp");
+        return GSMI_E_ARG;
+    }
+    Tables<float> tb{tables, tables32};
+    return adjoint_impl<float>(stamps, nstamps, N, max_ny, max_leaves, ops, leaves, tb, resid,
+                               scratch, partials, grad, stream);
+}
+
+}

@@ -1,0 +1,32 @@
+// Host-side helpers shared by the forward and adjoint translation units of libgsmi.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <string>
+
+#include "../../include/gsmi.h"
+#include "device_math.h"
+
+namespace gsmi {
+
+// Thread-local message returned by gsmi_last_error().
+void set_error(const std::string& msg);
+
+// GSMI_OK, or GSMI_E_HIP with the HIP error text recorded.
+int hip_check(hipError_t e, const char* what);
+
+// N is 2^k or 3*2^k and the two in-LDS transforms of a workgroup fit 64 KiB.
+bool fft_size_ok(int N, size_t elem);
+
+// Device table exp(+2 pi i m / N), m = 0..N-1, cached per device and N (nullptr on failure).
+template <typename T>
+const cplx<T>* twiddles(int N);
+
+// The separable axis-factor prepass of every stamp (X(kx), Y(ky) at each stamp's axis_offset).
+template <typename T>
+int launch_axis_factors(const gsmi_stamp* stamps, int nstamps, const gsmi_op* ops,
+                        const gsmi_leaf* leaves, void* scratch, void* stream);
+
+}  // namespace gsmi

@@ -186,6 +186,37 @@ int gsmi_irfft_f64(const double* kin, int N, double* out, void* scratch, void* s
  * ky=-N/2..N/2-1).  Both complex128 row-major. */
 int gsmi_wrap_hermx_f64(const double* kin, int Nk, int N, double* kout, void* stream);
 
+/* ---------------------------------------------------------------------------------------
+ * Adjoint of gsmi_draw_fft_* (no counterpart in the reference, which has no derivatives: it is
+ * the exact reverse-mode derivative of drawFFT, galsim/gsobject.py:2014-2045, as restated by
+ * the two kernels above).
+ *
+ * Given resid = dL/dI for every output pixel (same layout and precision as out of
+ * gsmi_draw_fft_*), writes grad[s * GSMI_GRAD_FIELDS + f] = dL/dfield_f of stamp s (stamps in
+ * the order of the stamps array), FP64:
+ *   f = 6 l + {0..5}: leaf slot l (the relative leaf index of a LEAF op of the program, l <
+ *                     GSMI_GRAD_MAXLEAF): flux, scale, jac[0], jac[1], jac[2], jac[3];
+ *   f = 24, 25:       ux, uy of a PHASE applying to the whole result (hoisted axis factor);
+ *   f = 26:           log fs of a SCALE / PHASE amplitude applying to the whole result.
+ * Only radial leaves (Exponential, Gaussian, Moffat, Kolmogorov, Sersic) have field gradients;
+ * a Box slot reads zero.  Exponential clip rows (FP64) are held fixed: clipped samples have
+ * zero derivative, as they have zero value.  max_leaves bounds the number of leaf slots the
+ * programs use (1..GSMI_GRAD_MAXLEAF); a stamp using more gets NaN gradients.  scratch is the
+ * forward's scratch layout (it is overwritten); partials holds gsmi_adjoint_partials() doubles.
+ * ------------------------------------------------------------------------------------- */
+#define GSMI_GRAD_MAXLEAF 4
+#define GSMI_GRAD_FIELDS 27
+
+int gsmi_adjoint_partials(int N, int nstamps, int64_t* ndoubles);
+int gsmi_adjoint_f64(const gsmi_stamp* stamps, int nstamps, int N, int max_ny, int max_leaves,
+                     const gsmi_op* ops, const gsmi_leaf* leaves, const double* tables,
+                     const int32_t* cliprows, const double* resid, void* scratch,
+                     double* partials, double* grad, void* stream);
+int gsmi_adjoint_f32(const gsmi_stamp* stamps, int nstamps, int N, int max_ny, int max_leaves,
+                     const gsmi_op* ops, const gsmi_leaf* leaves, const double* tables,
+                     const float* tables32, const float* resid, void* scratch, double* partials,
+                     double* grad, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
