@@ -1,0 +1,252 @@
+"""Differentiable batched rendering: gradients of a pixel loss through drawImage(method='fft').
+
+``DiffPlan`` is a BatchPlan whose stamp fields -- per program leaf its flux, scale and composite
+Jacobian, and per stamp the phase (ux, uy) and amplitude fs applying to the whole k-image -- can be
+rewritten on the device and differentiated.  ``render_fields`` is a ``torch.autograd.Function``
+over those fields: the forward runs the batched FFT drawing kernels, the backward runs the
+hand-written adjoint kernels (gsmi_adjoint_f32/f64), which recompute the k-space products rather
+than keeping any complex grid from the forward.
+
+Model parameters map to fields through plain torch code (so torch's autograd does the outer
+chain rule); ``ExpMoffat`` does this for Convolve(Exponential(hlr, flux).shear(g1, g2),
+Moffat(beta, fwhm)) drawn with offset (dx, dy) -- the 7 parameters (flux, hlr, g1, g2, dx, dy,
+psf_fwhm).
+
+The FFT grids (N, Nk per stamp), the Moffat/Sersic kind and tables, and the Exponential clip
+rows of the FP64 path are those of the profiles the plan was built from: gradients are those of
+the rendering on that fixed grid (the reference's size choices are piecewise constant in the
+parameters).
+"""
+import math
+
+import numpy as np
+
+from . import _lib
+from .batch import BatchPlan
+
+NFIELD = _lib.GRAD_FIELDS
+MAXLEAF = _lib.GRAD_MAXLEAF
+F_UX, F_UY, F_FS = 6 * MAXLEAF, 6 * MAXLEAF + 1, 6 * MAXLEAF + 2
+_LEAF_FIELDS = (('flux', 0), ('scale', 0), ('jac', 0), ('jac', 1), ('jac', 2), ('jac', 3))
+
+
+class DiffPlan:
+    """A device-resident, differentiable batch of stamps (see the module docstring)."""
+
+    def __init__(self, profiles, nx, ny, scale, offsets=None, precision='f32',
+                 use_true_center=True):
+        import torch
+        self.plan = BatchPlan(profiles, nx, ny, scale, offsets, precision, 'auto',
+                              use_true_center, differentiable=True)
+        p = self.plan
+        n = p.n
+        self.n = n
+        self.precision = precision
+        recs = p.recs                       # sorted by N
+        ops, leaves = p.host_ops, p.host_leaves
+        o_ops, o_leaves = p.offsets['ops'], p.offsets['leaves']
+        lf, of = _lib.LEAF_DTYPE.fields, _lib.OP_DTYPE.fields
+        dst = np.full((n, NFIELD), -1, dtype=np.int64)
+        f0 = np.zeros((n, NFIELD))
+        self.leaf_types = np.zeros((n, MAXLEAF), dtype=np.int32)
+        nl = np.zeros(n, dtype=np.int32)
+        inv = np.empty(n, dtype=np.int64)
+        inv[p.order] = np.arange(n)
+        for s in range(n):                  # s: sorted position
+            r = recs[s]
+            i = p.order[s]                  # input index
+            b, cnt = int(r['op_begin']), int(r['op_count'])
+            prog = ops[b:b + cnt]
+            args = prog['arg'][prog['code'] == _lib.OP_LEAF]
+            nleaf = int(args.max()) + 1
+            if nleaf > MAXLEAF:
+                raise _lib.GalSimAMDError("differentiable rendering supports at most %d leaves "
+                                          "per profile (got %d)" % (MAXLEAF, nleaf))
+            nl[i] = nleaf
+            for l in range(nleaf):
+                li = int(r['leaf_begin']) + l
+                rec = leaves[li]
+                self.leaf_types[i, l] = rec['type']
+                for f, (name, k) in enumerate(_LEAF_FIELDS):
+                    byte = o_leaves + li * _lib.LEAF_DTYPE.itemsize + lf[name][1] + 8 * k
+                    dst[i, 6 * l + f] = byte // 8
+                    f0[i, 6 * l + f] = rec[name][k] if name == 'jac' else rec[name]
+            axis = range(b + cnt, b + cnt + int(r['axis_count']))
+            ph = [k for k in axis if ops[k]['code'] == _lib.OP_PHASE][0]
+            for f, name in ((F_UX, 'a'), (F_UY, 'b'), (F_FS, 'c')):
+                dst[i, f] = (o_ops + ph * _lib.OP_DTYPE.itemsize + of[name][1]) // 8
+                f0[i, f] = ops[ph][name]
+        self.nleaves = nl
+        self._inv = torch.from_numpy(inv).cuda()          # input index -> sorted position
+        mask = dst.reshape(-1) >= 0
+        self._src = torch.from_numpy(np.nonzero(mask)[0]).cuda()
+        self._dst = torch.from_numpy(dst.reshape(-1)[mask]).cuda()
+        self.fields0 = torch.from_numpy(f0).cuda()
+        self._dev64 = p.dev.view(torch.float64)
+        lib = _lib.lib()
+        need = 1
+        self._bins = []
+        for N, first, count, max_ny in p.groups:
+            nd = np.zeros(1, dtype=np.int64)
+            _lib.check(lib.gsmi_adjoint_partials(int(N), int(count), _lib.ptr(nd)),
+                       "adjoint partials size")
+            need = max(need, int(nd[0]))
+            ml = int(nl[p.order[first:first + count]].max())
+            self._bins.append((N, first, count, max_ny, ml))
+        self._partials = torch.empty(need, dtype=torch.float64, device='cuda')
+        self._grad = torch.empty((n, NFIELD), dtype=torch.float64, device='cuda')
+
+    # -- fields <-> device records -------------------------------------------------------------
+    def write_fields(self, fields):
+        """Copy a (n, NFIELD) float64 device tensor of fields (input order) into the records."""
+        import torch
+        if fields.shape != (self.n, NFIELD) or fields.dtype != torch.float64:
+            raise ValueError("fields must be a (%d, %d) float64 tensor" % (self.n, NFIELD))
+        vals = fields.detach().reshape(-1).index_select(0, self._src)
+        self._dev64.index_copy_(0, self._dst, vals)
+
+    def forward(self):
+        """Run the drawing kernels on the current fields; returns the flat output tensor (the
+        plan's own buffer: clone it before the next run if it must be kept)."""
+        return self.plan.run()
+
+    def adjoint(self, resid):
+        """dL/dfield for every stamp (input order, (n, NFIELD) float64) given resid = dL/dI as a
+        flat tensor laid out like the plan's output.  Field F_FS holds d/dlog(fs)."""
+        import torch
+        p = self.plan
+        fdt = torch.float32 if self.precision == 'f32' else torch.float64
+        resid = resid.to(fdt).contiguous()
+        if resid.numel() != p.out.numel():
+            raise ValueError("residual has %d values, the batch has %d pixels"
+                             % (resid.numel(), p.out.numel()))
+        lib = _lib.lib()
+        base = p.dev.data_ptr()
+        o = p.offsets
+        s = torch.cuda.current_stream().cuda_stream
+        g = self._grad
+        for N, first, count, max_ny, ml in self._bins:
+            st = base + o['stamp'] + first * _lib.STAMP_DTYPE.itemsize
+            gptr = g.data_ptr() + first * NFIELD * 8
+            if self.precision == 'f32':
+                rc = lib.gsmi_adjoint_f32(st, count, N, max_ny, ml, base + o['ops'],
+                                          base + o['leaves'], base + o['tables'],
+                                          base + o['tables32'], resid.data_ptr(),
+                                          p.scratch.data_ptr(), self._partials.data_ptr(),
+                                          gptr, s)
+            else:
+                rc = lib.gsmi_adjoint_f64(st, count, N, max_ny, ml, base + o['ops'],
+                                          base + o['leaves'], base + o['tables'],
+                                          base + o['clip'], resid.data_ptr(),
+                                          p.scratch.data_ptr(), self._partials.data_ptr(),
+                                          gptr, s)
+            _lib.check(rc, "adjoint (N=%d)" % N)
+        return g.index_select(0, self._inv)
+
+    def stamps(self, flat):
+        """Split a flat output-layout tensor into per-stamp (ny, nx) views (input order)."""
+        out = []
+        for i in range(self.n):
+            off = int(self.plan.stamp_offset[i])
+            ny, nx = (int(v) for v in self.plan.stamp_shape[i])
+            out.append(flat[off:off + ny * nx].view(ny, nx))
+        return out
+
+
+def _function():
+    import torch
+
+    class RenderFields(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, fields, dplan):
+            dplan.write_fields(fields)
+            out = dplan.forward().clone()
+            ctx.dplan = dplan
+            ctx.save_for_backward(fields)
+            return out
+
+        @staticmethod
+        def backward(ctx, gout):
+            fields, = ctx.saved_tensors
+            dplan = ctx.dplan
+            dplan.write_fields(fields)        # another forward may have rewritten the records
+            g = dplan.adjoint(gout).clone()
+            g[:, F_FS] = g[:, F_FS] / fields[:, F_FS]
+            return g, None
+
+    return RenderFields
+
+
+_RenderFields = None
+
+
+def render_fields(fields, dplan):
+    """Differentiable render: flat output tensor of ``dplan``'s stamps for the given fields."""
+    global _RenderFields
+    if _RenderFields is None:
+        _RenderFields = _function()
+    return _RenderFields.apply(fields, dplan)
+
+
+class ExpMoffat:
+    """Convolve(Exponential(half_light_radius=hlr, flux=flux).shear(g1=g1, g2=g2),
+    Moffat(beta, fwhm=psf_fwhm)) drawn by drawImage(nx, ny, scale=scale, offset=(dx, dy)) for a
+    batch of parameter rows theta = (flux, hlr, g1, g2, dx, dy, psf_fwhm), differentiable in all
+    seven.  The grids are fixed by ``theta0``."""
+
+    PARAMS = ('flux', 'hlr', 'g1', 'g2', 'dx', 'dy', 'psf_fwhm')
+
+    def __init__(self, theta0, nx, ny, scale=0.2, beta=3.0, precision='f32'):
+        import torch
+        from .convolve import Convolve
+        from .exponential import Exponential
+        from .moffat import Moffat
+        theta0 = np.asarray(theta0, dtype=float).reshape(-1, 7)
+        n = len(theta0)
+        self.scale = float(scale)
+        self.beta = float(beta)
+        profs = []
+        for t in theta0:
+            gal = Exponential(half_light_radius=t[1], flux=t[0]).shear(g1=t[2], g2=t[3])
+            profs.append(Convolve([gal, Moffat(self.beta, fwhm=t[6])]))
+        self.dplan = DiffPlan(profs, nx, ny, scale, theta0[:, 4:6], precision)
+        lt = self.dplan.leaf_types
+        if not (np.all(lt[:, 0] == _lib.LEAF_EXPONENTIAL) and np.all(lt[:, 1] == _lib.LEAF_MOFFAT)):
+            raise _lib.GalSimAMDError("unexpected lowering of the Exponential x Moffat model")
+        f0 = self.dplan.fields0
+        t0 = torch.from_numpy(theta0).cuda()
+        self._base = f0
+        self._cx = f0[:, F_UX] - t0[:, 4]
+        self._cy = f0[:, F_UY] - t0[:, 5]
+        self._fwhm_to_rd = 1. / (2. * math.sqrt(2. ** (1. / self.beta) - 1.))
+        self.theta0 = t0
+
+    def fields(self, theta):
+        """(n, NFIELD) fields of parameter rows theta ((n, 7) float64 device tensor)."""
+        return exp_moffat_fields(theta, self._base, self._cx, self._cy, self.scale,
+                                 self._fwhm_to_rd)
+
+    def render(self, theta):
+        """Flat FP32/FP64 tensor of all stamps for theta; differentiable."""
+        return render_fields(self.fields(theta), self.dplan)
+
+
+def exp_moffat_fields(theta, base, cx, cy, scale, fwhm_to_rd):
+    """Fields of the Exponential x Moffat model as torch expressions of theta = (flux, hlr, g1,
+    g2, dx, dy, psf_fwhm), following the lowering: the Exponential leaf has scale radius
+    hlr / 1.6783469900166605 (galsim/exponential.py) and composite Jacobian diag(1/scale) S(g)
+    with S(g) = [[1+g1, g2], [g2, 1-g1]] / sqrt(1-|g|^2) (galsim/shear.py:365-386); the Moffat
+    leaf has rD = fwhm / (2 sqrt(2^(1/beta) - 1)) (galsim/moffat.py); the drawImage offset is
+    the whole-result phase (ux, uy) = (dx, dy) + the centring shift (cx, cy).  Fields the
+    parameters do not reach come from ``base``."""
+    import torch
+    flux, hlr, g1, g2, dx, dy, fw = theta.unbind(1)
+    inv = 1. / scale
+    den = torch.sqrt(1. - (g1 * g1 + g2 * g2))
+    b = base
+    cols = [flux, hlr / 1.6783469900166605,
+            inv * (1. + g1) / den, inv * g2 / den, inv * g2 / den, inv * (1. - g1) / den,
+            b[:, 6], fw * fwhm_to_rd, b[:, 8], b[:, 9], b[:, 10], b[:, 11]]
+    cols += [b[:, k] for k in range(12, F_UX)]
+    cols += [dx + cx, dy + cy, b[:, F_FS]]
+    return torch.stack(cols, dim=1)
