@@ -45,6 +45,91 @@ def kernel_bytes(rec, nx, fsize):
     return col + 64, col + nx * ny * fsize
 
 
+def fwd_bwd_bench(args, torch):
+    """Forward+backward of the chi^2 fit workload: 1024 Exponential x Moffat(beta=3) stamps of
+    64x64, FP32, noisy targets (sigma = 1% of each stamp's peak), gradient with respect to
+    (flux, hlr, g1, g2, dx, dy, psf_fwhm) through the adjoint kernels.  Also the same step on
+    the plain per-stamp FP64 GPU path (one stamp per launch sequence), and the FP32 gradients
+    against the FP64 ones."""
+    from galsim_amd import diff, workloads
+    n = args.fit_n
+    p = workloads.exp_moffat_params(n, seed=args.seed + 1)
+    theta = np.stack([p['flux'], p['hlr'], p['g1'], p['g2'], p['offset'][:, 0],
+                      p['offset'][:, 1], p['psf_fwhm']], axis=1)
+    t_true = torch.tensor(theta, dtype=torch.float64, device='cuda')
+    rng = np.random.default_rng(args.seed + 2)
+    t_fit = t_true * torch.tensor(1. + 0.02 * rng.standard_normal(theta.shape),
+                                  dtype=torch.float64, device='cuda')
+
+    def setup(prec, rows):
+        model = diff.ExpMoffat(theta[rows], 64, 64, scale=0.2, beta=3.0, precision=prec)
+        with torch.no_grad():
+            img = model.render(t_true[rows]).double()
+        sig = torch.empty_like(img)
+        gen = torch.Generator(device='cpu').manual_seed(args.seed + 3)
+        noise = torch.randn(img.shape, generator=gen, dtype=torch.float64).cuda()
+        for s_img, s_sig in zip(model.dplan.stamps(img), model.dplan.stamps(sig)):
+            s_sig.fill_(0.01 * float(s_img.abs().max()))
+        target = (img + sig * noise).to(img.dtype)
+        return model, target, 1. / sig
+
+    def step(model, t, target, isig):
+        t = t.detach().requires_grad_(True)
+        r = (model.render(t).double() - target) * isig
+        (r * r).sum().backward()
+        return t.grad
+
+    rows = np.arange(n)
+    m32, tg32, is32 = setup('f32', rows)
+    for _ in range(args.warmup):
+        step(m32, t_fit, tg32, is32)
+    torch.cuda.synchronize()
+    k = max(args.steps, 1)
+    t0 = time.perf_counter()
+    for _ in range(k):
+        g32 = step(m32, t_fit, tg32, is32)
+    torch.cuda.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / k
+    with torch.no_grad():
+        for _ in range(2):
+            m32.render(t_fit)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(k):
+            m32.render(t_fit)
+        torch.cuda.synchronize()
+        ms_fwd = 1e3 * (time.perf_counter() - t0) / k
+    rate = n / (ms * 1e-3)
+    log("fwd+bwd FP32 (%d stamps): %.3f ms/step, %.0f stamps/s (forward alone %.3f ms)"
+        % (n, ms, rate, ms_fwd))
+    # FP64 batched gradients for the accuracy figure
+    m64, tg64, is64 = setup('f64', rows)
+    g64 = step(m64, t_fit, tg64, is64)
+    scale = (g64.abs() * t_fit.abs().clamp(min=1.)).max(dim=1, keepdim=True).values
+    err = ((g32 - g64).abs() * t_fit.abs().clamp(min=1.) / scale).max().item()
+    log("FP32 vs FP64 gradient: max error / largest sensitivity = %.2e" % err)
+    # plain per-stamp FP64 path
+    mp = min(args.plain_sample, n, 64)
+    singles = [setup('f64', np.array([i])) for i in range(mp)]
+    for i in range(min(4, mp)):
+        step(singles[i][0], t_fit[i:i + 1], singles[i][1], singles[i][2])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i, (mm, tg, isg) in enumerate(singles):
+        step(mm, t_fit[i:i + 1], tg, isg)
+    torch.cuda.synchronize()
+    plain = mp / (time.perf_counter() - t0)
+    log("plain FP64 per-stamp fwd+bwd: %.0f stamps/s (%d stamps)" % (plain, mp))
+    return {"value": round(rate, 1), "unit": "stamps/s", "ms_per_step": round(ms, 4),
+            "forward_ms_per_step": round(ms_fwd, 4),
+            "workload": "%d Exponential x Moffat(beta=3) 64x64 stamps, FP32, chi^2 against "
+                        "noisy targets (sigma = 1%% of peak), gradient w.r.t. flux, hlr, g1, g2, "
+                        "dx, dy, psf_fwhm via the adjoint kernels" % n,
+            "plain_fp64_gpu_fwd_bwd_stamps_per_s": round(plain, 1),
+            "speedup_vs_plain_fp64_gpu": round(rate / plain, 2),
+            "grad_err_fp32_vs_fp64": err}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -55,6 +140,8 @@ def main():
     ap.add_argument('--plain-sample', type=int, default=256)
     ap.add_argument('--cpu-seconds', type=float, default=15.)
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--fit-n', type=int, default=1024)
+    ap.add_argument('--no-fit', action='store_true')
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='MIB',
                     help='only run a plain device copy of MIB MiB (3 times) and exit; used to '
                          'scale the FETCH_SIZE/WRITE_SIZE counters')
@@ -193,6 +280,8 @@ def main():
     drawimage_rate = m2 / (time.perf_counter() - t0)
     log("drawImage FP64 end to end: %.0f stamps/s" % drawimage_rate)
 
+    fwd_bwd = None if args.no_fit else fwd_bwd_bench(args, torch)
+
     # --- CPU baseline: the oracle (numpy FP64 restatement), one core ------------------------
     cpu = None
     if not args.no_cpu:
@@ -234,7 +323,7 @@ def main():
                    "seed": args.seed},
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "fwd_bwd": None,
+        "fwd_bwd": fwd_bwd,
         "plain_fp64_gpu_stamps_per_s": round(plain_rate, 1),
         "speedup_vs_plain_fp64_gpu": round(value / plain_rate, 2),
         "drawimage_fp64_end_to_end_stamps_per_s": round(drawimage_rate, 1),
