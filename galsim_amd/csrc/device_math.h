@@ -140,4 +140,58 @@ __device__ inline double bessel_k(double nu, double x, double* km1 = nullptr)
     return kmu;
 }
 
+// K0(x) and K1(x) in FP32 for x > 0 from the polynomial approximations of Abramowitz & Stegun,
+// Handbook of Mathematical Functions, 9.8.1, 9.8.3, 9.8.5-9.8.8 (absolute error below 2.2e-7 in
+// x^(1/2) e^x K(x) for x >= 2, below 1e-8 in K0 and x K1 for x <= 2).  Returns K0 and x K1, the
+// form the Moffat kernels need (x K1 -> 1 as x -> 0).
+__device__ inline void bessel_k0_xk1_f32(float x, float* k0, float* xk1)
+{
+    if (x <= 2.f) {
+        const float t = x * (1.f / 3.75f);
+        const float t2 = t * t;
+        const float i0 = 1.f + t2 * (3.5156229f + t2 * (3.0899424f + t2 * (1.2067492f +
+                         t2 * (0.2659732f + t2 * (0.0360768f + t2 * 0.0045813f)))));
+        const float i1_x = 0.5f + t2 * (0.87890594f + t2 * (0.51498869f + t2 * (0.15084934f +
+                           t2 * (0.02658733f + t2 * (0.00301532f + t2 * 0.00032411f)))));
+        const float y = 0.25f * x * x;
+        const float lg = __logf(0.5f * x);
+        *k0 = -lg * i0 + (-0.57721566f + y * (0.42278420f + y * (0.23069756f + y * (0.03488590f +
+              y * (0.00262698f + y * (0.00010750f + y * 0.0000074f))))));
+        *xk1 = x * x * lg * i1_x + (1.f + y * (0.15443144f + y * (-0.67278579f +
+               y * (-0.18156897f + y * (-0.01919402f + y * (-0.00110404f + y * -0.00004686f))))));
+    } else {
+        const float z = 2.f / x;
+        const float e = __expf(-x) * rsqrtf(x);
+        *k0 = e * (1.25331414f + z * (-0.07832358f + z * (0.02189568f + z * (-0.01062446f +
+              z * (0.00587872f + z * (-0.00251540f + z * 0.00053208f))))));
+        *xk1 = x * e * (1.25331414f + z * (0.23498619f + z * (-0.03655620f + z * (0.01504268f +
+               z * (-0.00780353f + z * (0.00325614f + z * -0.00068245f))))));
+    }
+}
+
+// Moffat k-profiles with integer Bessel order in FP32: g(k) = k K1 (beta = 2), k^2 K2 (3),
+// k^3 K3 (4), from K0 and k K1 by the recurrence K_{n+1} = K_{n-1} + (2n/k) K_n; dg = g'(k)/(2k)
+// = -K0/2, -k K1/2, -k^2 K2/2.  k = 0 gives g(0) = 1, 2, 8 and dg = 0.
+__device__ inline void moffat_int_kind_f32(int kind, float k, float* g, float* dg)
+{
+    if (k == 0.f) {
+        *g = (kind == 2) ? 1.f : (kind == 3) ? 2.f : 8.f;
+        *dg = 0.f;
+        return;
+    }
+    float k0, xk1;
+    bessel_k0_xk1_f32(k, &k0, &xk1);
+    const float k2k2 = k * k * k0 + 2.f * xk1;          // k^2 K2
+    if (kind == 2) {
+        *g = xk1;
+        *dg = -0.5f * k0;
+    } else if (kind == 3) {
+        *g = k2k2;
+        *dg = -0.5f * xk1;
+    } else {
+        *g = k * k * xk1 + 4.f * k2k2;                   // k^3 K3
+        *dg = -0.5f * k2k2;
+    }
+}
+
 }  // namespace gsmi

@@ -154,6 +154,13 @@ __device__ inline T eval_leaf(const gsmi_leaf* __restrict__ Lp, const Tables<T>&
           case 25: return flux * dexp(-k) * (T(1) + k);
           case 35: return flux * dexp(-k) * (T(3) + (T(3) + k) * k);
           default:
+            if constexpr (sizeof(T) == 4) {
+                if (L.kind == 2 || L.kind == 3 || L.kind == 4) {
+                    float g, dg;
+                    moffat_int_kind_f32(L.kind, k, &g, &dg);
+                    return flux * g;
+                }
+            }
             return flux * T(moffat_kv_bessel(L.kind, L.p[0] - 1., double(k), double(ksq), L.p[1]));
         }
       }

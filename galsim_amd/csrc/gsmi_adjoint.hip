@@ -173,6 +173,13 @@ __device__ inline LeafD<T> eval_leaf_d(const gsmi_leaf* __restrict__ Lp, const T
           case 35:
             return {e * (T(3) + (T(3) + k) * k), T(-0.5) * e * (T(1) + k)};
           default: {
+            if constexpr (sizeof(T) == 4) {
+                if (L.kind == 2 || L.kind == 3 || L.kind == 4) {
+                    float g, dg;
+                    moffat_int_kind_f32(L.kind, k, &g, &dg);
+                    return {g, dg};
+                }
+            }
             LeafD<double> m = moffat_bessel_d(L.kind, L.p[0] - 1., double(k), double(ksq), L.p[1]);
             return {T(m.phi), T(m.dphi)};
           }

@@ -69,6 +69,10 @@ def test_exponential_moffat_seeded_sweep():
     ("moffat_25_shift", lambda m: m.Moffat(2.5, half_light_radius=0.5).shift(0.1, -0.2),
      (48, 48)),
     ("moffat_generic", lambda m: m.Moffat(2.7, fwhm=0.9).shear(g1=-0.2, g2=0.15), (48, 48)),
+    ("moffat_2", lambda m: m.Moffat(2., fwhm=0.6, flux=4.), (48, 48)),
+    ("moffat_3_sheared", lambda m: m.Moffat(3., fwhm=0.7).shear(g1=0.25, g2=-0.1), (40, 44)),
+    ("moffat_4_conv", lambda m: m.Convolve([m.Exponential(half_light_radius=0.5, flux=30.),
+                                            m.Moffat(4., fwhm=0.8)]), (48, 48)),
     ("sersic_15", lambda m: m.Sersic(1.5, half_light_radius=0.6, flux=5.), (48, 48)),
     ("sersic_4_conv", lambda m: m.Convolve([m.Sersic(4., half_light_radius=0.8),
                                             m.Kolmogorov(fwhm=0.8)]), (64, 64)),
