@@ -31,74 +31,6 @@ struct Tables<float> {
     const float* tab32;
 };
 
-template <typename T>
-__device__ inline T sersic_spline(const gsmi_leaf& L, const Tables<T>& tb, T lk);
-
-template <>
-__device__ inline double sersic_spline<double>(const gsmi_leaf& L, const Tables<double>& tb,
-                                               double lk)
-{
-    // Table::lookup on equally spaced log-k knots (src/Table.cpp:77-92, 560-577)
-    const int n = L.ntab;
-    const double* args = tb.tab + L.table;
-    const double* vals = args + n;
-    const double* y2 = vals + n;
-    double a0 = args[0];
-    double da = (args[n - 1] - a0) / (n - 1);
-    if (lk < a0 || lk > args[n - 1]) return 0.;
-    int i = int(ceil((lk - a0) / da));
-    if (i >= n) i = n - 1;
-    if (i <= 0) i = 1;
-    double h = args[i] - args[i - 1];
-    double aa = args[i] - lk;
-    double bb = h - aa;
-    return (aa * vals[i - 1] + bb * vals[i] -
-            (1. / 6.) * aa * bb * ((aa + h) * y2[i - 1] + (bb + h) * y2[i])) / h;
-}
-
-template <>
-__device__ inline float sersic_spline<float>(const gsmi_leaf& L, const Tables<float>& tb, float lk)
-{
-    const int n = L.ntab;
-    const float* args = tb.tab32 + L.table;
-    const float* vals = args + n;
-    const float* y2 = vals + n;
-    float a0 = args[0];
-    float inv_da = float(n - 1) / (args[n - 1] - a0);
-    if (lk < a0 || lk > args[n - 1]) return 0.f;
-    int i = int(ceilf((lk - a0) * inv_da));
-    i = min(max(i, 1), n - 1);
-    float h = args[i] - args[i - 1];
-    float aa = args[i] - lk;
-    float bb = h - aa;
-    return (aa * vals[i - 1] + bb * vals[i] -
-            (1.f / 6.f) * aa * bb * ((aa + h) * y2[i - 1] + (bb + h) * y2[i])) / h;
-}
-
-template <typename T>
-__device__ inline bool exp_clipped(const gsmi_leaf& L, const Tables<T>& tb, int i, int j, int Nk,
-                                   T ksq);
-
-template <>
-__device__ inline bool exp_clipped<double>(const gsmi_leaf& L, const Tables<double>& tb, int i,
-                                           int j, int Nk, double ksq)
-{
-    if (L.cliprows >= 0) {
-        // Row ranges computed on the host exactly as GetKValueRange1d/2d
-        // (src/SBProfile.cpp:508-588) for the stored k-image rows.
-        const int32_t* r = tb.clip + L.cliprows + 2 * (j + Nk / 2);
-        return i < r[0] || i >= r[1];
-    }
-    return ksq > L.p[0];
-}
-
-template <>
-__device__ inline bool exp_clipped<float>(const gsmi_leaf& L, const Tables<float>&, int, int, int,
-                                          float ksq)
-{
-    return ksq > float(L.p[0]);
-}
-
 // Moffat kV for the Bessel-function cases, kept out of line so the common path stays small.
 __device__ __noinline__ double moffat_kv_bessel(int kind, double nu, double k, double ksq,
                                                 double at_zero)
@@ -111,57 +43,164 @@ __device__ __noinline__ double moffat_kv_bessel(int kind, double nu, double k, d
     }
 }
 
-// Value of leaf L at k-image sample (i, j) (kx = i dk, ky = j dk).
+// A leaf decoded once per stamp: its constants converted to the arithmetic type (LeafC, kept in
+// LDS by the batched kernels) and its integer fields (LeafI, uniform registers).
 template <typename T>
-__device__ inline T eval_leaf(const gsmi_leaf* __restrict__ Lp, const Tables<T>& tb, T kx, T ky,
-                              int i, int j, int Nk)
+struct LeafC {
+    T j0, j1, j2, j3;     // composite Jacobian (dudx, dudy, dvdx, dvdy)
+    T s, s2, flux, pad;   // scale, Box height scale, flux (Moffat: knorm)
+    T p[6];               // type-specific constants
+    T pad2, pad3;
+};
+
+struct LeafI {
+    int type, kind, sep, aux, ntab;   // aux: Sersic table offset, or Exponential clip-row offset
+};
+
+template <typename T>
+__device__ inline LeafC<T> leaf_consts(const gsmi_leaf& L)
 {
-    const gsmi_leaf& L = *Lp;
-    T kxp, kyp;
-    if (L.separable) {
-        kxp = T(L.jac[0]) * kx;
-        kyp = T(L.jac[3]) * ky;
-    } else {
-        kxp = T(L.jac[0]) * kx + T(L.jac[2]) * ky;
-        kyp = T(L.jac[1]) * kx + T(L.jac[3]) * ky;
+    LeafC<T> c;
+    c.j0 = T(L.jac[0]);
+    c.j1 = T(L.jac[1]);
+    c.j2 = T(L.jac[2]);
+    c.j3 = T(L.jac[3]);
+    c.s = T(L.scale);
+    c.s2 = T(L.scale2);
+    c.flux = T(L.flux);
+    c.pad = c.pad2 = c.pad3 = T(0);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) c.p[k] = T(L.p[k]);
+    return c;
+}
+
+__device__ inline LeafI leaf_ints(const gsmi_leaf& L)
+{
+    LeafI r;
+    r.type = L.type;
+    r.kind = L.kind;
+    r.sep = L.separable;
+    r.aux = (L.type == GSMI_LEAF_SERSIC) ? L.table : L.cliprows;
+    r.ntab = L.ntab;
+    return r;
+}
+
+// Table::lookup of the natural cubic spline on equally spaced log-k knots (src/Table.cpp:77-92,
+// 560-577), with its derivative when dS != nullptr.
+template <typename T>
+__device__ inline T spline_eq(const T* __restrict__ args, int n, T lk, T* dS)
+{
+    const T* vals = args + n;
+    const T* y2 = vals + n;
+    const T a0 = args[0];
+    if (lk < a0 || lk > args[n - 1]) {
+        if (dS) *dS = T(0);
+        return T(0);
     }
-    const T s = T(L.scale);
-    const T X = kxp * s;
-    const T Y = kyp * s;
-    const T flux = T(L.flux);
-    switch (L.type) {
+    int i;
+    if constexpr (sizeof(T) == 8) {
+        const double da = (args[n - 1] - a0) / (n - 1);
+        i = int(ceil((lk - a0) / da));
+    } else {
+        const float inv_da = float(n - 1) / (args[n - 1] - a0);
+        i = int(ceilf((lk - a0) * inv_da));
+    }
+    i = min(max(i, 1), n - 1);
+    const T h = args[i] - args[i - 1];
+    const T aa = args[i] - lk;
+    const T bb = h - aa;
+    const T cc = (aa + h) * y2[i - 1] + (bb + h) * y2[i];
+    if (dS)
+        *dS = (vals[i] - vals[i - 1] -
+               (T(1) / T(6)) * ((aa - bb) * cc + aa * bb * (y2[i] - y2[i - 1]))) / h;
+    return (aa * vals[i - 1] + bb * vals[i] - (T(1) / T(6)) * aa * bb * cc) / h;
+}
+
+template <typename T>
+__device__ inline const T* sersic_table(const Tables<T>& tb, int off);
+template <>
+__device__ inline const double* sersic_table<double>(const Tables<double>& tb, int off)
+{
+    return tb.tab + off;
+}
+template <>
+__device__ inline const float* sersic_table<float>(const Tables<float>& tb, int off)
+{
+    return tb.tab32 + off;
+}
+
+// Exponential clipping: FP64 uses the row ranges the host computed as GetKValueRange1d/2d
+// (src/SBProfile.cpp:508-588) for the stored k-image rows; FP32 clips at ksq > ksq_max.
+template <typename T>
+__device__ inline bool exp_clip(const LeafC<T>& C, const LeafI& I, const Tables<T>& tb, int i,
+                                int j, int Nk, T ksq)
+{
+    if constexpr (sizeof(T) == 8) {
+        if (I.aux >= 0) {
+            const int32_t* r = tb.clip + I.aux + 2 * (j + Nk / 2);
+            return i < r[0] || i >= r[1];
+        }
+    }
+    return ksq > C.p[0];
+}
+
+template <typename T>
+__device__ inline void leaf_kprime(const LeafC<T>& C, const LeafI& I, T kx, T ky, T* kxp, T* kyp)
+{
+    if (I.sep) {
+        *kxp = C.j0 * kx;
+        *kyp = C.j3 * ky;
+    } else {
+        *kxp = C.j0 * kx + C.j2 * ky;
+        *kyp = C.j1 * kx + C.j3 * ky;
+    }
+}
+
+// Value of a decoded leaf at k-image sample (i, j) (kx = i dk, ky = j dk).  rec is the leaf's
+// record, read only by the FP64 Bessel-function Moffat kinds.
+template <typename T>
+__device__ inline T eval_leafc(const LeafC<T>& C, const LeafI& I, const gsmi_leaf* rec,
+                               const Tables<T>& tb, T kx, T ky, int i, int j, int Nk)
+{
+    T kxp, kyp;
+    leaf_kprime(C, I, kx, ky, &kxp, &kyp);
+    const T X = kxp * C.s;
+    const T Y = kyp * C.s;
+    const T flux = C.flux;
+    switch (I.type) {
       case GSMI_LEAF_EXPONENTIAL: {
         // SBExponential::fillKImage (src/SBExponential.cpp:442-510)
         T ksq = X * X + Y * Y;
-        if (exp_clipped<T>(L, tb, i, j, Nk, ksq)) return T(0);
+        if (exp_clip<T>(C, I, tb, i, j, Nk, ksq)) return T(0);
         T q = T(1) + ksq;
         return flux / (q * dsqrt(q));
       }
       case GSMI_LEAF_GAUSSIAN: {
         // SBGaussian::fillKImage (src/SBGaussian.cpp:200-290)
-        if (L.separable) return flux * dexp(T(-0.5) * X * X) * dexp(T(-0.5) * Y * Y);
+        if (I.sep) return flux * dexp(T(-0.5) * X * X) * dexp(T(-0.5) * Y * Y);
         T ksq = X * X + Y * Y;
-        if (ksq > T(L.p[0])) return T(0);
-        if (ksq < T(L.p[1])) return flux * (T(1) - T(0.5) * ksq * (T(1) - T(0.25) * ksq));
+        if (ksq > C.p[0]) return T(0);
+        if (ksq < C.p[1]) return flux * (T(1) - T(0.5) * ksq * (T(1) - T(0.25) * ksq));
         return flux * dexp(T(-0.5) * ksq);
       }
       case GSMI_LEAF_MOFFAT: {
         // SBMoffat kV_* (src/SBMoffat.cpp:286-345); flux holds knorm
         T ksq = X * X + Y * Y;
         T k = dsqrt(ksq);
-        switch (L.kind) {
+        switch (I.kind) {
           case 15: return flux * dexp(-k);
           case 25: return flux * dexp(-k) * (T(1) + k);
           case 35: return flux * dexp(-k) * (T(3) + (T(3) + k) * k);
           default:
             if constexpr (sizeof(T) == 4) {
-                if (L.kind == 2 || L.kind == 3 || L.kind == 4) {
+                if (I.kind == 2 || I.kind == 3 || I.kind == 4) {
                     float g, dg;
-                    moffat_int_kind_f32(L.kind, k, &g, &dg);
+                    moffat_int_kind_f32(I.kind, k, &g, &dg);
                     return flux * g;
                 }
             }
-            return flux * T(moffat_kv_bessel(L.kind, L.p[0] - 1., double(k), double(ksq), L.p[1]));
+            return flux * T(moffat_kv_bessel(I.kind, rec->p[0] - 1., double(k), double(ksq),
+                                             rec->p[1]));
         }
       }
       case GSMI_LEAF_KOLMOGOROV: {
@@ -172,15 +211,15 @@ __device__ inline T eval_leaf(const gsmi_leaf* __restrict__ Lp, const Tables<T>&
       case GSMI_LEAF_SERSIC: {
         // SersicInfo::kValue (src/SBSersic.cpp:363-376)
         T ksq = X * X + Y * Y;
-        if (ksq >= T(L.p[1])) return flux * (T(L.p[2]) + T(L.p[3]) / dsqrt(ksq)) / ksq;
-        if (ksq < T(L.p[0])) return flux * (T(1) + ksq * (T(L.p[4]) + ksq * T(L.p[5])));
+        if (ksq >= C.p[1]) return flux * (C.p[2] + C.p[3] / dsqrt(ksq)) / ksq;
+        if (ksq < C.p[0]) return flux * (T(1) + ksq * (C.p[4] + ksq * C.p[5]));
         T lk = T(0.5) * dlog(ksq);
-        return flux * sersic_spline<T>(L, tb, lk) / ksq;
+        return flux * spline_eq<T>(sersic_table<T>(tb, I.aux), I.ntab, lk, nullptr) / ksq;
       }
       case GSMI_LEAF_BOX: {
         // SBBox::fillKImage with math::sinc (src/SBBox.cpp:151-224, src/math/Sinc.cpp:29-33)
-        const T Xb = kxp * T(L.scale);
-        const T Yb = kyp * T(L.scale2);
+        const T Xb = kxp * C.s;
+        const T Yb = kyp * C.s2;
         T sx = fabs(Xb) < T(1e-4) ? T(1) - T(M_PI * M_PI / 6.) * Xb * Xb
                                   : dsinpi(Xb) / (T(M_PI) * Xb);
         T sy = fabs(Yb) < T(1e-4) ? T(1) - T(M_PI * M_PI / 6.) * Yb * Yb
@@ -189,6 +228,14 @@ __device__ inline T eval_leaf(const gsmi_leaf* __restrict__ Lp, const Tables<T>&
       }
     }
     return T(0);
+}
+
+// Value of leaf record L at k-image sample (i, j), decoding it on the spot.
+template <typename T>
+__device__ inline T eval_leaf(const gsmi_leaf* __restrict__ Lp, const Tables<T>& tb, T kx, T ky,
+                              int i, int j, int Nk)
+{
+    return eval_leafc<T>(leaf_consts<T>(*Lp), leaf_ints(*Lp), Lp, tb, kx, ky, i, j, Nk);
 }
 
 // A small stack held in registers: push/pop shift every slot, so all indices are static.
@@ -216,6 +263,110 @@ template <>
 struct StackDepth<double> { static constexpr int value = GSMI_STACK_F64; };
 template <>
 struct StackDepth<float> { static constexpr int value = GSMI_STACK_F32; };
+
+// A stamp program decoded once per workgroup: op codes and arguments packed in uniform
+// registers (code | arg << 8), leaf integer fields in uniform registers, leaf constants in LDS.
+// Programs with more leaves or ops than this run on the record-reading interpreter.
+constexpr int kProgLeaves = 4;
+constexpr int kProgOps = 8;
+
+struct ProgramD {
+    int nl, nops;
+    int opc[kProgOps];
+    LeafI li[kProgLeaves];
+};
+
+// Number of leaf slots a program uses (1 + the largest LEAF argument).
+__device__ inline int program_leaves(const gsmi_op* __restrict__ ops, int nops)
+{
+    int nl = 0;
+    for (int o = 0; o < nops; ++o)
+        if (ops[o].code == GSMI_OP_LEAF) nl = max(nl, ops[o].arg + 1);
+    return nl;
+}
+
+// Decode the program (all threads) and write its leaf constants to lc[] (threads < nl).  The
+// caller synchronises before reading lc.  Returns false if the program does not fit.
+template <typename T>
+__device__ inline bool decode_program(const gsmi_op* __restrict__ ops, int nops,
+                                      const gsmi_leaf* __restrict__ leaves, ProgramD& d,
+                                      LeafC<T>* lc)
+{
+    const int nl = program_leaves(ops, nops);
+    d.nl = nl;
+    d.nops = nops;
+    if (nl > kProgLeaves || nops > kProgOps) return false;
+#pragma unroll
+    for (int o = 0; o < kProgOps; ++o)
+        d.opc[o] = (o < nops) ? (ops[o].code | (ops[o].arg << 8)) : 0;
+#pragma unroll
+    for (int l = 0; l < kProgLeaves; ++l)
+        d.li[l] = (l < nl) ? leaf_ints(leaves[l]) : LeafI{0, 0, 0, -1, 0};
+    if (int(threadIdx.x) < nl) lc[threadIdx.x] = leaf_consts<T>(leaves[threadIdx.x]);
+    return true;
+}
+
+// The decoded program on leaf values v[] (LEAF ops take v[arg]; slot ovl, if >= 0, replaced by
+// ovv).  SCALE and PHASE read their constants from the op records.
+template <typename T>
+__device__ inline cplx<T> run_ops(const ProgramD& d, const gsmi_op* __restrict__ ops,
+                                  const T (&v)[kProgLeaves], int ovl, T ovv, T kx, T ky)
+{
+    RegStack<T, StackDepth<T>::value> st;
+#pragma unroll
+    for (int o = 0; o < kProgOps; ++o) {
+        if (o < d.nops) {
+            const int code = d.opc[o] & 0xff;
+            const int a = d.opc[o] >> 8;
+            switch (code) {
+              case GSMI_OP_LEAF: {
+                T x = v[0];
+#pragma unroll
+                for (int l = 1; l < kProgLeaves; ++l)
+                    if (a == l) x = v[l];
+                if (a == ovl) x = ovv;
+                st.push({x, T(0)});
+                break;
+              }
+              case GSMI_OP_MUL: {
+                cplx<T> b = st.pop();
+                st.s[0] = cmul(st.s[0], b);
+                break;
+              }
+              case GSMI_OP_ADD: {
+                cplx<T> b = st.pop();
+                st.s[0] = cadd(st.s[0], b);
+                break;
+              }
+              case GSMI_OP_SCALE:
+                st.s[0].x *= T(ops[o].a);
+                st.s[0].y *= T(ops[o].a);
+                break;
+              case GSMI_OP_PHASE: {
+                T arg = T(ops[o].a) * kx + T(ops[o].b) * ky;
+                T sn, cs;
+                dsincos(arg, &sn, &cs);
+                const T fs = T(ops[o].c);
+                st.s[0] = cmul(st.s[0], cplx<T>{fs * cs, -fs * sn});
+                break;
+              }
+            }
+        }
+    }
+    return st.s[0];
+}
+
+// Leaf values of a decoded program at sample (i, j).
+template <typename T>
+__device__ inline void eval_leaves(const ProgramD& d, const LeafC<T>* __restrict__ lc,
+                                   const gsmi_leaf* __restrict__ leaves, const Tables<T>& tb,
+                                   T kx, T ky, int i, int j, int Nk, T (&v)[kProgLeaves])
+{
+#pragma unroll
+    for (int l = 0; l < kProgLeaves; ++l)
+        v[l] = (l < d.nl) ? eval_leafc<T>(lc[l], d.li[l], leaves + l, tb, kx, ky, i, j, Nk)
+                          : T(0);
+}
 
 // Evaluate the stamp program at k-image sample (i, j).
 template <typename T>

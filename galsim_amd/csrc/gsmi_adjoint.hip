@@ -77,87 +77,31 @@ __device__ __noinline__ LeafD<double> moffat_bessel_d(int kind, double nu, doubl
     }
 }
 
+// The forward's eval_leafc at unit flux, with dphi/dq.  The branches and clipping follow
+// eval_leafc exactly, so phi * flux is the forward value.
 template <typename T>
-__device__ inline void sersic_spline_d(const gsmi_leaf& L, const Tables<T>& tb, T lk, T* S, T* dS);
-
-template <>
-__device__ inline void sersic_spline_d<double>(const gsmi_leaf& L, const Tables<double>& tb,
-                                               double lk, double* S, double* dS)
+__device__ inline LeafD<T> eval_leafc_d(const LeafC<T>& C, const LeafI& I, const gsmi_leaf* rec,
+                                        const Tables<T>& tb, T kx, T ky, int i, int j, int Nk)
 {
-    const int n = L.ntab;
-    const double* args = tb.tab + L.table;
-    const double* vals = args + n;
-    const double* y2 = vals + n;
-    double a0 = args[0];
-    double da = (args[n - 1] - a0) / (n - 1);
-    if (lk < a0 || lk > args[n - 1]) { *S = 0.; *dS = 0.; return; }
-    int i = int(ceil((lk - a0) / da));
-    if (i >= n) i = n - 1;
-    if (i <= 0) i = 1;
-    double h = args[i] - args[i - 1];
-    double aa = args[i] - lk;
-    double bb = h - aa;
-    double cc = (aa + h) * y2[i - 1] + (bb + h) * y2[i];
-    *S = (aa * vals[i - 1] + bb * vals[i] - (1. / 6.) * aa * bb * cc) / h;
-    *dS = (vals[i] - vals[i - 1] -
-           (1. / 6.) * ((aa - bb) * cc + aa * bb * (y2[i] - y2[i - 1]))) / h;
-}
-
-template <>
-__device__ inline void sersic_spline_d<float>(const gsmi_leaf& L, const Tables<float>& tb,
-                                              float lk, float* S, float* dS)
-{
-    const int n = L.ntab;
-    const float* args = tb.tab32 + L.table;
-    const float* vals = args + n;
-    const float* y2 = vals + n;
-    float a0 = args[0];
-    float inv_da = float(n - 1) / (args[n - 1] - a0);
-    if (lk < a0 || lk > args[n - 1]) { *S = 0.f; *dS = 0.f; return; }
-    int i = int(ceilf((lk - a0) * inv_da));
-    i = min(max(i, 1), n - 1);
-    float h = args[i] - args[i - 1];
-    float aa = args[i] - lk;
-    float bb = h - aa;
-    float cc = (aa + h) * y2[i - 1] + (bb + h) * y2[i];
-    *S = (aa * vals[i - 1] + bb * vals[i] - (1.f / 6.f) * aa * bb * cc) / h;
-    *dS = (vals[i] - vals[i - 1] -
-           (1.f / 6.f) * ((aa - bb) * cc + aa * bb * (y2[i] - y2[i - 1]))) / h;
-}
-
-// The forward's eval_leaf at unit flux, with dphi/dq.  The branches and clipping follow
-// eval_leaf exactly, so phi * flux is the forward value.
-template <typename T>
-__device__ inline LeafD<T> eval_leaf_d(const gsmi_leaf* __restrict__ Lp, const Tables<T>& tb,
-                                       T kx, T ky, int i, int j, int Nk)
-{
-    const gsmi_leaf& L = *Lp;
     T kxp, kyp;
-    if (L.separable) {
-        kxp = T(L.jac[0]) * kx;
-        kyp = T(L.jac[3]) * ky;
-    } else {
-        kxp = T(L.jac[0]) * kx + T(L.jac[2]) * ky;
-        kyp = T(L.jac[1]) * kx + T(L.jac[3]) * ky;
-    }
-    const T s = T(L.scale);
-    const T X = kxp * s;
-    const T Y = kyp * s;
+    leaf_kprime(C, I, kx, ky, &kxp, &kyp);
+    const T X = kxp * C.s;
+    const T Y = kyp * C.s;
     const T ksq = X * X + Y * Y;
-    switch (L.type) {
+    switch (I.type) {
       case GSMI_LEAF_EXPONENTIAL: {
-        if (exp_clipped<T>(L, tb, i, j, Nk, ksq)) return {T(0), T(0)};
+        if (exp_clip<T>(C, I, tb, i, j, Nk, ksq)) return {T(0), T(0)};
         T q = T(1) + ksq;
         T phi = T(1) / (q * dsqrt(q));
         return {phi, T(-1.5) * phi / q};
       }
       case GSMI_LEAF_GAUSSIAN: {
-        if (L.separable) {
-          T phi = dexp(T(-0.5) * X * X) * dexp(T(-0.5) * Y * Y);
-          return {phi, T(-0.5) * phi};
+        if (I.sep) {
+            T phi = dexp(T(-0.5) * X * X) * dexp(T(-0.5) * Y * Y);
+            return {phi, T(-0.5) * phi};
         }
-        if (ksq > T(L.p[0])) return {T(0), T(0)};
-        if (ksq < T(L.p[1]))
+        if (ksq > C.p[0]) return {T(0), T(0)};
+        if (ksq < C.p[1])
             return {T(1) - T(0.5) * ksq * (T(1) - T(0.25) * ksq), T(-0.5) + T(0.25) * ksq};
         T phi = dexp(T(-0.5) * ksq);
         return {phi, T(-0.5) * phi};
@@ -165,7 +109,7 @@ __device__ inline LeafD<T> eval_leaf_d(const gsmi_leaf* __restrict__ Lp, const T
       case GSMI_LEAF_MOFFAT: {
         T k = dsqrt(ksq);
         T e = dexp(-k);
-        switch (L.kind) {
+        switch (I.kind) {
           case 15:
             return {e, ksq == T(0) ? T(0) : T(-0.5) * e / k};
           case 25:
@@ -174,13 +118,14 @@ __device__ inline LeafD<T> eval_leaf_d(const gsmi_leaf* __restrict__ Lp, const T
             return {e * (T(3) + (T(3) + k) * k), T(-0.5) * e * (T(1) + k)};
           default: {
             if constexpr (sizeof(T) == 4) {
-                if (L.kind == 2 || L.kind == 3 || L.kind == 4) {
+                if (I.kind == 2 || I.kind == 3 || I.kind == 4) {
                     float g, dg;
-                    moffat_int_kind_f32(L.kind, k, &g, &dg);
+                    moffat_int_kind_f32(I.kind, k, &g, &dg);
                     return {g, dg};
                 }
             }
-            LeafD<double> m = moffat_bessel_d(L.kind, L.p[0] - 1., double(k), double(ksq), L.p[1]);
+            LeafD<double> m = moffat_bessel_d(I.kind, rec->p[0] - 1., double(k), double(ksq),
+                                              rec->p[1]);
             return {T(m.phi), T(m.dphi)};
           }
         }
@@ -192,24 +137,21 @@ __device__ inline LeafD<T> eval_leaf_d(const gsmi_leaf* __restrict__ Lp, const T
         return {phi, T(-5. / 6.) * t / ksq * phi};
       }
       case GSMI_LEAF_SERSIC: {
-        const T kmax = T(L.p[1]);
-        if (ksq >= kmax) {
-            T a = T(L.p[2]), b = T(L.p[3]);
+        if (ksq >= C.p[1]) {
             T rk = dsqrt(ksq);
-            T phi = (a + b / rk) / ksq;
-            return {phi, -(a + T(1.5) * b / rk) / (ksq * ksq)};
+            T phi = (C.p[2] + C.p[3] / rk) / ksq;
+            return {phi, -(C.p[2] + T(1.5) * C.p[3] / rk) / (ksq * ksq)};
         }
-        if (ksq < T(L.p[0]))
-            return {T(1) + ksq * (T(L.p[4]) + ksq * T(L.p[5])),
-                    T(L.p[4]) + T(2) * ksq * T(L.p[5])};
+        if (ksq < C.p[0])
+            return {T(1) + ksq * (C.p[4] + ksq * C.p[5]), C.p[4] + T(2) * ksq * C.p[5]};
         T lk = T(0.5) * dlog(ksq);
-        T S, dS;
-        sersic_spline_d<T>(L, tb, lk, &S, &dS);
+        T dS;
+        T S = spline_eq<T>(sersic_table<T>(tb, I.aux), I.ntab, lk, &dS);
         return {S / ksq, (T(0.5) * dS - S) / (ksq * ksq)};
       }
       case GSMI_LEAF_BOX: {
-        const T Xb = kxp * T(L.scale);
-        const T Yb = kyp * T(L.scale2);
+        const T Xb = kxp * C.s;
+        const T Yb = kyp * C.s2;
         T sx = fabs(Xb) < T(1e-4) ? T(1) - T(M_PI * M_PI / 6.) * Xb * Xb
                                   : dsinpi(Xb) / (T(M_PI) * Xb);
         T sy = fabs(Yb) < T(1e-4) ? T(1) - T(M_PI * M_PI / 6.) * Yb * Yb
@@ -218,52 +160,6 @@ __device__ inline LeafD<T> eval_leaf_d(const gsmi_leaf* __restrict__ Lp, const T
       }
     }
     return {T(0), T(0)};
-}
-
-// The post-order program on cached leaf values; leaf slot ovl (if >= 0) replaced by ovv.
-template <typename T, int ML>
-__device__ inline cplx<T> eval_cached(const gsmi_op* __restrict__ ops, int nops, const T (&v)[ML],
-                                      int ovl, T ovv, T kx, T ky)
-{
-    RegStack<T, StackDepth<T>::value> st;
-    for (int o = 0; o < nops; ++o) {
-        const gsmi_op* op = ops + o;
-        switch (op->code) {
-          case GSMI_OP_LEAF: {
-            const int a = op->arg;
-            T x = v[0];
-#pragma unroll
-            for (int l = 1; l < ML; ++l)
-                if (a == l) x = v[l];
-            if (a == ovl) x = ovv;
-            st.push({x, T(0)});
-            break;
-          }
-          case GSMI_OP_MUL: {
-            cplx<T> b = st.pop();
-            st.s[0] = cmul(st.s[0], b);
-            break;
-          }
-          case GSMI_OP_ADD: {
-            cplx<T> b = st.pop();
-            st.s[0] = cadd(st.s[0], b);
-            break;
-          }
-          case GSMI_OP_SCALE:
-            st.s[0].x *= T(op->a);
-            st.s[0].y *= T(op->a);
-            break;
-          case GSMI_OP_PHASE: {
-            T arg = T(op->a) * kx + T(op->b) * ky;
-            T sn, cs;
-            dsincos(arg, &sn, &cs);
-            const T fs = T(op->c);
-            st.s[0] = cmul(st.s[0], cplx<T>{fs * cs, -fs * sn});
-            break;
-          }
-        }
-    }
-    return st.s[0];
 }
 
 __device__ inline double wave_sum(double x)
@@ -325,7 +221,7 @@ row_r2c(const gsmi_stamp* __restrict__ stamps, const T* __restrict__ resid,
 // ---------------------------------------------------------------------------------------------
 // Kernel B: column DFTs of the row coefficients -> Rhat, then the gradient over every aliased
 // k-image sample of the group's columns.  grid (ceil((N/2+1)/cpg), nstamps);
-// LDS 2 cpg N complex.  ML = leaf slots held in registers (2 or 4).
+// LDS 2 cpg N complex.  ML = leaf slots with gradient accumulators (2 or 4).
 // ---------------------------------------------------------------------------------------------
 template <typename T, int ML>
 __global__ void __launch_bounds__(kThreads)
@@ -358,11 +254,6 @@ grad_colfft(const gsmi_stamp* __restrict__ stamps, const gsmi_op* __restrict__ o
     cplx<T>* bufa = lds;
     cplx<T>* bufb = lds + cpg * N;
 
-    int nl = 0;
-    for (int o = 0; o < nops; ++o)
-        if (sops[o].code == GSMI_OP_LEAF) nl = max(nl, sops[o].arg + 1);
-    const bool ok = nl <= ML;
-
     for (int e = threadIdx.x; e < ncol * N; e += blockDim.x) bufa[e] = {T(0), T(0)};
     __syncthreads();
     for (int e = threadIdx.x; e < ncol * ny; e += blockDim.x) {
@@ -373,6 +264,13 @@ grad_colfft(const gsmi_stamp* __restrict__ stamps, const gsmi_op* __restrict__ o
     }
     __syncthreads();
     const cplx<T>* res = fft_lds<T>(bufa, bufb, N, ncol, tw, 1);
+
+    // the program decoded into the LDS buffer the transform result does not occupy
+    ProgramD d;
+    LeafC<T>* lc = reinterpret_cast<LeafC<T>*>(res == bufa ? bufb : bufa);
+    const bool ok = decode_program<T>(sops, nops, sleaves, d, lc) && d.nl <= ML;
+    const int nl = d.nl;
+    __syncthreads();
 
     T acc[NF];
 #pragma unroll
@@ -395,19 +293,21 @@ grad_colfft(const gsmi_stamp* __restrict__ stamps, const gsmi_op* __restrict__ o
                 const T ky = T(j) * dk;
                 const cplx<T> seed = neg ? cconj(A) : A;
                 const cplx<T> Z = cmul(seed, cmul(X[i], Y[j]));
-                T v[ML], phi[ML], dv[ML];
+                T v[kProgLeaves], phi[ML], dv[ML];
+#pragma unroll
+                for (int l = 0; l < kProgLeaves; ++l) v[l] = T(0);
 #pragma unroll
                 for (int l = 0; l < ML; ++l) {
-                    v[l] = phi[l] = dv[l] = T(0);
+                    phi[l] = dv[l] = T(0);
                     if (l < nl) {
-                        LeafD<T> d = eval_leaf_d<T>(sleaves + l, tb, kx, ky, i, j, Nk);
-                        const T fl = T(sleaves[l].flux);
-                        phi[l] = d.phi;
-                        v[l] = fl * d.phi;
-                        dv[l] = fl * d.dphi;
+                        LeafD<T> q = eval_leafc_d<T>(lc[l], d.li[l], sleaves + l, tb, kx, ky, i,
+                                                     j, Nk);
+                        phi[l] = q.phi;
+                        v[l] = lc[l].flux * q.phi;
+                        dv[l] = lc[l].flux * q.dphi;
                     }
                 }
-                const cplx<T> P = eval_cached<T, ML>(sops, nops, v, -1, T(0), kx, ky);
+                const cplx<T> P = run_ops<T>(d, sops, v, -1, T(0), kx, ky);
                 const cplx<T> FZ = cmul(P, Z);
                 acc[6 * ML] += kx * FZ.y;
                 acc[6 * ML + 1] += ky * FZ.y;
@@ -415,23 +315,17 @@ grad_colfft(const gsmi_stamp* __restrict__ stamps, const gsmi_op* __restrict__ o
 #pragma unroll
                 for (int l = 0; l < ML; ++l) {
                     if (l < nl) {
-                        const cplx<T> p1 = eval_cached<T, ML>(sops, nops, v, l, T(1), kx, ky);
-                        const cplx<T> p0 = eval_cached<T, ML>(sops, nops, v, l, T(0), kx, ky);
+                        const cplx<T> p1 = run_ops<T>(d, sops, v, l, T(1), kx, ky);
+                        const cplx<T> p0 = run_ops<T>(d, sops, v, l, T(0), kx, ky);
                         const cplx<T> t = cmul(csub(p1, p0), Z);
-                        const gsmi_leaf& L = sleaves[l];
+                        const LeafC<T>& L = lc[l];
                         T kxp, kyp;
-                        if (L.separable) {
-                            kxp = T(L.jac[0]) * kx;
-                            kyp = T(L.jac[3]) * ky;
-                        } else {
-                            kxp = T(L.jac[0]) * kx + T(L.jac[2]) * ky;
-                            kyp = T(L.jac[1]) * kx + T(L.jac[3]) * ky;
-                        }
-                        const T s = T(L.scale);
+                        leaf_kprime(L, d.li[l], kx, ky, &kxp, &kyp);
+                        const T sc = L.s;
                         const T g = t.x * dv[l];
-                        const T g2 = T(2) * s * s * g;
+                        const T g2 = T(2) * sc * sc * g;
                         acc[6 * l] += t.x * phi[l];
-                        acc[6 * l + 1] += T(2) * s * (kxp * kxp + kyp * kyp) * g;
+                        acc[6 * l + 1] += T(2) * sc * (kxp * kxp + kyp * kyp) * g;
                         acc[6 * l + 2] += g2 * kxp * kx;
                         acc[6 * l + 3] += g2 * kyp * kx;
                         acc[6 * l + 4] += g2 * kxp * ky;

@@ -47,36 +47,17 @@ axis_factors(const gsmi_stamp* __restrict__ stamps, const gsmi_op* __restrict__ 
     }
 }
 
-// ---------------------------------------------------------------------------------------------
-// Kernel 1: fused k-space evaluation + product/sinc/phase + Hermitian wrap + column inverse DFT
-// grid: (ceil((N/2+1)/cpg), nstamps); dynamic LDS: 2 * cpg * N complex
-// ---------------------------------------------------------------------------------------------
+// Every target cell (c, r) of the group's columns: the sum over the k-image samples congruent to
+// it modulo N (the stored sample, conjugated where it stands for kx < 0) of program x axis
+// factors X(kx) Y(ky), written to bufa.
 template <typename T>
-__global__ void __launch_bounds__(kThreads)
-fold_colfft(const gsmi_stamp* __restrict__ stamps, const gsmi_op* __restrict__ ops,
-            const gsmi_leaf* __restrict__ leaves, Tables<T> tb, const cplx<T>* __restrict__ tw,
-            cplx<T>* __restrict__ scratch)
+__device__ __forceinline__ void fold_cells(const gsmi_op* __restrict__ sops, int nops,
+                                           const gsmi_leaf* __restrict__ sleaves,
+                                           const Tables<T>& tb, const cplx<T>* __restrict__ X,
+                                           const cplx<T>* __restrict__ Y, T dk, int N, int Nk,
+                                           int c0, int ncol, cplx<T>* bufa)
 {
-    extern __shared__ unsigned char smem_raw[];
-    cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
-    const gsmi_stamp* st = stamps + blockIdx.y;
-    const int N = st->N;
-    const int Nk = st->Nk;
-    const int half = N / 2;
-    const int cpg = cols_per_group(N);
-    const int c0 = blockIdx.x * cpg;
-    const int ncol = min(cpg, half + 1 - c0);
-    if (ncol <= 0) return;
-    const T dk = T(st->dk);
-    const gsmi_op* sops = ops + st->op_begin;
-    const int nops = st->op_count;
-    const gsmi_leaf* sleaves = leaves + st->leaf_begin;
     const int hk = Nk / 2;
-    const cplx<T>* X = scratch + st->axis_offset;
-    const cplx<T>* Y = X + hk + 1 + hk;               // Y[ky] for ky in [-hk, hk]
-    cplx<T>* bufa = lds;
-    cplx<T>* bufb = lds + cpg * N;
-
     for (int e = threadIdx.x; e < ncol * N; e += blockDim.x) {
         const int cl = e / N;
         const int r = e - cl * N;
@@ -98,6 +79,48 @@ fold_colfft(const gsmi_stamp* __restrict__ stamps, const gsmi_op* __restrict__ o
         }
         bufa[cl * N + r] = acc;
     }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Kernel 1: fused k-space evaluation + product/sinc/phase + Hermitian wrap + column inverse DFT
+// grid: (ceil((N/2+1)/cpg), nstamps); dynamic LDS: 2 * cpg * N complex
+// ---------------------------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ void fold_colfft_body(const gsmi_stamp* __restrict__ stamps,
+                                                 const gsmi_op* __restrict__ ops,
+                                                 const gsmi_leaf* __restrict__ leaves,
+                                                 const Tables<T>& tb,
+                                                 const cplx<T>* __restrict__ tw,
+                                                 cplx<T>* __restrict__ scratch)
+{
+    extern __shared__ unsigned char smem_raw[];
+    cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
+    const gsmi_stamp* st = stamps + blockIdx.y;
+    const int N = st->N;
+    const int Nk = st->Nk;
+    const int half = N / 2;
+    const int cpg = cols_per_group(N);
+    const int c0 = blockIdx.x * cpg;
+    const int ncol = min(cpg, half + 1 - c0);
+    if (ncol <= 0) return;
+    const T dk = T(st->dk);
+    const gsmi_op* sops = ops + st->op_begin;
+    const int nops = st->op_count;
+    const gsmi_leaf* sleaves = leaves + st->leaf_begin;
+    const int hk = Nk / 2;
+    const cplx<T>* X = scratch + st->axis_offset;     // X[kx], kx in [0, hk]; then Y[ky]
+    cplx<T>* bufa = lds;
+    cplx<T>* bufb = lds + cpg * N;
+    const int naxis = 3 * hk + 2;
+    if (naxis <= cpg * N) {
+        // the axis factors staged in bufb, which the column DFT needs only afterwards
+        for (int e = threadIdx.x; e < naxis; e += blockDim.x) bufb[e] = X[e];
+        __syncthreads();
+        fold_cells<T>(sops, nops, sleaves, tb, bufb, bufb + 2 * hk + 1, dk, N, Nk, c0, ncol,
+                      bufa);
+    } else {
+        fold_cells<T>(sops, nops, sleaves, tb, X, X + 2 * hk + 1, dk, N, Nk, c0, ncol, bufa);
+    }
     __syncthreads();
     cplx<T>* res = fft_lds<T>(bufa, bufb, N, ncol, tw, 1);
     // rows of the output window: position p <-> DFT index p mod N
@@ -113,6 +136,31 @@ fold_colfft(const gsmi_stamp* __restrict__ stamps, const gsmi_op* __restrict__ o
         out[row * W + c0 + cl] = res[cl * N + y];
     }
 }
+
+// The FP32 kernel is held to 96 VGPRs (5 waves per SIMD): its sample loop is latency bound and
+// runs faster with the extra wave than without the few registers it gives up.
+__global__ void __launch_bounds__(kThreads, 5)
+fold_colfft_f32(const gsmi_stamp* __restrict__ stamps, const gsmi_op* __restrict__ ops,
+                const gsmi_leaf* __restrict__ leaves, Tables<float> tb,
+                const cplx<float>* __restrict__ tw, cplx<float>* __restrict__ scratch)
+{
+    fold_colfft_body<float>(stamps, ops, leaves, tb, tw, scratch);
+}
+
+__global__ void __launch_bounds__(kThreads)
+fold_colfft_f64(const gsmi_stamp* __restrict__ stamps, const gsmi_op* __restrict__ ops,
+                const gsmi_leaf* __restrict__ leaves, Tables<double> tb,
+                const cplx<double>* __restrict__ tw, cplx<double>* __restrict__ scratch)
+{
+    fold_colfft_body<double>(stamps, ops, leaves, tb, tw, scratch);
+}
+
+template <typename T>
+struct FoldKernel;
+template <>
+struct FoldKernel<float> { static constexpr auto fn = fold_colfft_f32; };
+template <>
+struct FoldKernel<double> { static constexpr auto fn = fold_colfft_f64; };
 
 // ---------------------------------------------------------------------------------------------
 // Kernel 2: real inverse DFT along x of the column-transformed rows, crop and scale.
@@ -356,7 +404,7 @@ int fold_impl(const gsmi_stamp* stamps, int nstamps, int N, const gsmi_op* ops,
     if (rc) return rc;
     dim3 g1((N / 2 + 1 + cpg - 1) / cpg, nstamps);
     size_t lds1 = 2 * size_t(cpg) * N * sizeof(cplx<T>);
-    hipLaunchKernelGGL(fold_colfft<T>, g1, dim3(kThreads), lds1,
+    hipLaunchKernelGGL(FoldKernel<T>::fn, g1, dim3(kThreads), lds1,
                        reinterpret_cast<hipStream_t>(stream), stamps, ops, leaves, tb, tw,
                        reinterpret_cast<cplx<T>*>(scratch));
     return hip_check(hipGetLastError(), "fold_colfft launch");
