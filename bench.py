@@ -176,10 +176,7 @@ def main():
     log("workload: %d stamps, build %.2fs, plan %.2fs, FFT sizes %s" % (
         args.n, t_build, t_plan, plan.fft_sizes()))
 
-    # --- timed region: K steps of the batched FP32 forward -----------------------------------
-    for _ in range(args.warmup):
-        plan.run()
-    torch.cuda.synchronize()
+    # --- the per-FFT-size launches (one fold + one row launch per size), event-timed --------
     evs = []
 
     def mk(kind, N):
@@ -187,24 +184,52 @@ def main():
         evs[-1].append((kind, N, e))
         return e
 
+    def kernel_times(p, steps):
+        del evs[:]
+        for _ in range(2):
+            p.run()
+        torch.cuda.synchronize()
+        for _ in range(steps):
+            evs.append([])
+            p.run(events=mk)
+        torch.cuda.synchronize()
+        fold, rows = {}, {}
+        for step in evs:
+            for k in range(len(step) - 1):
+                kind, N, e = step[k]
+                d = e.elapsed_time(step[k + 1][2])
+                (fold if kind == 'fold' else rows).setdefault(N, []).append(d)
+        return ({N: float(np.mean(v)) for N, v in fold.items()},
+                {N: float(np.mean(v)) for N, v in rows.items()})
+
+    bins_fold, bins_rows = kernel_times(plan, min(args.steps, 10))
+    t_bins = sum(bins_fold.values()) + sum(bins_rows.values())
+    log("per-size launches: fold %.3f ms + rows %.3f ms per step; fold per-N ms %s" % (
+        sum(bins_fold.values()), sum(bins_rows.values()),
+        {N: round(v, 4) for N, v in bins_fold.items()}))
+
+    # --- work lists: every FFT size in one fold and one row launch, tiles and grid tuned on
+    # this batch, the whole step captured in a HIP graph ---------------------------------------
+    t0 = time.perf_counter()
+    tuning = plan.autotune()
+    t_tune = time.perf_counter() - t0
+    log("autotune (%.1fs): cols %s rows %s grid %s -> %.3f ms" % (
+        t_tune, tuning['cols'], tuning['rows'], tuning['grid'], tuning['ms']))
+    fold_ms, rows_ms = kernel_times(plan, min(args.steps, 10))
+    plan.capture()
+
+    # --- timed region: K graph launches of the batched FP32 forward ---------------------------
+    for _ in range(args.warmup):
+        plan.replay()
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        evs.append([])
-        plan.run(events=mk)
+        plan.replay()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms_per_step = 1e3 * wall / args.steps
     value = args.n / (wall / args.steps)
-    log("batched FP32: %.3f ms/step, %.0f stamps/s" % (ms_per_step, value))
-
-    # per-kernel times from the events (same stream as the kernels)
-    fold_ms = {}
-    rows_ms = {}
-    for step in evs:
-        for k in range(len(step) - 1):
-            kind, N, e = step[k]
-            d = e.elapsed_time(step[k + 1][2])
-            (fold_ms if kind == 'fold' else rows_ms).setdefault(N, []).append(d)
+    log("batched FP32 (graph): %.3f ms/step, %.0f stamps/s" % (ms_per_step, value))
     recs = plan.recs
     nfac = {}
     flops = {'fold': 0., 'rows': 0.}
@@ -218,23 +243,21 @@ def main():
         bf, br = kernel_bytes((rec[0], rec[1], rec[2], 5), int(r['nx']), 4)
         nbytes['fold'] += bf
         nbytes['rows'] += br
-    t_fold = sum(np.mean(v) for v in fold_ms.values())
-    t_rows = sum(np.mean(v) for v in rows_ms.values())
+    t_fold = sum(fold_ms.values())
+    t_rows = sum(rows_ms.values())
     n_launch = len(fold_ms)
     dominant = 'fold' if t_fold >= t_rows else 'rows'
     t_dom = t_fold if dominant == 'fold' else t_rows
     achieved = flops[dominant] / (t_dom * 1e-3) / 1e12
-    log("fold_colfft: %.3f ms/step over %d launches; row_c2r: %.3f ms/step" % (
+    log("work lists: fold %.3f ms/step over %d launch(es); rows %.3f ms/step" % (
         t_fold, n_launch, t_rows))
-    log("fold per-N ms: %s" % {N: round(float(np.mean(v)), 4) for N, v in fold_ms.items()})
-    log("rows per-N ms: %s" % {N: round(float(np.mean(v)), 4) for N, v in rows_ms.items()})
     traffic = None
     tfile = os.path.join(ROOT, 'profiles', 'hbm_traffic_bulge_disk_4096.json')
     if os.path.exists(tfile):
         with open(tfile) as f:
-            traffic = json.load(f).get(dominant)
+            traffic = json.load(f).get(dominant + '_list')
     roofline = {
-        "kernel": "fold_colfft" if dominant == 'fold' else "row_c2r",
+        "kernel": "fold_list_f32" if dominant == 'fold' else "rows_list<float>",
         "bound": "mfma",
         "achieved": round(achieved, 4),
         "peak": FP32_PEAK_TFLOPS,
@@ -329,6 +352,13 @@ def main():
         "drawimage_fp64_end_to_end_stamps_per_s": round(drawimage_rate, 1),
         "max_peak_rel_err_fp32_vs_fp64": worst,
         "host_plan_s": round(t_plan, 3),
+        "schedule": {"launches": "axis prepass + 1 fold + 1 row launch per step, replayed "
+                                 "from one HIP graph",
+                     "cols_per_group": {str(k): v for k, v in tuning['cols'].items()},
+                     "rows_per_group": {str(k): v for k, v in tuning['rows'].items()},
+                     "grid": tuning['grid'], "autotune_s": round(t_tune, 2),
+                     "per_size_launches_ms_per_step": round(t_bins, 4),
+                     "work_list_ms_per_step": round(t_fold + t_rows, 4)},
     }
     print(json.dumps(out), flush=True)
 

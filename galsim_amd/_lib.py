@@ -34,6 +34,18 @@ ERRORS = {-1: "invalid argument", -2: "parameter out of range", -3: "solve/table
           -4: "HIP runtime failure", -5: "unsupported FFT size"}
 
 
+WORK_DTYPE = np.dtype([('stamp', np.int32), ('first', np.int32), ('count', np.int32),
+                       ('per_group', np.int32)])
+
+
+class Schedule(ctypes.Structure):
+    """gsmi_schedule: the two work lists of a batch and their launch shapes."""
+    _fields_ = [('fold', ctypes.c_void_p), ('rows', ctypes.c_void_p),
+                ('nfold', ctypes.c_int32), ('nrows', ctypes.c_int32),
+                ('fold_grid', ctypes.c_int32), ('rows_grid', ctypes.c_int32),
+                ('fold_lds', ctypes.c_int32), ('rows_lds', ctypes.c_int32)]
+
+
 class GalSimAMDError(RuntimeError):
     """Raised for failures inside the native library (mirrors GalSimError)."""
 
@@ -69,6 +81,12 @@ _SIGS = {
     'gsmi_draw_kimage_f64': (_I, [_P, _I, _P, _P, _P, _I, _I, _I, _I, _I, _D, _P, _P]),
     'gsmi_irfft_f64': (_I, [_P, _I, _P, _P, _P]),
     'gsmi_wrap_hermx_f64': (_I, [_P, _I, _I, _P, _P]),
+    'gsmi_draw_list_f64': (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P]),
+    'gsmi_draw_list_f32': (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P]),
+    'gsmi_fold_list_f64': (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P]),
+    'gsmi_fold_list_f32': (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P]),
+    'gsmi_rows_list_f64': (_I, [_P, _I, _P, _P, _P, _P]),
+    'gsmi_rows_list_f32': (_I, [_P, _I, _P, _P, _P, _P]),
     'gsmi_adjoint_partials': (_I, [_I, _I, _P]),
     'gsmi_adjoint_f64': (_I, [_P, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     'gsmi_adjoint_f32': (_I, [_P, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P]),

@@ -5,6 +5,8 @@ pixel convolution, drawFFT_makeKImage sizing) to every profile, lowers them all 
 buffer, bins the stamps by FFT size N and runs the two kernels once per bin on one stream.  The
 result stays on the GPU as one flat tensor plus per-stamp offsets and shapes.
 """
+import ctypes
+
 import numpy as np
 
 from . import _lib
@@ -98,19 +100,16 @@ class BatchPlan:
         sizes = (recs['nx'].astype(np.int64) * recs['ny'].astype(np.int64))
         out_off[1:] = np.cumsum(sizes)[:-1]
         recs['out_offset'] = out_off
+        # scratch: every stamp's column-transform rows, then every stamp's axis factors, with
+        # no region shared between stamps (a work-list launch runs all FFT sizes at once)
+        rows = recs['ny'].astype(np.int64) * (recs['N'].astype(np.int64) // 2 + 1)
+        axis = 3 * (recs['Nk'].astype(np.int64) // 2) + 2
+        recs['scratch_offset'] = np.concatenate([[0], np.cumsum(rows)[:-1]])
+        recs['axis_offset'] = int(rows.sum()) + np.concatenate([[0], np.cumsum(axis)[:-1]])
+        scratch_need = int(rows.sum() + axis.sum())
         self.groups = []
-        scratch_need = 0
         for N in np.unique(recs['N']):
             idx = np.nonzero(recs['N'] == N)[0]
-            w = N // 2 + 1
-            rows = recs['ny'][idx].astype(np.int64) * w
-            axis = 3 * (recs['Nk'][idx].astype(np.int64) // 2) + 2
-            s_off = np.zeros(len(idx), dtype=np.int64)
-            s_off[1:] = np.cumsum(rows)[:-1]
-            a_off = int(rows.sum()) + np.concatenate([[0], np.cumsum(axis)[:-1]])
-            recs['scratch_offset'][idx] = s_off
-            recs['axis_offset'][idx] = a_off
-            scratch_need = max(scratch_need, int(rows.sum() + axis.sum()))
             self.groups.append((int(N), int(idx[0]), len(idx), int(recs['ny'][idx].max())))
         pk = render._Packer()
         self._o_stamp = pk.add(recs)
@@ -137,16 +136,194 @@ class BatchPlan:
         self.stamp_shape = np.stack([recs['ny'][inv], recs['nx'][inv]], axis=1)
         self.nops = len(ctx.ops)
         self.nleaves = len(ctx.leaves)
+        self._sched = None
+        self._graph = None
+        self.tuning = None
+
+    # -- work-list schedule ----------------------------------------------------------------------
+    @staticmethod
+    def default_cols(N):
+        """Columns per fold workgroup of the per-size kernels: 1024 / N (one column from 1024)."""
+        return max(1, 1024 // N)
+
+    @staticmethod
+    def default_rows(N):
+        """Rows per row-transform workgroup of the per-size kernels: 1024 / (N/2)."""
+        return max(1, 1024 // (N // 2))
+
+    def _elem(self):
+        return 8 if self.precision == 'f32' else 16
+
+    def _work_lists(self, cols, rows, bins=None):
+        """Fold and row work items (gsmi_work) for the given tiles, largest FFT sizes first."""
+        recs = self.recs
+        fold, rw = [], []
+        groups = sorted(self.groups, key=lambda g: -g[0])
+        if bins is not None:
+            groups = [g for g in groups if g[0] in bins]
+        lds_f = lds_r = 0
+        for N, first, count, _ in groups:
+            cpg = int(cols.get(N, self.default_cols(N)))
+            rpg = int(rows.get(N, self.default_rows(N)))
+            W = N // 2 + 1
+            lds_f = max(lds_f, 2 * cpg * N * self._elem())
+            lds_r = max(lds_r, 2 * rpg * (N // 2) * self._elem())
+            for s in range(first, first + count):
+                for c0 in range(0, W, cpg):
+                    fold.append((s, c0, min(cpg, W - c0), cpg))
+                ny = int(recs['ny'][s])
+                for r0 in range(0, ny, rpg):
+                    rw.append((s, r0, min(rpg, ny - r0), rpg))
+        if max(lds_f, lds_r) > 65536:
+            raise _lib.GalSimAMDError("work-list tiles need %d bytes of LDS (limit 65536)"
+                                      % max(lds_f, lds_r))
+        return (np.array(fold, dtype=np.int32).reshape(-1, 4),
+                np.array(rw, dtype=np.int32).reshape(-1, 4), lds_f, lds_r)
+
+    def make_schedule(self, cols=None, rows=None, fold_grid=None, rows_grid=None, bins=None):
+        """(device item tensor, gsmi_schedule) for tiles ``cols``/``rows`` (dict N -> columns or
+        rows per workgroup; missing sizes take the defaults) and grids (None: one workgroup per
+        item; smaller: persistent workgroups).  ``bins`` restricts it to some FFT sizes."""
+        import torch
+        fold, rw, lds_f, lds_r = self._work_lists(cols or {}, rows or {}, bins)
+        items = torch.from_numpy(np.ascontiguousarray(np.concatenate([fold, rw]))).cuda()
+        sc = _lib.Schedule()
+        base = items.data_ptr()
+        sc.fold = base
+        sc.rows = base + len(fold) * 16
+        sc.nfold, sc.nrows = len(fold), len(rw)
+        sc.fold_grid = len(fold) if fold_grid is None else int(min(fold_grid, len(fold)))
+        sc.rows_grid = len(rw) if rows_grid is None else int(min(rows_grid, len(rw)))
+        sc.fold_grid, sc.rows_grid = max(sc.fold_grid, 1), max(sc.rows_grid, 1)
+        sc.fold_lds, sc.rows_lds = lds_f, lds_r
+        return items, sc
+
+    def use_schedule(self, cols=None, rows=None, fold_grid=None, rows_grid=None):
+        """Run every FFT size in one fold and one row launch from now on."""
+        self._sched = self.make_schedule(cols, rows, fold_grid, rows_grid)
+        self._graph = None
+
+    def _launch_list(self, sched, part, s):
+        lib = _lib.lib()
+        base = self.dev.data_ptr()
+        sc = ctypes.addressof(sched[1])
+        args_fold = (base + self._o_stamp, self.n, sc, base + self._o_ops, base + self._o_leaves,
+                     base + self._o_tab)
+        if part in ('fold', 'all'):
+            if self.precision == 'f32':
+                rc = lib.gsmi_fold_list_f32(*args_fold, base + self._o_tab32,
+                                            self.scratch.data_ptr(), s)
+            else:
+                rc = lib.gsmi_fold_list_f64(*args_fold, base + self._o_clip,
+                                            self.scratch.data_ptr(), s)
+            _lib.check(rc, "fold work list")
+        if part in ('rows', 'all'):
+            f = lib.gsmi_rows_list_f32 if self.precision == 'f32' else lib.gsmi_rows_list_f64
+            rc = f(base + self._o_stamp, self.n, sc, self.scratch.data_ptr(),
+                   self.out.data_ptr(), s)
+            _lib.check(rc, "row work list")
+
+    # -- tuning and graphs -----------------------------------------------------------------------
+    def _time(self, fn, reps):
+        import torch
+        fn()
+        torch.cuda.synchronize()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(reps):
+            fn()
+        b.record()
+        b.synchronize()
+        return a.elapsed_time(b) / reps
+
+    def autotune(self, reps=5, factors=(0.25, 0.5, 1, 2, 4), grids=(None, 1024, 2048, 4096)):
+        """Pick, by timing on this batch, the columns per fold workgroup and rows per row
+        workgroup of every FFT size (powers of two around the defaults, within the 64 KiB LDS
+        limit), then the fold and row grid sizes (one workgroup per item, or persistent), and
+        switch run() to the tuned work lists.  Returns the choice and its timings (ms)."""
+        import torch
+        s = torch.cuda.current_stream().cuda_stream
+        cols, rows, log = {}, {}, {}
+        elem = self._elem()
+        for N, first, count, _ in self.groups:
+            W = N // 2 + 1
+            best = None
+            for f in factors:
+                c = int(self.default_cols(N) * f)
+                if c < 1 or c > W or 2 * c * N * elem > 65536:
+                    continue
+                sched = self.make_schedule({N: c}, {}, bins=(N,))
+                t = self._time(lambda: self._launch_list(sched, 'fold', s), reps)
+                log[('cols', N, c)] = t
+                if best is None or t < best[0]:
+                    best = (t, c)
+            cols[N] = best[1]
+            best = None
+            for f in factors:
+                r = int(self.default_rows(N) * f)
+                if r < 1 or r > int(self.recs['ny'][first:first + count].max()) * 2 or \
+                        2 * r * (N // 2) * elem > 65536:
+                    continue
+                sched = self.make_schedule({N: cols[N]}, {N: r}, bins=(N,))
+                self._launch_list(sched, 'fold', s)
+                t = self._time(lambda: self._launch_list(sched, 'rows', s), reps)
+                log[('rows', N, r)] = t
+                if best is None or t < best[0]:
+                    best = (t, r)
+            rows[N] = best[1]
+        best = None
+        for g in grids:
+            sched = self.make_schedule(cols, rows, g, g)
+            t = self._time(lambda: self._launch_list(sched, 'all', s), reps)
+            log[('grid', g)] = t
+            if best is None or t < best[0]:
+                best = (t, g)
+        self.use_schedule(cols, rows, best[1], best[1])
+        self.tuning = {'cols': cols, 'rows': rows, 'grid': best[1], 'ms': best[0],
+                       'timings': {str(k): v for k, v in log.items()}}
+        return self.tuning
+
+    def capture(self):
+        """Capture one run() in a HIP graph (torch.cuda.CUDAGraph); replay() then launches the
+        whole batch with one graph launch."""
+        import torch
+        self.run()
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self.run()
+        torch.cuda.synchronize()
+        self._graph = g
+        return g
+
+    def replay(self):
+        if self._graph is None:
+            raise _lib.GalSimAMDError("capture() the batch first")
+        self._graph.replay()
+        return self.out
 
     def run(self, stream=None, events=None):
-        """Launch every bin's two kernels.  ``events``, if given, is a callable
-        ``events(kind, N)`` returning a torch.cuda.Event to record before each kernel
-        ('fold' / 'rows') and after the last ('end', 0)."""
+        """Launch the batch: with a work-list schedule (use_schedule / autotune), the axis
+        prepass, one fold and one row launch for every FFT size; otherwise each FFT size's
+        kernels in turn.  ``events``, if given, is a callable ``events(kind, N)`` returning a
+        torch.cuda.Event to record before each kernel ('fold' / 'rows', N = 0 for a work list)
+        and after the last ('end', 0)."""
         import torch
         lib = _lib.lib()
         base = self.dev.data_ptr()
         s = torch.cuda.current_stream().cuda_stream if stream is None else stream
         sc = self.scratch.data_ptr()
+        if self._sched is not None:
+            if events:
+                events('fold', 0).record()
+            self._launch_list(self._sched, 'fold', s)
+            if events:
+                events('rows', 0).record()
+            self._launch_list(self._sched, 'rows', s)
+            if events:
+                events('end', 0).record()
+            return self.out
         for N, first, count, max_ny in self.groups:
             st = base + self._o_stamp + first * _lib.STAMP_DTYPE.itemsize
             if events:

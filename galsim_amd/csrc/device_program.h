@@ -449,6 +449,26 @@ __device__ inline cplx<T> axis_factor(const gsmi_op* __restrict__ ops, int nops,
     return f;
 }
 
+// Twiddle tables exp(+2 pi i m / P) for the largest supported sizes of each family: a size-N
+// transform reads every (P / N)-th entry of the table of its family (2^k or 3 * 2^k).
+constexpr int kTwPow2 = 4096;
+constexpr int kTwPow3 = 3072;
+
+template <typename T>
+struct TwiddleSet {
+    const cplx<T>* p2;
+    const cplx<T>* p3;
+    __device__ __forceinline__ const cplx<T>* pick(int N, int* stride) const
+    {
+        if (N % 3 == 0) {
+            *stride = kTwPow3 / N;
+            return p3;
+        }
+        *stride = kTwPow2 / N;
+        return p2;
+    }
+};
+
 // ---------------------------------------------------------------------------------------------
 // In-LDS Stockham FFT (inverse, e^{+2 pi i}) over nseg contiguous segments of length n.
 // tw[m * twstride] = exp(+2 pi i m / n).  Mixed radix 4/2/3; returns the buffer with the result.

@@ -86,23 +86,15 @@ __device__ __forceinline__ void fold_cells(const gsmi_op* __restrict__ sops, int
 // grid: (ceil((N/2+1)/cpg), nstamps); dynamic LDS: 2 * cpg * N complex
 // ---------------------------------------------------------------------------------------------
 template <typename T>
-__device__ __forceinline__ void fold_colfft_body(const gsmi_stamp* __restrict__ stamps,
-                                                 const gsmi_op* __restrict__ ops,
-                                                 const gsmi_leaf* __restrict__ leaves,
-                                                 const Tables<T>& tb,
-                                                 const cplx<T>* __restrict__ tw,
-                                                 cplx<T>* __restrict__ scratch)
+__device__ __forceinline__ void fold_item(const gsmi_stamp* __restrict__ st, int c0, int ncol,
+                                          int cpg, const gsmi_op* __restrict__ ops,
+                                          const gsmi_leaf* __restrict__ leaves,
+                                          const Tables<T>& tb, const cplx<T>* __restrict__ tw,
+                                          int tws, cplx<T>* __restrict__ scratch, cplx<T>* lds)
 {
-    extern __shared__ unsigned char smem_raw[];
-    cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
-    const gsmi_stamp* st = stamps + blockIdx.y;
     const int N = st->N;
     const int Nk = st->Nk;
     const int half = N / 2;
-    const int cpg = cols_per_group(N);
-    const int c0 = blockIdx.x * cpg;
-    const int ncol = min(cpg, half + 1 - c0);
-    if (ncol <= 0) return;
     const T dk = T(st->dk);
     const gsmi_op* sops = ops + st->op_begin;
     const int nops = st->op_count;
@@ -122,7 +114,7 @@ __device__ __forceinline__ void fold_colfft_body(const gsmi_stamp* __restrict__ 
         fold_cells<T>(sops, nops, sleaves, tb, X, X + 2 * hk + 1, dk, N, Nk, c0, ncol, bufa);
     }
     __syncthreads();
-    cplx<T>* res = fft_lds<T>(bufa, bufb, N, ncol, tw, 1);
+    cplx<T>* res = fft_lds<T>(bufa, bufb, N, ncol, tw, tws);
     // rows of the output window: position p <-> DFT index p mod N
     cplx<T>* out = scratch + st->scratch_offset;
     const int W = half + 1;
@@ -134,6 +126,56 @@ __device__ __forceinline__ void fold_colfft_body(const gsmi_stamp* __restrict__ 
         const int p = y0 + row;
         const int y = (p % N + N) % N;
         out[row * W + c0 + cl] = res[cl * N + y];
+    }
+}
+
+// One stamp per blockIdx.y, column group blockIdx.x (all stamps of the launch share N).
+template <typename T>
+__device__ __forceinline__ void fold_colfft_body(const gsmi_stamp* __restrict__ stamps,
+                                                 const gsmi_op* __restrict__ ops,
+                                                 const gsmi_leaf* __restrict__ leaves,
+                                                 const Tables<T>& tb,
+                                                 const cplx<T>* __restrict__ tw,
+                                                 cplx<T>* __restrict__ scratch)
+{
+    extern __shared__ unsigned char smem_raw[];
+    const gsmi_stamp* st = stamps + blockIdx.y;
+    const int N = st->N;
+    const int cpg = cols_per_group(N);
+    const int c0 = blockIdx.x * cpg;
+    const int ncol = min(cpg, N / 2 + 1 - c0);
+    if (ncol <= 0) return;
+    fold_item<T>(st, c0, ncol, cpg, ops, leaves, tb, tw, 1, scratch,
+                 reinterpret_cast<cplx<T>*>(smem_raw));
+}
+
+// Work-list form: every FFT size in one launch.  Workgroup b takes items b, b + grid, ...
+// (persistent when the grid is smaller than the list); items are validated against the stamp
+// records and the launch's LDS so a malformed list skips items instead of overrunning memory.
+template <typename T>
+__device__ __forceinline__ void fold_list_body(const gsmi_stamp* __restrict__ stamps, int nstamps,
+                                               const gsmi_work* __restrict__ items, int nitems,
+                                               int lds_bytes, const gsmi_op* __restrict__ ops,
+                                               const gsmi_leaf* __restrict__ leaves,
+                                               const Tables<T>& tb, TwiddleSet<T> tws,
+                                               cplx<T>* __restrict__ scratch)
+{
+    extern __shared__ unsigned char smem_raw[];
+    cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
+    for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
+        const gsmi_work it = items[w];
+        if (it.stamp < 0 || it.stamp >= nstamps) continue;
+        const gsmi_stamp* st = stamps + it.stamp;
+        const int N = st->N;
+        if (it.count <= 0 || it.first < 0 || it.first + it.count > N / 2 + 1 ||
+            it.per_group < it.count ||
+            size_t(2) * it.per_group * N * sizeof(cplx<T>) > size_t(lds_bytes))
+            continue;
+        int stride;
+        const cplx<T>* tw = tws.pick(N, &stride);
+        fold_item<T>(st, it.first, it.count, it.per_group, ops, leaves, tb, tw, stride, scratch,
+                     lds);
+        __syncthreads();
     }
 }
 
@@ -155,12 +197,38 @@ fold_colfft_f64(const gsmi_stamp* __restrict__ stamps, const gsmi_op* __restrict
     fold_colfft_body<double>(stamps, ops, leaves, tb, tw, scratch);
 }
 
+__global__ void __launch_bounds__(kThreads, 5)
+fold_list_f32(const gsmi_stamp* __restrict__ stamps, int nstamps,
+              const gsmi_work* __restrict__ items, int nitems, int lds_bytes,
+              const gsmi_op* __restrict__ ops, const gsmi_leaf* __restrict__ leaves,
+              Tables<float> tb, TwiddleSet<float> tws, cplx<float>* __restrict__ scratch)
+{
+    fold_list_body<float>(stamps, nstamps, items, nitems, lds_bytes, ops, leaves, tb, tws,
+                          scratch);
+}
+
+__global__ void __launch_bounds__(kThreads)
+fold_list_f64(const gsmi_stamp* __restrict__ stamps, int nstamps,
+              const gsmi_work* __restrict__ items, int nitems, int lds_bytes,
+              const gsmi_op* __restrict__ ops, const gsmi_leaf* __restrict__ leaves,
+              Tables<double> tb, TwiddleSet<double> tws, cplx<double>* __restrict__ scratch)
+{
+    fold_list_body<double>(stamps, nstamps, items, nitems, lds_bytes, ops, leaves, tb, tws,
+                           scratch);
+}
+
 template <typename T>
 struct FoldKernel;
 template <>
-struct FoldKernel<float> { static constexpr auto fn = fold_colfft_f32; };
+struct FoldKernel<float> {
+    static constexpr auto fn = fold_colfft_f32;
+    static constexpr auto list = fold_list_f32;
+};
 template <>
-struct FoldKernel<double> { static constexpr auto fn = fold_colfft_f64; };
+struct FoldKernel<double> {
+    static constexpr auto fn = fold_colfft_f64;
+    static constexpr auto list = fold_list_f64;
+};
 
 // ---------------------------------------------------------------------------------------------
 // Kernel 2: real inverse DFT along x of the column-transformed rows, crop and scale.
@@ -171,7 +239,7 @@ struct FoldKernel<double> { static constexpr auto fn = fold_colfft_f64; };
 template <typename T>
 __device__ void row_c2r_body(cplx<T>* lds, int rpg, const cplx<T>* __restrict__ in, int nrow,
                              int N, int x0, int nx, const cplx<T>* __restrict__ tw,
-                             T* __restrict__ out)
+                             T* __restrict__ out, int tws = 1)
 {
     const int n2 = N / 2;
     const int W = n2 + 1;
@@ -185,11 +253,11 @@ __device__ void row_c2r_body(cplx<T>* lds, int rpg, const cplx<T>* __restrict__ 
         cplx<T> zm = z[n2 - c];
         if (c == 0) { zc.y = T(0); zm.y = T(0); }    // DC and Nyquist are real
         cplx<T> E = cadd(zc, cconj(zm));
-        cplx<T> O = cmul(csub(zc, cconj(zm)), tw[c]);
+        cplx<T> O = cmul(csub(zc, cconj(zm)), tw[c * tws]);
         bufa[rl * n2 + c] = {E.x - O.y, E.y + O.x};  // E + i O
     }
     __syncthreads();
-    cplx<T>* res = fft_lds<T>(bufa, bufb, n2, nrow, tw, 2);
+    cplx<T>* res = fft_lds<T>(bufa, bufb, n2, nrow, tw, 2 * tws);
     const T norm = T(1) / (T(N) * T(N));
     for (int e = threadIdx.x; e < nrow * nx; e += blockDim.x) {
         const int rl = e / nx;
@@ -217,6 +285,34 @@ row_c2r(const gsmi_stamp* __restrict__ stamps, const cplx<T>* __restrict__ scrat
     const int W = N / 2 + 1;
     row_c2r_body<T>(lds, rpg, scratch + st->scratch_offset + size_t(r0) * W, nrow, N, st->x0,
                     st->nx, tw, out + st->out_offset + size_t(r0) * st->nx);
+}
+
+// Work-list form of row_c2r: items (stamp, first row, rows, rows per group).
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+rows_list(const gsmi_stamp* __restrict__ stamps, int nstamps,
+          const gsmi_work* __restrict__ items, int nitems, int lds_bytes,
+          const cplx<T>* __restrict__ scratch, TwiddleSet<T> tws, T* __restrict__ out)
+{
+    extern __shared__ unsigned char smem_raw[];
+    cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
+    for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
+        const gsmi_work it = items[w];
+        if (it.stamp < 0 || it.stamp >= nstamps) continue;
+        const gsmi_stamp* st = stamps + it.stamp;
+        const int N = st->N;
+        if (it.count <= 0 || it.first < 0 || it.first + it.count > st->ny ||
+            it.per_group < it.count ||
+            size_t(2) * it.per_group * (N / 2) * sizeof(cplx<T>) > size_t(lds_bytes))
+            continue;
+        int stride;
+        const cplx<T>* tw = tws.pick(N, &stride);
+        const int W = N / 2 + 1;
+        row_c2r_body<T>(lds, it.per_group, scratch + st->scratch_offset + size_t(it.first) * W,
+                        it.count, N, st->x0, st->nx, tw,
+                        out + st->out_offset + size_t(it.first) * st->nx, stride);
+        __syncthreads();
+    }
 }
 
 // full N x N real image from N column-transformed rows (irfft entry point)
@@ -441,6 +537,59 @@ int draw_fft_impl(const gsmi_stamp* stamps, int nstamps, int N, const gsmi_op* o
     if (rc) return rc;
     return rows_impl<T>(stamps, nstamps, N, max_ny, scratch, out, stream);
 }
+template <typename T>
+int sched_check(const gsmi_stamp* stamps, int nstamps, const gsmi_schedule* sc)
+{
+    if (!stamps || !sc || nstamps < 0 || sc->nfold < 0 || sc->nrows < 0 ||
+        (sc->nfold > 0 && (!sc->fold || sc->fold_grid <= 0)) ||
+        (sc->nrows > 0 && (!sc->rows || sc->rows_grid <= 0)) || sc->fold_lds < 0 ||
+        sc->rows_lds < 0 || sc->fold_lds > 65536 || sc->rows_lds > 65536) {
+        set_error("bad schedule");
+        return GSMI_E_ARG;
+    }
+    return GSMI_OK;
+}
+
+template <typename T>
+int fold_list_impl(const gsmi_stamp* stamps, int nstamps, const gsmi_schedule* sc,
+                   const gsmi_op* ops, const gsmi_leaf* leaves, Tables<T> tb, void* scratch,
+                   void* stream)
+{
+    int rc = sched_check<T>(stamps, nstamps, sc);
+    if (rc) return rc;
+    if (nstamps == 0) return GSMI_OK;
+    TwiddleSet<T> tws = twiddle_set<T>();
+    if (!tws.p2 || !tws.p3) {
+        set_error("twiddle table allocation failed");
+        return GSMI_E_HIP;
+    }
+    rc = launch_axis_factors<T>(stamps, nstamps, ops, leaves, scratch, stream);
+    if (rc || sc->nfold == 0) return rc;
+    hipLaunchKernelGGL(FoldKernel<T>::list, dim3(min(sc->fold_grid, sc->nfold)), dim3(kThreads),
+                       sc->fold_lds, reinterpret_cast<hipStream_t>(stream), stamps, nstamps,
+                       sc->fold, sc->nfold, sc->fold_lds, ops, leaves, tb, tws,
+                       reinterpret_cast<cplx<T>*>(scratch));
+    return hip_check(hipGetLastError(), "fold list launch");
+}
+
+template <typename T>
+int rows_list_impl(const gsmi_stamp* stamps, int nstamps, const gsmi_schedule* sc,
+                   const void* scratch, T* out, void* stream)
+{
+    int rc = sched_check<T>(stamps, nstamps, sc);
+    if (rc) return rc;
+    if (nstamps == 0 || sc->nrows == 0) return GSMI_OK;
+    TwiddleSet<T> tws = twiddle_set<T>();
+    if (!tws.p2 || !tws.p3) {
+        set_error("twiddle table allocation failed");
+        return GSMI_E_HIP;
+    }
+    hipLaunchKernelGGL(rows_list<T>, dim3(min(sc->rows_grid, sc->nrows)), dim3(kThreads),
+                       sc->rows_lds, reinterpret_cast<hipStream_t>(stream), stamps, nstamps,
+                       sc->rows, sc->nrows, sc->rows_lds,
+                       reinterpret_cast<const cplx<T>*>(scratch), tws, out);
+    return hip_check(hipGetLastError(), "rows list launch");
+}
 }  // namespace
 
 void set_error(const std::string& msg) { g_err = msg; }
@@ -569,6 +718,58 @@ int gsmi_row_c2r_f32(const gsmi_stamp* stamps, int nstamps, int N, int max_ny, c
 {
     if (!stamps || !scratch || !out || max_ny <= 0) { set_error("bad argument"); return GSMI_E_ARG; }
     return rows_impl<float>(stamps, nstamps, N, max_ny, scratch, out, stream);
+}
+
+int gsmi_fold_list_f64(const gsmi_stamp* stamps, int nstamps, const gsmi_schedule* sched,
+                       const gsmi_op* ops, const gsmi_leaf* leaves, const double* tables,
+                       const int32_t* cliprows, void* scratch, void* stream)
+{
+    if (!ops || !leaves || !scratch) { set_error("null buffer"); return GSMI_E_ARG; }
+    Tables<double> tb{tables, cliprows};
+    return fold_list_impl<double>(stamps, nstamps, sched, ops, leaves, tb, scratch, stream);
+}
+
+int gsmi_fold_list_f32(const gsmi_stamp* stamps, int nstamps, const gsmi_schedule* sched,
+                       const gsmi_op* ops, const gsmi_leaf* leaves, const double* tables,
+                       const float* tables32, void* scratch, void* stream)
+{
+    if (!ops || !leaves || !scratch) { set_error("null buffer"); return GSMI_E_ARG; }
+    Tables<float> tb{tables, tables32};
+    return fold_list_impl<float>(stamps, nstamps, sched, ops, leaves, tb, scratch, stream);
+}
+
+int gsmi_rows_list_f64(const gsmi_stamp* stamps, int nstamps, const gsmi_schedule* sched,
+                       const void* scratch, double* out, void* stream)
+{
+    if (!scratch || !out) { set_error("null buffer"); return GSMI_E_ARG; }
+    return rows_list_impl<double>(stamps, nstamps, sched, scratch, out, stream);
+}
+
+int gsmi_rows_list_f32(const gsmi_stamp* stamps, int nstamps, const gsmi_schedule* sched,
+                       const void* scratch, float* out, void* stream)
+{
+    if (!scratch || !out) { set_error("null buffer"); return GSMI_E_ARG; }
+    return rows_list_impl<float>(stamps, nstamps, sched, scratch, out, stream);
+}
+
+int gsmi_draw_list_f64(const gsmi_stamp* stamps, int nstamps, const gsmi_schedule* sched,
+                       const gsmi_op* ops, const gsmi_leaf* leaves, const double* tables,
+                       const int32_t* cliprows, void* scratch, double* out, void* stream)
+{
+    int rc = gsmi_fold_list_f64(stamps, nstamps, sched, ops, leaves, tables, cliprows, scratch,
+                                stream);
+    if (rc) return rc;
+    return gsmi_rows_list_f64(stamps, nstamps, sched, scratch, out, stream);
+}
+
+int gsmi_draw_list_f32(const gsmi_stamp* stamps, int nstamps, const gsmi_schedule* sched,
+                       const gsmi_op* ops, const gsmi_leaf* leaves, const double* tables,
+                       const float* tables32, void* scratch, float* out, void* stream)
+{
+    int rc = gsmi_fold_list_f32(stamps, nstamps, sched, ops, leaves, tables, tables32, scratch,
+                                stream);
+    if (rc) return rc;
+    return gsmi_rows_list_f32(stamps, nstamps, sched, scratch, out, stream);
 }
 
 }

@@ -8,6 +8,7 @@
 
 #include "../../include/gsmi.h"
 #include "device_math.h"
+#include "device_program.h"
 
 namespace gsmi {
 
@@ -23,6 +24,13 @@ bool fft_size_ok(int N, size_t elem);
 // Device table exp(+2 pi i m / N), m = 0..N-1, cached per device and N (nullptr on failure).
 template <typename T>
 const cplx<T>* twiddles(int N);
+
+// The two large twiddle tables of the work-list kernels ({nullptr, nullptr} on failure).
+template <typename T>
+inline TwiddleSet<T> twiddle_set()
+{
+    return TwiddleSet<T>{twiddles<T>(kTwPow2), twiddles<T>(kTwPow3)};
+}
 
 // The separable axis-factor prepass of every stamp (X(kx), Y(ky) at each stamp's axis_offset).
 template <typename T>

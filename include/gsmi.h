@@ -95,6 +95,28 @@ typedef struct gsmi_stamp {
     double dk;            /* k spacing in pixel units: 2 pi / N                               */
 } gsmi_stamp;
 
+/* A work item of the work-list kernels: a group of columns (fold) or rows (row transform) of
+ * one stamp.  per_group sizes the workgroup's LDS tile (columns or rows per workgroup); count <=
+ * per_group is how many this item holds. */
+typedef struct gsmi_work {
+    int32_t stamp;        /* index into the stamps array                                      */
+    int32_t first;        /* first column (c = 0..N/2) or first output row                    */
+    int32_t count;        /* columns or rows in this item                                      */
+    int32_t per_group;    /* tile width the item was cut with                                  */
+} gsmi_work;
+
+/* A whole batch -- every FFT size -- as two work lists, run by one launch each.  Lists are
+ * device arrays; a grid smaller than the list makes the kernel persistent (each workgroup takes
+ * items b, b + grid, ...).  lds bytes must cover 2 per_group N (fold) and 2 per_group N/2 (rows)
+ * complex values of every item; items that do not fit are skipped. */
+typedef struct gsmi_schedule {
+    const gsmi_work* fold;
+    const gsmi_work* rows;
+    int32_t nfold, nrows;
+    int32_t fold_grid, rows_grid;
+    int32_t fold_lds, rows_lds;
+} gsmi_schedule;
+
 const char* gsmi_last_error(void);
 int gsmi_version(void);
 
@@ -167,6 +189,27 @@ int gsmi_row_c2r_f64(const gsmi_stamp* stamps, int nstamps, int N, int max_ny,
                      const void* scratch, double* out, void* stream);
 int gsmi_row_c2r_f32(const gsmi_stamp* stamps, int nstamps, int N, int max_ny,
                      const void* scratch, float* out, void* stream);
+
+/* The same drawing as gsmi_draw_fft_* for a batch of any mix of FFT sizes, scheduled by work
+ * lists (see gsmi_schedule): the axis-factor prepass over all nstamps, one fold launch and one
+ * row launch.  Scratch and output offsets are those of the stamp records.  gsmi_fold_list_* /
+ * gsmi_rows_list_* are the two halves (the prepass runs with the fold). */
+int gsmi_draw_list_f64(const gsmi_stamp* stamps, int nstamps, const gsmi_schedule* sched,
+                       const gsmi_op* ops, const gsmi_leaf* leaves, const double* tables,
+                       const int32_t* cliprows, void* scratch, double* out, void* stream);
+int gsmi_draw_list_f32(const gsmi_stamp* stamps, int nstamps, const gsmi_schedule* sched,
+                       const gsmi_op* ops, const gsmi_leaf* leaves, const double* tables,
+                       const float* tables32, void* scratch, float* out, void* stream);
+int gsmi_fold_list_f64(const gsmi_stamp* stamps, int nstamps, const gsmi_schedule* sched,
+                       const gsmi_op* ops, const gsmi_leaf* leaves, const double* tables,
+                       const int32_t* cliprows, void* scratch, void* stream);
+int gsmi_fold_list_f32(const gsmi_stamp* stamps, int nstamps, const gsmi_schedule* sched,
+                       const gsmi_op* ops, const gsmi_leaf* leaves, const double* tables,
+                       const float* tables32, void* scratch, void* stream);
+int gsmi_rows_list_f64(const gsmi_stamp* stamps, int nstamps, const gsmi_schedule* sched,
+                       const void* scratch, double* out, void* stream);
+int gsmi_rows_list_f32(const gsmi_stamp* stamps, int nstamps, const gsmi_schedule* sched,
+                       const void* scratch, float* out, void* stream);
 
 /* Evaluate a stamp program on a k-image with bounds (xmin..xmin+nx-1, ymin..ymin+ny-1) and
  * spacing dk, writing complex128 values row-major (SBProfile::drawK src/SBProfile.cpp:383-425

@@ -39,3 +39,87 @@ def test_batch_exp_moffat_offsets():
                                                                offset=tuple(offs[i]))
         b = plan.stamp(i).cpu().numpy()
         assert np.max(np.abs(b - ref)) / np.max(np.abs(ref)) < 1e-10, i
+
+
+def _peak_rel_all(plan_a, plan_b, n):
+    worst = 0.
+    for i in range(n):
+        a = plan_a.stamp(i).double()
+        b = plan_b.stamp(i).double()
+        worst = max(worst, float((a - b).abs().max() / b.abs().max()))
+    return worst
+
+
+@pytest.mark.parametrize("precision,tol", [("f32", 2e-6), ("f64", 1e-12)])
+def test_work_list_schedule_matches_per_size_launches(precision, tol):
+    """One fold and one row launch over a work list of every FFT size (default tiles, tuned
+    tiles, a persistent grid) against the per-size launches of the same batch."""
+    profs, nx, ny, offs, _ = workloads.build('bulge_disk', 40, seed=5)
+    ref = BatchPlan(profs, nx, ny, 0.2, offs, precision=precision)
+    ref.run()
+    assert len(ref.fft_sizes()) >= 2
+    plan = BatchPlan(profs, nx, ny, 0.2, offs, precision=precision)
+    plan.use_schedule()
+    plan.run()
+    assert _peak_rel_all(plan, ref, len(profs)) < tol
+    plan.use_schedule(cols={N: max(1, BatchPlan.default_cols(N) // 2) for N in ref.fft_sizes()},
+                      rows={N: BatchPlan.default_rows(N) * 2 for N in ref.fft_sizes()},
+                      fold_grid=37, rows_grid=11)
+    plan.out.zero_()
+    plan.run()
+    assert _peak_rel_all(plan, ref, len(profs)) < tol
+    tuned = plan.autotune(reps=2)
+    print("tuned:", {k: tuned[k] for k in ('cols', 'rows', 'grid', 'ms')})
+    plan.out.zero_()
+    plan.run()
+    assert _peak_rel_all(plan, ref, len(profs)) < tol
+
+
+def test_graph_replay_matches_eager():
+    """The batch captured in a HIP graph: every replay (with eager work and synchronisation
+    between replays) equals the eager result."""
+    import torch
+    profs, nx, ny, offs, _ = workloads.build('bulge_disk', 24, seed=9)
+    eager = BatchPlan(profs, nx, ny, 0.2, offs, precision='f32')
+    eager.use_schedule()
+    eager.run()
+    ref = eager.out.clone()
+    plan = BatchPlan(profs, nx, ny, 0.2, offs, precision='f32')
+    plan.use_schedule()
+    plan.capture()
+    other = BatchPlan(profs[:5], nx[:5], ny[:5], 0.2, offs[:5], precision='f32')
+    for k in range(3):
+        plan.out.fill_(float('nan'))
+        torch.cuda.synchronize()
+        plan.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(plan.out, ref), k
+        other.run()                     # eager work between replays
+        torch.cuda.synchronize()
+
+
+def test_work_list_skips_malformed_items():
+    """Items naming a stamp outside the batch or an empty range are skipped by the kernels."""
+    import torch
+    from galsim_amd import _lib
+    profs, nx, ny, offs, _ = workloads.build('exp_moffat', 3, seed=1)
+    plan = BatchPlan(profs, nx, ny, 0.2, offs, precision='f32')
+    plan.use_schedule()
+    plan.run()
+    ref = plan.out.clone()
+    items, sc = plan._sched
+    bad = torch.tensor([[7, 0, 4, 4], [0, 0, 0, 4], [-1, 0, 1, 1]], dtype=torch.int32,
+                       device='cuda')
+    fold = torch.cat([bad, items[:sc.nfold]])
+    rows = torch.cat([bad, items[sc.nfold:]])
+    allitems = torch.cat([fold, rows]).contiguous()
+    sc2 = _lib.Schedule()
+    sc2.fold = allitems.data_ptr()
+    sc2.rows = allitems.data_ptr() + len(fold) * 16
+    sc2.nfold, sc2.nrows = len(fold), len(rows)
+    sc2.fold_grid, sc2.rows_grid = len(fold), len(rows)
+    sc2.fold_lds, sc2.rows_lds = sc.fold_lds, sc.rows_lds
+    plan._sched = (allitems, sc2)
+    plan.out.zero_()
+    plan.run()
+    assert torch.equal(plan.out, ref)
