@@ -55,6 +55,39 @@ def _prepare_differentiable(ctx, op0, nops, naxis, leaf0):
     return 1
 
 
+def work_lists(recs, groups, cols, rows, elem, bins=None):
+    """Fold and row work items (gsmi_work rows: stamp, first, count, per_group) of a batch whose
+    stamp records ``recs`` are sorted into FFT-size ``groups`` (N, first, count, max_ny), for
+    tiles ``cols`` / ``rows`` (dict N -> columns / rows per workgroup; defaults 1024/N and
+    1024/(N/2)), largest FFT sizes first, and the LDS bytes each list needs (elem: bytes per
+    complex value).  Every column 0..N/2 and every output row of every stamp is in exactly one
+    item."""
+    fold, rw = [], []
+    groups = sorted(groups, key=lambda g: -g[0])
+    if bins is not None:
+        groups = [g for g in groups if g[0] in bins]
+    lds_f = lds_r = 0
+    for N, first, count, _ in groups:
+        cpg = int(cols.get(N, BatchPlan.default_cols(N)))
+        rpg = int(rows.get(N, BatchPlan.default_rows(N)))
+        if cpg < 1 or rpg < 1:
+            raise ValueError("tiles must hold at least one column and one row")
+        W = N // 2 + 1
+        lds_f = max(lds_f, 2 * cpg * N * elem)
+        lds_r = max(lds_r, 2 * rpg * (N // 2) * elem)
+        for s in range(first, first + count):
+            for c0 in range(0, W, cpg):
+                fold.append((s, c0, min(cpg, W - c0), cpg))
+            ny = int(recs['ny'][s])
+            for r0 in range(0, ny, rpg):
+                rw.append((s, r0, min(rpg, ny - r0), rpg))
+    if max(lds_f, lds_r) > 65536:
+        raise _lib.GalSimAMDError("work-list tiles need %d bytes of LDS (limit 65536)"
+                                  % max(lds_f, lds_r))
+    return (np.array(fold, dtype=np.int32).reshape(-1, 4),
+            np.array(rw, dtype=np.int32).reshape(-1, 4), lds_f, lds_r)
+
+
 class BatchPlan:
     """A lowered batch, resident on the device.  ``run()`` launches the kernels on torch's
     current stream (no allocation, no host transfer: it can be captured in a HIP graph)."""
@@ -155,30 +188,7 @@ class BatchPlan:
         return 8 if self.precision == 'f32' else 16
 
     def _work_lists(self, cols, rows, bins=None):
-        """Fold and row work items (gsmi_work) for the given tiles, largest FFT sizes first."""
-        recs = self.recs
-        fold, rw = [], []
-        groups = sorted(self.groups, key=lambda g: -g[0])
-        if bins is not None:
-            groups = [g for g in groups if g[0] in bins]
-        lds_f = lds_r = 0
-        for N, first, count, _ in groups:
-            cpg = int(cols.get(N, self.default_cols(N)))
-            rpg = int(rows.get(N, self.default_rows(N)))
-            W = N // 2 + 1
-            lds_f = max(lds_f, 2 * cpg * N * self._elem())
-            lds_r = max(lds_r, 2 * rpg * (N // 2) * self._elem())
-            for s in range(first, first + count):
-                for c0 in range(0, W, cpg):
-                    fold.append((s, c0, min(cpg, W - c0), cpg))
-                ny = int(recs['ny'][s])
-                for r0 in range(0, ny, rpg):
-                    rw.append((s, r0, min(rpg, ny - r0), rpg))
-        if max(lds_f, lds_r) > 65536:
-            raise _lib.GalSimAMDError("work-list tiles need %d bytes of LDS (limit 65536)"
-                                      % max(lds_f, lds_r))
-        return (np.array(fold, dtype=np.int32).reshape(-1, 4),
-                np.array(rw, dtype=np.int32).reshape(-1, 4), lds_f, lds_r)
+        return work_lists(self.recs, self.groups, cols, rows, self._elem(), bins)
 
     def make_schedule(self, cols=None, rows=None, fold_grid=None, rows_grid=None, bins=None):
         """(device item tensor, gsmi_schedule) for tiles ``cols``/``rows`` (dict N -> columns or

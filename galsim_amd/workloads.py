@@ -63,6 +63,21 @@ def bulge_disk_profile(m, p, i):
     return m.Convolve([gal, m.Kolmogorov(fwhm=float(p['psf_fwhm'][i]))])
 
 
+# -- large Sersic stamps (x) Kolmogorov: FFT grids of 512..4096 --------------------------------
+def large_sersic_params(n, seed=0):
+    """Sersic n ~ U(3, 6) quantised to 0.05, hlr ~ U(2, 6)", Kolmogorov fwhm 0.8", flux 1e4."""
+    rng = np.random.default_rng(seed)
+    sn = np.round(rng.uniform(3.0, 6.0, n) / 0.05) * 0.05
+    return dict(n=np.round(sn, 2), hlr=rng.uniform(2., 6., n), psf_fwhm=np.full(n, 0.8),
+                flux=np.full(n, 1e4))
+
+
+def large_sersic_profile(m, p, i):
+    gal = m.Sersic(float(p['n'][i]), half_light_radius=float(p['hlr'][i]),
+                   flux=float(p['flux'][i]))
+    return m.Convolve([gal, m.Kolmogorov(fwhm=float(p['psf_fwhm'][i]))])
+
+
 def stamp_size(prof, scale, lo=32, hi=256):
     """goodFFTSize(getGoodImageSize(scale)) clipped to [lo, hi]."""
     from .image import Image
@@ -70,7 +85,8 @@ def stamp_size(prof, scale, lo=32, hi=256):
 
 
 def build(kind, n, seed=0, m=None):
-    """(profiles, nx, ny, offsets, params) for workload ``kind`` ('exp_moffat' or 'bulge_disk')."""
+    """(profiles, nx, ny, offsets, params) for workload ``kind`` ('exp_moffat', 'bulge_disk' or
+    'large_sersic')."""
     if m is None:
         import galsim_amd as m
     if kind == 'exp_moffat':
@@ -83,5 +99,13 @@ def build(kind, n, seed=0, m=None):
         profs = [bulge_disk_profile(m, p, i) for i in range(n)]
         import galsim_amd as g
         sizes = np.array([stamp_size(bulge_disk_profile(g, p, i), 0.2) for i in range(n)])
+        return profs, sizes, sizes.copy(), np.zeros((n, 2)), p
+    if kind == 'large_sersic':
+        # drawImage's own size for each profile (getGoodImageSize at the pixel scale)
+        p = large_sersic_params(n, seed)
+        profs = [large_sersic_profile(m, p, i) for i in range(n)]
+        import galsim_amd as g
+        sizes = np.array([large_sersic_profile(g, p, i).getGoodImageSize(0.2)
+                          for i in range(n)])
         return profs, sizes, sizes.copy(), np.zeros((n, 2)), p
     raise ValueError(kind)

@@ -1,0 +1,52 @@
+"""CPU checks of the work-list schedule: every column of every stamp's wrapped grid and every
+output row is covered by exactly one item, tiles respect the LDS limit, and the largest FFT
+sizes come first."""
+import numpy as np
+import pytest
+
+from galsim_amd import _lib
+from galsim_amd.batch import BatchPlan, work_lists
+
+
+def _recs(spec):
+    recs = np.zeros(len(spec), dtype=_lib.STAMP_DTYPE)
+    for k, (N, ny) in enumerate(spec):
+        recs[k]['N'] = N
+        recs[k]['ny'] = ny
+    groups = []
+    for N in np.unique(recs['N']):
+        idx = np.nonzero(recs['N'] == N)[0]
+        groups.append((int(N), int(idx[0]), len(idx), int(recs['ny'][idx].max())))
+    return recs, groups
+
+
+@pytest.mark.parametrize("cols,rows", [({}, {}), ({128: 3, 768: 1}, {128: 5, 384: 7})])
+def test_work_lists_cover_every_column_and_row_once(cols, rows):
+    recs, groups = _recs([(128, 48), (128, 128), (192, 77), (384, 200), (768, 9)])
+    fold, rw, lds_f, lds_r = work_lists(recs, groups, cols, rows, 8)
+    for s in range(len(recs)):
+        N, ny = int(recs['N'][s]), int(recs['ny'][s])
+        f = fold[fold[:, 0] == s]
+        seen = np.zeros(N // 2 + 1, int)
+        for _, c0, cnt, per in f:
+            assert 1 <= cnt <= per
+            seen[c0:c0 + cnt] += 1
+        assert np.all(seen == 1)
+        r = rw[rw[:, 0] == s]
+        seen = np.zeros(ny, int)
+        for _, r0, cnt, per in r:
+            assert 1 <= cnt <= per
+            seen[r0:r0 + cnt] += 1
+        assert np.all(seen == 1)
+    Ns = recs['N'][fold[:, 0]]
+    assert np.all(np.diff(Ns) <= 0)                       # largest FFT sizes first
+    assert lds_f == max(2 * per * int(recs['N'][s]) * 8 for s, _, _, per in fold)
+    assert lds_r == max(2 * per * (int(recs['N'][s]) // 2) * 8 for s, _, _, per in rw)
+
+
+def test_work_lists_reject_tiles_beyond_lds():
+    recs, groups = _recs([(1024, 10)])
+    with pytest.raises(_lib.GalSimAMDError):
+        work_lists(recs, groups, {1024: 8}, {}, 16)
+    assert BatchPlan.default_cols(128) == 8 and BatchPlan.default_cols(2048) == 1
+    assert BatchPlan.default_rows(128) == 16 and BatchPlan.default_rows(4096) == 1
