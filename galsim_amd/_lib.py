@@ -88,8 +88,8 @@ _SIGS = {
     'gsmi_rows_list_f64': (_I, [_P, _I, _P, _P, _P, _P]),
     'gsmi_rows_list_f32': (_I, [_P, _I, _P, _P, _P, _P]),
     'gsmi_adjoint_partials': (_I, [_I, _I, _P]),
-    'gsmi_adjoint_f64': (_I, [_P, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
-    'gsmi_adjoint_f32': (_I, [_P, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    'gsmi_adjoint_f64': (_I, [_P, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    'gsmi_adjoint_f32': (_I, [_P, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
 }
 
 

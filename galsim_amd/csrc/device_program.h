@@ -268,10 +268,11 @@ struct StackDepth<float> { static constexpr int value = GSMI_STACK_F32; };
 // registers (code | arg << 8), leaf integer fields in uniform registers, leaf constants in LDS.
 // Programs with more leaves or ops than this run on the record-reading interpreter.
 constexpr int kProgLeaves = 4;
-constexpr int kProgOps = 8;
+constexpr int kProgOps = 12;
 
 struct ProgramD {
     int nl, nops;
+    bool has_phase;       // an in-program PHASE: the program value is complex
     int opc[kProgOps];
     LeafI li[kProgLeaves];
 };
@@ -295,7 +296,10 @@ __device__ inline bool decode_program(const gsmi_op* __restrict__ ops, int nops,
     const int nl = program_leaves(ops, nops);
     d.nl = nl;
     d.nops = nops;
+    d.has_phase = false;
     if (nl > kProgLeaves || nops > kProgOps) return false;
+    for (int o = 0; o < nops; ++o)
+        if (ops[o].code == GSMI_OP_PHASE) d.has_phase = true;
 #pragma unroll
     for (int o = 0; o < kProgOps; ++o)
         d.opc[o] = (o < nops) ? (ops[o].code | (ops[o].arg << 8)) : 0;
@@ -354,6 +358,42 @@ __device__ inline cplx<T> run_ops(const ProgramD& d, const gsmi_op* __restrict__
         }
     }
     return st.s[0];
+}
+
+// The same for a program without PHASE ops, whose value is real (radial leaves are real in
+// k-space): half the registers of the complex form.
+template <typename T>
+__device__ inline T run_ops_real(const ProgramD& d, const gsmi_op* __restrict__ ops,
+                                 const T (&v)[kProgLeaves], int ovl, T ovv)
+{
+    T st[StackDepth<T>::value];
+#pragma unroll
+    for (int k = 0; k < StackDepth<T>::value; ++k) st[k] = T(0);
+#pragma unroll
+    for (int o = 0; o < kProgOps; ++o) {
+        if (o < d.nops) {
+            const int code = d.opc[o] & 0xff;
+            const int a = d.opc[o] >> 8;
+            if (code == GSMI_OP_LEAF) {
+                T x = v[0];
+#pragma unroll
+                for (int l = 1; l < kProgLeaves; ++l)
+                    if (a == l) x = v[l];
+                if (a == ovl) x = ovv;
+#pragma unroll
+                for (int k = StackDepth<T>::value - 1; k > 0; --k) st[k] = st[k - 1];
+                st[0] = x;
+            } else if (code == GSMI_OP_MUL || code == GSMI_OP_ADD) {
+                const T b = st[0];
+#pragma unroll
+                for (int k = 0; k < StackDepth<T>::value - 1; ++k) st[k] = st[k + 1];
+                st[0] = (code == GSMI_OP_MUL) ? st[0] * b : st[0] + b;
+            } else if (code == GSMI_OP_SCALE) {
+                st[0] *= T(ops[o].a);
+            }
+        }
+    }
+    return st[0];
 }
 
 // Leaf values of a decoded program at sample (i, j).

@@ -172,7 +172,8 @@ __device__ inline double wave_sum(double x)
 // ---------------------------------------------------------------------------------------------
 // Kernel A: row DFTs of the residual.  grid (ceil(ny/rpg), nstamps); LDS 2 rpg N/2 complex.
 // H(c) = sum_q R(q) e^{+2 pi i c q / N}, c = 0..N/2, from Z = DFT_{N/2}(R(2m) + i R(2m+1)):
-// H(c) = E(c) + e^{2 pi i c/N} O(c), E = (Z(c) + conj Z(N/2-c)) / 2, O = (Z(c) - conj Z(N/2-c)) / 2i.
+// H(c) = E(c) + e^{2 pi i c/N} O(c), E = (Z(c) + conj Z(N/2-c)) / 2,
+// O = (Z(c) - conj Z(N/2-c)) / 2i.
 // ---------------------------------------------------------------------------------------------
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
@@ -223,7 +224,7 @@ row_r2c(const gsmi_stamp* __restrict__ stamps, const T* __restrict__ resid,
 // k-image sample of the group's columns.  grid (ceil((N/2+1)/cpg), nstamps);
 // LDS 2 cpg N complex.  ML = leaf slots with gradient accumulators (2 or 4).
 // ---------------------------------------------------------------------------------------------
-template <typename T, int ML>
+template <typename T, int ML, bool CPLX>
 __global__ void __launch_bounds__(kThreads)
 grad_colfft(const gsmi_stamp* __restrict__ stamps, const gsmi_op* __restrict__ ops,
             const gsmi_leaf* __restrict__ leaves, Tables<T> tb, const cplx<T>* __restrict__ tw,
@@ -268,7 +269,8 @@ grad_colfft(const gsmi_stamp* __restrict__ stamps, const gsmi_op* __restrict__ o
     // the program decoded into the LDS buffer the transform result does not occupy
     ProgramD d;
     LeafC<T>* lc = reinterpret_cast<LeafC<T>*>(res == bufa ? bufb : bufa);
-    const bool ok = decode_program<T>(sops, nops, sleaves, d, lc) && d.nl <= ML;
+    const bool ok = decode_program<T>(sops, nops, sleaves, d, lc) && d.nl <= ML &&
+                    (CPLX || !d.has_phase);
     const int nl = d.nl;
     __syncthreads();
 
@@ -307,24 +309,35 @@ grad_colfft(const gsmi_stamp* __restrict__ stamps, const gsmi_op* __restrict__ o
                         dv[l] = lc[l].flux * q.dphi;
                     }
                 }
-                const cplx<T> P = run_ops<T>(d, sops, v, -1, T(0), kx, ky);
-                const cplx<T> FZ = cmul(P, Z);
+                cplx<T> FZ;
+                if constexpr (CPLX) {
+                    FZ = cmul(run_ops<T>(d, sops, v, -1, T(0), kx, ky), Z);
+                } else {
+                    const T P = run_ops_real<T>(d, sops, v, -1, T(0));
+                    FZ = {P * Z.x, P * Z.y};
+                }
                 acc[6 * ML] += kx * FZ.y;
                 acc[6 * ML + 1] += ky * FZ.y;
                 acc[6 * ML + 2] += FZ.x;
 #pragma unroll
                 for (int l = 0; l < ML; ++l) {
                     if (l < nl) {
-                        const cplx<T> p1 = run_ops<T>(d, sops, v, l, T(1), kx, ky);
-                        const cplx<T> p0 = run_ops<T>(d, sops, v, l, T(0), kx, ky);
-                        const cplx<T> t = cmul(csub(p1, p0), Z);
+                        T tx;          // Re[(dP/dv_l) Z]
+                        if constexpr (CPLX) {
+                            const cplx<T> p1 = run_ops<T>(d, sops, v, l, T(1), kx, ky);
+                            const cplx<T> p0 = run_ops<T>(d, sops, v, l, T(0), kx, ky);
+                            tx = cmul(csub(p1, p0), Z).x;
+                        } else {
+                            tx = (run_ops_real<T>(d, sops, v, l, T(1)) -
+                                  run_ops_real<T>(d, sops, v, l, T(0))) * Z.x;
+                        }
                         const LeafC<T>& L = lc[l];
                         T kxp, kyp;
                         leaf_kprime(L, d.li[l], kx, ky, &kxp, &kyp);
                         const T sc = L.s;
-                        const T g = t.x * dv[l];
+                        const T g = tx * dv[l];
                         const T g2 = T(2) * sc * sc * g;
-                        acc[6 * l] += t.x * phi[l];
+                        acc[6 * l] += tx * phi[l];
                         acc[6 * l + 1] += T(2) * sc * (kxp * kxp + kyp * kyp) * g;
                         acc[6 * l + 2] += g2 * kxp * kx;
                         acc[6 * l + 3] += g2 * kyp * kx;
@@ -370,15 +383,31 @@ grad_reduce(const double* __restrict__ partials, int ngroups, int nstamps,
     const int s = idx / GSMI_GRAD_FIELDS;
     const int f = idx - s * GSMI_GRAD_FIELDS;
     double x = 0.;
-    for (int g = 0; g < ngroups; ++g) x += partials[(size_t(s) * ngroups + g) * GSMI_GRAD_FIELDS + f];
+    for (int g = 0; g < ngroups; ++g)
+        x += partials[(size_t(s) * ngroups + g) * GSMI_GRAD_FIELDS + f];
     grad[idx] = x;
 }
 
 int adjoint_groups(int N) { return (N / 2 + 1 + cols_per_group(N) - 1) / cols_per_group(N); }
 
+template <typename T, bool CPLX>
+void launch_grad(dim3 gb, size_t ldsb, hipStream_t s, int max_leaves, const gsmi_stamp* stamps,
+                 const gsmi_op* ops, const gsmi_leaf* leaves, Tables<T> tb,
+                 const cplx<T>* tw, const void* scratch, double* partials)
+{
+    const cplx<T>* sc = reinterpret_cast<const cplx<T>*>(scratch);
+    if (max_leaves <= 2)
+        hipLaunchKernelGGL((grad_colfft<T, 2, CPLX>), gb, dim3(kThreads), ldsb, s, stamps, ops,
+                           leaves, tb, tw, sc, partials);
+    else
+        hipLaunchKernelGGL((grad_colfft<T, 4, CPLX>), gb, dim3(kThreads), ldsb, s, stamps, ops,
+                           leaves, tb, tw, sc, partials);
+}
+
 template <typename T>
 int adjoint_impl(const gsmi_stamp* stamps, int nstamps, int N, int max_ny, int max_leaves,
-                 const gsmi_op* ops, const gsmi_leaf* leaves, Tables<T> tb, const T* resid,
+                 int complex_program, const gsmi_op* ops, const gsmi_leaf* leaves, Tables<T> tb,
+                 const T* resid,
                  void* scratch, double* partials, double* grad, void* stream)
 {
     if (!fft_size_ok(N, sizeof(cplx<T>))) {
@@ -411,12 +440,12 @@ int adjoint_impl(const gsmi_stamp* stamps, int nstamps, int N, int max_ny, int m
     const int ng = adjoint_groups(N);
     dim3 gb(ng, nstamps);
     size_t ldsb = 2 * size_t(cpg) * N * sizeof(cplx<T>);
-    if (max_leaves <= 2)
-        hipLaunchKernelGGL((grad_colfft<T, 2>), gb, dim3(kThreads), ldsb, s, stamps, ops, leaves,
-                           tb, tw, reinterpret_cast<const cplx<T>*>(scratch), partials);
+    if (complex_program)
+        launch_grad<T, true>(gb, ldsb, s, max_leaves, stamps, ops, leaves, tb, tw, scratch,
+                             partials);
     else
-        hipLaunchKernelGGL((grad_colfft<T, 4>), gb, dim3(kThreads), ldsb, s, stamps, ops, leaves,
-                           tb, tw, reinterpret_cast<const cplx<T>*>(scratch), partials);
+        launch_grad<T, false>(gb, ldsb, s, max_leaves, stamps, ops, leaves, tb, tw, scratch,
+                              partials);
     rc = hip_check(hipGetLastError(), "grad_colfft launch");
     if (rc) return rc;
     const int total = nstamps * GSMI_GRAD_FIELDS;
@@ -443,30 +472,32 @@ int gsmi_adjoint_partials(int N, int nstamps, int64_t* ndoubles)
 }
 
 int gsmi_adjoint_f64(const gsmi_stamp* stamps, int nstamps, int N, int max_ny, int max_leaves,
-                     const gsmi_op* ops, const gsmi_leaf* leaves, const double* tables,
-                     const int32_t* cliprows, const double* resid, void* scratch,
-                     double* partials, double* grad, void* stream)
+                     int complex_program, const gsmi_op* ops, const gsmi_leaf* leaves,
+                     const double* tables, const int32_t* cliprows, const double* resid,
+                     void* scratch, double* partials, double* grad, void* stream)
 {
     if (!stamps || !ops || !leaves || !resid || !scratch || !partials || !grad || max_ny <= 0) {
         set_error("gsmi_adjoint_f64: null buffer or empty stamp");
         return GSMI_E_ARG;
     }
     Tables<double> tb{tables, cliprows};
-    return adjoint_impl<double>(stamps, nstamps, N, max_ny, max_leaves, ops, leaves, tb, resid,
+    return adjoint_impl<double>(stamps, nstamps, N, max_ny, max_leaves, complex_program, ops,
+                                leaves, tb, resid,
                                 scratch, partials, grad, stream);
 }
 
 int gsmi_adjoint_f32(const gsmi_stamp* stamps, int nstamps, int N, int max_ny, int max_leaves,
-                     const gsmi_op* ops, const gsmi_leaf* leaves, const double* tables,
-                     const float* tables32, const float* resid, void* scratch, double* partials,
-                     double* grad, void* stream)
+                     int complex_program, const gsmi_op* ops, const gsmi_leaf* leaves,
+                     const double* tables, const float* tables32, const float* resid,
+                     void* scratch, double* partials, double* grad, void* stream)
 {
     if (!stamps || !ops || !leaves || !resid || !scratch || !partials || !grad || max_ny <= 0) {
         set_error("gsmi_adjoint_f32: null buffer or empty stamp");
         return GSMI_E_ARG;
     }
     Tables<float> tb{tables, tables32};
-    return adjoint_impl<float>(stamps, nstamps, N, max_ny, max_leaves, ops, leaves, tb, resid,
+    return adjoint_impl<float>(stamps, nstamps, N, max_ny, max_leaves, complex_program, ops,
+                               leaves, tb, resid,
                                scratch, partials, grad, stream);
 }
 

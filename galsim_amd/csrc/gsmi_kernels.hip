@@ -709,14 +709,20 @@ int gsmi_fold_colfft_f32(const gsmi_stamp* stamps, int nstamps, int N, const gsm
 int gsmi_row_c2r_f64(const gsmi_stamp* stamps, int nstamps, int N, int max_ny, const void* scratch,
                      double* out, void* stream)
 {
-    if (!stamps || !scratch || !out || max_ny <= 0) { set_error("bad argument"); return GSMI_E_ARG; }
+    if (!stamps || !scratch || !out || max_ny <= 0) {
+        set_error("bad argument");
+        return GSMI_E_ARG;
+    }
     return rows_impl<double>(stamps, nstamps, N, max_ny, scratch, out, stream);
 }
 
 int gsmi_row_c2r_f32(const gsmi_stamp* stamps, int nstamps, int N, int max_ny, const void* scratch,
                      float* out, void* stream)
 {
-    if (!stamps || !scratch || !out || max_ny <= 0) { set_error("bad argument"); return GSMI_E_ARG; }
+    if (!stamps || !scratch || !out || max_ny <= 0) {
+        set_error("bad argument");
+        return GSMI_E_ARG;
+    }
     return rows_impl<float>(stamps, nstamps, N, max_ny, scratch, out, stream);
 }
 

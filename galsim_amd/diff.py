@@ -26,6 +26,7 @@ from .batch import BatchPlan
 
 NFIELD = _lib.GRAD_FIELDS
 MAXLEAF = _lib.GRAD_MAXLEAF
+MAXOPS = 12
 F_UX, F_UY, F_FS = 6 * MAXLEAF, 6 * MAXLEAF + 1, 6 * MAXLEAF + 2
 _LEAF_FIELDS = (('flux', 0), ('scale', 0), ('jac', 0), ('jac', 1), ('jac', 2), ('jac', 3))
 
@@ -50,6 +51,7 @@ class DiffPlan:
         f0 = np.zeros((n, NFIELD))
         self.leaf_types = np.zeros((n, MAXLEAF), dtype=np.int32)
         nl = np.zeros(n, dtype=np.int32)
+        cplx = np.zeros(n, dtype=bool)
         inv = np.empty(n, dtype=np.int64)
         inv[p.order] = np.arange(n)
         for s in range(n):                  # s: sorted position
@@ -59,10 +61,12 @@ class DiffPlan:
             prog = ops[b:b + cnt]
             args = prog['arg'][prog['code'] == _lib.OP_LEAF]
             nleaf = int(args.max()) + 1
-            if nleaf > MAXLEAF:
-                raise _lib.GalSimAMDError("differentiable rendering supports at most %d leaves "
-                                          "per profile (got %d)" % (MAXLEAF, nleaf))
+            if nleaf > MAXLEAF or cnt > MAXOPS:
+                raise _lib.GalSimAMDError(
+                    "differentiable rendering supports at most %d leaves and %d ops per profile "
+                    "(got %d and %d)" % (MAXLEAF, MAXOPS, nleaf, cnt))
             nl[i] = nleaf
+            cplx[i] = bool(np.any(prog['code'] == _lib.OP_PHASE))
             for l in range(nleaf):
                 li = int(r['leaf_begin']) + l
                 rec = leaves[li]
@@ -91,8 +95,9 @@ class DiffPlan:
             _lib.check(lib.gsmi_adjoint_partials(int(N), int(count), _lib.ptr(nd)),
                        "adjoint partials size")
             need = max(need, int(nd[0]))
-            ml = int(nl[p.order[first:first + count]].max())
-            self._bins.append((N, first, count, max_ny, ml))
+            members = p.order[first:first + count]
+            self._bins.append((N, first, count, max_ny, int(nl[members].max()),
+                               int(cplx[members].any())))
         self._partials = torch.empty(need, dtype=torch.float64, device='cuda')
         self._grad = torch.empty((n, NFIELD), dtype=torch.float64, device='cuda')
 
@@ -125,17 +130,17 @@ class DiffPlan:
         o = p.offsets
         s = torch.cuda.current_stream().cuda_stream
         g = self._grad
-        for N, first, count, max_ny, ml in self._bins:
+        for N, first, count, max_ny, ml, cx in self._bins:
             st = base + o['stamp'] + first * _lib.STAMP_DTYPE.itemsize
             gptr = g.data_ptr() + first * NFIELD * 8
             if self.precision == 'f32':
-                rc = lib.gsmi_adjoint_f32(st, count, N, max_ny, ml, base + o['ops'],
+                rc = lib.gsmi_adjoint_f32(st, count, N, max_ny, ml, cx, base + o['ops'],
                                           base + o['leaves'], base + o['tables'],
                                           base + o['tables32'], resid.data_ptr(),
                                           p.scratch.data_ptr(), self._partials.data_ptr(),
                                           gptr, s)
             else:
-                rc = lib.gsmi_adjoint_f64(st, count, N, max_ny, ml, base + o['ops'],
+                rc = lib.gsmi_adjoint_f64(st, count, N, max_ny, ml, cx, base + o['ops'],
                                           base + o['leaves'], base + o['tables'],
                                           base + o['clip'], resid.data_ptr(),
                                           p.scratch.data_ptr(), self._partials.data_ptr(),

@@ -244,7 +244,9 @@ int gsmi_wrap_hermx_f64(const double* kin, int Nk, int N, double* kout, void* st
  * Only radial leaves (Exponential, Gaussian, Moffat, Kolmogorov, Sersic) have field gradients;
  * a Box slot reads zero.  Exponential clip rows (FP64) are held fixed: clipped samples have
  * zero derivative, as they have zero value.  max_leaves bounds the number of leaf slots the
- * programs use (1..GSMI_GRAD_MAXLEAF); a stamp using more gets NaN gradients.  scratch is the
+ * programs use (1..GSMI_GRAD_MAXLEAF); complex_program must be nonzero if any program holds a
+ * PHASE op (a shift inside the tree rather than on the whole result).  A stamp whose program
+ * breaks either promise, or has more than 12 ops, gets NaN gradients.  scratch is the
  * forward's scratch layout (it is overwritten); partials holds gsmi_adjoint_partials() doubles.
  * ------------------------------------------------------------------------------------- */
 #define GSMI_GRAD_MAXLEAF 4
@@ -252,13 +254,13 @@ int gsmi_wrap_hermx_f64(const double* kin, int Nk, int N, double* kout, void* st
 
 int gsmi_adjoint_partials(int N, int nstamps, int64_t* ndoubles);
 int gsmi_adjoint_f64(const gsmi_stamp* stamps, int nstamps, int N, int max_ny, int max_leaves,
-                     const gsmi_op* ops, const gsmi_leaf* leaves, const double* tables,
-                     const int32_t* cliprows, const double* resid, void* scratch,
-                     double* partials, double* grad, void* stream);
+                     int complex_program, const gsmi_op* ops, const gsmi_leaf* leaves,
+                     const double* tables, const int32_t* cliprows, const double* resid,
+                     void* scratch, double* partials, double* grad, void* stream);
 int gsmi_adjoint_f32(const gsmi_stamp* stamps, int nstamps, int N, int max_ny, int max_leaves,
-                     const gsmi_op* ops, const gsmi_leaf* leaves, const double* tables,
-                     const float* tables32, const float* resid, void* scratch, double* partials,
-                     double* grad, void* stream);
+                     int complex_program, const gsmi_op* ops, const gsmi_leaf* leaves,
+                     const double* tables, const float* tables32, const float* resid,
+                     void* scratch, double* partials, double* grad, void* stream);
 
 #ifdef __cplusplus
 }

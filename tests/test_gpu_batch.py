@@ -151,3 +151,21 @@ def test_large_sersic_stamps_fp32_against_fp64_and_oracle():
     ref = workloads.large_sersic_profile(od, p, i).drawImage(int(nx[i]), int(ny[i]), scale=0.2)
     b = plan64.stamp(small.index(i)).cpu().numpy()
     assert np.max(np.abs(b - ref)) / np.max(np.abs(ref)) < 1e-10
+
+
+def test_plain_fp64_sweep_256_against_oracle():
+    """The plain FP64 workload: 256 seeded Exponential x Moffat stamps (hlr ~ U(0.8, 2.5)",
+    |g| ~ U(0, 0.5), sub-pixel offsets), 64x64 at 0.2"/px, batched on the GPU, every stamp
+    within 1e-10 of the peak of the oracle's drawImage."""
+    profs, nx, ny, offs, p = workloads.build('exp_moffat', 256, seed=0)
+    plan = BatchPlan(profs, nx, ny, 0.2, offs, precision='f64')
+    plan.use_schedule()
+    plan.run()
+    worst = 0.
+    for i in range(256):
+        ref = workloads.exp_moffat_profile(od, p, i).drawImage(64, 64, scale=0.2,
+                                                               offset=tuple(offs[i]))
+        b = plan.stamp(i).cpu().numpy()
+        worst = max(worst, np.max(np.abs(b - ref)) / np.max(np.abs(ref)))
+    print("256-stamp FP64 sweep: worst peak-relative error %.2e" % worst)
+    assert worst < 1e-10
