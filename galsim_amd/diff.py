@@ -197,7 +197,7 @@ class ExpMoffat:
     """Convolve(Exponential(half_light_radius=hlr, flux=flux).shear(g1=g1, g2=g2),
     Moffat(beta, fwhm=psf_fwhm)) drawn by drawImage(nx, ny, scale=scale, offset=(dx, dy)) for a
     batch of parameter rows theta = (flux, hlr, g1, g2, dx, dy, psf_fwhm), differentiable in all
-    seven.  The grids are fixed by ``theta0``."""
+    seven.  ``beta``, ``nx`` and ``ny`` may be per stamp.  The grids are fixed by ``theta0``."""
 
     PARAMS = ('flux', 'hlr', 'g1', 'g2', 'dx', 'dy', 'psf_fwhm')
 
@@ -209,11 +209,11 @@ class ExpMoffat:
         theta0 = np.asarray(theta0, dtype=float).reshape(-1, 7)
         n = len(theta0)
         self.scale = float(scale)
-        self.beta = float(beta)
+        self.beta = np.broadcast_to(np.asarray(beta, dtype=float), (n,)).copy()
         profs = []
-        for t in theta0:
+        for t, b in zip(theta0, self.beta):
             gal = Exponential(half_light_radius=t[1], flux=t[0]).shear(g1=t[2], g2=t[3])
-            profs.append(Convolve([gal, Moffat(self.beta, fwhm=t[6])]))
+            profs.append(Convolve([gal, Moffat(float(b), fwhm=t[6])]))
         self.dplan = DiffPlan(profs, nx, ny, scale, theta0[:, 4:6], precision)
         lt = self.dplan.leaf_types
         if not (np.all(lt[:, 0] == _lib.LEAF_EXPONENTIAL) and np.all(lt[:, 1] == _lib.LEAF_MOFFAT)):
@@ -223,7 +223,8 @@ class ExpMoffat:
         self._base = f0
         self._cx = f0[:, F_UX] - t0[:, 4]
         self._cy = f0[:, F_UY] - t0[:, 5]
-        self._fwhm_to_rd = 1. / (2. * math.sqrt(2. ** (1. / self.beta) - 1.))
+        self._fwhm_to_rd = torch.from_numpy(
+            1. / (2. * np.sqrt(2. ** (1. / self.beta) - 1.))).cuda()
         self.theta0 = t0
 
     def fields(self, theta):
