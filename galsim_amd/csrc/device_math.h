@@ -140,6 +140,83 @@ __device__ inline double bessel_k(double nu, double x, double* km1 = nullptr)
     return kmu;
 }
 
+// K_nu(x) in FP32 for nu = nl + mu (nl >= 0 integer, |mu| <= 1/2), x > 0, by the same method as
+// bessel_k (Temme's series for x < 2, Steed's CF2 above, upward recurrence in the order).  The
+// mu-only constants of Temme's series come in precomputed (in FP64, once per profile):
+// c0 = fact gam1, c1 = fact gam2, c2 = Gamma(1 + mu) / 2, c3 = Gamma(1 - mu) / 2 with
+// fact = pi mu / sin(pi mu).  With km1 != nullptr also returns K_{nu-1}(x).
+__device__ inline float bessel_k_f32(int nl, float mu, float c0, float c1, float c2, float c3,
+                                     float x, float* km1)
+{
+    const float EPS = 1e-7f;
+    const int MAXIT = 1000;
+    const float mu2 = mu * mu;
+    const float xi = 1.f / x, xi2 = 2.f * xi;
+    float rkmu, rk1;
+    if (x < 2.f) {
+        const float x2 = 0.5f * x;
+        float d = -__logf(x2);
+        float e = mu * d;
+        const float fact2 = (fabsf(e) < 1e-4f) ? 1.f + e * e * (1.f / 6.f) : sinhf(e) / e;
+        float ff = c0 * coshf(e) + c1 * fact2 * d;
+        float sum = ff;
+        e = __expf(e);
+        float p = c2 * e;
+        float q = c3 / e;
+        float c = 1.f;
+        d = x2 * x2;
+        float sum1 = p;
+        for (int i = 1; i <= MAXIT; ++i) {
+            const float fi = float(i);
+            ff = (fi * ff + p + q) / (fi * fi - mu2);
+            c *= d / fi;
+            p /= (fi - mu);
+            q /= (fi + mu);
+            const float del = c * ff;
+            sum += del;
+            sum1 += c * (p - fi * ff);
+            if (fabsf(del) < fabsf(sum) * EPS) break;
+        }
+        rkmu = sum;
+        rk1 = sum1 * xi2;
+    } else {
+        float b = 2.f * (1.f + x);
+        float d = 1.f / b;
+        float h = d, delh = d;
+        float q1 = 0.f, q2 = 1.f;
+        const float a1 = 0.25f - mu2;
+        float q = a1, c = a1;
+        float a = -a1;
+        float s = 1.f + q * delh;
+        for (int i = 1; i <= MAXIT; ++i) {
+            a -= 2 * i;
+            c = -a * c / (i + 1.f);
+            const float qnew = (q1 - b * q2) / a;
+            q1 = q2;
+            q2 = qnew;
+            q += c * qnew;
+            b += 2.f;
+            d = 1.f / (b + a * d);
+            delh = (b * d - 1.f) * delh;
+            h += delh;
+            const float dels = q * delh;
+            s += dels;
+            if (fabsf(dels / s) < EPS) break;
+        }
+        rkmu = sqrtf(float(M_PI) / (2.f * x)) * __expf(-x) / s;
+        rk1 = rkmu * (mu + x + 0.5f - a1 * h) * xi;
+    }
+    float kmu = rkmu, kmu1 = rk1, kprev = 0.f;
+    for (int i = 1; i <= nl; ++i) {
+        const float knext = (mu + i) * xi2 * kmu1 + kmu;
+        kprev = kmu;
+        kmu = kmu1;
+        kmu1 = knext;
+    }
+    if (km1) *km1 = (nl >= 1) ? kprev : kmu1 - mu * xi2 * kmu;
+    return kmu;
+}
+
 // K0(x) and K1(x) in FP32 for x > 0 from the polynomial approximations of Abramowitz & Stegun,
 // Handbook of Mathematical Functions, 9.8.1, 9.8.3, 9.8.5-9.8.8 (absolute error below 2.2e-7 in
 // x^(1/2) e^x K(x) for x >= 2, below 1e-8 in K0 and x K1 for x <= 2).  Returns K0 and x K1, the

@@ -156,6 +156,28 @@ __device__ inline void leaf_kprime(const LeafC<T>& C, const LeafI& I, T kx, T ky
     }
 }
 
+// Moffat of non-special beta in FP32: g(k) = k^nu K_nu(k), nu = beta - 1, and dg = g'(k)/(2k) =
+// -k^(nu-1) K_{nu-1}(k) / 2, with the mu-only constants of Temme's series in C.p[2..5].  At k = 0:
+// g = C.p[1] (the kV value at the origin, flux / knorm), dg = 0.
+template <typename T>
+__device__ inline void moffat_generic_f32(const LeafC<T>& C, float k, float* g, float* dg)
+{
+    if (k == 0.f) {
+        *g = float(C.p[1]);
+        *dg = 0.f;
+        return;
+    }
+    const float nu = float(C.p[0]) - 1.f;
+    const int nl = int(nu + 0.5f);
+    const float mu = nu - float(nl);
+    float km1;
+    const float kn = bessel_k_f32(nl, mu, float(C.p[2]), float(C.p[3]), float(C.p[4]),
+                                  float(C.p[5]), k, &km1);
+    const float pk = __expf((nu - 1.f) * __logf(k));       // k^(nu - 1)
+    *g = kn * pk * k;
+    *dg = -0.5f * pk * km1;
+}
+
 // Value of a decoded leaf at k-image sample (i, j) (kx = i dk, ky = j dk).  rec is the leaf's
 // record, read only by the FP64 Bessel-function Moffat kinds.
 template <typename T>
@@ -196,6 +218,11 @@ __device__ inline T eval_leafc(const LeafC<T>& C, const LeafI& I, const gsmi_lea
                 if (I.kind == 2 || I.kind == 3 || I.kind == 4) {
                     float g, dg;
                     moffat_int_kind_f32(I.kind, k, &g, &dg);
+                    return flux * g;
+                }
+                if (I.kind == 0) {
+                    float g, dg;
+                    moffat_generic_f32(C, k, &g, &dg);
                     return flux * g;
                 }
             }

@@ -123,6 +123,11 @@ __device__ inline LeafD<T> eval_leafc_d(const LeafC<T>& C, const LeafI& I, const
                     moffat_int_kind_f32(I.kind, k, &g, &dg);
                     return {g, dg};
                 }
+                if (I.kind == 0) {
+                    float g, dg;
+                    moffat_generic_f32(C, k, &g, &dg);
+                    return {g, dg};
+                }
             }
             LeafD<double> m = moffat_bessel_d(I.kind, rec->p[0] - 1., double(k), double(ksq),
                                               rec->p[1]);

@@ -9,6 +9,29 @@ from ._leaf import Leaf
 from .gsparams import GSParams
 
 
+def temme_constants(nu):
+    """The order-only constants of Temme's series for K_nu(x), x < 2 (N. M. Temme, J. Comput.
+    Phys. 19, 324 (1975)), which the FP32 kernels take precomputed: with mu = nu - round(nu),
+    fact = pi mu / sin(pi mu), gam1 = (1/Gamma(1-mu) - 1/Gamma(1+mu)) / (2 mu),
+    gam2 = (1/Gamma(1-mu) + 1/Gamma(1+mu)) / 2, returns (fact gam1, fact gam2,
+    Gamma(1+mu) / 2, Gamma(1-mu) / 2)."""
+    nl = int(nu + 0.5)
+    mu = nu - nl
+    gampl = 1. / math.gamma(1. + mu)
+    gammi = 1. / math.gamma(1. - mu)
+    if abs(mu) < 1e-4:
+        # Taylor coefficients of 1/Gamma(1+z) (Abramowitz & Stegun 6.1.34)
+        m2 = mu * mu
+        gam1 = -(0.5772156649015329 - 0.0420026350340952 * m2)
+        gam2 = 1. - 0.6558780715202538 * m2 + 0.1665386113822915 * m2 * m2
+        fact = 1. + (math.pi * mu) ** 2 / 6.
+    else:
+        gam1 = (gammi - gampl) / (2. * mu)
+        gam2 = (gammi + gampl) / 2.
+        fact = math.pi * mu / math.sin(math.pi * mu)
+    return fact * gam1, fact * gam2, 0.5 / gampl, 0.5 / gammi
+
+
 class Moffat(Leaf):
     _leaf_type = _lib.LEAF_MOFFAT
     _is_analytic_x = True
@@ -76,6 +99,8 @@ class Moffat(Leaf):
         rec['kind'] = int(info[4])
         rec['p'][0] = self._beta
         rec['p'][1] = self._flux / info[2] if info[2] != 0 else 0.   # kV_gen at k = 0
+        if int(info[4]) == 0:
+            rec['p'][2:6] = temme_constants(self._beta - 1.)
 
     def withFlux(self, flux):
         return Moffat(self._beta, scale_radius=self._r0, flux=flux, gsparams=self.gsparams)

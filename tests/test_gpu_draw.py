@@ -70,6 +70,9 @@ def test_exponential_moffat_seeded_sweep():
      (48, 48)),
     ("moffat_generic", lambda m: m.Moffat(2.7, fwhm=0.9).shear(g1=-0.2, g2=0.15), (48, 48)),
     ("moffat_2", lambda m: m.Moffat(2., fwhm=0.6, flux=4.), (48, 48)),
+    ("moffat_37_sheared", lambda m: m.Moffat(3.7, fwhm=0.8).shear(g1=0.1, g2=0.2), (48, 48)),
+    ("moffat_44_conv", lambda m: m.Convolve([m.Exponential(half_light_radius=0.7, flux=9.),
+                                             m.Moffat(4.4, fwhm=0.9)]), (56, 56)),
     ("moffat_3_sheared", lambda m: m.Moffat(3., fwhm=0.7).shear(g1=0.25, g2=-0.1), (40, 44)),
     ("moffat_4_conv", lambda m: m.Convolve([m.Exponential(half_light_radius=0.5, flux=30.),
                                             m.Moffat(4., fwhm=0.8)]), (48, 48)),
