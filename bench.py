@@ -130,6 +130,114 @@ def fwd_bwd_bench(args, torch):
             "grad_err_fp32_vs_fp64": err}
 
 
+def fit_bench(args, torch):
+    """The batched fit workload: 8192 Exponential x Moffat stamps with sides in {32, 48, 64, 96,
+    128}, beta ~ U(2.5, 4.5), noisy targets at S/N ~ U(20, 200); 50 Adam steps on (flux, hlr,
+    g1, g2, dx, dy, psf_fwhm) (beta held at its true value), each step one HIP graph launch.
+    Reports fit throughput, peak device memory, and the parameters against the same fit run
+    with the FP64 kernels on a subset."""
+    from galsim_amd import fit
+    n = args.fit_batch
+    steps = 50
+    torch.cuda.synchronize()
+    torch.cuda.reset_peak_memory_stats()
+    base_mem = torch.cuda.memory_allocated()
+    t0 = time.perf_counter()
+    model, theta, init, target, isig, scale, betas, sides = fit.exp_moffat_fit_problem(
+        n, seed=args.seed + 7, precision='f32')
+    f = fit.BatchFit(model, init, target, isig, scale)
+    f.capture()
+    t_setup = time.perf_counter() - t0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    f.replay(steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    peak = torch.cuda.max_memory_allocated() - base_mem
+    rate = n / wall
+    est32 = f.theta.cpu().numpy()
+    chi = (f.chi2().cpu().numpy() / sides.astype(float) ** 2)
+    log("fit: %d stamps x %d Adam steps in %.3f s (%.1f ms/step) -> %.0f stamps/s; peak "
+        "device memory %.2f GB; median chi2/pixel %.3f" % (n, steps, wall, 1e3 * wall / steps,
+                                                           rate, peak / 1e9, np.median(chi)))
+    # the same fit with the FP64 kernels on a subset (graph replays as well)
+    m = min(n, args.fit_compare)
+    sub = fit.exp_moffat_fit_problem(m, seed=args.seed + 7, precision='f64')
+    f64 = fit.BatchFit(sub[0], sub[2], sub[3], sub[4], sub[5])
+    f64.capture()
+    f64.replay(steps)
+    torch.cuda.synchronize()
+    est64 = f64.theta.cpu().numpy()
+    sub32 = fit.exp_moffat_fit_problem(m, seed=args.seed + 7, precision='f32')
+    f32 = fit.BatchFit(sub32[0], sub32[2], sub32[3], sub32[4], sub32[5])
+    f32.capture()
+    f32.replay(steps)
+    torch.cuda.synchronize()
+    d = np.abs(f32.theta.cpu().numpy() - est64) / scale
+    err_truth = np.abs(est32 - theta) / scale
+    log("fit FP32 vs FP64 (%d stamps): max |diff| / scale per parameter %s" % (
+        m, np.round(d.max(axis=0), 6)))
+    return {"value": round(rate, 1), "unit": "stamps/s (50 Adam steps each)",
+            "n_stamps": n, "adam_steps": steps, "ms_per_step": round(1e3 * wall / steps, 3),
+            "setup_s": round(t_setup, 2), "peak_device_memory_GB": round(peak / 1e9, 3),
+            "median_chi2_per_pixel": round(float(np.median(chi)), 4),
+            "param_diff_fp32_vs_fp64_fit_max_over_scale": {
+                k: float(v) for k, v in zip(('flux', 'hlr', 'g1', 'g2', 'dx', 'dy', 'psf_fwhm'),
+                                            d.max(axis=0))},
+            "param_scale": scale.tolist(),
+            "median_abs_error_vs_truth_over_scale": float(np.median(err_truth)),
+            "workload": "%d Exponential x Moffat(beta ~ U(2.5, 4.5)) stamps, sides in {32, 48, "
+                        "64, 96, 128}, FP32, S/N ~ U(20, 200); 7 parameters fitted, beta held "
+                        "at truth" % n}
+
+
+def large_bench(args, torch):
+    """Large stamps: 64 Sersic galaxies (n ~ U(3, 6) in steps of 0.05, hlr ~ U(2, 6)") under
+    Kolmogorov(0.8") at drawImage's own stamp sizes, FP32 batched (FFT grids of 512 to 4096),
+    with the error of every stamp whose grid the FP64 kernels take (N <= 2048) against them."""
+    from galsim_amd import workloads
+    from galsim_amd.batch import BatchPlan
+    n = args.large_n
+    t0 = time.perf_counter()
+    profs, nx, ny, offs, p = workloads.build('large_sersic', n, seed=args.seed + 11)
+    plan = BatchPlan(profs, nx, ny, 0.2, offs, precision='f32')
+    plan.autotune(reps=2)
+    plan.capture()
+    t_setup = time.perf_counter() - t0
+    for _ in range(2):
+        plan.replay()
+    torch.cuda.synchronize()
+    k = 5
+    t0 = time.perf_counter()
+    for _ in range(k):
+        plan.replay()
+    torch.cuda.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / k
+    Ns = np.empty(n, dtype=int)
+    Ns[plan.order] = plan.recs['N']
+    small = [i for i in range(n) if Ns[i] <= 2048]
+    plan64 = BatchPlan([profs[i] for i in small], nx[small], ny[small], 0.2, offs[small],
+                       precision='f64')
+    plan64.run()
+    torch.cuda.synchronize()
+    errs = []
+    for k2, i in enumerate(small):
+        a = plan.stamp(i).double()
+        b = plan64.stamp(k2)
+        errs.append(float((a - b).abs().max() / b.abs().max()))
+    log("large Sersic: %d stamps FP32 in %.2f ms (%.0f stamps/s), FFT sizes %s; FP32 vs FP64 "
+        "peak-rel error over %d stamps: max %.2e" % (n, ms, n / (ms * 1e-3), plan.fft_sizes(),
+                                                      len(small), max(errs)))
+    return {"value": round(n / (ms * 1e-3), 1), "unit": "stamps/s", "ms_per_step": round(ms, 3),
+            "n_stamps": n, "fft_sizes": {str(a): b for a, b in plan.fft_sizes().items()},
+            "setup_s": round(t_setup, 2),
+            "peak_rel_err_fp32_vs_fp64": {"max": max(errs), "median": float(np.median(errs)),
+                                          "stamps_compared": len(small),
+                                          "per_stamp": [round(e, 10) for e in errs]},
+            "workload": "Sersic n ~ U(3, 6) step 0.05, hlr ~ U(2, 6) arcsec, x Kolmogorov "
+                        "0.8 arcsec x Pixel, drawImage's own stamp sizes, FP32"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -142,6 +250,11 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--fit-n', type=int, default=1024)
     ap.add_argument('--no-fit', action='store_true')
+    ap.add_argument('--fit-batch', type=int, default=8192)
+    ap.add_argument('--fit-compare', type=int, default=256)
+    ap.add_argument('--no-batch-fit', action='store_true')
+    ap.add_argument('--large-n', type=int, default=64)
+    ap.add_argument('--no-large', action='store_true')
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='MIB',
                     help='only run a plain device copy of MIB MiB (3 times) and exit; used to '
                          'scale the FETCH_SIZE/WRITE_SIZE counters')
@@ -304,6 +417,8 @@ def main():
     log("drawImage FP64 end to end: %.0f stamps/s" % drawimage_rate)
 
     fwd_bwd = None if args.no_fit else fwd_bwd_bench(args, torch)
+    batch_fit = None if (args.no_fit or args.no_batch_fit) else fit_bench(args, torch)
+    large = None if args.no_large else large_bench(args, torch)
 
     # --- CPU baseline: the oracle (numpy FP64 restatement), one core ------------------------
     cpu = None
@@ -347,6 +462,8 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "fwd_bwd": fwd_bwd,
+        "batch_fit": batch_fit,
+        "large_stamps": large,
         "plain_fp64_gpu_stamps_per_s": round(plain_rate, 1),
         "speedup_vs_plain_fp64_gpu": round(value / plain_rate, 2),
         "drawimage_fp64_end_to_end_stamps_per_s": round(drawimage_rate, 1),
