@@ -37,6 +37,21 @@ __device__ __forceinline__ void dsincos(float x, float* s, float* c) { sincosf(x
 __device__ __forceinline__ double dsinpi(double x) { return sinpi(x); }
 __device__ __forceinline__ float dsinpi(float x) { return sinpif(x); }
 
+// Exact integer quotient t / d for 0 <= t, 1 <= d, t + d < 2^21, from inv_d = 1.f / d: the
+// product (t + 1/2) / d lies at least 1/(2d) away from an integer and its float error is below
+// 2^-23 (q + 1), so truncation gives q.  The kernels' index arithmetic divides by FFT sizes and
+// tile widths that are not powers of two (3 * 2^k); this replaces ~40-instruction integer
+// divisions by one multiply.
+__device__ __forceinline__ int fdiv(int t, float inv_d)
+{
+    return int((float(t) + 0.5f) * inv_d);
+}
+
+// Division for the leaf formulas: exact in FP64 (the FP64 path must reproduce the reference),
+// the hardware reciprocal (1 ulp) in FP32.
+__device__ __forceinline__ double ddiv(double a, double b) { return a / b; }
+__device__ __forceinline__ float ddiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+
 // 1/Gamma(1+mu) +- 1/Gamma(1-mu) combinations for Temme's series (|mu| <= 1/2).  For small mu
 // the Taylor coefficients of 1/Gamma(1+z) (Abramowitz & Stegun 6.1.34) avoid cancellation.
 __device__ inline void temme_gammas(double mu, double* gam1, double* gam2, double* gampl,

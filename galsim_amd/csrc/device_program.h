@@ -111,9 +111,9 @@ __device__ inline T spline_eq(const T* __restrict__ args, int n, T lk, T* dS)
     const T bb = h - aa;
     const T cc = (aa + h) * y2[i - 1] + (bb + h) * y2[i];
     if (dS)
-        *dS = (vals[i] - vals[i - 1] -
-               (T(1) / T(6)) * ((aa - bb) * cc + aa * bb * (y2[i] - y2[i - 1]))) / h;
-    return (aa * vals[i - 1] + bb * vals[i] - (T(1) / T(6)) * aa * bb * cc) / h;
+        *dS = ddiv(vals[i] - vals[i - 1] -
+                   (T(1) / T(6)) * ((aa - bb) * cc + aa * bb * (y2[i] - y2[i - 1])), h);
+    return ddiv(aa * vals[i - 1] + bb * vals[i] - (T(1) / T(6)) * aa * bb * cc, h);
 }
 
 template <typename T>
@@ -195,7 +195,7 @@ __device__ inline T eval_leafc(const LeafC<T>& C, const LeafI& I, const gsmi_lea
         T ksq = X * X + Y * Y;
         if (exp_clip<T>(C, I, tb, i, j, Nk, ksq)) return T(0);
         T q = T(1) + ksq;
-        return flux / (q * dsqrt(q));
+        return ddiv(flux, q * dsqrt(q));
       }
       case GSMI_LEAF_GAUSSIAN: {
         // SBGaussian::fillKImage (src/SBGaussian.cpp:200-290)
@@ -238,19 +238,19 @@ __device__ inline T eval_leafc(const LeafC<T>& C, const LeafI& I, const gsmi_lea
       case GSMI_LEAF_SERSIC: {
         // SersicInfo::kValue (src/SBSersic.cpp:363-376)
         T ksq = X * X + Y * Y;
-        if (ksq >= C.p[1]) return flux * (C.p[2] + C.p[3] / dsqrt(ksq)) / ksq;
+        if (ksq >= C.p[1]) return ddiv(flux * (C.p[2] + ddiv(C.p[3], dsqrt(ksq))), ksq);
         if (ksq < C.p[0]) return flux * (T(1) + ksq * (C.p[4] + ksq * C.p[5]));
         T lk = T(0.5) * dlog(ksq);
-        return flux * spline_eq<T>(sersic_table<T>(tb, I.aux), I.ntab, lk, nullptr) / ksq;
+        return ddiv(flux * spline_eq<T>(sersic_table<T>(tb, I.aux), I.ntab, lk, nullptr), ksq);
       }
       case GSMI_LEAF_BOX: {
         // SBBox::fillKImage with math::sinc (src/SBBox.cpp:151-224, src/math/Sinc.cpp:29-33)
         const T Xb = kxp * C.s;
         const T Yb = kyp * C.s2;
         T sx = fabs(Xb) < T(1e-4) ? T(1) - T(M_PI * M_PI / 6.) * Xb * Xb
-                                  : dsinpi(Xb) / (T(M_PI) * Xb);
+                                  : ddiv(dsinpi(Xb), T(M_PI) * Xb);
         T sy = fabs(Yb) < T(1e-4) ? T(1) - T(M_PI * M_PI / 6.) * Yb * Yb
-                                  : dsinpi(Yb) / (T(M_PI) * Yb);
+                                  : ddiv(dsinpi(Yb), T(M_PI) * Yb);
         return flux * sx * sy;
       }
     }
@@ -578,10 +578,13 @@ __device__ inline void stockham_pass(const cplx<T>* src, cplx<T>* dst, int n, in
 {
     const int nb = n / R;
     const int tstep = (n / (Ns * R)) * twstride;
+    const float inv_nb = 1.f / float(nb);
+    const float inv_ns = 1.f / float(Ns);
     for (int t = threadIdx.x; t < nseg * nb; t += blockDim.x) {
-        const int seg = t / nb;
+        const int seg = fdiv(t, inv_nb);
         const int tt = t - seg * nb;
-        const int jj = tt % Ns;
+        const int tq = fdiv(tt, inv_ns);
+        const int jj = tt - tq * Ns;
         const cplx<T>* s = src + seg * n + tt;
         cplx<T> v[R];
 #pragma unroll
@@ -593,7 +596,7 @@ __device__ inline void stockham_pass(const cplx<T>* src, cplx<T>* dst, int n, in
         if (R == 2) bfly2(v);
         if (R == 3) bfly3(v);
         if (R == 4) bfly4(v);
-        cplx<T>* d = dst + seg * n + (tt / Ns) * Ns * R + jj;
+        cplx<T>* d = dst + seg * n + tq * Ns * R + jj;
 #pragma unroll
         for (int r = 0; r < R; ++r) d[r * Ns] = v[r];
     }

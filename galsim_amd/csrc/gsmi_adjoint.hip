@@ -92,8 +92,8 @@ __device__ inline LeafD<T> eval_leafc_d(const LeafC<T>& C, const LeafI& I, const
       case GSMI_LEAF_EXPONENTIAL: {
         if (exp_clip<T>(C, I, tb, i, j, Nk, ksq)) return {T(0), T(0)};
         T q = T(1) + ksq;
-        T phi = T(1) / (q * dsqrt(q));
-        return {phi, T(-1.5) * phi / q};
+        T phi = ddiv(T(1), q * dsqrt(q));
+        return {phi, ddiv(T(-1.5) * phi, q)};
       }
       case GSMI_LEAF_GAUSSIAN: {
         if (I.sep) {
@@ -111,7 +111,7 @@ __device__ inline LeafD<T> eval_leafc_d(const LeafC<T>& C, const LeafI& I, const
         T e = dexp(-k);
         switch (I.kind) {
           case 15:
-            return {e, ksq == T(0) ? T(0) : T(-0.5) * e / k};
+            return {e, ksq == T(0) ? T(0) : ddiv(T(-0.5) * e, k)};
           case 25:
             return {e * (T(1) + k), T(-0.5) * e};
           case 35:
@@ -139,28 +139,29 @@ __device__ inline LeafD<T> eval_leafc_d(const LeafC<T>& C, const LeafI& I, const
         if (ksq == T(0)) return {T(1), T(0)};
         T t = dexp(T(5. / 6.) * dlog(ksq));
         T phi = dexp(-t);
-        return {phi, T(-5. / 6.) * t / ksq * phi};
+        return {phi, ddiv(T(-5. / 6.) * t, ksq) * phi};
       }
       case GSMI_LEAF_SERSIC: {
         if (ksq >= C.p[1]) {
             T rk = dsqrt(ksq);
-            T phi = (C.p[2] + C.p[3] / rk) / ksq;
-            return {phi, -(C.p[2] + T(1.5) * C.p[3] / rk) / (ksq * ksq)};
+            const T irk = ddiv(T(1), rk);
+            T phi = ddiv(C.p[2] + C.p[3] * irk, ksq);
+            return {phi, -ddiv(C.p[2] + T(1.5) * C.p[3] * irk, ksq * ksq)};
         }
         if (ksq < C.p[0])
             return {T(1) + ksq * (C.p[4] + ksq * C.p[5]), C.p[4] + T(2) * ksq * C.p[5]};
         T lk = T(0.5) * dlog(ksq);
         T dS;
         T S = spline_eq<T>(sersic_table<T>(tb, I.aux), I.ntab, lk, &dS);
-        return {S / ksq, (T(0.5) * dS - S) / (ksq * ksq)};
+        return {ddiv(S, ksq), ddiv(T(0.5) * dS - S, ksq * ksq)};
       }
       case GSMI_LEAF_BOX: {
         const T Xb = kxp * C.s;
         const T Yb = kyp * C.s2;
         T sx = fabs(Xb) < T(1e-4) ? T(1) - T(M_PI * M_PI / 6.) * Xb * Xb
-                                  : dsinpi(Xb) / (T(M_PI) * Xb);
+                                  : ddiv(dsinpi(Xb), T(M_PI) * Xb);
         T sy = fabs(Yb) < T(1e-4) ? T(1) - T(M_PI * M_PI / 6.) * Yb * Yb
-                                  : dsinpi(Yb) / (T(M_PI) * Yb);
+                                  : ddiv(dsinpi(Yb), T(M_PI) * Yb);
         return {sx * sy, T(0)};
       }
     }
@@ -203,17 +204,21 @@ row_r2c(const gsmi_stamp* __restrict__ stamps, const T* __restrict__ resid,
     for (int e = threadIdx.x; e < nrow * n2; e += blockDim.x) bufa[e] = {T(0), T(0)};
     __syncthreads();
     T* za = reinterpret_cast<T*>(bufa);
+    const float inv_nx = 1.f / float(nx);
+    const float inv_n = 1.f / float(N);
     for (int e = threadIdx.x; e < nrow * nx; e += blockDim.x) {
-        const int rl = e / nx;
+        const int rl = fdiv(e, inv_nx);
         const int col = e - rl * nx;
-        const int q = ((x0 + col) % N + N) % N;
+        const int pq = x0 + col + N;                  // >= 0: x0 >= -N/2
+        const int q = pq - N * fdiv(pq, inv_n);
         za[2 * (rl * n2 + (q >> 1)) + (q & 1)] = in[size_t(rl) * nx + col];
     }
     __syncthreads();
     const cplx<T>* res = fft_lds<T>(bufa, bufb, n2, nrow, tw, 2);
     cplx<T>* out = scratch + st->scratch_offset + size_t(r0) * W;
+    const float inv_w = 1.f / float(W);
     for (int e = threadIdx.x; e < nrow * W; e += blockDim.x) {
-        const int rl = e / W;
+        const int rl = fdiv(e, inv_w);
         const int c = e - rl * W;
         const cplx<T> zc = res[rl * n2 + (c == n2 ? 0 : c)];
         const cplx<T> zm = res[rl * n2 + (c == 0 ? 0 : n2 - c)];
@@ -262,10 +267,13 @@ grad_colfft(const gsmi_stamp* __restrict__ stamps, const gsmi_op* __restrict__ o
 
     for (int e = threadIdx.x; e < ncol * N; e += blockDim.x) bufa[e] = {T(0), T(0)};
     __syncthreads();
+    const float inv_ncol = 1.f / float(max(ncol, 1));
+    const float inv_n = 1.f / float(N);
     for (int e = threadIdx.x; e < ncol * ny; e += blockDim.x) {
-        const int row = e / ncol;
+        const int row = fdiv(e, inv_ncol);
         const int cl = e - row * ncol;
-        const int y = ((y0 + row) % N + N) % N;
+        const int py = y0 + row + N;                  // >= 0: y0 >= -N/2
+        const int y = py - N * fdiv(py, inv_n);
         bufa[cl * N + y] = H[size_t(row) * W + c0 + cl];
     }
     __syncthreads();
@@ -284,13 +292,13 @@ grad_colfft(const gsmi_stamp* __restrict__ stamps, const gsmi_op* __restrict__ o
     for (int f = 0; f < NF; ++f) acc[f] = T(0);
     const T inv_n2 = T(1) / (T(N) * T(N));
     for (int e = threadIdx.x; ok && e < ncol * N; e += blockDim.x) {
-        const int cl = e / N;
+        const int cl = fdiv(e, inv_n);
         const int r = e - cl * N;
         const int c = c0 + cl;
         const T wc = (c == 0 || c == half) ? inv_n2 : T(2) * inv_n2;
         const cplx<T> A = {res[cl * N + r].x * wc, res[cl * N + r].y * wc};
-        const int kx0 = c - N * ((c + hk) / N);
-        const int ky0 = r - N * ((r + hk) / N);
+        const int kx0 = c - N * fdiv(c + hk, inv_n);
+        const int ky0 = r - N * fdiv(r + hk, inv_n);
         for (int kxa = kx0; kxa <= hk; kxa += N) {
             for (int kya = ky0; kya <= hk; kya += N) {
                 const bool neg = kxa < 0;

@@ -58,13 +58,14 @@ __device__ __forceinline__ void fold_cells(const gsmi_op* __restrict__ sops, int
                                            int c0, int ncol, cplx<T>* bufa)
 {
     const int hk = Nk / 2;
+    const float inv_n = 1.f / float(N);
     for (int e = threadIdx.x; e < ncol * N; e += blockDim.x) {
-        const int cl = e / N;
+        const int cl = fdiv(e, inv_n);
         const int r = e - cl * N;
         const int c = c0 + cl;
         cplx<T> acc = {T(0), T(0)};
-        const int kx0 = c - N * ((c + hk) / N);
-        const int ky0 = r - N * ((r + hk) / N);
+        const int kx0 = c - N * fdiv(c + hk, inv_n);
+        const int ky0 = r - N * fdiv(r + hk, inv_n);
         for (int kx = kx0; kx <= hk; kx += N) {
             for (int ky = ky0; ky <= hk; ky += N) {
                 // the stored sample (kx >= 0); its conjugate stands for (kx, ky) when kx < 0
@@ -120,11 +121,13 @@ __device__ __forceinline__ void fold_item(const gsmi_stamp* __restrict__ st, int
     const int W = half + 1;
     const int ny = st->ny;
     const int y0 = st->y0;
+    const float inv_ncol = 1.f / float(ncol);
+    const float inv_n = 1.f / float(N);
     for (int e = threadIdx.x; e < ncol * ny; e += blockDim.x) {
-        const int row = e / ncol;
+        const int row = fdiv(e, inv_ncol);
         const int cl = e - row * ncol;
-        const int p = y0 + row;
-        const int y = (p % N + N) % N;
+        const int p = y0 + row + N;                   // >= 0: y0 >= -N/2
+        const int y = p - N * fdiv(p, inv_n);
         out[row * W + c0 + cl] = res[cl * N + y];
     }
 }
@@ -245,8 +248,9 @@ __device__ void row_c2r_body(cplx<T>* lds, int rpg, const cplx<T>* __restrict__ 
     const int W = n2 + 1;
     cplx<T>* bufa = lds;
     cplx<T>* bufb = lds + rpg * n2;
+    const float inv_n2 = 1.f / float(n2);
     for (int e = threadIdx.x; e < nrow * n2; e += blockDim.x) {
-        const int rl = e / n2;
+        const int rl = fdiv(e, inv_n2);
         const int c = e - rl * n2;
         const cplx<T>* z = in + size_t(rl) * W;
         cplx<T> zc = z[c];
@@ -259,10 +263,13 @@ __device__ void row_c2r_body(cplx<T>* lds, int rpg, const cplx<T>* __restrict__ 
     __syncthreads();
     cplx<T>* res = fft_lds<T>(bufa, bufb, n2, nrow, tw, 2 * tws);
     const T norm = T(1) / (T(N) * T(N));
+    const float inv_nx = 1.f / float(nx);
+    const float inv_n = 1.f / float(N);
     for (int e = threadIdx.x; e < nrow * nx; e += blockDim.x) {
-        const int rl = e / nx;
+        const int rl = fdiv(e, inv_nx);
         const int col = e - rl * nx;
-        const int q = ((x0 + col) % N + N) % N;
+        const int p = x0 + col + N;                   // >= 0: x0 >= -N/2
+        const int q = p - N * fdiv(p, inv_n);
         const cplx<T> v = res[rl * n2 + (q >> 1)];
         out[size_t(rl) * nx + col] = ((q & 1) ? v.y : v.x) * norm;
     }

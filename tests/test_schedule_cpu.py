@@ -50,3 +50,13 @@ def test_work_lists_reject_tiles_beyond_lds():
         work_lists(recs, groups, {1024: 8}, {}, 16)
     assert BatchPlan.default_cols(128) == 8 and BatchPlan.default_cols(2048) == 1
     assert BatchPlan.default_rows(128) == 16 and BatchPlan.default_rows(4096) == 1
+
+
+def test_float_reciprocal_index_division_is_exact():
+    """The kernels' index arithmetic divides by FFT sizes and tile widths with
+    int((t + 0.5) * (1.f / d)) in FP32; it equals t // d over the ranges they use."""
+    t = np.arange(0, 20000, dtype=np.int64)
+    for d in list(range(1, 1100)) + [1536, 2048, 3072, 4096, 6144, 8192]:
+        inv = np.float32(1.0) / np.float32(d)
+        q = ((t.astype(np.float32) + np.float32(0.5)) * inv).astype(np.int64)
+        assert np.array_equal(q, t // d), d
