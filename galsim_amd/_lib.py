@@ -25,7 +25,10 @@ STAMP_DTYPE = np.dtype([('N', np.int32), ('Nk', np.int32), ('nx', np.int32), ('n
 assert OP_DTYPE.itemsize == 32 and LEAF_DTYPE.itemsize == 128 and STAMP_DTYPE.itemsize == 72
 STACK_DEPTH = {'f64': 6, 'f32': 4}
 GRAD_MAXLEAF = 4
-GRAD_FIELDS = 27
+GRAD_PER_LEAF = 7
+GRAD_FIELDS = 31
+ADJ_COMPLEX = 1
+ADJ_SHAPE = 2
 
 OP_LEAF, OP_MUL, OP_ADD, OP_SCALE, OP_PHASE = 1, 2, 3, 4, 5
 LEAF_EXPONENTIAL, LEAF_GAUSSIAN, LEAF_MOFFAT, LEAF_KOLMOGOROV, LEAF_SERSIC, LEAF_BOX = range(1, 7)

@@ -6,6 +6,7 @@ buffer, bins the stamps by FFT size N and runs the two kernels once per bin on o
 result stays on the GPU as one flat tensor plus per-stamp offsets and shapes.
 """
 import ctypes
+import math
 
 import numpy as np
 
@@ -34,20 +35,36 @@ def _image_profile(prof, nx, ny, scale, offset, use_true_center, method):
     return p, b.shift(PositionI(-c.x, -c.y)), flux_scale
 
 
+def _special_knorm2(kind):
+    """knorm / flux of the closed-form Moffat kinds (src/SBMoffat.cpp:497-549)."""
+    return {15: 1., 2: 1., 25: 1., 3: 0.5, 35: 1. / 3., 4: 0.125}[kind]
+
+
 _RADIAL = (_lib.LEAF_EXPONENTIAL, _lib.LEAF_GAUSSIAN, _lib.LEAF_MOFFAT, _lib.LEAF_KOLMOGOROV,
            _lib.LEAF_SERSIC)
 
 
-def _prepare_differentiable(ctx, op0, nops, naxis, leaf0):
+def _prepare_differentiable(ctx, op0, nops, naxis, leaf0, moffat_generic=False):
     """Make a lowered stamp's fields writable and differentiable: radial leaves of the program
     take the general-Jacobian path (so every jac entry is live), and the hoisted factors get a
-    whole-result PHASE (ux = uy = 0, fs = 1, an exact no-op) if they have none.  Returns the
-    number of ops appended.  Exponential clip rows (FP64) stay those of the lowered Jacobian."""
+    whole-result PHASE (ux = uy = 0, fs = 1, an exact no-op) if they have none.  With
+    ``moffat_generic`` Moffat leaves use the k^nu K_nu form for every beta (the special-beta
+    closed forms equal it at their beta, but only there), so beta can move.  Returns the number
+    of ops appended.  Exponential clip rows (FP64) stay those of the lowered Jacobian."""
+    from .moffat import temme_constants
     for code, arg, *_ in ctx.ops[op0:op0 + nops]:
         if code == _lib.OP_LEAF:
             rec = ctx.leaves[leaf0 + arg]
             if rec['type'] in _RADIAL:
                 rec['separable'] = 0
+            if moffat_generic and rec['type'] == _lib.LEAF_MOFFAT and rec['kind'] != 0:
+                beta = float(rec['p'][0])
+                knorm2 = 4. / (math.gamma(beta - 1.) * 2. ** beta)
+                # knorm = profile flux x knorm2: swap the closed form's knorm2 for the generic one
+                rec['flux'] = float(rec['flux']) * knorm2 / _special_knorm2(int(rec['kind']))
+                rec['kind'] = 0
+                rec['p'][1] = 1. / knorm2
+                rec['p'][2:6] = temme_constants(beta - 1.)
     axis = ctx.ops[op0 + nops:op0 + nops + naxis]
     if any(op[0] == _lib.OP_PHASE for op in axis):
         return 0
@@ -93,7 +110,7 @@ class BatchPlan:
     current stream (no allocation, no host transfer: it can be captured in a HIP graph)."""
 
     def __init__(self, profiles, nx, ny, scale, offsets=None, precision='f32', method='auto',
-                 use_true_center=True, differentiable=False):
+                 use_true_center=True, differentiable=False, moffat_generic=False):
         import torch
         if precision not in ('f32', 'f64'):
             raise ValueError("precision must be 'f32' or 'f64'")
@@ -121,7 +138,7 @@ class BatchPlan:
                     ctx.ops[k] = (code, arg - leaf0, a, bb, c)
             render.check_stack(ctx.ops, op0, nops, precision)
             if differentiable:
-                naxis += _prepare_differentiable(ctx, op0, nops, naxis, leaf0)
+                naxis += _prepare_differentiable(ctx, op0, nops, naxis, leaf0, moffat_generic)
             recs[i] = (N, Nk, nx[i], ny[i], b.xmin, b.ymin, op0, nops, leaf0, naxis,
                        0, 0, 0, dk)
             Ns[i] = N

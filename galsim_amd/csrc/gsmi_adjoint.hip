@@ -24,9 +24,9 @@
 //   3. grad_reduce: per stamp, sums the workgroup partials in a fixed order (deterministic).
 //
 // Fields, per stamp (GSMI_GRAD_FIELDS doubles): for each leaf slot l < GSMI_GRAD_MAXLEAF
-// (relative leaf index of the program's LEAF ops) d/d(flux, scale, jac[0..3]) at 6 l .. 6 l + 5,
-// then d/dux, d/duy of a phase applying to the whole result and d/dlog(fs) of a factor applying
-// to the whole result.  Radial leaves only (Exponential, Gaussian, Moffat, Kolmogorov, Sersic):
+// (relative leaf index of the program's LEAF ops) d/d(flux, scale, jac[0..3], shape) at
+// 7 l .. 7 l + 6, then d/dux, d/duy of a phase applying to the whole result and d/dlog(fs) of a
+// factor applying to the whole result.  The shape field is the Moffat beta.  Radial leaves only (Exponential, Gaussian, Moffat, Kolmogorov, Sersic):
 // a Box leaf in the program contributes its value, and its slot reads zero.
 #include <hip/hip_runtime.h>
 
@@ -79,6 +79,25 @@ __device__ __noinline__ LeafD<double> moffat_bessel_d(int kind, double nu, doubl
 
 // The forward's eval_leafc at unit flux, with dphi/dq.  The branches and clipping follow
 // eval_leafc exactly, so phi * flux is the forward value.
+// d/dnu [k^nu K_nu(k)] (nu = beta - 1 >= 0.1) in FP64: k^nu (ln k K_nu + dK_nu/dnu), the order
+// derivative by a central difference with step 1e-4 (truncation ~1e-9 relative, rounding
+// ~1e-12); at k = 0 the limit 2^(nu-1) Gamma(nu) (ln 2 + psi(nu)), psi by the difference of
+// lgamma.
+__device__ __noinline__ double moffat_dg_dnu(double nu, double k)
+{
+    const double h = 1e-4;
+    if (k == 0.) {
+        const double g0 = exp((nu - 1.) * M_LN2 + lgamma(nu));
+        const double psi = (lgamma(nu + h) - lgamma(nu - h)) / (2. * h);
+        return g0 * (M_LN2 + psi);
+    }
+    const double lk = log(k);
+    const double kp = bessel_k(nu + h, k);
+    const double km = bessel_k(nu - h, k);
+    const double kn = bessel_k(nu, k);
+    return exp(nu * lk) * (lk * kn + (kp - km) / (2. * h));
+}
+
 template <typename T>
 __device__ inline LeafD<T> eval_leafc_d(const LeafC<T>& C, const LeafI& I, const gsmi_leaf* rec,
                                         const Tables<T>& tb, T kx, T ky, int i, int j, int Nk)
@@ -234,13 +253,14 @@ row_r2c(const gsmi_stamp* __restrict__ stamps, const T* __restrict__ resid,
 // k-image sample of the group's columns.  grid (ceil((N/2+1)/cpg), nstamps);
 // LDS 2 cpg N complex.  ML = leaf slots with gradient accumulators (2 or 4).
 // ---------------------------------------------------------------------------------------------
-template <typename T, int ML, bool CPLX>
+template <typename T, int ML, bool CPLX, bool SHAPE>
 __global__ void __launch_bounds__(kThreads)
 grad_colfft(const gsmi_stamp* __restrict__ stamps, const gsmi_op* __restrict__ ops,
             const gsmi_leaf* __restrict__ leaves, Tables<T> tb, const cplx<T>* __restrict__ tw,
             const cplx<T>* __restrict__ scratch, double* __restrict__ partials)
 {
-    constexpr int NF = 6 * ML + 3;
+    constexpr int PL = GSMI_GRAD_PER_LEAF;
+    constexpr int NF = PL * ML + 3;
     extern __shared__ unsigned char smem_raw[];
     cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
     const gsmi_stamp* st = stamps + blockIdx.y;
@@ -329,9 +349,9 @@ grad_colfft(const gsmi_stamp* __restrict__ stamps, const gsmi_op* __restrict__ o
                     const T P = run_ops_real<T>(d, sops, v, -1, T(0));
                     FZ = {P * Z.x, P * Z.y};
                 }
-                acc[6 * ML] += kx * FZ.y;
-                acc[6 * ML + 1] += ky * FZ.y;
-                acc[6 * ML + 2] += FZ.x;
+                acc[PL * ML] += kx * FZ.y;
+                acc[PL * ML + 1] += ky * FZ.y;
+                acc[PL * ML + 2] += FZ.x;
 #pragma unroll
                 for (int l = 0; l < ML; ++l) {
                     if (l < nl) {
@@ -350,12 +370,19 @@ grad_colfft(const gsmi_stamp* __restrict__ stamps, const gsmi_op* __restrict__ o
                         const T sc = L.s;
                         const T g = tx * dv[l];
                         const T g2 = T(2) * sc * sc * g;
-                        acc[6 * l] += tx * phi[l];
-                        acc[6 * l + 1] += T(2) * sc * (kxp * kxp + kyp * kyp) * g;
-                        acc[6 * l + 2] += g2 * kxp * kx;
-                        acc[6 * l + 3] += g2 * kyp * kx;
-                        acc[6 * l + 4] += g2 * kxp * ky;
-                        acc[6 * l + 5] += g2 * kyp * ky;
+                        acc[PL * l] += tx * phi[l];
+                        acc[PL * l + 1] += T(2) * sc * (kxp * kxp + kyp * kyp) * g;
+                        acc[PL * l + 2] += g2 * kxp * kx;
+                        acc[PL * l + 3] += g2 * kyp * kx;
+                        acc[PL * l + 4] += g2 * kxp * ky;
+                        acc[PL * l + 5] += g2 * kyp * ky;
+                        if constexpr (SHAPE) {
+                            if (d.li[l].type == GSMI_LEAF_MOFFAT) {
+                                const T kk = sc * dsqrt(kxp * kxp + kyp * kyp);
+                                acc[PL * l + 6] += tx * L.flux *
+                                    T(moffat_dg_dnu(sleaves[l].p[0] - 1., double(kk)));
+                            }
+                        }
                     }
                 }
             }
@@ -373,11 +400,11 @@ grad_colfft(const gsmi_stamp* __restrict__ stamps, const gsmi_op* __restrict__ o
     }
     __syncthreads();
     for (int f = threadIdx.x; f < GSMI_GRAD_FIELDS; f += blockDim.x) {
-        // output layout: leaf slots at 6 l for l < GSMI_GRAD_MAXLEAF, then the 3 whole-result
+        // output layout: leaf slots at PL l for l < GSMI_GRAD_MAXLEAF, then the 3 whole-result
         // fields; slots beyond ML read zero
         int src = -1;
-        if (f < 6 * ML) src = f;
-        else if (f >= 6 * GSMI_GRAD_MAXLEAF) src = 6 * ML + (f - 6 * GSMI_GRAD_MAXLEAF);
+        if (f < PL * ML) src = f;
+        else if (f >= PL * GSMI_GRAD_MAXLEAF) src = PL * ML + (f - PL * GSMI_GRAD_MAXLEAF);
         double x = 0.;
         if (!ok) x = NAN;
         else if (src >= 0)
@@ -403,23 +430,23 @@ grad_reduce(const double* __restrict__ partials, int ngroups, int nstamps,
 
 int adjoint_groups(int N) { return (N / 2 + 1 + cols_per_group(N) - 1) / cols_per_group(N); }
 
-template <typename T, bool CPLX>
+template <typename T, bool CPLX, bool SHAPE>
 void launch_grad(dim3 gb, size_t ldsb, hipStream_t s, int max_leaves, const gsmi_stamp* stamps,
                  const gsmi_op* ops, const gsmi_leaf* leaves, Tables<T> tb,
                  const cplx<T>* tw, const void* scratch, double* partials)
 {
     const cplx<T>* sc = reinterpret_cast<const cplx<T>*>(scratch);
     if (max_leaves <= 2)
-        hipLaunchKernelGGL((grad_colfft<T, 2, CPLX>), gb, dim3(kThreads), ldsb, s, stamps, ops,
-                           leaves, tb, tw, sc, partials);
+        hipLaunchKernelGGL((grad_colfft<T, 2, CPLX, SHAPE>), gb, dim3(kThreads), ldsb, s,
+                           stamps, ops, leaves, tb, tw, sc, partials);
     else
-        hipLaunchKernelGGL((grad_colfft<T, 4, CPLX>), gb, dim3(kThreads), ldsb, s, stamps, ops,
-                           leaves, tb, tw, sc, partials);
+        hipLaunchKernelGGL((grad_colfft<T, 4, CPLX, SHAPE>), gb, dim3(kThreads), ldsb, s,
+                           stamps, ops, leaves, tb, tw, sc, partials);
 }
 
 template <typename T>
 int adjoint_impl(const gsmi_stamp* stamps, int nstamps, int N, int max_ny, int max_leaves,
-                 int complex_program, const gsmi_op* ops, const gsmi_leaf* leaves, Tables<T> tb,
+                 int flags, const gsmi_op* ops, const gsmi_leaf* leaves, Tables<T> tb,
                  const T* resid,
                  void* scratch, double* partials, double* grad, void* stream)
 {
@@ -453,12 +480,20 @@ int adjoint_impl(const gsmi_stamp* stamps, int nstamps, int N, int max_ny, int m
     const int ng = adjoint_groups(N);
     dim3 gb(ng, nstamps);
     size_t ldsb = 2 * size_t(cpg) * N * sizeof(cplx<T>);
-    if (complex_program)
-        launch_grad<T, true>(gb, ldsb, s, max_leaves, stamps, ops, leaves, tb, tw, scratch,
-                             partials);
+    const bool cx = flags & GSMI_ADJ_COMPLEX;
+    const bool sh = flags & GSMI_ADJ_SHAPE;
+    if (cx && sh)
+        launch_grad<T, true, true>(gb, ldsb, s, max_leaves, stamps, ops, leaves, tb, tw,
+                                   scratch, partials);
+    else if (cx)
+        launch_grad<T, true, false>(gb, ldsb, s, max_leaves, stamps, ops, leaves, tb, tw,
+                                    scratch, partials);
+    else if (sh)
+        launch_grad<T, false, true>(gb, ldsb, s, max_leaves, stamps, ops, leaves, tb, tw,
+                                    scratch, partials);
     else
-        launch_grad<T, false>(gb, ldsb, s, max_leaves, stamps, ops, leaves, tb, tw, scratch,
-                              partials);
+        launch_grad<T, false, false>(gb, ldsb, s, max_leaves, stamps, ops, leaves, tb, tw,
+                                     scratch, partials);
     rc = hip_check(hipGetLastError(), "grad_colfft launch");
     if (rc) return rc;
     const int total = nstamps * GSMI_GRAD_FIELDS;
@@ -485,7 +520,7 @@ int gsmi_adjoint_partials(int N, int nstamps, int64_t* ndoubles)
 }
 
 int gsmi_adjoint_f64(const gsmi_stamp* stamps, int nstamps, int N, int max_ny, int max_leaves,
-                     int complex_program, const gsmi_op* ops, const gsmi_leaf* leaves,
+                     int flags, const gsmi_op* ops, const gsmi_leaf* leaves,
                      const double* tables, const int32_t* cliprows, const double* resid,
                      void* scratch, double* partials, double* grad, void* stream)
 {
@@ -494,13 +529,13 @@ int gsmi_adjoint_f64(const gsmi_stamp* stamps, int nstamps, int N, int max_ny, i
         return GSMI_E_ARG;
     }
     Tables<double> tb{tables, cliprows};
-    return adjoint_impl<double>(stamps, nstamps, N, max_ny, max_leaves, complex_program, ops,
+    return adjoint_impl<double>(stamps, nstamps, N, max_ny, max_leaves, flags, ops,
                                 leaves, tb, resid,
                                 scratch, partials, grad, stream);
 }
 
 int gsmi_adjoint_f32(const gsmi_stamp* stamps, int nstamps, int N, int max_ny, int max_leaves,
-                     int complex_program, const gsmi_op* ops, const gsmi_leaf* leaves,
+                     int flags, const gsmi_op* ops, const gsmi_leaf* leaves,
                      const double* tables, const float* tables32, const float* resid,
                      void* scratch, double* partials, double* grad, void* stream)
 {
@@ -509,7 +544,7 @@ int gsmi_adjoint_f32(const gsmi_stamp* stamps, int nstamps, int N, int max_ny, i
         return GSMI_E_ARG;
     }
     Tables<float> tb{tables, tables32};
-    return adjoint_impl<float>(stamps, nstamps, N, max_ny, max_leaves, complex_program, ops,
+    return adjoint_impl<float>(stamps, nstamps, N, max_ny, max_leaves, flags, ops,
                                leaves, tb, resid,
                                scratch, partials, grad, stream);
 }

@@ -76,14 +76,15 @@ class BatchFit:
 
 
 def exp_moffat_fit_problem(n, sizes=(32, 48, 64, 96, 128), seed=0, precision='f32',
-                           snr=(20., 200.), beta=None):
+                           snr=(20., 200.), beta=None, fit_beta=True):
     """The batched-fit workload: n stamps with sides drawn from ``sizes``, Exponential (x)
     Moffat with seeded true parameters (hlr ~ U(0.8, 2.5)", |g| ~ U(0, 0.5), sub-pixel offsets,
     flux 1e4, psf fwhm ~ U(0.6, 1.0)", beta ~ U(2.5, 4.5) unless given), noisy targets with a
     sky sigma giving S/N ~ U(snr) (S/N = flux / (sigma sqrt(4 pi r_eff^2)), r_eff the
     half-light radius in pixels of the PSF-convolved profile, approximately), and an initial
-    guess perturbed from the truth.  Returns (model, theta_true, theta_init, target, inv_sigma,
-    scale, betas, sides)."""
+    guess perturbed from the truth.  With ``fit_beta`` the Moffat beta is the eighth parameter
+    (initial guess off by up to 0.2); otherwise it is held at the truth.  Returns (model,
+    theta_true, theta_init, target, inv_sigma, scale, betas, sides)."""
     import torch
     from . import diff
     rng = np.random.default_rng(seed)
@@ -102,10 +103,17 @@ def exp_moffat_fit_problem(n, sizes=(32, 48, 64, 96, 128), seed=0, precision='f3
                              rng.uniform(0.95, 1.05, n)], axis=1)
     init[:, 2:4] += rng.normal(0., 0.03, (n, 2))
     init[:, 4:6] += rng.normal(0., 0.1, (n, 2))
-    model = diff.ExpMoffat(init, sides, sides, scale=0.2, beta=betas, precision=precision)
+    scale = np.array([1e3, 0.1, 0.05, 0.05, 0.2, 0.2, 0.05])
+    if fit_beta:
+        theta = np.concatenate([theta, betas[:, None]], axis=1)
+        init = np.concatenate([init, (betas + rng.uniform(-0.2, 0.2, n))[:, None]], axis=1)
+        scale = np.append(scale, 0.1)
+    model = diff.ExpMoffat(init, sides, sides, scale=0.2, beta=betas, precision=precision,
+                           fit_beta=fit_beta)
     t_true = torch.tensor(theta, dtype=torch.float64, device='cuda')
     with torch.no_grad():
-        truth = diff.ExpMoffat(theta, sides, sides, scale=0.2, beta=betas, precision='f64')
+        truth = diff.ExpMoffat(theta, sides, sides, scale=0.2, beta=betas, precision='f64',
+                               fit_beta=fit_beta)
         timg = truth.render(t_true)
         # the truth's stamps in the fit model's layout (FFT sizes, hence order, may differ)
         img = torch.empty(model.dplan.plan.out.numel(), dtype=torch.float64, device='cuda')
@@ -120,5 +128,4 @@ def exp_moffat_fit_problem(n, sizes=(32, 48, 64, 96, 128), seed=0, precision='f3
     gen = torch.Generator(device='cpu').manual_seed(seed + 1)
     noise = torch.randn(img.shape, generator=gen, dtype=torch.float64).cuda()
     target = img + noise / inv_sigma
-    scale = np.array([1e3, 0.1, 0.05, 0.05, 0.2, 0.2, 0.05])
     return model, theta, init, target, inv_sigma, scale, betas, sides

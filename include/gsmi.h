@@ -237,28 +237,35 @@ int gsmi_wrap_hermx_f64(const double* kin, int Nk, int N, double* kout, void* st
  * Given resid = dL/dI for every output pixel (same layout and precision as out of
  * gsmi_draw_fft_*), writes grad[s * GSMI_GRAD_FIELDS + f] = dL/dfield_f of stamp s (stamps in
  * the order of the stamps array), FP64:
- *   f = 6 l + {0..5}: leaf slot l (the relative leaf index of a LEAF op of the program, l <
- *                     GSMI_GRAD_MAXLEAF): flux, scale, jac[0], jac[1], jac[2], jac[3];
- *   f = 24, 25:       ux, uy of a PHASE applying to the whole result (hoisted axis factor);
- *   f = 26:           log fs of a SCALE / PHASE amplitude applying to the whole result.
+ *   f = 7 l + {0..6}: leaf slot l (the relative leaf index of a LEAF op of the program, l <
+ *                     GSMI_GRAD_MAXLEAF): flux, scale, jac[0], jac[1], jac[2], jac[3], and the
+ *                     shape p[0] -- for a Moffat leaf its beta, the derivative of the Moffat
+ *                     k-profile in beta at fixed knorm and scale (GSMI_ADJ_SHAPE only);
+ *   f = 28, 29:       ux, uy of a PHASE applying to the whole result (hoisted axis factor);
+ *   f = 30:           log fs of a SCALE / PHASE amplitude applying to the whole result.
  * Only radial leaves (Exponential, Gaussian, Moffat, Kolmogorov, Sersic) have field gradients;
- * a Box slot reads zero.  Exponential clip rows (FP64) are held fixed: clipped samples have
- * zero derivative, as they have zero value.  max_leaves bounds the number of leaf slots the
- * programs use (1..GSMI_GRAD_MAXLEAF); complex_program must be nonzero if any program holds a
- * PHASE op (a shift inside the tree rather than on the whole result).  A stamp whose program
- * breaks either promise, or has more than 12 ops, gets NaN gradients.  scratch is the
- * forward's scratch layout (it is overwritten); partials holds gsmi_adjoint_partials() doubles.
+ * a Box slot reads zero, and so does the shape field of every leaf but Moffat.  Exponential clip
+ * rows (FP64) are held fixed: clipped samples have zero derivative, as they have zero value.
+ * max_leaves bounds the number of leaf slots the programs use (1..GSMI_GRAD_MAXLEAF).  flags:
+ * GSMI_ADJ_COMPLEX if any program holds a PHASE op (a shift inside the tree rather than on the
+ * whole result), GSMI_ADJ_SHAPE to compute the Moffat beta derivatives (three FP64 Bessel
+ * evaluations per Moffat sample).  A stamp whose program breaks a promise, or has more than 12
+ * ops, gets NaN gradients.  scratch is the forward's scratch layout (it is overwritten);
+ * partials holds gsmi_adjoint_partials() doubles.
  * ------------------------------------------------------------------------------------- */
 #define GSMI_GRAD_MAXLEAF 4
-#define GSMI_GRAD_FIELDS 27
+#define GSMI_GRAD_PER_LEAF 7
+#define GSMI_GRAD_FIELDS 31
+#define GSMI_ADJ_COMPLEX 1
+#define GSMI_ADJ_SHAPE 2
 
 int gsmi_adjoint_partials(int N, int nstamps, int64_t* ndoubles);
 int gsmi_adjoint_f64(const gsmi_stamp* stamps, int nstamps, int N, int max_ny, int max_leaves,
-                     int complex_program, const gsmi_op* ops, const gsmi_leaf* leaves,
+                     int flags, const gsmi_op* ops, const gsmi_leaf* leaves,
                      const double* tables, const int32_t* cliprows, const double* resid,
                      void* scratch, double* partials, double* grad, void* stream);
 int gsmi_adjoint_f32(const gsmi_stamp* stamps, int nstamps, int N, int max_ny, int max_leaves,
-                     int complex_program, const gsmi_op* ops, const gsmi_leaf* leaves,
+                     int flags, const gsmi_op* ops, const gsmi_leaf* leaves,
                      const double* tables, const float* tables32, const float* resid,
                      void* scratch, double* partials, double* grad, void* stream);
 

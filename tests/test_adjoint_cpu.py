@@ -155,9 +155,10 @@ def test_exp_moffat_field_map_matches_lowering():
         naxis += _prepare_differentiable(ctx, 0, nops, naxis, 0)
         e, m = ctx.leaves[0], ctx.leaves[1]
         ph = [op for op in ctx.ops[nops:nops + naxis] if op[0] == g._lib.OP_PHASE][0]
+        P = diff.PER_LEAF
         base = torch.zeros(1, diff.NFIELD, dtype=torch.float64)
-        base[0, 6] = float(m['flux'])
-        base[0, 8:12] = torch.from_numpy(np.array(m['jac'], dtype=float))
+        base[0, P] = float(m['flux'])
+        base[0, P + 2:P + 6] = torch.from_numpy(np.array(m['jac'], dtype=float))
         base[0, diff.F_FS] = ph[4]
         t = torch.tensor(th[None], dtype=torch.float64)
         cx = torch.tensor([ph[2] - th[4]])
@@ -167,6 +168,39 @@ def test_exp_moffat_field_map_matches_lowering():
         np.testing.assert_allclose(f[0], e['flux'], rtol=1e-15)
         np.testing.assert_allclose(f[1], e['scale'], rtol=1e-15)
         np.testing.assert_allclose(f[2:6], np.array(e['jac']), rtol=1e-14, atol=1e-15)
-        np.testing.assert_allclose(f[7], m['scale'], rtol=1e-15)
+        np.testing.assert_allclose(f[P + 1], m['scale'], rtol=1e-15)
+        np.testing.assert_allclose(f[P + 2:P + 6], np.array(m['jac']), rtol=1e-15)
         np.testing.assert_allclose(f[diff.F_UX], ph[2], rtol=1e-14, atol=1e-15)
         np.testing.assert_allclose(f[diff.F_UY], ph[3], rtol=1e-14, atol=1e-15)
+
+
+def test_moffat_beta_field_map_and_aux_match_lowering():
+    """With beta as a parameter: the knorm and rD the torch map gives, and the auxiliary record
+    values (k = 0 value, Temme constants), equal what the lowering writes for a Moffat put on
+    the k^nu K_nu form at that beta."""
+    import torch
+    from galsim_amd import diff
+    from galsim_amd.moffat import temme_constants
+    for beta in (2.5, 2.73, 3.0, 3.5, 4.21):
+        prof = g.Moffat(beta, fwhm=0.8)
+        p, b, _ = _image_profile(prof, 40, 40, 0.2, (0., 0.), True, 'auto')
+        N, Nk, dk = render.plan_sizes(p, b)
+        ctx = render.Lowering('f64', Nk, dk)
+        p._lower(ctx, None)
+        nops, naxis = render.hoist_axis_factors(ctx, 0)
+        _prepare_differentiable(ctx, 0, nops, naxis, 0, moffat_generic=True)
+        rec = ctx.leaves[0]
+        assert rec['kind'] == 0
+        tb = torch.tensor([beta], dtype=torch.float64)
+        aux = diff.moffat_aux(tb)[0].numpy()
+        np.testing.assert_allclose(aux[0], rec['p'][1], rtol=1e-13)
+        np.testing.assert_allclose(aux[1:], rec['p'][2:6], rtol=1e-12, atol=1e-15)
+        np.testing.assert_allclose(rec['p'][2:6], temme_constants(beta - 1.), rtol=1e-15)
+        th = torch.tensor([[1., 1., 0., 0., 0., 0., 0.8]], dtype=torch.float64)
+        base = torch.zeros(1, diff.NFIELD, dtype=torch.float64)
+        f = diff.exp_moffat_fields(th, base, torch.zeros(1), torch.zeros(1), 0.2, None,
+                                   beta=tb)[0].numpy()
+        P = diff.PER_LEAF
+        np.testing.assert_allclose(f[P], rec['flux'], rtol=1e-13)
+        np.testing.assert_allclose(f[P + 1], rec['scale'], rtol=1e-14)
+        assert f[P + diff.F_SHAPE] == beta

@@ -29,7 +29,7 @@ def _live_fields(dplan, i):
     out = []
     for l in range(int(dplan.nleaves[i])):
         if dplan.leaf_types[i, l] != _lib.LEAF_BOX:
-            out += [6 * l + k for k in range(6)]
+            out += [diff.PER_LEAF * l + k for k in range(6)]
     return out + [diff.F_UX, diff.F_UY, diff.F_FS]
 
 
@@ -228,3 +228,44 @@ def test_adjoint_rejects_bad_residual_size():
     d = diff.DiffPlan(profs, 16, 16, 0.2, None, precision='f32')
     with pytest.raises(ValueError):
         d.adjoint(torch.zeros(10, device='cuda'))
+
+
+def test_autograd_beta_gradient_matches_central_fd_f64():
+    """The Moffat beta as an eighth parameter (k^nu K_nu form, shape gradients in the adjoint):
+    autograd against central differences of the chi^2 loss, every parameter."""
+    from galsim_amd import workloads
+    p = workloads.exp_moffat_params(3, seed=13)
+    beta = np.array([2.7, 3.0, 4.2])
+    theta = np.stack([p['flux'], p['hlr'], p['g1'], p['g2'], p['offset'][:, 0],
+                      p['offset'][:, 1], p['psf_fwhm'], beta], axis=1)
+    model = diff.ExpMoffat(theta, 48, 48, scale=0.2, precision='f64', fit_beta=True)
+    t0 = torch.tensor(theta, dtype=torch.float64, device='cuda')
+    with torch.no_grad():
+        img = model.render(t0).double()
+    gen = torch.Generator(device='cpu').manual_seed(2)
+    sig = 0.01 * img.abs().max()
+    target = img + sig * torch.randn(img.shape, generator=gen, dtype=torch.float64).cuda()
+    t1 = t0.clone()
+    t1[:, 1] *= 0.97
+    t1[:, 6] *= 1.04
+    t1[:, 7] += 0.15
+
+    def losses(t):
+        r = ((model.render(t).double() - target) / sig) ** 2
+        return torch.stack([s.sum() for s in model.dplan.stamps(r)])
+
+    t = t1.clone().requires_grad_(True)
+    losses(t).sum().backward()
+    gr = t.grad.cpu().numpy()
+    with torch.no_grad():
+        for k in range(8):
+            h = 1e-6 * max(1., float(t1[:, k].abs().max()))
+            tp = t1.clone()
+            tp[:, k] += h
+            tm = t1.clone()
+            tm[:, k] -= h
+            fd = ((losses(tp) - losses(tm)) / (2 * h)).cpu().numpy()
+            for i in range(3):
+                scale = np.max(np.abs(gr[i]))
+                assert abs(gr[i, k] - fd[i]) <= 2e-5 * max(abs(fd[i]), 1e-3 * scale), \
+                    (i, k, gr[i, k], fd[i])
