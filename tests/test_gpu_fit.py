@@ -11,8 +11,9 @@ torch = pytest.importorskip("torch")
 from galsim_amd import fit  # noqa: E402
 
 
-def _problem(n, seed, precision='f32'):
-    return fit.exp_moffat_fit_problem(n, sizes=(32, 48, 64), seed=seed, precision=precision)
+def _problem(n, seed, precision='f32', snr=(20., 200.)):
+    return fit.exp_moffat_fit_problem(n, sizes=(32, 48, 64), seed=seed, precision=precision,
+                                      snr=snr)
 
 
 def test_graph_steps_equal_eager_steps():
@@ -31,7 +32,8 @@ def test_graph_steps_equal_eager_steps():
 
 def test_fit_converges_toward_truth():
     n = 12
-    model, theta, init, target, isig, scale, betas, sides = _problem(n, 5)
+    # high S/N, so that the starting guess is many noise sigmas from the optimum
+    model, theta, init, target, isig, scale, betas, sides = _problem(n, 5, snr=(400., 800.))
     f = fit.BatchFit(model, init, target, isig, scale)
     chi0 = f.chi2().cpu().numpy()
     f.capture()
