@@ -46,7 +46,18 @@ class Schedule(ctypes.Structure):
     _fields_ = [('fold', ctypes.c_void_p), ('rows', ctypes.c_void_p),
                 ('nfold', ctypes.c_int32), ('nrows', ctypes.c_int32),
                 ('fold_grid', ctypes.c_int32), ('rows_grid', ctypes.c_int32),
-                ('fold_lds', ctypes.c_int32), ('rows_lds', ctypes.c_int32)]
+                ('fold_lds', ctypes.c_int32), ('rows_lds', ctypes.c_int32),
+                ('program', ctypes.c_int32)]
+
+
+# Program shapes with specialised fold kernels (gsmi_schedule.program): post-order ops
+# ((code, arg), ...) and the leaf type of each slot.
+STATIC_PROGRAMS = {
+    1: (((1, 0), (1, 1), (3, 0), (1, 2), (2, 0)), (1, 5, 4)),   # (Exp + Sersic) x Kolmogorov
+    2: (((1, 0), (1, 1), (2, 0)), (1, 3)),                     # Exponential x Moffat
+    3: (((1, 0), (1, 1), (2, 0)), (5, 4)),                     # Sersic x Kolmogorov
+    4: (((1, 0), (1, 1), (2, 0)), (1, 4)),                     # Exponential x Kolmogorov
+}
 
 
 class GalSimAMDError(RuntimeError):

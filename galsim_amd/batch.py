@@ -72,6 +72,24 @@ def _prepare_differentiable(ctx, op0, nops, naxis, leaf0, moffat_generic=False):
     return 1
 
 
+def _program_shape(recs, ops, leaves):
+    """The specialised program shape (_lib.STATIC_PROGRAMS id) every stamp of the batch has, or
+    0 if they do not all share one."""
+    found = None
+    for r in recs:
+        b, cnt, lb = int(r['op_begin']), int(r['op_count']), int(r['leaf_begin'])
+        prog = tuple((int(o['code']), int(o['arg'])) for o in ops[b:b + cnt])
+        shape = 0
+        for sid, (sops, types) in _lib.STATIC_PROGRAMS.items():
+            if prog == sops and all(int(leaves[lb + l]['type']) == t for l, t in enumerate(types)):
+                shape = sid
+                break
+        if shape == 0 or (found is not None and shape != found):
+            return 0
+        found = shape
+    return found or 0
+
+
 def work_lists(recs, groups, cols, rows, elem, bins=None):
     """Fold and row work items (gsmi_work rows: stamp, first, count, per_group) of a batch whose
     stamp records ``recs`` are sorted into FFT-size ``groups`` (N, first, count, max_ny), for
@@ -186,6 +204,7 @@ class BatchPlan:
         self.stamp_shape = np.stack([recs['ny'][inv], recs['nx'][inv]], axis=1)
         self.nops = len(ctx.ops)
         self.nleaves = len(ctx.leaves)
+        self.program_shape = _program_shape(recs, self.host_ops, self.host_leaves)
         self._sched = None
         self._graph = None
         self.tuning = None
@@ -222,7 +241,13 @@ class BatchPlan:
         sc.fold_grid = len(fold) if fold_grid is None else int(min(fold_grid, len(fold)))
         sc.rows_grid = len(rw) if rows_grid is None else int(min(rows_grid, len(rw)))
         sc.fold_grid, sc.rows_grid = max(sc.fold_grid, 1), max(sc.rows_grid, 1)
-        sc.fold_lds, sc.rows_lds = lds_f, lds_r
+        # the specialised fold kernel of the batch's program shape keeps 4 decoded leaves
+        # (16 values each) in LDS beside the transform buffers
+        extra = 4 * 16 * (4 if self.precision == 'f32' else 8)
+        program = self.program_shape if lds_f + extra <= 65536 else 0
+        sc.fold_lds = lds_f + (extra if program else 0)
+        sc.rows_lds = lds_r
+        sc.program = program
         return items, sc
 
     def use_schedule(self, cols=None, rows=None, fold_grid=None, rows_grid=None):

@@ -423,6 +423,85 @@ __device__ inline T run_ops_real(const ProgramD& d, const gsmi_op* __restrict__ 
     return st[0];
 }
 
+// Program shapes compiled into specialised fold kernels: the op sequence and the leaf types are
+// compile-time constants there, so the sample loop has no interpreter (no op fetch or dispatch)
+// and every leaf branch but the right one folds away.  The kernels check each stamp against the
+// shape and take the interpreter for any stamp that does not match.
+struct StaticProgram {
+    int nops;
+    int code[5];
+    int arg[5];
+    int nl;
+    int type[3];
+};
+
+// index 0 is unused (the interpreter); the table's order is the gsmi_schedule program id
+constexpr StaticProgram kStaticPrograms[] = {
+    {0, {0, 0, 0, 0, 0}, {0, 0, 0, 0, 0}, 0, {0, 0, 0}},
+    // (Exponential + Sersic) x Kolmogorov: a bulge+disk galaxy under atmospheric seeing
+    {5, {GSMI_OP_LEAF, GSMI_OP_LEAF, GSMI_OP_ADD, GSMI_OP_LEAF, GSMI_OP_MUL}, {0, 1, 0, 2, 0}, 3,
+     {GSMI_LEAF_EXPONENTIAL, GSMI_LEAF_SERSIC, GSMI_LEAF_KOLMOGOROV}},
+    // Exponential x Moffat
+    {3, {GSMI_OP_LEAF, GSMI_OP_LEAF, GSMI_OP_MUL, 0, 0}, {0, 1, 0, 0, 0}, 2,
+     {GSMI_LEAF_EXPONENTIAL, GSMI_LEAF_MOFFAT, 0}},
+    // Sersic x Kolmogorov
+    {3, {GSMI_OP_LEAF, GSMI_OP_LEAF, GSMI_OP_MUL, 0, 0}, {0, 1, 0, 0, 0}, 2,
+     {GSMI_LEAF_SERSIC, GSMI_LEAF_KOLMOGOROV, 0}},
+    // Exponential x Kolmogorov
+    {3, {GSMI_OP_LEAF, GSMI_OP_LEAF, GSMI_OP_MUL, 0, 0}, {0, 1, 0, 0, 0}, 2,
+     {GSMI_LEAF_EXPONENTIAL, GSMI_LEAF_KOLMOGOROV, 0}},
+};
+constexpr int kNumStaticPrograms = 5;
+
+// Does the stamp's program (ops, leaves) have shape SIG?
+template <int SIG>
+__device__ inline bool matches_static(const gsmi_op* __restrict__ ops, int nops,
+                                      const gsmi_leaf* __restrict__ leaves)
+{
+    constexpr StaticProgram P = kStaticPrograms[SIG];
+    if (nops != P.nops) return false;
+#pragma unroll
+    for (int o = 0; o < P.nops; ++o)
+        if (ops[o].code != P.code[o] || ops[o].arg != P.arg[o]) return false;
+#pragma unroll
+    for (int l = 0; l < P.nl; ++l)
+        if (leaves[l].type != P.type[l]) return false;
+    return true;
+}
+
+// The value of program shape SIG at sample (i, j), its leaf constants in lc[] (LDS) and integer
+// fields in li[] (their type overridden by the compile-time one).
+template <typename T, int SIG>
+__device__ __forceinline__ T eval_static(const LeafC<T>* __restrict__ lc,
+                                         const LeafI (&li)[kProgLeaves],
+                                         const gsmi_leaf* __restrict__ leaves,
+                                         const Tables<T>& tb, T kx, T ky, int i, int j, int Nk)
+{
+    constexpr StaticProgram P = kStaticPrograms[SIG];
+    T v[3];
+#pragma unroll
+    for (int l = 0; l < P.nl; ++l) {
+        LeafI I = li[l];
+        I.type = P.type[l];
+        v[l] = eval_leafc<T>(lc[l], I, leaves + l, tb, kx, ky, i, j, Nk);
+    }
+    T st[3];
+    int sp = 0;
+#pragma unroll
+    for (int o = 0; o < P.nops; ++o) {
+        if (P.code[o] == GSMI_OP_LEAF) {
+            st[sp++] = v[P.arg[o]];
+        } else if (P.code[o] == GSMI_OP_MUL) {
+            --sp;
+            st[sp - 1] = st[sp - 1] * st[sp];
+        } else if (P.code[o] == GSMI_OP_ADD) {
+            --sp;
+            st[sp - 1] = st[sp - 1] + st[sp];
+        }
+    }
+    return st[0];
+}
+
 // Leaf values of a decoded program at sample (i, j).
 template <typename T>
 __device__ inline void eval_leaves(const ProgramD& d, const LeafC<T>* __restrict__ lc,

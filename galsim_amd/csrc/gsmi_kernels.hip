@@ -82,11 +82,46 @@ __device__ __forceinline__ void fold_cells(const gsmi_op* __restrict__ sops, int
     }
 }
 
+// fold_cells for a stamp whose program has the compile-time shape SIG (leaf constants in lc).
+template <typename T, int SIG>
+__device__ __forceinline__ void fold_cells_static(const LeafC<T>* __restrict__ lc,
+                                                  const LeafI (&li)[kProgLeaves],
+                                                  const gsmi_leaf* __restrict__ sleaves,
+                                                  const Tables<T>& tb,
+                                                  const cplx<T>* __restrict__ X,
+                                                  const cplx<T>* __restrict__ Y, T dk, int N,
+                                                  int Nk, int c0, int ncol, cplx<T>* bufa)
+{
+    const int hk = Nk / 2;
+    const float inv_n = 1.f / float(N);
+    for (int e = threadIdx.x; e < ncol * N; e += blockDim.x) {
+        const int cl = fdiv(e, inv_n);
+        const int r = e - cl * N;
+        const int c = c0 + cl;
+        cplx<T> acc = {T(0), T(0)};
+        const int kx0 = c - N * fdiv(c + hk, inv_n);
+        const int ky0 = r - N * fdiv(r + hk, inv_n);
+        for (int kx = kx0; kx <= hk; kx += N) {
+            for (int ky = ky0; ky <= hk; ky += N) {
+                const bool neg = kx < 0;
+                const int i = neg ? -kx : kx;
+                const int j = neg ? -ky : ky;
+                const T P = eval_static<T, SIG>(lc, li, sleaves, tb, T(i) * dk, T(j) * dk, i, j,
+                                                Nk);
+                const cplx<T> xy = cmul(X[i], Y[j]);
+                acc.x += P * xy.x;
+                acc.y += neg ? -(P * xy.y) : P * xy.y;
+            }
+        }
+        bufa[cl * N + r] = acc;
+    }
+}
+
 // ---------------------------------------------------------------------------------------------
 // Kernel 1: fused k-space evaluation + product/sinc/phase + Hermitian wrap + column inverse DFT
 // grid: (ceil((N/2+1)/cpg), nstamps); dynamic LDS: 2 * cpg * N complex
 // ---------------------------------------------------------------------------------------------
-template <typename T>
+template <typename T, int SIG = 0>
 __device__ __forceinline__ void fold_item(const gsmi_stamp* __restrict__ st, int c0, int ncol,
                                           int cpg, const gsmi_op* __restrict__ ops,
                                           const gsmi_leaf* __restrict__ leaves,
@@ -105,14 +140,41 @@ __device__ __forceinline__ void fold_item(const gsmi_stamp* __restrict__ st, int
     cplx<T>* bufa = lds;
     cplx<T>* bufb = lds + cpg * N;
     const int naxis = 3 * hk + 2;
-    if (naxis <= cpg * N) {
-        // the axis factors staged in bufb, which the column DFT needs only afterwards
-        for (int e = threadIdx.x; e < naxis; e += blockDim.x) bufb[e] = X[e];
-        __syncthreads();
-        fold_cells<T>(sops, nops, sleaves, tb, bufb, bufb + 2 * hk + 1, dk, N, Nk, c0, ncol,
-                      bufa);
-    } else {
-        fold_cells<T>(sops, nops, sleaves, tb, X, X + 2 * hk + 1, dk, N, Nk, c0, ncol, bufa);
+    bool specialised = false;
+    if constexpr (SIG > 0) {
+        // the program of shape SIG: its leaf constants converted once into LDS after bufb
+        specialised = matches_static<SIG>(sops, nops, sleaves);
+        if (specialised) {
+            LeafC<T>* lc = reinterpret_cast<LeafC<T>*>(bufb + cpg * N);
+            LeafI li[kProgLeaves];
+#pragma unroll
+            for (int l = 0; l < kProgLeaves; ++l)
+                li[l] = (l < kStaticPrograms[SIG].nl) ? leaf_ints(sleaves[l])
+                                                      : LeafI{0, 0, 0, -1, 0};
+            if (int(threadIdx.x) < kStaticPrograms[SIG].nl)
+                lc[threadIdx.x] = leaf_consts<T>(sleaves[threadIdx.x]);
+            if (naxis <= cpg * N) {
+                for (int e = threadIdx.x; e < naxis; e += blockDim.x) bufb[e] = X[e];
+                __syncthreads();
+                fold_cells_static<T, SIG>(lc, li, sleaves, tb, bufb, bufb + 2 * hk + 1, dk, N,
+                                          Nk, c0, ncol, bufa);
+            } else {
+                __syncthreads();
+                fold_cells_static<T, SIG>(lc, li, sleaves, tb, X, X + 2 * hk + 1, dk, N, Nk,
+                                          c0, ncol, bufa);
+            }
+        }
+    }
+    if (!specialised) {
+        if (naxis <= cpg * N) {
+            // the axis factors staged in bufb, which the column DFT needs only afterwards
+            for (int e = threadIdx.x; e < naxis; e += blockDim.x) bufb[e] = X[e];
+            __syncthreads();
+            fold_cells<T>(sops, nops, sleaves, tb, bufb, bufb + 2 * hk + 1, dk, N, Nk, c0, ncol,
+                          bufa);
+        } else {
+            fold_cells<T>(sops, nops, sleaves, tb, X, X + 2 * hk + 1, dk, N, Nk, c0, ncol, bufa);
+        }
     }
     __syncthreads();
     cplx<T>* res = fft_lds<T>(bufa, bufb, N, ncol, tw, tws);
@@ -152,10 +214,15 @@ __device__ __forceinline__ void fold_colfft_body(const gsmi_stamp* __restrict__ 
                  reinterpret_cast<cplx<T>*>(smem_raw));
 }
 
+// LDS a work-list fold launch needs beyond its two transform buffers (the decoded leaf constants
+// of a specialised program).
+template <typename T>
+constexpr size_t kFoldLdsExtra = kProgLeaves * sizeof(LeafC<T>);
+
 // Work-list form: every FFT size in one launch.  Workgroup b takes items b, b + grid, ...
 // (persistent when the grid is smaller than the list); items are validated against the stamp
 // records and the launch's LDS so a malformed list skips items instead of overrunning memory.
-template <typename T>
+template <typename T, int SIG>
 __device__ __forceinline__ void fold_list_body(const gsmi_stamp* __restrict__ stamps, int nstamps,
                                                const gsmi_work* __restrict__ items, int nitems,
                                                int lds_bytes, const gsmi_op* __restrict__ ops,
@@ -172,12 +239,13 @@ __device__ __forceinline__ void fold_list_body(const gsmi_stamp* __restrict__ st
         const int N = st->N;
         if (it.count <= 0 || it.first < 0 || it.first + it.count > N / 2 + 1 ||
             it.per_group < it.count ||
-            size_t(2) * it.per_group * N * sizeof(cplx<T>) > size_t(lds_bytes))
+            size_t(2) * it.per_group * N * sizeof(cplx<T>) + (SIG > 0 ? kFoldLdsExtra<T> : 0) >
+                size_t(lds_bytes))
             continue;
         int stride;
         const cplx<T>* tw = tws.pick(N, &stride);
-        fold_item<T>(st, it.first, it.count, it.per_group, ops, leaves, tb, tw, stride, scratch,
-                     lds);
+        fold_item<T, SIG>(st, it.first, it.count, it.per_group, ops, leaves, tb, tw, stride,
+                          scratch, lds);
         __syncthreads();
     }
 }
@@ -200,24 +268,26 @@ fold_colfft_f64(const gsmi_stamp* __restrict__ stamps, const gsmi_op* __restrict
     fold_colfft_body<double>(stamps, ops, leaves, tb, tw, scratch);
 }
 
+template <int SIG>
 __global__ void __launch_bounds__(kThreads, 5)
 fold_list_f32(const gsmi_stamp* __restrict__ stamps, int nstamps,
               const gsmi_work* __restrict__ items, int nitems, int lds_bytes,
               const gsmi_op* __restrict__ ops, const gsmi_leaf* __restrict__ leaves,
               Tables<float> tb, TwiddleSet<float> tws, cplx<float>* __restrict__ scratch)
 {
-    fold_list_body<float>(stamps, nstamps, items, nitems, lds_bytes, ops, leaves, tb, tws,
-                          scratch);
+    fold_list_body<float, SIG>(stamps, nstamps, items, nitems, lds_bytes, ops, leaves, tb, tws,
+                               scratch);
 }
 
+template <int SIG>
 __global__ void __launch_bounds__(kThreads)
 fold_list_f64(const gsmi_stamp* __restrict__ stamps, int nstamps,
               const gsmi_work* __restrict__ items, int nitems, int lds_bytes,
               const gsmi_op* __restrict__ ops, const gsmi_leaf* __restrict__ leaves,
               Tables<double> tb, TwiddleSet<double> tws, cplx<double>* __restrict__ scratch)
 {
-    fold_list_body<double>(stamps, nstamps, items, nitems, lds_bytes, ops, leaves, tb, tws,
-                           scratch);
+    fold_list_body<double, SIG>(stamps, nstamps, items, nitems, lds_bytes, ops, leaves, tb, tws,
+                                scratch);
 }
 
 template <typename T>
@@ -225,12 +295,14 @@ struct FoldKernel;
 template <>
 struct FoldKernel<float> {
     static constexpr auto fn = fold_colfft_f32;
-    static constexpr auto list = fold_list_f32;
+    template <int SIG>
+    static constexpr auto list = fold_list_f32<SIG>;
 };
 template <>
 struct FoldKernel<double> {
     static constexpr auto fn = fold_colfft_f64;
-    static constexpr auto list = fold_list_f64;
+    template <int SIG>
+    static constexpr auto list = fold_list_f64<SIG>;
 };
 
 // ---------------------------------------------------------------------------------------------
@@ -572,10 +644,26 @@ int fold_list_impl(const gsmi_stamp* stamps, int nstamps, const gsmi_schedule* s
     }
     rc = launch_axis_factors<T>(stamps, nstamps, ops, leaves, scratch, stream);
     if (rc || sc->nfold == 0) return rc;
-    hipLaunchKernelGGL(FoldKernel<T>::list, dim3(min(sc->fold_grid, sc->nfold)), dim3(kThreads),
-                       sc->fold_lds, reinterpret_cast<hipStream_t>(stream), stamps, nstamps,
-                       sc->fold, sc->nfold, sc->fold_lds, ops, leaves, tb, tws,
-                       reinterpret_cast<cplx<T>*>(scratch));
+    const dim3 grid(min(sc->fold_grid, sc->nfold));
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    cplx<T>* scr = reinterpret_cast<cplx<T>*>(scratch);
+    switch (sc->program) {
+#define GSMI_FOLD_LIST(SIG)                                                                      \
+      case SIG:                                                                                  \
+        hipLaunchKernelGGL(FoldKernel<T>::template list<SIG>, grid, dim3(kThreads), sc->fold_lds, \
+                           s, stamps, nstamps, sc->fold, sc->nfold, sc->fold_lds, ops, leaves,  \
+                           tb, tws, scr);                                                       \
+        break;
+      GSMI_FOLD_LIST(0)
+      GSMI_FOLD_LIST(1)
+      GSMI_FOLD_LIST(2)
+      GSMI_FOLD_LIST(3)
+      GSMI_FOLD_LIST(4)
+#undef GSMI_FOLD_LIST
+      default:
+        set_error("unknown program shape in schedule");
+        return GSMI_E_ARG;
+    }
     return hip_check(hipGetLastError(), "fold list launch");
 }
 

@@ -115,6 +115,13 @@ typedef struct gsmi_schedule {
     int32_t nfold, nrows;
     int32_t fold_grid, rows_grid;
     int32_t fold_lds, rows_lds;
+    int32_t program;      /* 0, or a program shape compiled into a specialised fold kernel:
+                             1 (Exponential + Sersic) x Kolmogorov, 2 Exponential x Moffat,
+                             3 Sersic x Kolmogorov, 4 Exponential x Kolmogorov (leaf slots in
+                             that order).  Stamps of another shape still draw correctly (the
+                             kernel checks every stamp) at the interpreter's speed.  With a
+                             nonzero program fold_lds must also hold 4 decoded leaves (64 B
+                             each in FP32, 128 B in FP64). */
 } gsmi_schedule;
 
 const char* gsmi_last_error(void);

@@ -60,3 +60,25 @@ def test_float_reciprocal_index_division_is_exact():
         inv = np.float32(1.0) / np.float32(d)
         q = ((t.astype(np.float32) + np.float32(0.5)) * inv).astype(np.int64)
         assert np.array_equal(q, t // d), d
+
+
+def test_program_shape_detection():
+    """A batch whose stamps all lower to one specialised shape gets its id; a mixed batch, or a
+    shape without a specialised kernel, gets 0 (the interpreter)."""
+    from galsim_amd.batch import _program_shape
+    L = _lib
+    ops = np.zeros(8, dtype=L.OP_DTYPE)
+    for k, (c, a) in enumerate([(1, 0), (1, 1), (3, 0), (1, 2), (2, 0), (1, 0), (1, 1), (2, 0)]):
+        ops[k]['code'], ops[k]['arg'] = c, a
+    leaves = np.zeros(7, dtype=L.LEAF_DTYPE)
+    for k, t in enumerate([L.LEAF_EXPONENTIAL, L.LEAF_SERSIC, L.LEAF_KOLMOGOROV, L.LEAF_BOX,
+                           L.LEAF_EXPONENTIAL, L.LEAF_MOFFAT, L.LEAF_BOX]):
+        leaves[k]['type'] = t
+    recs = np.zeros(2, dtype=L.STAMP_DTYPE)
+    recs[0]['op_begin'], recs[0]['op_count'], recs[0]['leaf_begin'] = 0, 5, 0
+    recs[1]['op_begin'], recs[1]['op_count'], recs[1]['leaf_begin'] = 5, 3, 4
+    assert _program_shape(recs[:1], ops, leaves) == 1
+    assert _program_shape(recs[1:], ops, leaves) == 2
+    assert _program_shape(recs, ops, leaves) == 0
+    leaves[5]['type'] = L.LEAF_GAUSSIAN
+    assert _program_shape(recs[1:], ops, leaves) == 0
