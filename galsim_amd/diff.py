@@ -111,6 +111,9 @@ class DiffPlan:
             self._bins.append((N, first, count, max_ny, int(nl[members].max()), flags))
         self._partials = torch.empty(need, dtype=torch.float64, device='cuda')
         self._grad = torch.empty((n, NFIELD), dtype=torch.float64, device='cuda')
+        # the forward as one fold and one row launch over work lists (specialised to the
+        # batch's program shape when it has one)
+        p.use_schedule()
 
     # -- fields <-> device records -------------------------------------------------------------
     def use_aux(self, slot, first, count):
