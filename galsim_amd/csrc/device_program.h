@@ -469,28 +469,18 @@ __device__ inline bool matches_static(const gsmi_op* __restrict__ ops, int nops,
     return true;
 }
 
-// The value of program shape SIG at sample (i, j), its leaf constants in lc[] (LDS) and integer
-// fields in li[] (their type overridden by the compile-time one).
+// The ops of program shape SIG on leaf values v[] (slot ovl, if >= 0, replaced by ovv); with
+// constant arguments the stack and the selects fold into straight-line arithmetic.
 template <typename T, int SIG>
-__device__ __forceinline__ T eval_static(const LeafC<T>* __restrict__ lc,
-                                         const LeafI (&li)[kProgLeaves],
-                                         const gsmi_leaf* __restrict__ leaves,
-                                         const Tables<T>& tb, T kx, T ky, int i, int j, int Nk)
+__device__ __forceinline__ T static_ops(const T (&v)[kProgLeaves], int ovl, T ovv)
 {
     constexpr StaticProgram P = kStaticPrograms[SIG];
-    T v[3];
-#pragma unroll
-    for (int l = 0; l < P.nl; ++l) {
-        LeafI I = li[l];
-        I.type = P.type[l];
-        v[l] = eval_leafc<T>(lc[l], I, leaves + l, tb, kx, ky, i, j, Nk);
-    }
-    T st[3];
+    T st[3] = {};
     int sp = 0;
 #pragma unroll
     for (int o = 0; o < P.nops; ++o) {
         if (P.code[o] == GSMI_OP_LEAF) {
-            st[sp++] = v[P.arg[o]];
+            st[sp++] = (P.arg[o] == ovl) ? ovv : v[P.arg[o]];
         } else if (P.code[o] == GSMI_OP_MUL) {
             --sp;
             st[sp - 1] = st[sp - 1] * st[sp];
@@ -500,6 +490,25 @@ __device__ __forceinline__ T eval_static(const LeafC<T>* __restrict__ lc,
         }
     }
     return st[0];
+}
+
+// The value of program shape SIG at sample (i, j), its leaf constants in lc[] (LDS) and integer
+// fields in li[] (their type overridden by the compile-time one).
+template <typename T, int SIG>
+__device__ __forceinline__ T eval_static(const LeafC<T>* __restrict__ lc,
+                                         const LeafI (&li)[kProgLeaves],
+                                         const gsmi_leaf* __restrict__ leaves,
+                                         const Tables<T>& tb, T kx, T ky, int i, int j, int Nk)
+{
+    constexpr StaticProgram P = kStaticPrograms[SIG];
+    T v[kProgLeaves] = {};
+#pragma unroll
+    for (int l = 0; l < P.nl; ++l) {
+        LeafI I = li[l];
+        I.type = P.type[l];
+        v[l] = eval_leafc<T>(lc[l], I, leaves + l, tb, kx, ky, i, j, Nk);
+    }
+    return static_ops<T, SIG>(v, -1, T(0));
 }
 
 // Leaf values of a decoded program at sample (i, j).

@@ -253,7 +253,7 @@ row_r2c(const gsmi_stamp* __restrict__ stamps, const T* __restrict__ resid,
 // k-image sample of the group's columns.  grid (ceil((N/2+1)/cpg), nstamps);
 // LDS 2 cpg N complex.  ML = leaf slots with gradient accumulators (2 or 4).
 // ---------------------------------------------------------------------------------------------
-template <typename T, int ML, bool CPLX, bool SHAPE>
+template <typename T, int ML, bool CPLX, bool SHAPE, int SIG>
 __global__ void __launch_bounds__(kThreads)
 grad_colfft(const gsmi_stamp* __restrict__ stamps, const gsmi_op* __restrict__ ops,
             const gsmi_leaf* __restrict__ leaves, Tables<T> tb, const cplx<T>* __restrict__ tw,
@@ -302,9 +302,10 @@ grad_colfft(const gsmi_stamp* __restrict__ stamps, const gsmi_op* __restrict__ o
     // the program decoded into the LDS buffer the transform result does not occupy
     ProgramD d;
     LeafC<T>* lc = reinterpret_cast<LeafC<T>*>(res == bufa ? bufb : bufa);
-    const bool ok = decode_program<T>(sops, nops, sleaves, d, lc) && d.nl <= ML &&
-                    (CPLX || !d.has_phase);
-    const int nl = d.nl;
+    bool ok = decode_program<T>(sops, nops, sleaves, d, lc) && d.nl <= ML &&
+              (CPLX || !d.has_phase);
+    if constexpr (SIG > 0) ok = ok && matches_static<SIG>(sops, nops, sleaves);
+    const int nl = (SIG > 0) ? kStaticPrograms[SIG].nl : d.nl;
     __syncthreads();
 
     T acc[NF];
@@ -335,15 +336,20 @@ grad_colfft(const gsmi_stamp* __restrict__ stamps, const gsmi_op* __restrict__ o
                 for (int l = 0; l < ML; ++l) {
                     phi[l] = dv[l] = T(0);
                     if (l < nl) {
-                        LeafD<T> q = eval_leafc_d<T>(lc[l], d.li[l], sleaves + l, tb, kx, ky, i,
-                                                     j, Nk);
+                        LeafI I = d.li[l];
+                        if constexpr (SIG > 0) I.type = kStaticPrograms[SIG].type[l];
+                        LeafD<T> q = eval_leafc_d<T>(lc[l], I, sleaves + l, tb, kx, ky, i, j,
+                                                     Nk);
                         phi[l] = q.phi;
                         v[l] = lc[l].flux * q.phi;
                         dv[l] = lc[l].flux * q.dphi;
                     }
                 }
                 cplx<T> FZ;
-                if constexpr (CPLX) {
+                if constexpr (SIG > 0) {
+                    const T P = static_ops<T, SIG>(v, -1, T(0));
+                    FZ = {P * Z.x, P * Z.y};
+                } else if constexpr (CPLX) {
                     FZ = cmul(run_ops<T>(d, sops, v, -1, T(0), kx, ky), Z);
                 } else {
                     const T P = run_ops_real<T>(d, sops, v, -1, T(0));
@@ -356,7 +362,10 @@ grad_colfft(const gsmi_stamp* __restrict__ stamps, const gsmi_op* __restrict__ o
                 for (int l = 0; l < ML; ++l) {
                     if (l < nl) {
                         T tx;          // Re[(dP/dv_l) Z]
-                        if constexpr (CPLX) {
+                        if constexpr (SIG > 0) {
+                            tx = (static_ops<T, SIG>(v, l, T(1)) -
+                                  static_ops<T, SIG>(v, l, T(0))) * Z.x;
+                        } else if constexpr (CPLX) {
                             const cplx<T> p1 = run_ops<T>(d, sops, v, l, T(1), kx, ky);
                             const cplx<T> p0 = run_ops<T>(d, sops, v, l, T(0), kx, ky);
                             tx = cmul(csub(p1, p0), Z).x;
@@ -377,7 +386,9 @@ grad_colfft(const gsmi_stamp* __restrict__ stamps, const gsmi_op* __restrict__ o
                         acc[PL * l + 4] += g2 * kxp * ky;
                         acc[PL * l + 5] += g2 * kyp * ky;
                         if constexpr (SHAPE) {
-                            if (d.li[l].type == GSMI_LEAF_MOFFAT) {
+                            const int type = (SIG > 0) ? kStaticPrograms[SIG].type[l]
+                                                       : d.li[l].type;
+                            if (type == GSMI_LEAF_MOFFAT) {
                                 const T kk = sc * dsqrt(kxp * kxp + kyp * kyp);
                                 acc[PL * l + 6] += tx * L.flux *
                                     T(moffat_dg_dnu(sleaves[l].p[0] - 1., double(kk)));
@@ -437,11 +448,44 @@ void launch_grad(dim3 gb, size_t ldsb, hipStream_t s, int max_leaves, const gsmi
 {
     const cplx<T>* sc = reinterpret_cast<const cplx<T>*>(scratch);
     if (max_leaves <= 2)
-        hipLaunchKernelGGL((grad_colfft<T, 2, CPLX, SHAPE>), gb, dim3(kThreads), ldsb, s,
+        hipLaunchKernelGGL((grad_colfft<T, 2, CPLX, SHAPE, 0>), gb, dim3(kThreads), ldsb, s,
                            stamps, ops, leaves, tb, tw, sc, partials);
     else
-        hipLaunchKernelGGL((grad_colfft<T, 4, CPLX, SHAPE>), gb, dim3(kThreads), ldsb, s,
+        hipLaunchKernelGGL((grad_colfft<T, 4, CPLX, SHAPE, 0>), gb, dim3(kThreads), ldsb, s,
                            stamps, ops, leaves, tb, tw, sc, partials);
+}
+
+// The specialised adjoint of program shape sig (phase-free by construction); false if there is
+// no such kernel and the interpreter should run.
+template <typename T>
+bool launch_grad_static(int sig, bool shape, dim3 gb, size_t ldsb, hipStream_t s,
+                        const gsmi_stamp* stamps, const gsmi_op* ops, const gsmi_leaf* leaves,
+                        Tables<T> tb, const cplx<T>* tw, const void* scratch, double* partials)
+{
+    const cplx<T>* sc = reinterpret_cast<const cplx<T>*>(scratch);
+    switch (sig) {
+      case 1:
+        hipLaunchKernelGGL((grad_colfft<T, 4, false, false, 1>), gb, dim3(kThreads), ldsb, s,
+                           stamps, ops, leaves, tb, tw, sc, partials);
+        return true;
+      case 2:
+        if (shape)
+            hipLaunchKernelGGL((grad_colfft<T, 2, false, true, 2>), gb, dim3(kThreads), ldsb, s,
+                               stamps, ops, leaves, tb, tw, sc, partials);
+        else
+            hipLaunchKernelGGL((grad_colfft<T, 2, false, false, 2>), gb, dim3(kThreads), ldsb,
+                               s, stamps, ops, leaves, tb, tw, sc, partials);
+        return true;
+      case 3:
+        hipLaunchKernelGGL((grad_colfft<T, 2, false, false, 3>), gb, dim3(kThreads), ldsb, s,
+                           stamps, ops, leaves, tb, tw, sc, partials);
+        return true;
+      case 4:
+        hipLaunchKernelGGL((grad_colfft<T, 2, false, false, 4>), gb, dim3(kThreads), ldsb, s,
+                           stamps, ops, leaves, tb, tw, sc, partials);
+        return true;
+    }
+    return false;
 }
 
 template <typename T>
@@ -482,7 +526,12 @@ int adjoint_impl(const gsmi_stamp* stamps, int nstamps, int N, int max_ny, int m
     size_t ldsb = 2 * size_t(cpg) * N * sizeof(cplx<T>);
     const bool cx = flags & GSMI_ADJ_COMPLEX;
     const bool sh = flags & GSMI_ADJ_SHAPE;
-    if (cx && sh)
+    const int sig = (flags >> 8) & 0xff;
+    if (!cx && sig > 0 &&
+        launch_grad_static<T>(sig, sh, gb, ldsb, s, stamps, ops, leaves, tb, tw, scratch,
+                              partials)) {
+        // specialised kernel launched
+    } else if (cx && sh)
         launch_grad<T, true, true>(gb, ldsb, s, max_leaves, stamps, ops, leaves, tb, tw,
                                    scratch, partials);
     else if (cx)

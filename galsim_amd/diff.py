@@ -107,7 +107,8 @@ class DiffPlan:
             need = max(need, int(nd[0]))
             members = p.order[first:first + count]
             flags = (_lib.ADJ_COMPLEX if cplx[members].any() else 0) | \
-                (_lib.ADJ_SHAPE if self.shape_gradients else 0)
+                (_lib.ADJ_SHAPE if self.shape_gradients else 0) | \
+                (p.program_shape << 8)
             self._bins.append((N, first, count, max_ny, int(nl[members].max()), flags))
         self._partials = torch.empty(need, dtype=torch.float64, device='cuda')
         self._grad = torch.empty((n, NFIELD), dtype=torch.float64, device='cuda')

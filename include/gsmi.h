@@ -256,8 +256,9 @@ int gsmi_wrap_hermx_f64(const double* kin, int Nk, int N, double* kout, void* st
  * max_leaves bounds the number of leaf slots the programs use (1..GSMI_GRAD_MAXLEAF).  flags:
  * GSMI_ADJ_COMPLEX if any program holds a PHASE op (a shift inside the tree rather than on the
  * whole result), GSMI_ADJ_SHAPE to compute the Moffat beta derivatives (three FP64 Bessel
- * evaluations per Moffat sample).  A stamp whose program breaks a promise, or has more than 12
- * ops, gets NaN gradients.  scratch is the forward's scratch layout (it is overwritten);
+ * evaluations per Moffat sample), and in bits 8-15 a program shape of gsmi_schedule.program
+ * that every stamp has (a specialised kernel; 0 = the interpreter).  A stamp whose program
+ * breaks a promise, or has more than 12 ops, gets NaN gradients.  scratch is the forward's scratch layout (it is overwritten);
  * partials holds gsmi_adjoint_partials() doubles.
  * ------------------------------------------------------------------------------------- */
 #define GSMI_GRAD_MAXLEAF 4
