@@ -359,8 +359,8 @@ def main():
     t0 = time.perf_counter()
     tuning = plan.autotune()
     t_tune = time.perf_counter() - t0
-    log("autotune (%.1fs): cols %s rows %s grid %s -> %.3f ms" % (
-        t_tune, tuning['cols'], tuning['rows'], tuning['grid'], tuning['ms']))
+    log("autotune (%.1fs): cols %s chunks %s rows %s grid %s -> %.3f ms" % (
+        t_tune, tuning['cols'], tuning['chunks'], tuning['rows'], tuning['grid'], tuning['ms']))
     fold_ms, rows_ms = kernel_times(plan, min(args.steps, 10))
     plan.capture()
 
@@ -506,6 +506,7 @@ def main():
                                  "from one HIP graph",
                      "cols_per_group": {str(k): v for k, v in tuning['cols'].items()},
                      "rows_per_group": {str(k): v for k, v in tuning['rows'].items()},
+                     "fold_chunks_per_item": {str(k): v for k, v in tuning['chunks'].items()},
                      "grid": tuning['grid'], "autotune_s": round(t_tune, 2),
                      "per_size_launches_ms_per_step": round(t_bins, 4),
                      "work_list_ms_per_step": round(t_fold + t_rows, 4)},

@@ -90,17 +90,28 @@ def _program_shape(recs, ops, leaves):
     return found or 0
 
 
-def work_lists(recs, groups, cols, rows, elem, bins=None):
+FOLD_ITEM_CELLS = 16384       # default size of a fold work item (columns x N), in cells
+STAGE_LDS = 32768             # LDS per fold workgroup allowed for staging (5 workgroups per CU)
+
+
+def work_lists(recs, groups, cols, rows, elem, bins=None, chunks=None, extra=0):
     """Fold and row work items (gsmi_work rows: stamp, first, count, per_group) of a batch whose
     stamp records ``recs`` are sorted into FFT-size ``groups`` (N, first, count, max_ny), for
-    tiles ``cols`` / ``rows`` (dict N -> columns / rows per workgroup; defaults 1024/N and
-    1024/(N/2)), largest FFT sizes first, and the LDS bytes each list needs (elem: bytes per
-    complex value).  Every column 0..N/2 and every output row of every stamp is in exactly one
-    item."""
+    tiles ``cols`` / ``rows`` (dict N -> columns / rows per workgroup chunk; defaults 1024/N and
+    1024/(N/2)), largest FFT sizes first, and the LDS bytes of a fold and of a row workgroup
+    (elem: bytes per complex value; extra: bytes the fold kernel keeps beside its buffers).
+    Every column 0..N/2 and every output row of every stamp is in exactly one item.
+
+    A fold item of a size whose two transform buffers, N twiddles and 3 Nk/2 + 2 axis factors
+    fit in STAGE_LDS spans ``chunks[N]`` chunks of ``cols[N]`` columns (default: about
+    FOLD_ITEM_CELLS cells; the columns split evenly over the items), which share its set-up and
+    its staged tables; the fold LDS then covers that staging.  Other sizes get one-chunk
+    items."""
     fold, rw = [], []
     groups = sorted(groups, key=lambda g: -g[0])
     if bins is not None:
         groups = [g for g in groups if g[0] in bins]
+    chunks = chunks or {}
     lds_f = lds_r = 0
     for N, first, count, _ in groups:
         cpg = int(cols.get(N, BatchPlan.default_cols(N)))
@@ -108,11 +119,24 @@ def work_lists(recs, groups, cols, rows, elem, bins=None):
         if cpg < 1 or rpg < 1:
             raise ValueError("tiles must hold at least one column and one row")
         W = N // 2 + 1
-        lds_f = max(lds_f, 2 * cpg * N * elem)
+        nk = int(recs['Nk'][first:first + count].max()) if count else 0
+        bufs = 2 * cpg * N * elem + extra
+        staged = bufs + (N + 3 * (nk // 2) + 2) * elem
+        nchunk = -(-W // cpg)
+        if staged <= STAGE_LDS:
+            k = int(chunks.get(N, max(1, FOLD_ITEM_CELLS // (cpg * N))))
+            if k < 1:
+                raise ValueError("a fold item holds at least one chunk")
+            lds_f = max(lds_f, staged)
+        else:
+            k = 1
+            lds_f = max(lds_f, bufs)
+        nitem = -(-nchunk // k)
+        per_item = -(-nchunk // nitem) * cpg            # columns per item, a multiple of cpg
         lds_r = max(lds_r, 2 * rpg * (N // 2) * elem)
         for s in range(first, first + count):
-            for c0 in range(0, W, cpg):
-                fold.append((s, c0, min(cpg, W - c0), cpg))
+            for c0 in range(0, W, per_item):
+                fold.append((s, c0, min(per_item, W - c0), cpg))
             ny = int(recs['ny'][s])
             for r0 in range(0, ny, rpg):
                 rw.append((s, r0, min(rpg, ny - r0), rpg))
@@ -223,15 +247,23 @@ class BatchPlan:
     def _elem(self):
         return 8 if self.precision == 'f32' else 16
 
-    def _work_lists(self, cols, rows, bins=None):
-        return work_lists(self.recs, self.groups, cols, rows, self._elem(), bins)
+    def _fold_extra(self):
+        """LDS bytes the specialised fold kernel of the batch's program shape keeps beside its
+        buffers: 4 decoded leaves of 16 values."""
+        return 4 * 16 * (4 if self.precision == 'f32' else 8) if self.program_shape else 0
 
-    def make_schedule(self, cols=None, rows=None, fold_grid=None, rows_grid=None, bins=None):
+    def _work_lists(self, cols, rows, bins=None, chunks=None):
+        return work_lists(self.recs, self.groups, cols, rows, self._elem(), bins, chunks,
+                          self._fold_extra())
+
+    def make_schedule(self, cols=None, rows=None, fold_grid=None, rows_grid=None, bins=None,
+                      chunks=None):
         """(device item tensor, gsmi_schedule) for tiles ``cols``/``rows`` (dict N -> columns or
-        rows per workgroup; missing sizes take the defaults) and grids (None: one workgroup per
-        item; smaller: persistent workgroups).  ``bins`` restricts it to some FFT sizes."""
+        rows per workgroup chunk; missing sizes take the defaults), fold chunks per item
+        (``chunks``, dict N -> count) and grids (None: one workgroup per item; smaller:
+        persistent workgroups).  ``bins`` restricts it to some FFT sizes."""
         import torch
-        fold, rw, lds_f, lds_r = self._work_lists(cols or {}, rows or {}, bins)
+        fold, rw, lds_f, lds_r = self._work_lists(cols or {}, rows or {}, bins, chunks)
         items = torch.from_numpy(np.ascontiguousarray(np.concatenate([fold, rw]))).cuda()
         sc = _lib.Schedule()
         base = items.data_ptr()
@@ -241,18 +273,14 @@ class BatchPlan:
         sc.fold_grid = len(fold) if fold_grid is None else int(min(fold_grid, len(fold)))
         sc.rows_grid = len(rw) if rows_grid is None else int(min(rows_grid, len(rw)))
         sc.fold_grid, sc.rows_grid = max(sc.fold_grid, 1), max(sc.rows_grid, 1)
-        # the specialised fold kernel of the batch's program shape keeps 4 decoded leaves
-        # (16 values each) in LDS beside the transform buffers
-        extra = 4 * 16 * (4 if self.precision == 'f32' else 8)
-        program = self.program_shape if lds_f + extra <= 65536 else 0
-        sc.fold_lds = lds_f + (extra if program else 0)
+        sc.fold_lds = lds_f
         sc.rows_lds = lds_r
-        sc.program = program
+        sc.program = self.program_shape
         return items, sc
 
-    def use_schedule(self, cols=None, rows=None, fold_grid=None, rows_grid=None):
+    def use_schedule(self, cols=None, rows=None, fold_grid=None, rows_grid=None, chunks=None):
         """Run every FFT size in one fold and one row launch from now on."""
-        self._sched = self.make_schedule(cols, rows, fold_grid, rows_grid)
+        self._sched = self.make_schedule(cols, rows, fold_grid, rows_grid, chunks=chunks)
         self._graph = None
 
     def _launch_list(self, sched, part, s):
@@ -289,10 +317,12 @@ class BatchPlan:
         b.synchronize()
         return a.elapsed_time(b) / reps
 
-    def autotune(self, reps=5, factors=(0.25, 0.5, 1, 2, 4), grids=(None, 1024, 2048, 4096)):
-        """Pick, by timing on this batch, the columns per fold workgroup and rows per row
-        workgroup of every FFT size (powers of two around the defaults, within the 64 KiB LDS
-        limit), then the fold and row grid sizes (one workgroup per item, or persistent), and
+    def autotune(self, reps=5, factors=(0.25, 0.5, 1, 2, 4), grids=(None, 1024, 2048, 4096),
+                 item_cells=(1, 4096, FOLD_ITEM_CELLS, 65536)):
+        """Pick, by timing on this batch, the columns per fold chunk and the rows per row
+        workgroup of every FFT size (powers of two around the defaults, within the LDS budget),
+        then the size of a fold item (``item_cells``: cells per item, 1 = one chunk) and the fold
+        and row grid sizes (one workgroup per item, or persistent) on the whole launch, and
         switch run() to the tuned work lists.  Returns the choice and its timings (ms)."""
         import torch
         s = torch.cuda.current_stream().cuda_stream
@@ -303,7 +333,9 @@ class BatchPlan:
             best = None
             for f in factors:
                 c = int(self.default_cols(N) * f)
-                if c < 1 or c > W or 2 * c * N * elem > 65536:
+                # the launch's LDS is the largest size's: keep tiles within the staging budget
+                if c < 1 or c > W or 2 * c * N * elem + self._fold_extra() > \
+                        (65536 if c == 1 else STAGE_LDS):
                     continue
                 sched = self.make_schedule({N: c}, {}, bins=(N,))
                 t = self._time(lambda: self._launch_list(sched, 'fold', s), reps)
@@ -324,15 +356,26 @@ class BatchPlan:
                 if best is None or t < best[0]:
                     best = (t, r)
             rows[N] = best[1]
+        # fold item size (cells per item, the same for every FFT size), on the whole launch
+        best = None
+        for cells in item_cells:
+            ck = {N: max(1, cells // (cols[N] * N)) for N, _, _, _ in self.groups}
+            sched = self.make_schedule(cols, rows, chunks=ck)
+            t = self._time(lambda: self._launch_list(sched, 'fold', s), reps)
+            log[('item_cells', cells)] = t
+            if best is None or t < best[0]:
+                best = (t, ck)
+        chunks = best[1]
         best = None
         for g in grids:
-            sched = self.make_schedule(cols, rows, g, g)
+            sched = self.make_schedule(cols, rows, g, g, chunks=chunks)
             t = self._time(lambda: self._launch_list(sched, 'all', s), reps)
             log[('grid', g)] = t
             if best is None or t < best[0]:
                 best = (t, g)
-        self.use_schedule(cols, rows, best[1], best[1])
-        self.tuning = {'cols': cols, 'rows': rows, 'grid': best[1], 'ms': best[0],
+        self.use_schedule(cols, rows, best[1], best[1], chunks)
+        self.tuning = {'cols': cols, 'rows': rows, 'chunks': chunks, 'grid': best[1],
+                       'ms': best[0],
                        'timings': {str(k): v for k, v in log.items()}}
         return self.tuning
 

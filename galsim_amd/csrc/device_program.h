@@ -661,7 +661,7 @@ __device__ inline void bfly3(cplx<T>* v)
 }
 
 template <typename T, int R>
-__device__ inline void stockham_pass(const cplx<T>* src, cplx<T>* dst, int n, int nseg, int Ns,
+__device__ __forceinline__ void stockham_pass(const cplx<T>* src, cplx<T>* dst, int n, int nseg, int Ns,
                                      const cplx<T>* tw, int twstride)
 {
     const int nb = n / R;
@@ -691,7 +691,7 @@ __device__ inline void stockham_pass(const cplx<T>* src, cplx<T>* dst, int n, in
 }
 
 template <typename T>
-__device__ cplx<T>* fft_lds(cplx<T>* a, cplx<T>* b, int n, int nseg, const cplx<T>* tw,
+__device__ __forceinline__ cplx<T>* fft_lds(cplx<T>* a, cplx<T>* b, int n, int nseg, const cplx<T>* tw,
                             int twstride)
 {
     int Ns = 1;

@@ -121,77 +121,130 @@ __device__ __forceinline__ void fold_cells_static(const LeafC<T>* __restrict__ l
 // Kernel 1: fused k-space evaluation + product/sinc/phase + Hermitian wrap + column inverse DFT
 // grid: (ceil((N/2+1)/cpg), nstamps); dynamic LDS: 2 * cpg * N complex
 // ---------------------------------------------------------------------------------------------
-template <typename T, int SIG = 0>
-__device__ __forceinline__ void fold_item(const gsmi_stamp* __restrict__ st, int c0, int ncol,
-                                          int cpg, const gsmi_op* __restrict__ ops,
-                                          const gsmi_leaf* __restrict__ leaves,
-                                          const Tables<T>& tb, const cplx<T>* __restrict__ tw,
-                                          int tws, cplx<T>* __restrict__ scratch, cplx<T>* lds)
+
+// Columns [first, first + count) of one stamp in chunks of at most cpg: each chunk's cells are
+// evaluated into bufa, transformed along ky in LDS (twiddles twp, stride twstride) and the rows
+// of the output window stored.  STATIC: the program has the compile-time shape SIG (leaf
+// constants lc); otherwise the interpreter runs.  X holds the axis factors X(kx), kx in [0, hk],
+// then Y(ky), ky in [-hk, hk].
+template <typename T, int SIG, bool STATIC>
+__device__ __forceinline__ void fold_chunks(const gsmi_stamp* __restrict__ st, int first,
+                                            int count, int cpg, const LeafC<T>* lc,
+                                            const LeafI (&li)[kProgLeaves],
+                                            const gsmi_op* __restrict__ sops, int nops,
+                                            const gsmi_leaf* __restrict__ sleaves,
+                                            const Tables<T>& tb, const cplx<T>* X,
+                                            const cplx<T>* twp, int twstride,
+                                            cplx<T>* __restrict__ scratch, cplx<T>* bufa,
+                                            cplx<T>* bufb)
 {
     const int N = st->N;
     const int Nk = st->Nk;
-    const int half = N / 2;
+    const int hk = Nk / 2;
     const T dk = T(st->dk);
+    cplx<T>* out = scratch + st->scratch_offset;
+    const int W = N / 2 + 1;
+    const int ny = st->ny;
+    const int y0 = st->y0;
+    const float inv_n = 1.f / float(N);
+    for (int c0 = first; c0 < first + count; c0 += cpg) {
+        const int ncol = min(cpg, first + count - c0);
+        if constexpr (STATIC)
+            fold_cells_static<T, SIG>(lc, li, sleaves, tb, X, X + 2 * hk + 1, dk, N, Nk, c0,
+                                      ncol, bufa);
+        else
+            fold_cells<T>(sops, nops, sleaves, tb, X, X + 2 * hk + 1, dk, N, Nk, c0, ncol, bufa);
+        __syncthreads();
+        cplx<T>* res = fft_lds<T>(bufa, bufb, N, ncol, twp, twstride);
+        // rows of the output window: position p <-> DFT index p mod N
+        const float inv_ncol = 1.f / float(ncol);
+        for (int e = threadIdx.x; e < ncol * ny; e += blockDim.x) {
+            const int row = fdiv(e, inv_ncol);
+            const int cl = e - row * ncol;
+            const int p = y0 + row + N;               // >= 0: y0 >= -N/2
+            const int y = p - N * fdiv(p, inv_n);
+            out[row * W + c0 + cl] = res[cl * N + y];
+        }
+        __syncthreads();                          // bufa / bufb are reused by the next chunk
+    }
+}
+
+// One fold work item: columns [first, first + count) of stamp st, cpg columns at a time, in the
+// workgroup's LDS of lds_elems complex values.  Set-up is done once per item: the program check
+// and its leaf constants (shape SIG), and, when they fit after the two transform buffers, the
+// stamp's axis factors and the N twiddles in LDS; otherwise a one-chunk item stages the axis
+// factors in bufb (free until the column DFT), and the rest read them from global memory.
+template <typename T, int SIG = 0>
+__device__ __forceinline__ void fold_item(const gsmi_stamp* __restrict__ st, int first,
+                                          int count, int cpg, const gsmi_op* __restrict__ ops,
+                                          const gsmi_leaf* __restrict__ leaves,
+                                          const Tables<T>& tb, const cplx<T>* __restrict__ tw,
+                                          int tws, cplx<T>* __restrict__ scratch, cplx<T>* lds,
+                                          size_t lds_elems)
+{
+    const int N = st->N;
     const gsmi_op* sops = ops + st->op_begin;
     const int nops = st->op_count;
     const gsmi_leaf* sleaves = leaves + st->leaf_begin;
-    const int hk = Nk / 2;
-    const cplx<T>* X = scratch + st->axis_offset;     // X[kx], kx in [0, hk]; then Y[ky]
+    const int hk = st->Nk / 2;
+    const cplx<T>* Xg = scratch + st->axis_offset;    // X[kx], kx in [0, hk]; then Y[ky]
     cplx<T>* bufa = lds;
     cplx<T>* bufb = lds + cpg * N;
     const int naxis = 3 * hk + 2;
+    size_t used = size_t(2) * cpg * N;
+    LeafC<T>* lc = reinterpret_cast<LeafC<T>*>(lds + used);
+    LeafI li[kProgLeaves];
     bool specialised = false;
     if constexpr (SIG > 0) {
-        // the program of shape SIG: its leaf constants converted once into LDS after bufb
+        used += kProgLeaves * sizeof(LeafC<T>) / sizeof(cplx<T>);
         specialised = matches_static<SIG>(sops, nops, sleaves);
         if (specialised) {
-            LeafC<T>* lc = reinterpret_cast<LeafC<T>*>(bufb + cpg * N);
-            LeafI li[kProgLeaves];
 #pragma unroll
             for (int l = 0; l < kProgLeaves; ++l)
                 li[l] = (l < kStaticPrograms[SIG].nl) ? leaf_ints(sleaves[l])
                                                       : LeafI{0, 0, 0, -1, 0};
             if (int(threadIdx.x) < kStaticPrograms[SIG].nl)
                 lc[threadIdx.x] = leaf_consts<T>(sleaves[threadIdx.x]);
-            if (naxis <= cpg * N) {
-                for (int e = threadIdx.x; e < naxis; e += blockDim.x) bufb[e] = X[e];
-                __syncthreads();
-                fold_cells_static<T, SIG>(lc, li, sleaves, tb, bufb, bufb + 2 * hk + 1, dk, N,
-                                          Nk, c0, ncol, bufa);
-            } else {
-                __syncthreads();
-                fold_cells_static<T, SIG>(lc, li, sleaves, tb, X, X + 2 * hk + 1, dk, N, Nk,
-                                          c0, ncol, bufa);
-            }
         }
     }
-    if (!specialised) {
-        if (naxis <= cpg * N) {
-            // the axis factors staged in bufb, which the column DFT needs only afterwards
-            for (int e = threadIdx.x; e < naxis; e += blockDim.x) bufb[e] = X[e];
-            __syncthreads();
-            fold_cells<T>(sops, nops, sleaves, tb, bufb, bufb + 2 * hk + 1, dk, N, Nk, c0, ncol,
-                          bufa);
-        } else {
-            fold_cells<T>(sops, nops, sleaves, tb, X, X + 2 * hk + 1, dk, N, Nk, c0, ncol, bufa);
-        }
+    cplx<T>* twl = lds + used;
+    cplx<T>* axl = twl + N;
+    if (used + N + naxis <= lds_elems) {
+        for (int e = threadIdx.x; e < N; e += blockDim.x) twl[e] = tw[e * tws];
+        for (int e = threadIdx.x; e < naxis; e += blockDim.x) axl[e] = Xg[e];
+        __syncthreads();
+        if constexpr (SIG > 0)
+            if (specialised) {
+                fold_chunks<T, SIG, true>(st, first, count, cpg, lc, li, sops, nops, sleaves, tb,
+                                          axl, twl, 1, scratch, bufa, bufb);
+                return;
+            }
+        fold_chunks<T, SIG, false>(st, first, count, cpg, lc, li, sops, nops, sleaves, tb, axl,
+                                   twl, 1, scratch, bufa, bufb);
+        return;
+    }
+    if (count <= cpg && naxis <= cpg * N) {
+        for (int e = threadIdx.x; e < naxis; e += blockDim.x) bufb[e] = Xg[e];
+        __syncthreads();
+        if constexpr (SIG > 0)
+            if (specialised) {
+                fold_chunks<T, SIG, true>(st, first, count, cpg, lc, li, sops, nops, sleaves, tb,
+                                          bufb, tw, tws, scratch, bufa, bufb);
+                return;
+            }
+        fold_chunks<T, SIG, false>(st, first, count, cpg, lc, li, sops, nops, sleaves, tb, bufb,
+                                   tw, tws, scratch, bufa, bufb);
+        return;
     }
     __syncthreads();
-    cplx<T>* res = fft_lds<T>(bufa, bufb, N, ncol, tw, tws);
-    // rows of the output window: position p <-> DFT index p mod N
-    cplx<T>* out = scratch + st->scratch_offset;
-    const int W = half + 1;
-    const int ny = st->ny;
-    const int y0 = st->y0;
-    const float inv_ncol = 1.f / float(ncol);
-    const float inv_n = 1.f / float(N);
-    for (int e = threadIdx.x; e < ncol * ny; e += blockDim.x) {
-        const int row = fdiv(e, inv_ncol);
-        const int cl = e - row * ncol;
-        const int p = y0 + row + N;                   // >= 0: y0 >= -N/2
-        const int y = p - N * fdiv(p, inv_n);
-        out[row * W + c0 + cl] = res[cl * N + y];
-    }
+    if constexpr (SIG > 0)
+        if (specialised) {
+            fold_chunks<T, SIG, true>(st, first, count, cpg, lc, li, sops, nops, sleaves, tb, Xg,
+                                      tw, tws, scratch, bufa, bufb);
+            return;
+        }
+    fold_chunks<T, SIG, false>(st, first, count, cpg, lc, li, sops, nops, sleaves, tb, Xg, tw,
+                               tws, scratch, bufa, bufb);
 }
 
 // One stamp per blockIdx.y, column group blockIdx.x (all stamps of the launch share N).
@@ -211,7 +264,7 @@ __device__ __forceinline__ void fold_colfft_body(const gsmi_stamp* __restrict__ 
     const int ncol = min(cpg, N / 2 + 1 - c0);
     if (ncol <= 0) return;
     fold_item<T>(st, c0, ncol, cpg, ops, leaves, tb, tw, 1, scratch,
-                 reinterpret_cast<cplx<T>*>(smem_raw));
+                 reinterpret_cast<cplx<T>*>(smem_raw), size_t(2) * cpg * N);
 }
 
 // LDS a work-list fold launch needs beyond its two transform buffers (the decoded leaf constants
@@ -220,8 +273,10 @@ template <typename T>
 constexpr size_t kFoldLdsExtra = kProgLeaves * sizeof(LeafC<T>);
 
 // Work-list form: every FFT size in one launch.  Workgroup b takes items b, b + grid, ...
-// (persistent when the grid is smaller than the list); items are validated against the stamp
-// records and the launch's LDS so a malformed list skips items instead of overrunning memory.
+// (persistent when the grid is smaller than the list); an item (stamp, first column, columns,
+// columns per chunk) may span several chunks, which share its set-up.  Items are validated
+// against the stamp records and the launch's LDS so a malformed list skips items instead of
+// overrunning memory.
 template <typename T, int SIG>
 __device__ __forceinline__ void fold_list_body(const gsmi_stamp* __restrict__ stamps, int nstamps,
                                                const gsmi_work* __restrict__ items, int nitems,
@@ -238,14 +293,14 @@ __device__ __forceinline__ void fold_list_body(const gsmi_stamp* __restrict__ st
         const gsmi_stamp* st = stamps + it.stamp;
         const int N = st->N;
         if (it.count <= 0 || it.first < 0 || it.first + it.count > N / 2 + 1 ||
-            it.per_group < it.count ||
+            it.per_group <= 0 ||
             size_t(2) * it.per_group * N * sizeof(cplx<T>) + (SIG > 0 ? kFoldLdsExtra<T> : 0) >
                 size_t(lds_bytes))
             continue;
         int stride;
         const cplx<T>* tw = tws.pick(N, &stride);
         fold_item<T, SIG>(st, it.first, it.count, it.per_group, ops, leaves, tb, tw, stride,
-                          scratch, lds);
+                          scratch, lds, size_t(lds_bytes) / sizeof(cplx<T>));
         __syncthreads();
     }
 }

@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 from galsim_amd import _lib
-from galsim_amd.batch import BatchPlan, work_lists
+from galsim_amd.batch import STAGE_LDS, BatchPlan, work_lists
 
 
 def _recs(spec):
@@ -20,18 +20,28 @@ def _recs(spec):
     return recs, groups
 
 
-@pytest.mark.parametrize("cols,rows", [({}, {}), ({128: 3, 768: 1}, {128: 5, 384: 7})])
-def test_work_lists_cover_every_column_and_row_once(cols, rows):
+@pytest.mark.parametrize("cols,rows,chunks", [
+    ({}, {}, None), ({128: 3, 768: 1}, {128: 5, 384: 7}, None),
+    ({}, {}, {128: 1, 192: 1, 384: 1, 768: 1}),
+    ({128: 3}, {}, {128: 4, 192: 1000, 768: 7})])
+def test_work_lists_cover_every_column_and_row_once(cols, rows, chunks):
     recs, groups = _recs([(128, 48), (128, 128), (192, 77), (384, 200), (768, 9)])
-    fold, rw, lds_f, lds_r = work_lists(recs, groups, cols, rows, 8)
+    recs['Nk'] = 2 * recs['N']
+    fold, rw, lds_f, lds_r = work_lists(recs, groups, cols, rows, 8, chunks=chunks)
     for s in range(len(recs)):
         N, ny = int(recs['N'][s]), int(recs['ny'][s])
         f = fold[fold[:, 0] == s]
         seen = np.zeros(N // 2 + 1, int)
+        staged = (2 * int(f[0, 3]) * N + N + 3 * N + 2) * 8 <= STAGE_LDS
+        k = (chunks or {}).get(N) if staged else 1
         for _, c0, cnt, per in f:
-            assert 1 <= cnt <= per
+            assert cnt >= 1 and per >= 1
+            if k is not None:
+                assert cnt <= k * per                     # at most k chunks per item
             seen[c0:c0 + cnt] += 1
         assert np.all(seen == 1)
+        # items split the columns evenly: every item but the last spans a whole number of chunks
+        assert all(cnt % per == 0 for _, _, cnt, per in f[:-1])
         r = rw[rw[:, 0] == s]
         seen = np.zeros(ny, int)
         for _, r0, cnt, per in r:
@@ -40,7 +50,11 @@ def test_work_lists_cover_every_column_and_row_once(cols, rows):
         assert np.all(seen == 1)
     Ns = recs['N'][fold[:, 0]]
     assert np.all(np.diff(Ns) <= 0)                       # largest FFT sizes first
-    assert lds_f == max(2 * per * int(recs['N'][s]) * 8 for s, _, _, per in fold)
+    need = []
+    for (_, _, _, per), N in zip(fold, recs['N'][fold[:, 0]]):
+        staged = (2 * per * N + N + 3 * N + 2) * 8
+        need.append(staged if staged <= STAGE_LDS else 2 * per * N * 8)
+    assert lds_f == max(need)
     assert lds_r == max(2 * per * (int(recs['N'][s]) // 2) * 8 for s, _, _, per in rw)
 
 
