@@ -96,19 +96,23 @@ typedef struct gsmi_stamp {
 } gsmi_stamp;
 
 /* A work item of the work-list kernels: a group of columns (fold) or rows (row transform) of
- * one stamp.  per_group sizes the workgroup's LDS tile (columns or rows per workgroup); count <=
- * per_group is how many this item holds. */
+ * one stamp.  per_group sizes the workgroup's LDS tile (columns or rows per chunk).  A row item
+ * holds count <= per_group rows; a fold item may hold more columns than per_group, which the
+ * workgroup takes in chunks of per_group after one set-up of the stamp (program, leaf
+ * constants, axis factors and twiddles, staged in LDS when fold_lds has room for them). */
 typedef struct gsmi_work {
     int32_t stamp;        /* index into the stamps array                                      */
     int32_t first;        /* first column (c = 0..N/2) or first output row                    */
     int32_t count;        /* columns or rows in this item                                      */
-    int32_t per_group;    /* tile width the item was cut with                                  */
+    int32_t per_group;    /* columns or rows per chunk                                         */
 } gsmi_work;
 
 /* A whole batch -- every FFT size -- as two work lists, run by one launch each.  Lists are
  * device arrays; a grid smaller than the list makes the kernel persistent (each workgroup takes
  * items b, b + grid, ...).  lds bytes must cover 2 per_group N (fold) and 2 per_group N/2 (rows)
- * complex values of every item; items that do not fit are skipped. */
+ * complex values of every item; items that do not fit are skipped.  A fold item stages its
+ * stamp's N twiddles and 3 Nk/2 + 2 axis factors in LDS when fold_lds also covers them (after
+ * the decoded leaves of a specialised program), and otherwise reads them from memory. */
 typedef struct gsmi_schedule {
     const gsmi_work* fold;
     const gsmi_work* rows;
