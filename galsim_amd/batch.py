@@ -253,8 +253,18 @@ class BatchPlan:
         return 4 * 16 * (4 if self.precision == 'f32' else 8) if self.program_shape else 0
 
     def _work_lists(self, cols, rows, bins=None, chunks=None):
+        """work_lists with the specialised kernel's LDS extra, or, if a tile leaves no room for
+        it, without (the batch then runs the interpreter kernel).  Returns the lists, the LDS
+        sizes and the program shape to launch."""
+        extra = self._fold_extra()
+        try:
+            return work_lists(self.recs, self.groups, cols, rows, self._elem(), bins, chunks,
+                              extra) + (self.program_shape,)
+        except _lib.GalSimAMDError:
+            if not extra:
+                raise
         return work_lists(self.recs, self.groups, cols, rows, self._elem(), bins, chunks,
-                          self._fold_extra())
+                          0) + (0,)
 
     def make_schedule(self, cols=None, rows=None, fold_grid=None, rows_grid=None, bins=None,
                       chunks=None):
@@ -263,7 +273,8 @@ class BatchPlan:
         (``chunks``, dict N -> count) and grids (None: one workgroup per item; smaller:
         persistent workgroups).  ``bins`` restricts it to some FFT sizes."""
         import torch
-        fold, rw, lds_f, lds_r = self._work_lists(cols or {}, rows or {}, bins, chunks)
+        fold, rw, lds_f, lds_r, program = self._work_lists(cols or {}, rows or {}, bins,
+                                                           chunks)
         items = torch.from_numpy(np.ascontiguousarray(np.concatenate([fold, rw]))).cuda()
         sc = _lib.Schedule()
         base = items.data_ptr()
@@ -275,7 +286,7 @@ class BatchPlan:
         sc.fold_grid, sc.rows_grid = max(sc.fold_grid, 1), max(sc.rows_grid, 1)
         sc.fold_lds = lds_f
         sc.rows_lds = lds_r
-        sc.program = self.program_shape
+        sc.program = program
         return items, sc
 
     def use_schedule(self, cols=None, rows=None, fold_grid=None, rows_grid=None, chunks=None):

@@ -48,9 +48,9 @@ __device__ __noinline__ double moffat_kv_bessel(int kind, double nu, double k, d
 template <typename T>
 struct LeafC {
     T j0, j1, j2, j3;     // composite Jacobian (dudx, dudy, dvdx, dvdy)
-    T s, s2, flux, pad;   // scale, Box height scale, flux (Moffat: knorm)
+    T s, s2, flux, t0;    // scale, Box height scale, flux (Moffat: knorm); t0..t2: Sersic
     T p[6];               // type-specific constants
-    T pad2, pad3;
+    T t1, t2;             // Sersic: t0, t1 = first and last log-k knot, t2 = 1 / knot spacing
 };
 
 struct LeafI {
@@ -68,7 +68,7 @@ __device__ inline LeafC<T> leaf_consts(const gsmi_leaf& L)
     c.s = T(L.scale);
     c.s2 = T(L.scale2);
     c.flux = T(L.flux);
-    c.pad = c.pad2 = c.pad3 = T(0);
+    c.t0 = c.t1 = c.t2 = T(0);
 #pragma unroll
     for (int k = 0; k < 6; ++k) c.p[k] = T(L.p[k]);
     return c;
@@ -127,6 +127,66 @@ template <>
 __device__ inline const float* sersic_table<float>(const Tables<float>& tb, int off)
 {
     return tb.tab32 + off;
+}
+
+// A wave-uniform value moved to a scalar register (the compiler cannot prove LDS loads uniform).
+__device__ __forceinline__ float uniform(float x)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, x)));
+}
+__device__ __forceinline__ double uniform(double x)
+{
+    const long long b = __builtin_bit_cast(long long, x);
+    const int lo = __builtin_amdgcn_readfirstlane(int(b & 0xffffffffll));
+    const int hi = __builtin_amdgcn_readfirstlane(int(b >> 32));
+    return __builtin_bit_cast(double, (static_cast<long long>(hi) << 32) |
+                                          static_cast<long long>(static_cast<unsigned>(lo)));
+}
+
+// A decoded leaf copied out of LDS into scalar registers: the sample loops then read its
+// constants as instruction operands instead of reloading them from LDS.
+template <typename T>
+__device__ __forceinline__ LeafC<T> uniform_leaf(const LeafC<T>& in)
+{
+    LeafC<T> out;
+    out.j0 = uniform(in.j0); out.j1 = uniform(in.j1); out.j2 = uniform(in.j2);
+    out.j3 = uniform(in.j3); out.s = uniform(in.s); out.s2 = uniform(in.s2);
+    out.flux = uniform(in.flux); out.t0 = uniform(in.t0);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) out.p[k] = uniform(in.p[k]);
+    out.t1 = uniform(in.t1); out.t2 = uniform(in.t2);
+    return out;
+}
+
+// leaf_consts plus, for a Sersic leaf, the ends and the inverse spacing of its log-k knots in the
+// arithmetic type (so the FP32 lookup needs neither those loads nor a division per sample).
+template <typename T>
+__device__ inline LeafC<T> leaf_consts_t(const gsmi_leaf& L, const Tables<T>& tb)
+{
+    LeafC<T> c = leaf_consts<T>(L);
+    if (L.type == GSMI_LEAF_SERSIC && L.ntab > 1) {
+        const T* args = sersic_table<T>(tb, L.table);
+        c.t0 = args[0];
+        c.t1 = args[L.ntab - 1];
+        c.t2 = T(L.ntab - 1) / (c.t1 - c.t0);
+    }
+    return c;
+}
+
+// spline_eq in FP32 with the knot range and inverse spacing precomputed (leaf_consts_t).
+__device__ inline float spline_eq32(const float* __restrict__ args, int n, float lk, float a0,
+                                    float alast, float inv_da)
+{
+    if (lk < a0 || lk > alast) return 0.f;
+    int i = int(ceilf((lk - a0) * inv_da));
+    i = min(max(i, 1), n - 1);
+    const float* vals = args + n;
+    const float* y2 = vals + n;
+    const float h = args[i] - args[i - 1];
+    const float aa = args[i] - lk;
+    const float bb = h - aa;
+    const float cc = (aa + h) * y2[i - 1] + (bb + h) * y2[i];
+    return ddiv(aa * vals[i - 1] + bb * vals[i] - (1.f / 6.f) * aa * bb * cc, h);
 }
 
 // Exponential clipping: FP64 uses the row ranges the host computed as GetKValueRange1d/2d
@@ -195,6 +255,10 @@ __device__ inline T eval_leafc(const LeafC<T>& C, const LeafI& I, const gsmi_lea
         T ksq = X * X + Y * Y;
         if (exp_clip<T>(C, I, tb, i, j, Nk, ksq)) return T(0);
         T q = T(1) + ksq;
+        if constexpr (sizeof(T) == 4) {
+            const float r = __builtin_amdgcn_rsqf(q);         // q^(-3/2) = r^3
+            return flux * r * r * r;
+        }
         return ddiv(flux, q * dsqrt(q));
       }
       case GSMI_LEAF_GAUSSIAN: {
@@ -233,11 +297,26 @@ __device__ inline T eval_leafc(const LeafC<T>& C, const LeafI& I, const gsmi_lea
       case GSMI_LEAF_KOLMOGOROV: {
         // SBKolmogorov::fillKImage, KolmogorovInfo::kValue (src/SBKolmogorov.cpp:185-259)
         T ksq = X * X + Y * Y;
+        if constexpr (sizeof(T) == 4) {
+            // in base 2: ksq^(5/6) = 2^(5/6 log2 ksq), e^-t = 2^(-t log2 e); ksq = 0 gives 1
+            const float t = __builtin_amdgcn_exp2f((5.f / 6.f) * __builtin_amdgcn_logf(ksq));
+            return flux * __builtin_amdgcn_exp2f(-1.44269504088896341f * t);
+        }
         return flux * dexp(-dexp(T(5. / 6.) * dlog(ksq)));
       }
       case GSMI_LEAF_SERSIC: {
         // SersicInfo::kValue (src/SBSersic.cpp:363-376)
         T ksq = X * X + Y * Y;
+        if constexpr (sizeof(T) == 4) {
+            if (ksq >= C.p[1]) {
+                const float r = __builtin_amdgcn_rsqf(ksq);
+                return flux * (C.p[2] + C.p[3] * r) * (r * r);
+            }
+            if (ksq < C.p[0]) return flux * (1.f + ksq * (C.p[4] + ksq * C.p[5]));
+            const float lk = 0.34657359027997264f * __builtin_amdgcn_logf(ksq);   // ln(ksq)/2
+            return ddiv(flux * spline_eq32(sersic_table<float>(tb, I.aux), I.ntab, lk, C.t0,
+                                           C.t1, C.t2), ksq);
+        }
         if (ksq >= C.p[1]) return ddiv(flux * (C.p[2] + ddiv(C.p[3], dsqrt(ksq))), ksq);
         if (ksq < C.p[0]) return flux * (T(1) + ksq * (C.p[4] + ksq * C.p[5]));
         T lk = T(0.5) * dlog(ksq);
@@ -262,7 +341,7 @@ template <typename T>
 __device__ inline T eval_leaf(const gsmi_leaf* __restrict__ Lp, const Tables<T>& tb, T kx, T ky,
                               int i, int j, int Nk)
 {
-    return eval_leafc<T>(leaf_consts<T>(*Lp), leaf_ints(*Lp), Lp, tb, kx, ky, i, j, Nk);
+    return eval_leafc<T>(leaf_consts_t<T>(*Lp, tb), leaf_ints(*Lp), Lp, tb, kx, ky, i, j, Nk);
 }
 
 // A small stack held in registers: push/pop shift every slot, so all indices are static.
@@ -317,8 +396,8 @@ __device__ inline int program_leaves(const gsmi_op* __restrict__ ops, int nops)
 // caller synchronises before reading lc.  Returns false if the program does not fit.
 template <typename T>
 __device__ inline bool decode_program(const gsmi_op* __restrict__ ops, int nops,
-                                      const gsmi_leaf* __restrict__ leaves, ProgramD& d,
-                                      LeafC<T>* lc)
+                                      const gsmi_leaf* __restrict__ leaves, const Tables<T>& tb,
+                                      ProgramD& d, LeafC<T>* lc)
 {
     const int nl = program_leaves(ops, nops);
     d.nl = nl;
@@ -333,7 +412,7 @@ __device__ inline bool decode_program(const gsmi_op* __restrict__ ops, int nops,
 #pragma unroll
     for (int l = 0; l < kProgLeaves; ++l)
         d.li[l] = (l < nl) ? leaf_ints(leaves[l]) : LeafI{0, 0, 0, -1, 0};
-    if (int(threadIdx.x) < nl) lc[threadIdx.x] = leaf_consts<T>(leaves[threadIdx.x]);
+    if (int(threadIdx.x) < nl) lc[threadIdx.x] = leaf_consts_t<T>(leaves[threadIdx.x], tb);
     return true;
 }
 

@@ -358,7 +358,7 @@ grad_colfft(const gsmi_stamp* __restrict__ stamps, const gsmi_op* __restrict__ o
     // the program decoded into the LDS buffer the transform result does not occupy
     ProgramD d;
     LeafC<T>* lc = reinterpret_cast<LeafC<T>*>(res == bufa ? bufb : bufa);
-    bool ok = decode_program<T>(sops, nops, sleaves, d, lc) && d.nl <= ML &&
+    bool ok = decode_program<T>(sops, nops, sleaves, tb, d, lc) && d.nl <= ML &&
               (CPLX || !d.has_phase);
     if constexpr (SIG > 0) ok = ok && matches_static<SIG>(sops, nops, sleaves);
     const int nl = (SIG > 0) ? kStaticPrograms[SIG].nl : d.nl;

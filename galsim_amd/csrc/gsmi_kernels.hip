@@ -94,6 +94,9 @@ __device__ __forceinline__ void fold_cells_static(const LeafC<T>* __restrict__ l
 {
     const int hk = Nk / 2;
     const float inv_n = 1.f / float(N);
+    LeafC<T> lr[kProgLeaves];
+#pragma unroll
+    for (int l = 0; l < kStaticPrograms[SIG].nl; ++l) lr[l] = uniform_leaf(lc[l]);
     for (int e = threadIdx.x; e < ncol * N; e += blockDim.x) {
         const int cl = fdiv(e, inv_n);
         const int r = e - cl * N;
@@ -106,7 +109,7 @@ __device__ __forceinline__ void fold_cells_static(const LeafC<T>* __restrict__ l
                 const bool neg = kx < 0;
                 const int i = neg ? -kx : kx;
                 const int j = neg ? -ky : ky;
-                const T P = eval_static<T, SIG>(lc, li, sleaves, tb, T(i) * dk, T(j) * dk, i, j,
+                const T P = eval_static<T, SIG>(lr, li, sleaves, tb, T(i) * dk, T(j) * dk, i, j,
                                                 Nk);
                 const cplx<T> xy = cmul(X[i], Y[j]);
                 acc.x += P * xy.x;
@@ -204,7 +207,7 @@ __device__ __forceinline__ void fold_item(const gsmi_stamp* __restrict__ st, int
                 li[l] = (l < kStaticPrograms[SIG].nl) ? leaf_ints(sleaves[l])
                                                       : LeafI{0, 0, 0, -1, 0};
             if (int(threadIdx.x) < kStaticPrograms[SIG].nl)
-                lc[threadIdx.x] = leaf_consts<T>(sleaves[threadIdx.x]);
+                lc[threadIdx.x] = leaf_consts_t<T>(sleaves[threadIdx.x], tb);
         }
     }
     cplx<T>* twl = lds + used;

@@ -119,6 +119,7 @@ def test_work_list_skips_malformed_items():
     sc2.nfold, sc2.nrows = len(fold), len(rows)
     sc2.fold_grid, sc2.rows_grid = len(fold), len(rows)
     sc2.fold_lds, sc2.rows_lds = sc.fold_lds, sc.rows_lds
+    sc2.program = sc.program                        # the same (specialised) fold kernel
     plan._sched = (allitems, sc2)
     plan.out.zero_()
     plan.run()
