@@ -189,6 +189,28 @@ __device__ inline float spline_eq32(const float* __restrict__ args, int n, float
     return ddiv(aa * vals[i - 1] + bb * vals[i] - (1.f / 6.f) * aa * bb * cc, h);
 }
 
+// spline_eq32 with the derivative dS/dlk (spline_eq's formula).
+__device__ inline float spline_eq32_d(const float* __restrict__ args, int n, float lk, float a0,
+                                      float alast, float inv_da, float* dS)
+{
+    if (lk < a0 || lk > alast) {
+        *dS = 0.f;
+        return 0.f;
+    }
+    int i = int(ceilf((lk - a0) * inv_da));
+    i = min(max(i, 1), n - 1);
+    const float* vals = args + n;
+    const float* y2 = vals + n;
+    const float h = args[i] - args[i - 1];
+    const float ih = __builtin_amdgcn_rcpf(h);
+    const float aa = args[i] - lk;
+    const float bb = h - aa;
+    const float cc = (aa + h) * y2[i - 1] + (bb + h) * y2[i];
+    *dS = (vals[i] - vals[i - 1] - (1.f / 6.f) * ((aa - bb) * cc + aa * bb * (y2[i] - y2[i - 1])))
+          * ih;
+    return (aa * vals[i - 1] + bb * vals[i] - (1.f / 6.f) * aa * bb * cc) * ih;
+}
+
 // Exponential clipping: FP64 uses the row ranges the host computed as GetKValueRange1d/2d
 // (src/SBProfile.cpp:508-588) for the stored k-image rows; FP32 clips at ksq > ksq_max.
 template <typename T>

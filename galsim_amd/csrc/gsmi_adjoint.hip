@@ -167,6 +167,11 @@ __device__ inline LeafD<T> eval_leafc_d(const LeafC<T>& C, const LeafI& I, const
       case GSMI_LEAF_EXPONENTIAL: {
         if (exp_clip<T>(C, I, tb, i, j, Nk, ksq)) return {T(0), T(0)};
         T q = T(1) + ksq;
+        if constexpr (sizeof(T) == 4) {
+            const float r = __builtin_amdgcn_rsqf(q);         // q^(-1/2)
+            const float phi = r * r * r;
+            return {phi, -1.5f * phi * (r * r)};
+        }
         T phi = ddiv(T(1), q * dsqrt(q));
         return {phi, ddiv(T(-1.5) * phi, q)};
       }
@@ -182,7 +187,9 @@ __device__ inline LeafD<T> eval_leafc_d(const LeafC<T>& C, const LeafI& I, const
         return {phi, T(-0.5) * phi};
       }
       case GSMI_LEAF_MOFFAT: {
-        T k = dsqrt(ksq);
+        T k;
+        if constexpr (sizeof(T) == 4) k = __builtin_amdgcn_sqrtf(ksq);
+        else k = dsqrt(ksq);
         T e = dexp(-k);
         switch (I.kind) {
           case 15:
@@ -212,22 +219,33 @@ __device__ inline LeafD<T> eval_leafc_d(const LeafC<T>& C, const LeafI& I, const
       }
       case GSMI_LEAF_KOLMOGOROV: {
         if (ksq == T(0)) return {T(1), T(0)};
+        if constexpr (sizeof(T) == 4) {
+            const float t = __builtin_amdgcn_exp2f((5.f / 6.f) * __builtin_amdgcn_logf(ksq));
+            const float phi = __builtin_amdgcn_exp2f(-1.44269504088896341f * t);
+            return {phi, ddiv(-5.f / 6.f * t, ksq) * phi};
+        }
         T t = dexp(T(5. / 6.) * dlog(ksq));
         T phi = dexp(-t);
         return {phi, ddiv(T(-5. / 6.) * t, ksq) * phi};
       }
       case GSMI_LEAF_SERSIC: {
         if (ksq >= C.p[1]) {
-            T rk = dsqrt(ksq);
-            const T irk = ddiv(T(1), rk);
+            T irk;
+            if constexpr (sizeof(T) == 4) irk = __builtin_amdgcn_rsqf(ksq);
+            else irk = ddiv(T(1), dsqrt(ksq));
             T phi = ddiv(C.p[2] + C.p[3] * irk, ksq);
             return {phi, -ddiv(C.p[2] + T(1.5) * C.p[3] * irk, ksq * ksq)};
         }
         if (ksq < C.p[0])
             return {T(1) + ksq * (C.p[4] + ksq * C.p[5]), C.p[4] + T(2) * ksq * C.p[5]};
-        T lk = T(0.5) * dlog(ksq);
-        T dS;
-        T S = spline_eq<T>(sersic_table<T>(tb, I.aux), I.ntab, lk, &dS);
+        T lk, dS, S;
+        if constexpr (sizeof(T) == 4) {
+            lk = 0.34657359027997264f * __builtin_amdgcn_logf(ksq);     // ln(ksq)/2
+            S = spline_eq32_d(sersic_table<float>(tb, I.aux), I.ntab, lk, C.t0, C.t1, C.t2, &dS);
+        } else {
+            lk = T(0.5) * dlog(ksq);
+            S = spline_eq<T>(sersic_table<T>(tb, I.aux), I.ntab, lk, &dS);
+        }
         return {ddiv(S, ksq), ddiv(T(0.5) * dS - S, ksq * ksq)};
       }
       case GSMI_LEAF_BOX: {
@@ -355,110 +373,135 @@ grad_colfft(const gsmi_stamp* __restrict__ stamps, const gsmi_op* __restrict__ o
     __syncthreads();
     const cplx<T>* res = fft_lds<T>(bufa, bufb, N, ncol, tw, 1);
 
-    // the program decoded into the LDS buffer the transform result does not occupy
+    // the program decoded into the LDS buffer the transform result does not occupy, and the
+    // stamp's axis factors after it when they fit
     ProgramD d;
-    LeafC<T>* lc = reinterpret_cast<LeafC<T>*>(res == bufa ? bufb : bufa);
+    cplx<T>* freeb = const_cast<cplx<T>*>(res == bufa ? bufb : bufa);
+    LeafC<T>* lc = reinterpret_cast<LeafC<T>*>(freeb);
     bool ok = decode_program<T>(sops, nops, sleaves, tb, d, lc) && d.nl <= ML &&
               (CPLX || !d.has_phase);
     if constexpr (SIG > 0) ok = ok && matches_static<SIG>(sops, nops, sleaves);
     const int nl = (SIG > 0) ? kStaticPrograms[SIG].nl : d.nl;
+    constexpr int kLeafSlots = kProgLeaves * sizeof(LeafC<T>) / sizeof(cplx<T>);
+    const int naxis = 3 * hk + 2;
+    const bool staged = kLeafSlots + naxis <= cpg * N;
+    if (staged)
+        for (int e = threadIdx.x; e < naxis; e += blockDim.x) freeb[kLeafSlots + e] = X[e];
     __syncthreads();
+    // a specialised program's leaf constants in scalar registers; the interpreter reads LDS
+    LeafC<T> lr[(SIG > 0) ? ML : 1];
+    if constexpr (SIG > 0) {
+#pragma unroll
+        for (int l = 0; l < ML; ++l)
+            if (l < nl) lr[l] = uniform_leaf(lc[l]);
+    }
+    auto leafc = [&](int l) -> const LeafC<T>& {
+        if constexpr (SIG > 0) return lr[l];
+        else return lc[l];
+    };
 
     T acc[NF];
 #pragma unroll
     for (int f = 0; f < NF; ++f) acc[f] = T(0);
     const T inv_n2 = T(1) / (T(N) * T(N));
-    for (int e = threadIdx.x; ok && e < ncol * N; e += blockDim.x) {
-        const int cl = fdiv(e, inv_n);
-        const int r = e - cl * N;
-        const int c = c0 + cl;
-        const T wc = (c == 0 || c == half) ? inv_n2 : T(2) * inv_n2;
-        const cplx<T> A = {res[cl * N + r].x * wc, res[cl * N + r].y * wc};
-        const int kx0 = c - N * fdiv(c + hk, inv_n);
-        const int ky0 = r - N * fdiv(r + hk, inv_n);
-        for (int kxa = kx0; kxa <= hk; kxa += N) {
-            for (int kya = ky0; kya <= hk; kya += N) {
-                const bool neg = kxa < 0;
-                const int i = neg ? -kxa : kxa;
-                const int j = neg ? -kya : kya;
-                const T kx = T(i) * dk;
-                const T ky = T(j) * dk;
-                const cplx<T> seed = neg ? cconj(A) : A;
-                const cplx<T> Z = cmul(seed, cmul(X[i], Y[j]));
-                T v[kProgLeaves], phi[ML], dv[ML];
-#pragma unroll
-                for (int l = 0; l < kProgLeaves; ++l) v[l] = T(0);
-#pragma unroll
-                for (int l = 0; l < ML; ++l) {
-                    phi[l] = dv[l] = T(0);
-                    if (l < nl) {
-                        LeafI I = d.li[l];
-                        if constexpr (SIG > 0) I.type = kStaticPrograms[SIG].type[l];
-                        LeafD<T> q = eval_leafc_d<T>(lc[l], I, sleaves + l, tb, kx, ky, i, j,
-                                                     Nk);
-                        phi[l] = q.phi;
-                        v[l] = lc[l].flux * q.phi;
-                        dv[l] = lc[l].flux * q.dphi;
-                    }
-                }
-                cplx<T> FZ;
-                if constexpr (SIG > 0) {
-                    const T P = static_ops<T, SIG>(v, -1, T(0));
-                    FZ = {P * Z.x, P * Z.y};
-                } else if constexpr (CPLX) {
-                    FZ = cmul(run_ops<T>(d, sops, v, -1, T(0), kx, ky), Z);
-                } else {
-                    const T P = run_ops_real<T>(d, sops, v, -1, T(0));
-                    FZ = {P * Z.x, P * Z.y};
-                }
-                acc[PL * ML] += kx * FZ.y;
-                acc[PL * ML + 1] += ky * FZ.y;
-                acc[PL * ML + 2] += FZ.x;
-#pragma unroll
-                for (int l = 0; l < ML; ++l) {
-                    if (l < nl) {
-                        T tx;          // Re[(dP/dv_l) Z]
-                        if constexpr (SIG > 0) {
-                            tx = (static_ops<T, SIG>(v, l, T(1)) -
-                                  static_ops<T, SIG>(v, l, T(0))) * Z.x;
-                        } else if constexpr (CPLX) {
-                            const cplx<T> p1 = run_ops<T>(d, sops, v, l, T(1), kx, ky);
-                            const cplx<T> p0 = run_ops<T>(d, sops, v, l, T(0), kx, ky);
-                            tx = cmul(csub(p1, p0), Z).x;
-                        } else {
-                            tx = (run_ops_real<T>(d, sops, v, l, T(1)) -
-                                  run_ops_real<T>(d, sops, v, l, T(0))) * Z.x;
+    // the sample loop, with the axis factors in LDS or in global memory
+    auto samples = [&](const cplx<T>* Xs, const cplx<T>* Ys) {
+        for (int e = threadIdx.x; ok && e < ncol * N; e += blockDim.x) {
+            const int cl = fdiv(e, inv_n);
+            const int r = e - cl * N;
+            const int c = c0 + cl;
+            const T wc = (c == 0 || c == half) ? inv_n2 : T(2) * inv_n2;
+            const cplx<T> A = {res[cl * N + r].x * wc, res[cl * N + r].y * wc};
+            const int kx0 = c - N * fdiv(c + hk, inv_n);
+            const int ky0 = r - N * fdiv(r + hk, inv_n);
+            for (int kxa = kx0; kxa <= hk; kxa += N) {
+                for (int kya = ky0; kya <= hk; kya += N) {
+                    const bool neg = kxa < 0;
+                    const int i = neg ? -kxa : kxa;
+                    const int j = neg ? -kya : kya;
+                    const T kx = T(i) * dk;
+                    const T ky = T(j) * dk;
+                    const cplx<T> seed = neg ? cconj(A) : A;
+                    const cplx<T> Z = cmul(seed, cmul(Xs[i], Ys[j]));
+                    T v[kProgLeaves], phi[ML], dv[ML];
+    #pragma unroll
+                    for (int l = 0; l < kProgLeaves; ++l) v[l] = T(0);
+    #pragma unroll
+                    for (int l = 0; l < ML; ++l) {
+                        phi[l] = dv[l] = T(0);
+                        if (l < nl) {
+                            LeafI I = d.li[l];
+                            if constexpr (SIG > 0) I.type = kStaticPrograms[SIG].type[l];
+                            LeafD<T> q = eval_leafc_d<T>(leafc(l), I, sleaves + l, tb, kx, ky,
+                                                         i, j, Nk);
+                            phi[l] = q.phi;
+                            v[l] = leafc(l).flux * q.phi;
+                            dv[l] = leafc(l).flux * q.dphi;
                         }
-                        const LeafC<T>& L = lc[l];
-                        T kxp, kyp;
-                        leaf_kprime(L, d.li[l], kx, ky, &kxp, &kyp);
-                        const T sc = L.s;
-                        const T g = tx * dv[l];
-                        const T g2 = T(2) * sc * sc * g;
-                        acc[PL * l] += tx * phi[l];
-                        acc[PL * l + 1] += T(2) * sc * (kxp * kxp + kyp * kyp) * g;
-                        acc[PL * l + 2] += g2 * kxp * kx;
-                        acc[PL * l + 3] += g2 * kyp * kx;
-                        acc[PL * l + 4] += g2 * kxp * ky;
-                        acc[PL * l + 5] += g2 * kyp * ky;
-                        if constexpr (SHAPE) {
-                            const int type = (SIG > 0) ? kStaticPrograms[SIG].type[l]
-                                                       : d.li[l].type;
-                            if (type == GSMI_LEAF_MOFFAT) {
-                                const T kk = sc * dsqrt(kxp * kxp + kyp * kyp);
-                                T dg;
-                                if constexpr (std::is_same<T, float>::value)
-                                    dg = moffat_dg_dnu_f32(float(sleaves[l].p[0]) - 1.f, kk);
-                                else
-                                    dg = moffat_dg_dnu(sleaves[l].p[0] - 1., kk);
-                                acc[PL * l + 6] += tx * L.flux * dg;
+                    }
+                    cplx<T> FZ;
+                    if constexpr (SIG > 0) {
+                        const T P = static_ops<T, SIG>(v, -1, T(0));
+                        FZ = {P * Z.x, P * Z.y};
+                    } else if constexpr (CPLX) {
+                        FZ = cmul(run_ops<T>(d, sops, v, -1, T(0), kx, ky), Z);
+                    } else {
+                        const T P = run_ops_real<T>(d, sops, v, -1, T(0));
+                        FZ = {P * Z.x, P * Z.y};
+                    }
+                    acc[PL * ML] += kx * FZ.y;
+                    acc[PL * ML + 1] += ky * FZ.y;
+                    acc[PL * ML + 2] += FZ.x;
+    #pragma unroll
+                    for (int l = 0; l < ML; ++l) {
+                        if (l < nl) {
+                            T tx;          // Re[(dP/dv_l) Z]
+                            if constexpr (SIG > 0) {
+                                tx = (static_ops<T, SIG>(v, l, T(1)) -
+                                      static_ops<T, SIG>(v, l, T(0))) * Z.x;
+                            } else if constexpr (CPLX) {
+                                const cplx<T> p1 = run_ops<T>(d, sops, v, l, T(1), kx, ky);
+                                const cplx<T> p0 = run_ops<T>(d, sops, v, l, T(0), kx, ky);
+                                tx = cmul(csub(p1, p0), Z).x;
+                            } else {
+                                tx = (run_ops_real<T>(d, sops, v, l, T(1)) -
+                                      run_ops_real<T>(d, sops, v, l, T(0))) * Z.x;
+                            }
+                            const LeafC<T>& L = leafc(l);
+                            T kxp, kyp;
+                            leaf_kprime(L, d.li[l], kx, ky, &kxp, &kyp);
+                            const T sc = L.s;
+                            const T g = tx * dv[l];
+                            const T g2 = T(2) * sc * sc * g;
+                            acc[PL * l] += tx * phi[l];
+                            acc[PL * l + 1] += T(2) * sc * (kxp * kxp + kyp * kyp) * g;
+                            acc[PL * l + 2] += g2 * kxp * kx;
+                            acc[PL * l + 3] += g2 * kyp * kx;
+                            acc[PL * l + 4] += g2 * kxp * ky;
+                            acc[PL * l + 5] += g2 * kyp * ky;
+                            if constexpr (SHAPE) {
+                                const int type = (SIG > 0) ? kStaticPrograms[SIG].type[l]
+                                                           : d.li[l].type;
+                                if (type == GSMI_LEAF_MOFFAT) {
+                                    const T kk = sc * dsqrt(kxp * kxp + kyp * kyp);
+                                    T dg;
+                                    if constexpr (std::is_same<T, float>::value)
+                                        dg = moffat_dg_dnu_f32(float(sleaves[l].p[0]) - 1.f, kk);
+                                    else
+                                        dg = moffat_dg_dnu(sleaves[l].p[0] - 1., kk);
+                                    acc[PL * l + 6] += tx * L.flux * dg;
+                                }
                             }
                         }
                     }
                 }
             }
         }
-    }
+    };
+    if (staged)
+        samples(freeb + kLeafSlots, freeb + kLeafSlots + 2 * hk + 1);
+    else
+        samples(X, Y);
     __syncthreads();                       // LDS is reused for the reduction
     double* red = reinterpret_cast<double*>(smem_raw);
     const int wave = threadIdx.x / 64;
