@@ -91,10 +91,12 @@ def _program_shape(recs, ops, leaves):
 
 
 FOLD_ITEM_CELLS = 16384       # default size of a fold work item (columns x N), in cells
+ROW_ITEM_CELLS = 1            # default size of a row work item (rows x N/2): one chunk
 STAGE_LDS = 32768             # LDS per fold workgroup allowed for staging (5 workgroups per CU)
 
 
-def work_lists(recs, groups, cols, rows, elem, bins=None, chunks=None, extra=0):
+def work_lists(recs, groups, cols, rows, elem, bins=None, chunks=None, extra=0,
+               row_chunks=None):
     """Fold and row work items (gsmi_work rows: stamp, first, count, per_group) of a batch whose
     stamp records ``recs`` are sorted into FFT-size ``groups`` (N, first, count, max_ny), for
     tiles ``cols`` / ``rows`` (dict N -> columns / rows per workgroup chunk; defaults 1024/N and
@@ -106,12 +108,14 @@ def work_lists(recs, groups, cols, rows, elem, bins=None, chunks=None, extra=0):
     fit in STAGE_LDS spans ``chunks[N]`` chunks of ``cols[N]`` columns (default: about
     FOLD_ITEM_CELLS cells; the columns split evenly over the items), which share its set-up and
     its staged tables; the fold LDS then covers that staging.  Other sizes get one-chunk
-    items."""
+    items.  A row item likewise spans ``row_chunks[N]`` chunks of ``rows[N]`` rows (default:
+    about ROW_ITEM_CELLS cells) and stages the N twiddles, which the row LDS covers."""
     fold, rw = [], []
     groups = sorted(groups, key=lambda g: -g[0])
     if bins is not None:
         groups = [g for g in groups if g[0] in bins]
     chunks = chunks or {}
+    row_chunks = row_chunks or {}
     lds_f = lds_r = 0
     for N, first, count, _ in groups:
         cpg = int(cols.get(N, BatchPlan.default_cols(N)))
@@ -133,13 +137,19 @@ def work_lists(recs, groups, cols, rows, elem, bins=None, chunks=None, extra=0):
             lds_f = max(lds_f, bufs)
         nitem = -(-nchunk // k)
         per_item = -(-nchunk // nitem) * cpg            # columns per item, a multiple of cpg
-        lds_r = max(lds_r, 2 * rpg * (N // 2) * elem)
+        rbufs = 2 * rpg * (N // 2) * elem
+        lds_r = max(lds_r, rbufs + N * elem if rbufs + N * elem <= 65536 else rbufs)
+        kr = int(row_chunks.get(N, max(1, ROW_ITEM_CELLS // (rpg * (N // 2)))))
+        if kr < 1:
+            raise ValueError("a row item holds at least one chunk")
         for s in range(first, first + count):
             for c0 in range(0, W, per_item):
                 fold.append((s, c0, min(per_item, W - c0), cpg))
             ny = int(recs['ny'][s])
-            for r0 in range(0, ny, rpg):
-                rw.append((s, r0, min(rpg, ny - r0), rpg))
+            nrc = -(-ny // rpg)
+            rows_item = -(-nrc // -(-nrc // kr)) * rpg     # rows per item, a multiple of rpg
+            for r0 in range(0, ny, rows_item):
+                rw.append((s, r0, min(rows_item, ny - r0), rpg))
     if max(lds_f, lds_r) > 65536:
         raise _lib.GalSimAMDError("work-list tiles need %d bytes of LDS (limit 65536)"
                                   % max(lds_f, lds_r))

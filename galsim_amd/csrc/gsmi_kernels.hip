@@ -370,7 +370,7 @@ struct FoldKernel<double> {
 // E(c) = Z(c) + conj(Z(N/2-c)) and O(c) = (Z(c) - conj(Z(N/2-c))) e^{2 pi i c / N}.
 // ---------------------------------------------------------------------------------------------
 template <typename T>
-__device__ void row_c2r_body(cplx<T>* lds, int rpg, const cplx<T>* __restrict__ in, int nrow,
+__device__ __forceinline__ void row_c2r_body(cplx<T>* lds, int rpg, const cplx<T>* __restrict__ in, int nrow,
                              int N, int x0, int nx, const cplx<T>* __restrict__ tw,
                              T* __restrict__ out, int tws = 1)
 {
@@ -424,7 +424,9 @@ row_c2r(const gsmi_stamp* __restrict__ stamps, const cplx<T>* __restrict__ scrat
                     st->nx, tw, out + st->out_offset + size_t(r0) * st->nx);
 }
 
-// Work-list form of row_c2r: items (stamp, first row, rows, rows per group).
+// Work-list form of row_c2r: items (stamp, first row, rows, rows per chunk).  An item may span
+// several chunks of per_group rows; the N twiddles are staged in LDS once per item when the
+// launch's LDS has room for them after the two transform buffers.
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
 rows_list(const gsmi_stamp* __restrict__ stamps, int nstamps,
@@ -438,17 +440,32 @@ rows_list(const gsmi_stamp* __restrict__ stamps, int nstamps,
         if (it.stamp < 0 || it.stamp >= nstamps) continue;
         const gsmi_stamp* st = stamps + it.stamp;
         const int N = st->N;
+        const size_t bufs = size_t(2) * it.per_group * (N / 2);
         if (it.count <= 0 || it.first < 0 || it.first + it.count > st->ny ||
-            it.per_group < it.count ||
-            size_t(2) * it.per_group * (N / 2) * sizeof(cplx<T>) > size_t(lds_bytes))
+            it.per_group <= 0 || bufs * sizeof(cplx<T>) > size_t(lds_bytes))
             continue;
         int stride;
         const cplx<T>* tw = tws.pick(N, &stride);
         const int W = N / 2 + 1;
-        row_c2r_body<T>(lds, it.per_group, scratch + st->scratch_offset + size_t(it.first) * W,
-                        it.count, N, st->x0, st->nx, tw,
-                        out + st->out_offset + size_t(it.first) * st->nx, stride);
-        __syncthreads();
+        const int end = it.first + it.count;
+        if ((bufs + N) * sizeof(cplx<T>) <= size_t(lds_bytes)) {
+            cplx<T>* twl = lds + bufs;
+            for (int e = threadIdx.x; e < N; e += blockDim.x) twl[e] = tw[e * stride];
+            __syncthreads();
+            for (int r0 = it.first; r0 < end; r0 += it.per_group) {
+                row_c2r_body<T>(lds, it.per_group, scratch + st->scratch_offset + size_t(r0) * W,
+                                min(it.per_group, end - r0), N, st->x0, st->nx, twl,
+                                out + st->out_offset + size_t(r0) * st->nx, 1);
+                __syncthreads();
+            }
+        } else {
+            for (int r0 = it.first; r0 < end; r0 += it.per_group) {
+                row_c2r_body<T>(lds, it.per_group, scratch + st->scratch_offset + size_t(r0) * W,
+                                min(it.per_group, end - r0), N, st->x0, st->nx, tw,
+                                out + st->out_offset + size_t(r0) * st->nx, stride);
+                __syncthreads();
+            }
+        }
     }
 }
 

@@ -45,9 +45,10 @@ def test_work_lists_cover_every_column_and_row_once(cols, rows, chunks):
         r = rw[rw[:, 0] == s]
         seen = np.zeros(ny, int)
         for _, r0, cnt, per in r:
-            assert 1 <= cnt <= per
+            assert cnt >= 1 and per >= 1
             seen[r0:r0 + cnt] += 1
         assert np.all(seen == 1)
+        assert all(cnt % per == 0 for _, _, cnt, per in r[:-1])
     Ns = recs['N'][fold[:, 0]]
     assert np.all(np.diff(Ns) <= 0)                       # largest FFT sizes first
     need = []
@@ -55,7 +56,8 @@ def test_work_lists_cover_every_column_and_row_once(cols, rows, chunks):
         staged = (2 * per * N + N + 3 * N + 2) * 8
         need.append(staged if staged <= STAGE_LDS else 2 * per * N * 8)
     assert lds_f == max(need)
-    assert lds_r == max(2 * per * (int(recs['N'][s]) // 2) * 8 for s, _, _, per in rw)
+    assert lds_r == max((2 * per * (int(recs['N'][s]) // 2) + int(recs['N'][s])) * 8
+                        for s, _, _, per in rw)
 
 
 def test_work_lists_reject_tiles_beyond_lds():
