@@ -92,6 +92,7 @@ class DiffPlan:
         self.nleaves = nl
         self._inv = torch.from_numpy(inv).cuda()          # input index -> sorted position
         mask = dst.reshape(-1) >= 0
+        self._dst_table = torch.from_numpy(dst).cuda()          # (n, NFIELD), -1: none
         self._src = torch.from_numpy(np.nonzero(mask)[0]).cuda()
         self._dst = torch.from_numpy(dst.reshape(-1)[mask]).cuda()
         self.fields0 = torch.from_numpy(f0).cuda()
@@ -128,7 +129,7 @@ class DiffPlan:
             raise ValueError("leaf slot %d is not used by every stamp" % slot)
         base = self.plan.offsets['leaves'] + li * _lib.LEAF_DTYPE.itemsize + lf['p'][1]
         idx = base[:, None] // 8 + first + np.arange(count)[None, :]
-        self._aux_dst = torch.from_numpy(idx.reshape(-1)).cuda()
+        self._aux_dst = torch.from_numpy(np.ascontiguousarray(idx.reshape(-1))).cuda()
 
     def write_fields(self, fields, aux=None):
         """Copy a (n, NFIELD) float64 device tensor of fields (input order), and the declared
@@ -149,6 +150,11 @@ class DiffPlan:
     def adjoint(self, resid):
         """dL/dfield for every stamp (input order, (n, NFIELD) float64) given resid = dL/dI as a
         flat tensor laid out like the plan's output.  Field F_FS holds d/dlog(fs)."""
+        return self.adjoint_sorted(resid).index_select(0, self._inv)
+
+    def adjoint_sorted(self, resid):
+        """adjoint() in the plan's sorted stamp order (row self._inv[i] is stamp i); the tensor
+        is the plan's own buffer, overwritten by the next call."""
         import torch
         p = self.plan
         fdt = torch.float32 if self.precision == 'f32' else torch.float64
@@ -177,7 +183,7 @@ class DiffPlan:
                                           p.scratch.data_ptr(), self._partials.data_ptr(),
                                           gptr, s)
             _lib.check(rc, "adjoint (N=%d)" % N)
-        return g.index_select(0, self._inv)
+        return g
 
     def stamps(self, flat):
         """Split a flat output-layout tensor into per-stamp (ny, nx) views (input order)."""
@@ -235,7 +241,8 @@ class ExpMoffat:
 
     PARAMS = ('flux', 'hlr', 'g1', 'g2', 'dx', 'dy', 'psf_fwhm')
 
-    def __init__(self, theta0, nx, ny, scale=0.2, beta=3.0, precision='f32', fit_beta=False):
+    def __init__(self, theta0, nx, ny, scale=0.2, beta=3.0, precision='f32', fit_beta=False,
+                 fused=True):
         import torch
         from .convolve import Convolve
         from .exponential import Exponential
@@ -265,7 +272,9 @@ class ExpMoffat:
         self._cy = f0[:, F_UY] - t0[:, 5]
         self._fwhm_to_rd = torch.from_numpy(
             1. / (2. * np.sqrt(2. ** (1. / self.beta) - 1.))).cuda()
+        self._cxcy = torch.stack([self._cx, self._cy], dim=1).contiguous()
         self.theta0 = t0
+        self.fused = bool(fused)
 
     def fields(self, theta):
         """(n, NFIELD) fields of parameter rows theta ((n, 7 or 8) float64 device tensor)."""
@@ -276,9 +285,75 @@ class ExpMoffat:
                                  self._fwhm_to_rd)
 
     def render(self, theta):
-        """Flat FP32/FP64 tensor of all stamps for theta; differentiable."""
+        """Flat FP32/FP64 tensor of all stamps for theta; differentiable.  With ``fused`` (the
+        default) the parameter map and its chain rule are one device kernel each
+        (gsmi_exp_moffat_fields / _grad); otherwise they are the torch expressions of
+        ``fields`` under torch's autograd."""
+        if self.fused:
+            global _ExpMoffatRender
+            if _ExpMoffatRender is None:
+                _ExpMoffatRender = _exp_moffat_function()
+            return _ExpMoffatRender.apply(theta, self)
         aux = moffat_aux(theta[:, 7].detach()) if self.fit_beta else None
         return render_fields(self.fields(theta), self.dplan, aux)
+
+    def _check_theta(self, theta):
+        import torch
+        ncol = 8 if self.fit_beta else 7
+        if theta.shape != (self.dplan.n, ncol) or theta.dtype != torch.float64 or \
+                not theta.is_cuda:
+            raise ValueError("theta must be a (%d, %d) float64 device tensor"
+                             % (self.dplan.n, ncol))
+        return theta.detach().contiguous()
+
+    def _write(self, theta):
+        """Fields (and the Moffat values that follow beta) of theta into the device records."""
+        import torch
+        d = self.dplan
+        aux = d._aux_dst.data_ptr() if (self.fit_beta and d._aux_dst is not None) else None
+        rc = _lib.lib().gsmi_exp_moffat_fields(
+            theta.data_ptr(), d.n, int(self.fit_beta), self._base.data_ptr(),
+            self._cxcy.data_ptr(), 1. / self.scale, self._fwhm_to_rd.data_ptr(),
+            d._dst_table.data_ptr(), aux, d._dev64.data_ptr(),
+            torch.cuda.current_stream().cuda_stream)
+        _lib.check(rc, "exp_moffat_fields")
+
+    def _grad(self, theta, gsorted):
+        import torch
+        d = self.dplan
+        out = torch.empty_like(theta)
+        rc = _lib.lib().gsmi_exp_moffat_grad(
+            theta.data_ptr(), d.n, int(self.fit_beta), self._fwhm_to_rd.data_ptr(),
+            gsorted.data_ptr(), d._inv.data_ptr(), 1. / self.scale, out.data_ptr(),
+            torch.cuda.current_stream().cuda_stream)
+        _lib.check(rc, "exp_moffat_grad")
+        return out
+
+
+def _exp_moffat_function():
+    import torch
+
+    class ExpMoffatRender(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, theta, model):
+            theta = model._check_theta(theta)
+            model._write(theta)
+            out = model.dplan.forward().clone()
+            ctx.model = model
+            ctx.save_for_backward(theta)
+            return out
+
+        @staticmethod
+        def backward(ctx, gout):
+            theta, = ctx.saved_tensors
+            model = ctx.model
+            model._write(theta)                  # another forward may have rewritten them
+            return model._grad(theta, model.dplan.adjoint_sorted(gout)), None
+
+    return ExpMoffatRender
+
+
+_ExpMoffatRender = None
 
 
 def moffat_aux(beta):

@@ -281,6 +281,28 @@ int gsmi_adjoint_f32(const gsmi_stamp* stamps, int nstamps, int N, int max_ny, i
                      const double* tables, const float* tables32, const float* resid,
                      void* scratch, double* partials, double* grad, void* stream);
 
+/* -------------------------------------------------------------------------------------------
+ * Parameter map of the differentiable Exponential (x) Moffat model (no counterpart in the
+ * reference: it restates on the device what the host lowering does with galsim/exponential.py,
+ * galsim/shear.py:365-386 and galsim/moffat.py, so a fit step needs no host work).  theta rows
+ * (n x 7, or n x 8 with fit_beta): flux, hlr, g1, g2, dx, dy, psf_fwhm [, beta], FP64 device
+ * arrays.
+ *   gsmi_exp_moffat_fields: the GSMI_GRAD_FIELDS fields of each stamp (base[i] where no
+ *     parameter reaches; cxcy[i] the centring shift added to (dx, dy); fwhm_to_rd[i] the fixed
+ *     rD / fwhm without fit_beta) written into the device records at the double indices
+ *     dst[i * GSMI_GRAD_FIELDS + f] (< 0: not written), and with fit_beta the 5 Moffat values
+ *     that follow beta (the k = 0 value and Temme's constants) at aux_dst[i * 5 + k].
+ *   gsmi_exp_moffat_grad: dL/dtheta (n x 7 or 8) from dL/dfields (gsmi_adjoint_* output, row
+ *     inv[i] for stamp i).
+ * ------------------------------------------------------------------------------------- */
+int gsmi_exp_moffat_fields(const double* theta, int n, int fit_beta, const double* base,
+                           const double* cxcy, double inv_scale, const double* fwhm_to_rd,
+                           const int64_t* dst, const int64_t* aux_dst, double* records,
+                           void* stream);
+int gsmi_exp_moffat_grad(const double* theta, int n, int fit_beta, const double* fwhm_to_rd,
+                         const double* gfields, const int64_t* inv, double inv_scale,
+                         double* dtheta, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -303,3 +303,49 @@ def test_fp32_beta_gradient_matches_fp64():
     print("FP32 vs FP64 adjoint with beta: worst field error / largest sensitivity = %.2e "
           "(beta field %.2e)" % (worst, worst_shape))
     assert worst <= 3e-4, worst
+
+
+@pytest.mark.parametrize("fit_beta", [False, True])
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_fused_parameter_map_matches_torch_expressions(fit_beta, precision):
+    """ExpMoffat's device parameter map and chain rule (gsmi_exp_moffat_fields / _grad) against
+    the same model with the map written as torch expressions under torch's autograd: equal
+    images and equal gradients of a chi^2 loss, every parameter (beta included)."""
+    from galsim_amd import workloads
+    p = workloads.exp_moffat_params(6, seed=29)
+    beta = np.array([2.5, 3.0, 3.3, 4.1, 2.8, 3.7])
+    cols = [p['flux'], p['hlr'], p['g1'], p['g2'], p['offset'][:, 0], p['offset'][:, 1],
+            p['psf_fwhm']]
+    if fit_beta:
+        cols.append(beta)
+    theta = np.stack(cols, axis=1)
+    sides = np.array([32, 48, 64, 48, 96, 64])
+    kw = dict(scale=0.2, beta=beta, precision=precision, fit_beta=fit_beta)
+    fused = diff.ExpMoffat(theta, sides, sides, fused=True, **kw)
+    plain = diff.ExpMoffat(theta, sides, sides, fused=False, **kw)
+    t = torch.tensor(theta, dtype=torch.float64, device='cuda')
+    t1 = t.clone()
+    t1[:, 1] *= 1.05
+    t1[:, 2] += 0.02
+    t1[:, 6] *= 0.97
+    if fit_beta:
+        t1[:, 7] += 0.1
+    with torch.no_grad():
+        target = plain.render(t).double()
+    grads, imgs = [], []
+    for m in (fused, plain):
+        tt = t1.clone().requires_grad_(True)
+        img = m.render(tt)
+        ((img.double() - target) ** 2).sum().backward()
+        grads.append(tt.grad.cpu().numpy())
+        imgs.append(img.detach().double().cpu().numpy())
+    tol = 1e-12 if precision == 'f64' else 1e-6
+    assert np.max(np.abs(imgs[0] - imgs[1])) <= tol * np.max(np.abs(imgs[1]))
+    # per stamp, each gradient times its parameter's size (the loss change for a relative
+    # step), against the stamp's largest such sensitivity
+    w = np.maximum(np.abs(theta), 0.1)
+    sens = np.max(np.abs(grads[1]) * w, axis=1, keepdims=True)
+    err = np.abs(grads[0] - grads[1]) * w / sens
+    print("fused vs torch parameter map: worst gradient difference %.2e of the largest "
+          "sensitivity" % err.max())
+    assert err.max() <= (1e-9 if precision == 'f64' else 1e-4), err
