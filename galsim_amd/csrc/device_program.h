@@ -816,6 +816,75 @@ __device__ __forceinline__ cplx<T>* fft_lds(cplx<T>* a, cplx<T>* b, int n, int n
     return src;
 }
 
+// The same transform with the FFT length a compile-time constant: the flat-index divisions of
+// each pass become multiplications by constants, and the twiddles of a butterfly are loaded
+// together before they are used.  Passes and arithmetic are those of fft_lds (radix 3 first,
+// then 4, then 2), so the results are the same.
+template <typename T, int N, int R, int NS>
+__device__ __forceinline__ void stockham_pass_c(const cplx<T>* src, cplx<T>* dst, int nseg,
+                                                const cplx<T>* tw, int twstride)
+{
+    constexpr unsigned nb = N / R;
+    const int tstep = (N / (NS * R)) * twstride;
+    const unsigned total = unsigned(nseg) * nb;
+    for (unsigned t = threadIdx.x; t < total; t += blockDim.x) {
+        const unsigned seg = t / nb;
+        const unsigned tt = t - seg * nb;
+        const unsigned tq = tt / NS;
+        const unsigned jj = tt - tq * NS;
+        const cplx<T>* s = src + seg * N + tt;
+        cplx<T> v[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = s[r * nb];
+        if constexpr (NS > 1) {
+            cplx<T> w[R];
+#pragma unroll
+            for (int r = 1; r < R; ++r) w[r] = tw[int(jj) * r * tstep];
+#pragma unroll
+            for (int r = 1; r < R; ++r) v[r] = cmul(v[r], w[r]);
+        }
+        if constexpr (R == 2) bfly2(v);
+        if constexpr (R == 3) bfly3(v);
+        if constexpr (R == 4) bfly4(v);
+        cplx<T>* d = dst + seg * N + tq * (NS * R) + jj;
+#pragma unroll
+        for (int r = 0; r < R; ++r) d[r * NS] = v[r];
+    }
+}
+
+template <typename T, int N, int NS>
+__device__ __forceinline__ cplx<T>* fft_fixed(cplx<T>* src, cplx<T>* dst, int nseg,
+                                              const cplx<T>* tw, int twstride)
+{
+    constexpr int rem = N / NS;
+    if constexpr (rem == 1) {
+        return src;
+    } else {
+        constexpr int R = (rem % 3 == 0) ? 3 : (rem % 4 == 0) ? 4 : 2;
+        stockham_pass_c<T, N, R, NS>(src, dst, nseg, tw, twstride);
+        __syncthreads();
+        return fft_fixed<T, N, NS * R>(dst, src, nseg, tw, twstride);
+    }
+}
+
+// fft_lds with the common lengths compiled for their size.
+template <typename T>
+__device__ __forceinline__ cplx<T>* fft_lds_c(cplx<T>* a, cplx<T>* b, int n, int nseg,
+                                              const cplx<T>* tw, int twstride)
+{
+    switch (n) {
+      case 64: return fft_fixed<T, 64, 1>(a, b, nseg, tw, twstride);
+      case 96: return fft_fixed<T, 96, 1>(a, b, nseg, tw, twstride);
+      case 128: return fft_fixed<T, 128, 1>(a, b, nseg, tw, twstride);
+      case 192: return fft_fixed<T, 192, 1>(a, b, nseg, tw, twstride);
+      case 256: return fft_fixed<T, 256, 1>(a, b, nseg, tw, twstride);
+      case 384: return fft_fixed<T, 384, 1>(a, b, nseg, tw, twstride);
+      case 512: return fft_fixed<T, 512, 1>(a, b, nseg, tw, twstride);
+      case 768: return fft_fixed<T, 768, 1>(a, b, nseg, tw, twstride);
+      default: return fft_lds<T>(a, b, n, nseg, tw, twstride);
+    }
+}
+
 __host__ __device__ inline int cols_per_group(int N)
 {
     return N >= 1024 ? 1 : 1024 / N;

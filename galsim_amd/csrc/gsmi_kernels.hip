@@ -158,7 +158,7 @@ __device__ __forceinline__ void fold_chunks(const gsmi_stamp* __restrict__ st, i
         else
             fold_cells<T>(sops, nops, sleaves, tb, X, X + 2 * hk + 1, dk, N, Nk, c0, ncol, bufa);
         __syncthreads();
-        cplx<T>* res = fft_lds<T>(bufa, bufb, N, ncol, twp, twstride);
+        cplx<T>* res = fft_lds_c<T>(bufa, bufb, N, ncol, twp, twstride);
         // rows of the output window: position p <-> DFT index p mod N
         const float inv_ncol = 1.f / float(ncol);
         for (int e = threadIdx.x; e < ncol * ny; e += blockDim.x) {
@@ -391,7 +391,7 @@ __device__ __forceinline__ void row_c2r_body(cplx<T>* lds, int rpg, const cplx<T
         bufa[rl * n2 + c] = {E.x - O.y, E.y + O.x};  // E + i O
     }
     __syncthreads();
-    cplx<T>* res = fft_lds<T>(bufa, bufb, n2, nrow, tw, 2 * tws);
+    cplx<T>* res = fft_lds_c<T>(bufa, bufb, n2, nrow, tw, 2 * tws);
     const T norm = T(1) / (T(N) * T(N));
     const float inv_nx = 1.f / float(nx);
     const float inv_n = 1.f / float(N);
