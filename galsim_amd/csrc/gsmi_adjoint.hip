@@ -307,7 +307,7 @@ row_r2c(const gsmi_stamp* __restrict__ stamps, const T* __restrict__ resid,
         za[2 * (rl * n2 + (q >> 1)) + (q & 1)] = in[size_t(rl) * nx + col];
     }
     __syncthreads();
-    const cplx<T>* res = fft_lds<T>(bufa, bufb, n2, nrow, tw, 2);
+    const cplx<T>* res = fft_lds_c<T>(bufa, bufb, n2, nrow, tw, 2);
     cplx<T>* out = scratch + st->scratch_offset + size_t(r0) * W;
     const float inv_w = 1.f / float(W);
     for (int e = threadIdx.x; e < nrow * W; e += blockDim.x) {
@@ -371,7 +371,7 @@ grad_colfft(const gsmi_stamp* __restrict__ stamps, const gsmi_op* __restrict__ o
         bufa[cl * N + y] = H[size_t(row) * W + c0 + cl];
     }
     __syncthreads();
-    const cplx<T>* res = fft_lds<T>(bufa, bufb, N, ncol, tw, 1);
+    const cplx<T>* res = fft_lds_c<T>(bufa, bufb, N, ncol, tw, 1);
 
     // the program decoded into the LDS buffer the transform result does not occupy, and the
     // stamp's axis factors after it when they fit
