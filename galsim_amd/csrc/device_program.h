@@ -749,6 +749,29 @@ __device__ inline void bfly4(cplx<T>* v)
     v[3] = csub(a1, a3);
 }
 
+// Radix 8 as two radix-4 transforms of the even and odd samples joined by the eighth roots of
+// unity w^k = e^{+2 pi i k / 8}.
+template <typename T>
+__device__ inline void bfly8(cplx<T>* v)
+{
+    const T h = T(0.70710678118654752440);   // 1/sqrt(2)
+    cplx<T> e[4] = {v[0], v[2], v[4], v[6]};
+    cplx<T> o[4] = {v[1], v[3], v[5], v[7]};
+    bfly4(e);
+    bfly4(o);
+    const cplx<T> t1 = {h * (o[1].x - o[1].y), h * (o[1].x + o[1].y)};    // w   o1
+    const cplx<T> t2 = {-o[2].y, o[2].x};                                 // w^2 o2 = i o2
+    const cplx<T> t3 = {-h * (o[3].x + o[3].y), h * (o[3].x - o[3].y)};   // w^3 o3
+    v[0] = cadd(e[0], o[0]);
+    v[4] = csub(e[0], o[0]);
+    v[1] = cadd(e[1], t1);
+    v[5] = csub(e[1], t1);
+    v[2] = cadd(e[2], t2);
+    v[6] = csub(e[2], t2);
+    v[3] = cadd(e[3], t3);
+    v[7] = csub(e[3], t3);
+}
+
 template <typename T>
 __device__ inline void bfly3(cplx<T>* v)
 {
@@ -817,9 +840,9 @@ __device__ __forceinline__ cplx<T>* fft_lds(cplx<T>* a, cplx<T>* b, int n, int n
 }
 
 // The same transform with the FFT length a compile-time constant: the flat-index divisions of
-// each pass become multiplications by constants, and the twiddles of a butterfly are loaded
-// together before they are used.  Passes and arithmetic are those of fft_lds (radix 3 first,
-// then 4, then 2), so the results are the same.
+// each pass become multiplications by constants, the twiddles of a butterfly are loaded together
+// before they are used, and radix-8 passes (radix 3 first, then 8, 4, 2) cut the number of LDS
+// round trips (128 = 8 8 2, 192 = 3 8 8, 256 = 8 8 4, 512 = 8 8 8).
 template <typename T, int N, int R, int NS>
 __device__ __forceinline__ void stockham_pass_c(const cplx<T>* src, cplx<T>* dst, int nseg,
                                                 const cplx<T>* tw, int twstride)
@@ -846,6 +869,7 @@ __device__ __forceinline__ void stockham_pass_c(const cplx<T>* src, cplx<T>* dst
         if constexpr (R == 2) bfly2(v);
         if constexpr (R == 3) bfly3(v);
         if constexpr (R == 4) bfly4(v);
+        if constexpr (R == 8) bfly8(v);
         cplx<T>* d = dst + seg * N + tq * (NS * R) + jj;
 #pragma unroll
         for (int r = 0; r < R; ++r) d[r * NS] = v[r];
@@ -860,7 +884,7 @@ __device__ __forceinline__ cplx<T>* fft_fixed(cplx<T>* src, cplx<T>* dst, int ns
     if constexpr (rem == 1) {
         return src;
     } else {
-        constexpr int R = (rem % 3 == 0) ? 3 : (rem % 4 == 0) ? 4 : 2;
+        constexpr int R = (rem % 3 == 0) ? 3 : (rem % 8 == 0) ? 8 : (rem % 4 == 0) ? 4 : 2;
         stockham_pass_c<T, N, R, NS>(src, dst, nseg, tw, twstride);
         __syncthreads();
         return fft_fixed<T, N, NS * R>(dst, src, nseg, tw, twstride);
