@@ -84,6 +84,9 @@ _SIGS = {
     'gsmi_kolmogorov_info': (_I, [_P, _P]),
     'gsmi_moffat_info': (_I, [_D, _D, _D, _P, _P]),
     'gsmi_moffat_scale_radius_from_hlr': (_I, [_D, _D, _P]),
+    'gsmi_moffat_scale_radius_from_hlr_trunc': (_I, [_D, _D, _D, _P]),
+    'gsmi_moffat_trunc_info': (_I, [_D, _D, _D, _P, _P]),
+    'gsmi_moffat_trunc_table': (_I, [_D, _D, _D, _P, _P, _P, _P, _I]),
     'gsmi_exponential_cliprows': (_I, [_D, _P, _P, _I, _D, _P]),
     'gsmi_sersic_kvalue': (_I, [_D, _P, _P, _P, _I]),
     'gsmi_draw_fft_f64': (_I, [_P, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P]),

@@ -57,6 +57,8 @@ def _prepare_differentiable(ctx, op0, nops, naxis, leaf0, moffat_generic=False):
             rec = ctx.leaves[leaf0 + arg]
             if rec['type'] in _RADIAL:
                 rec['separable'] = 0
+            if moffat_generic and rec['type'] == _lib.LEAF_MOFFAT and rec['kind'] == 1:
+                raise NotImplementedError("the beta of a truncated Moffat is not differentiable")
             if moffat_generic and rec['type'] == _lib.LEAF_MOFFAT and rec['kind'] != 0:
                 beta = float(rec['p'][0])
                 knorm2 = 4. / (math.gamma(beta - 1.) * 2. ** beta)

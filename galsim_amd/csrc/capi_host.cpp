@@ -208,4 +208,48 @@ int gsmi_moffat_scale_radius_from_hlr(double hlr, double beta, double* out)
     return GSMI_OK;
 }
 
+int gsmi_moffat_scale_radius_from_hlr_trunc(double hlr, double trunc, double beta, double* out)
+{
+    if (!out || !(hlr > 0) || !(trunc > 0) || !is_finite(beta)) {
+        set_error("gsmi_moffat_scale_radius_from_hlr_trunc: bad argument");
+        return GSMI_E_ARG;
+    }
+    return guarded("gsmi_moffat_scale_radius_from_hlr_trunc",
+                   [&] { *out = moffat_scale_radius_from_hlr_trunc(hlr, trunc, beta); });
+}
+
+int gsmi_moffat_trunc_info(double beta, double scale_radius, double trunc, const double* gsp,
+                           double* out)
+{
+    if (!out || !is_finite(beta) || !(scale_radius > 0) || !(trunc > 0)) {
+        set_error("gsmi_moffat_trunc_info: bad argument");
+        return GSMI_E_ARG;
+    }
+    return guarded("gsmi_moffat_trunc_info", [&] {
+        const MoffatTruncInfo& m = moffat_trunc_info(beta, trunc / scale_radius, GSP::from(gsp));
+        out[0] = m.maxk_rD / scale_radius;
+        out[1] = m.stepk_rD / scale_radius;
+        out[2] = m.flux_factor;
+        out[3] = m.hlr_rD * scale_radius;
+        out[4] = double(m.ft.args.size());
+        out[5] = m.dk;
+    });
+}
+
+int gsmi_moffat_trunc_table(double beta, double scale_radius, double trunc, const double* gsp,
+                            double* args, double* vals, double* y2, int cap)
+{
+    if (!args || !vals || !y2 || !is_finite(beta) || !(scale_radius > 0) || !(trunc > 0)) {
+        set_error("gsmi_moffat_trunc_table: bad argument");
+        return GSMI_E_ARG;
+    }
+    return guarded("gsmi_moffat_trunc_table", [&] {
+        const MoffatTruncInfo& m = moffat_trunc_info(beta, trunc / scale_radius, GSP::from(gsp));
+        if (int(m.ft.args.size()) > cap) throw std::runtime_error("table capacity too small");
+        std::copy(m.ft.args.begin(), m.ft.args.end(), args);
+        std::copy(m.ft.vals.begin(), m.ft.vals.end(), vals);
+        std::copy(m.ft.y2.begin(), m.ft.y2.end(), y2);
+    });
+}
+
 }

@@ -80,7 +80,8 @@ __device__ inline LeafI leaf_ints(const gsmi_leaf& L)
     r.type = L.type;
     r.kind = L.kind;
     r.sep = L.separable;
-    r.aux = (L.type == GSMI_LEAF_SERSIC) ? L.table : L.cliprows;
+    r.aux = (L.type == GSMI_LEAF_SERSIC || (L.type == GSMI_LEAF_MOFFAT && L.kind == 1))
+                ? L.table : L.cliprows;
     r.ntab = L.ntab;
     return r;
 }
@@ -156,6 +157,35 @@ __device__ __forceinline__ LeafC<T> uniform_leaf(const LeafC<T>& in)
     for (int k = 0; k < 6; ++k) out.p[k] = uniform(in.p[k]);
     out.t1 = uniform(in.t1); out.t2 = uniform(in.t2);
     return out;
+}
+
+// Truncated Moffat kV_trunc (src/SBMoffat.cpp:340-345): the natural spline of its Hankel table
+// on x = (k rD)^2, knots at the squares of k = 0, dk, 2 dk, ... (a running sum), zero beyond the
+// last knot.  The interval comes from k / dk, stepped to the one holding x (the reference's
+// search gives it, or its neighbour when x is a knot, where the spline is continuous).  With dS,
+// also dS/dx.
+template <typename T>
+__device__ inline T moffat_trunc_spline(const T* __restrict__ args, int n, T x, T k, T inv_dk,
+                                        T* dS)
+{
+    if (x > args[n - 1]) {
+        if (dS) *dS = T(0);
+        return T(0);
+    }
+    int i = int(ceil(k * inv_dk));
+    i = min(max(i, 1), n - 1);
+    if (i < n - 1 && args[i] < x) ++i;
+    if (i > 1 && args[i - 1] > x) --i;
+    const T* vals = args + n;
+    const T* y2 = vals + n;
+    const T h = args[i] - args[i - 1];
+    const T aa = args[i] - x;
+    const T bb = h - aa;
+    const T cc = (aa + h) * y2[i - 1] + (bb + h) * y2[i];
+    if (dS)
+        *dS = ddiv(vals[i] - vals[i - 1] -
+                   (T(1) / T(6)) * ((aa - bb) * cc + aa * bb * (y2[i] - y2[i - 1])), h);
+    return ddiv(aa * vals[i - 1] + bb * vals[i] - (T(1) / T(6)) * aa * bb * cc, h);
 }
 
 // leaf_consts plus, for a Sersic leaf, the ends and the inverse spacing of its log-k knots in the
@@ -296,6 +326,9 @@ __device__ inline T eval_leafc(const LeafC<T>& C, const LeafI& I, const gsmi_lea
         T ksq = X * X + Y * Y;
         T k = dsqrt(ksq);
         switch (I.kind) {
+          case 1:
+            return flux * moffat_trunc_spline<T>(sersic_table<T>(tb, I.aux), I.ntab, ksq, k,
+                                                 C.p[1], nullptr);
           case 15: return flux * dexp(-k);
           case 25: return flux * dexp(-k) * (T(1) + k);
           case 35: return flux * dexp(-k) * (T(3) + (T(3) + k) * k);

@@ -192,6 +192,12 @@ __device__ inline LeafD<T> eval_leafc_d(const LeafC<T>& C, const LeafI& I, const
         else k = dsqrt(ksq);
         T e = dexp(-k);
         switch (I.kind) {
+          case 1: {
+            T dS;
+            const T S = moffat_trunc_spline<T>(sersic_table<T>(tb, I.aux), I.ntab, ksq, k,
+                                               C.p[1], &dS);
+            return {S, dS};
+          }
           case 15:
             return {e, ksq == T(0) ? T(0) : ddiv(T(-0.5) * e, k)};
           case 25:

@@ -716,4 +716,175 @@ double moffat_scale_radius_from_hlr(double re, double beta)
     return re / rerd;
 }
 
+// ---------------------------------------------------------------------------------------------
+// Truncated Moffat: src/SBMoffat.cpp:103-161 (rD from hlr with truncation), 163-244 (flux
+// factor, maxRrD, the (1+r^2)^-beta forms), 340-345 (kV_trunc), 497-548 (maxk, stepk),
+// 551-626 (setupFT: the Hankel table on k^2 knots, maxk refined between two knots).
+// ---------------------------------------------------------------------------------------------
+namespace {
+
+// (1+r^2)^-beta as the reference evaluates it: the closed forms when beta is within
+// xvalue_accuracy of 1, 1.5, ..., 4, otherwise exp(-beta log x) through the table-driven exp.
+double moffat_pow_mbeta(double x, double beta, double xva)
+{
+    if (std::abs(beta - 1.) < xva) return 1. / x;
+    if (std::abs(beta - 1.5) < xva) return 1. / (x * std::sqrt(x));
+    if (std::abs(beta - 2.) < xva) return 1. / (x * x);
+    if (std::abs(beta - 2.5) < xva) return 1. / (x * x * std::sqrt(x));
+    if (std::abs(beta - 3.) < xva) return 1. / (x * x * x);
+    if (std::abs(beta - 3.5) < xva) return 1. / (x * x * x * std::sqrt(x));
+    if (std::abs(beta - 4.) < xva) { double x2 = x * x; return 1. / (x2 * x2); }
+    return fast_pow(x, -beta);
+}
+
+// Gauss-Legendre rule of 16 points on [-1, 1] (nodes by Newton's method on P_16).
+struct GaussLegendre16 {
+    double x[16], w[16];
+    GaussLegendre16()
+    {
+        const int n = 16;
+        auto legendre = [n](double z, double* dp) {       // P_n(z) and P_n'(z)
+            double p0 = 1., p1 = z;
+            for (int j = 2; j <= n; ++j) {
+                double p2 = ((2. * j - 1.) * z * p1 - (j - 1.) * p0) / j;
+                p0 = p1;
+                p1 = p2;
+            }
+            *dp = n * (z * p1 - p0) / (z * z - 1.);
+            return p1;
+        };
+        for (int i = 0; i < n; ++i) {
+            double z = std::cos(M_PI * (i + 0.75) / (n + 0.5));
+            double dp = 1.;
+            for (int it = 0; it < 100; ++it) {
+                double dz = legendre(z, &dp) / dp;
+                z -= dz;
+                if (std::abs(dz) < 1e-16) break;
+            }
+            legendre(z, &dp);
+            x[i] = z;
+            w[i] = 2. / ((1. - z * z) * dp * dp);
+        }
+    }
+};
+
+template <class F>
+double gl16(const F& f, double a, double b)
+{
+    static const GaussLegendre16 g;
+    const double c = 0.5 * (a + b), h = 0.5 * (b - a);
+    double s = 0.;
+    for (int i = 0; i < 16; ++i) s += g.w[i] * f(c + h * g.x[i]);
+    return s * h;
+}
+
+// Adaptive 16-point Gauss-Legendre: a panel is accepted when its two halves agree with it to
+// tol; otherwise both halves recurse (depth-limited).
+template <class F>
+double gl_adapt(const F& f, double a, double b, double whole, double tol, int depth)
+{
+    const double m = 0.5 * (a + b);
+    const double l = gl16(f, a, m), r = gl16(f, m, b);
+    if (std::abs(l + r - whole) <= tol || depth >= 40) return l + r;
+    return gl_adapt(f, a, m, l, 0.5 * tol, depth + 1) + gl_adapt(f, m, b, r, 0.5 * tol, depth + 1);
+}
+
+// int_0^rmax f(r) J0(k r) r dr with the reference's split points (the first 10 zeros of J0
+// below k rmax, src/math/Hankel.cpp:37-63), each panel integrated to ~1e-15 of scale.
+template <class F>
+double hankel_trunc_gl(const F& f, double k, double rmax, double scale)
+{
+    auto g = [&](double r) { return r * f(r) * bessel_j0(k * r); };
+    std::vector<double> cuts{0.};
+    for (int s = 1; s <= 10; ++s) {
+        double root = bessel_j0_zero(s);
+        if (root > k * rmax) break;
+        cuts.push_back(root / k);
+    }
+    cuts.push_back(rmax);
+    double sum = 0.;
+    for (size_t i = 0; i + 1 < cuts.size(); ++i)
+        sum += gl_adapt(g, cuts[i], cuts[i + 1], gl16(g, cuts[i], cuts[i + 1]), 1e-15 * scale, 0);
+    return sum;
+}
+
+struct MoffatTruncKey {
+    double beta, maxRrD;
+    GSP gsp;
+    bool operator<(const MoffatTruncKey& o) const
+    {
+        if (beta != o.beta) return beta < o.beta;
+        if (maxRrD != o.maxRrD) return maxRrD < o.maxRrD;
+        return gsp < o.gsp;
+    }
+};
+}  // namespace
+
+const MoffatTruncInfo& moffat_trunc_info(double beta, double maxRrD, const GSP& gsp)
+{
+    static std::map<MoffatTruncKey, MoffatTruncInfo> cache;
+    std::lock_guard<std::recursive_mutex> lock(table_mutex());
+    MoffatTruncKey key{beta, maxRrD, gsp};
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    if (!(maxRrD > 0.)) throw std::runtime_error("Moffat truncation radius must be > 0");
+    MoffatTruncInfo m;
+    m.maxRrD = maxRrD;
+    m.flux_factor = 1. - std::pow(1 + maxRrD * maxRrD, (1. - beta));
+    const double xva = gsp.xvalue_accuracy;
+    auto f = [&](double r) { return moffat_pow_mbeta(1. + r * r, beta, xva); };
+    const double prefactor = 2. * (beta - 1.) / m.flux_factor;
+    // the k = 0 integral, the scale the panel tolerances are relative to
+    const double scale = std::abs(1. / prefactor);
+    const double maxk_val = gsp.maxk_threshold;
+    const double dk = gsp.table_spacing * std::sqrt(std::sqrt(gsp.kvalue_accuracy / 10.));
+    m.dk = dk;
+    int n_below = 0;
+    double maxk = 0.;
+    for (double k = 0.; k < 50; k += dk) {
+        double val = hankel_trunc_gl(f, k, maxRrD, scale) * prefactor;
+        m.ft.args.push_back(k * k);
+        m.ft.vals.push_back(val);
+        if (std::abs(val) > maxk_val) maxk = k;
+        if (std::abs(val) > gsp.kvalue_accuracy) n_below = 0;
+        else ++n_below;
+        if (n_below == 5) break;
+    }
+    m.ft.build();
+    double k1 = maxk, k2 = maxk + dk;
+    if (k2 < 50) {
+        double f1 = std::abs(m.ft(k1 * k1));
+        double f2 = std::abs(m.ft(k2 * k2));
+        maxk = (k1 * (f2 - maxk_val) + k2 * (maxk_val - f1)) / (f2 - f1);
+    }
+    m.maxk_rD = maxk;
+    // stepK in units of rD (maxR = maxRrD rD)
+    m.hlr_rD = std::sqrt(std::pow(1. - 0.5 * m.flux_factor, 1. / (1. - beta)) - 1.);
+    double R;
+    if (beta <= 1.1) {
+        R = maxRrD;
+    } else {
+        R = std::pow(gsp.folding_threshold, 0.5 / (1. - beta));
+        if (R > maxRrD) R = maxRrD;
+        R = std::max(R, gsp.stepk_minimum_hlr * m.hlr_rD);
+    }
+    m.stepk_rD = M_PI / R;
+    return cache.emplace(key, std::move(m)).first->second;
+}
+
+double moffat_scale_radius_from_hlr_trunc(double re, double rm, double beta)
+{
+    if (rm <= std::sqrt(2.) * re)
+        throw std::runtime_error("Moffat truncation radius must be > sqrt(2) * half_light_radius.");
+    auto func = [=](double rd) {
+        double fre = 1. - fast_pow(1. + (re * re) / (rd * rd), 1. - beta);
+        double frm = 1. - fast_pow(1. + (rm * rm) / (rd * rd), 1. - beta);
+        return 2. * fre - frm;
+    };
+    double r1 = re / std::sqrt(std::pow(0.5, 1. / (1. - beta)) - 1.);
+    auto s = make_brent(func, r1, 2. * r1);
+    s.bracket_upper();
+    return s.root();
+}
+
 }  // namespace gsmi

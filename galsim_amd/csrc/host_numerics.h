@@ -56,6 +56,15 @@ MoffatInfo moffat_info(double beta, double rD, double flux, const GSP& gsp);
 double moffat_kv(int kind, double beta, double knorm2, double flux_over_knorm, double ksq);
 double moffat_scale_radius_from_hlr(double re, double beta);
 
+// Truncated Moffat (trunc > 0): the Hankel-transform table kV_trunc on x = (k rD)^2 and the sizes,
+// in units of rD (maxk * rD, stepk * rD); depends on (beta, trunc / rD, gsparams), cached.
+struct MoffatTruncInfo {
+    double flux_factor = 0., maxRrD = 0., maxk_rD = 0., stepk_rD = 0., hlr_rD = 0., dk = 0.;
+    Spline ft;
+};
+const MoffatTruncInfo& moffat_trunc_info(double beta, double maxRrD, const GSP& gsp);
+double moffat_scale_radius_from_hlr_trunc(double re, double rm, double beta);
+
 double bessel_k(double nu, double x);
 
 }  // namespace gsmi

@@ -1,4 +1,4 @@
-"""Moffat, untruncated (galsim/moffat.py; src/SBMoffat.cpp)."""
+"""Moffat, untruncated or truncated (galsim/moffat.py; src/SBMoffat.cpp)."""
 import ctypes
 import math
 
@@ -42,17 +42,26 @@ class Moffat(Leaf):
         self._trunc = float(trunc)
         self._flux = float(flux)
         self._gsparams = GSParams.check(gsparams)
-        if self._trunc != 0.:
-            raise NotImplementedError("truncated Moffat profiles are not part of galsim_amd yet")
-        if self._beta <= 1.1:
+        if self._trunc == 0. and self._beta <= 1.1:
             raise _lib.GalSimAMDRangeError("Moffat profiles with beta <= 1.1 must be truncated")
+        if self._trunc < 0.:
+            raise _lib.GalSimAMDRangeError("Moffat trunc must be >= 0")
         given = [v is not None for v in (scale_radius, half_light_radius, fwhm)]
         if sum(given) != 1:
             raise ValueError("One of scale_radius, half_light_radius, or fwhm must be specified")
         if half_light_radius is not None:
+            hlr = float(half_light_radius)
             out = np.zeros(1)
-            _lib.check(_lib.lib().gsmi_moffat_scale_radius_from_hlr(
-                float(half_light_radius), self._beta, _lib.ptr(out)), "Moffat hlr")
+            if self._trunc > 0.:
+                # galsim/moffat.py: trunc must exceed sqrt(2) hlr; MoffatCalculateSRFromHLR
+                if self._trunc <= math.sqrt(2.) * hlr:
+                    raise _lib.GalSimAMDRangeError(
+                        "Moffat trunc must be > sqrt(2) * half_light_radius.")
+                _lib.check(_lib.lib().gsmi_moffat_scale_radius_from_hlr_trunc(
+                    hlr, self._trunc, self._beta, _lib.ptr(out)), "Moffat hlr")
+            else:
+                _lib.check(_lib.lib().gsmi_moffat_scale_radius_from_hlr(
+                    hlr, self._beta, _lib.ptr(out)), "Moffat hlr")
             self._r0 = float(out[0])
         elif fwhm is not None:
             self._r0 = float(fwhm) / (2. * math.sqrt(2. ** (1. / self._beta) - 1.))
@@ -64,13 +73,36 @@ class Moffat(Leaf):
         self._info = None
 
     def _get_info(self):
+        """Untruncated: (maxK, stepK, knorm, knorm2, kind).  Truncated: (maxK, stepK, flux
+        factor, half-light radius, knots, knot spacing) and the kV_trunc table."""
         if self._info is None:
-            out = np.zeros(5)
-            _lib.check(_lib.lib().gsmi_moffat_info(self._beta, self._r0, self._flux,
-                                                   _lib.ptr(_lib.gsp_array(self.gsparams)),
-                                                   _lib.ptr(out)), "Moffat info")
+            g = _lib.ptr(_lib.gsp_array(self.gsparams))
+            if self._trunc > 0.:
+                out = np.zeros(6)
+                _lib.check(_lib.lib().gsmi_moffat_trunc_info(self._beta, self._r0, self._trunc,
+                                                             g, _lib.ptr(out)), "Moffat info")
+                n = int(out[4])
+                args, vals, y2 = np.zeros(n), np.zeros(n), np.zeros(n)
+                _lib.check(_lib.lib().gsmi_moffat_trunc_table(
+                    self._beta, self._r0, self._trunc, g, _lib.ptr(args), _lib.ptr(vals),
+                    _lib.ptr(y2), n), "Moffat table")
+                self._table = (args, vals, y2)
+            else:
+                out = np.zeros(5)
+                _lib.check(_lib.lib().gsmi_moffat_info(self._beta, self._r0, self._flux, g,
+                                                       _lib.ptr(out)), "Moffat info")
             self._info = out
         return self._info
+
+    @property
+    def trunc(self):
+        return self._trunc
+
+    @property
+    def half_light_radius(self):
+        if self._trunc > 0.:
+            return float(self._get_info()[3])
+        return self._r0 * math.sqrt(2. ** (1. / (self._beta - 1.)) - 1.)
 
     @property
     def beta(self):
@@ -95,6 +127,17 @@ class Moffat(Leaf):
     def _fill(self, rec, ctx, jac):
         info = self._get_info()
         rec['scale'] = self._r0
+        if self._trunc > 0.:
+            # kV_trunc: knorm = flux (knorm2 = 1), the table on (k rD)^2
+            args, vals, y2 = self._table
+            rec['flux'] = self._flux
+            rec['kind'] = 1
+            rec['table'] = ctx.add_table(('moffat', self._beta, self._trunc / self._r0,
+                                          self.gsparams), args, vals, y2)
+            rec['ntab'] = len(args)
+            rec['p'][0] = self._beta
+            rec['p'][1] = 1. / float(info[5])
+            return
         rec['flux'] = info[2]                 # knorm
         rec['kind'] = int(info[4])
         rec['p'][0] = self._beta
@@ -103,8 +146,9 @@ class Moffat(Leaf):
             rec['p'][2:6] = temme_constants(self._beta - 1.)
 
     def withFlux(self, flux):
-        return Moffat(self._beta, scale_radius=self._r0, flux=flux, gsparams=self.gsparams)
+        return Moffat(self._beta, scale_radius=self._r0, trunc=self._trunc, flux=flux,
+                      gsparams=self.gsparams)
 
     def __repr__(self):
-        return 'galsim_amd.Moffat(beta=%r, scale_radius=%r, flux=%r)' % (
-            self._beta, self._r0, self._flux)
+        return 'galsim_amd.Moffat(beta=%r, scale_radius=%r, trunc=%r, flux=%r)' % (
+            self._beta, self._r0, self._trunc, self._flux)

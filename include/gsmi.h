@@ -64,9 +64,11 @@ typedef struct gsmi_op {
 typedef struct gsmi_leaf {
     int32_t type;         /* GSMI_LEAF_* */
     int32_t separable;    /* 1: diagonal/no Jacobian (the reference's quadrant fill) */
-    int32_t kind;         /* Moffat: 15,2,25,3,35,4 or 0; others 0 */
-    int32_t table;        /* Sersic: offset of its table in the table buffer (doubles) */
-    int32_t ntab;         /* Sersic: number of knots */
+    int32_t kind;         /* Moffat: 15,2,25,3,35,4 or 0; 1 = truncated (kV_trunc table,
+                             p[0] = beta, p[1] = 1 / knot spacing in k rD); others 0 */
+    int32_t table;        /* Sersic, truncated Moffat: offset of its table in the table buffer
+                             (doubles: knots, values, spline second derivatives) */
+    int32_t ntab;         /* Sersic, truncated Moffat: number of knots */
     int32_t cliprows;     /* Exponential (FP64): offset of its int2 row ranges, -1 if none */
     double jac[4];        /* composite Jacobian (dudx, dudy, dvdx, dvdy) of the leaf       */
     double scale;         /* r0, sigma, rD, 1/k0, r0, or width/2pi                         */
@@ -156,6 +158,18 @@ int gsmi_moffat_info(double beta, double scale_radius, double flux, const double
                      double* out);
 /* MoffatCalculateScaleRadiusFromHLR, untruncated (src/SBMoffat.cpp:103-126). */
 int gsmi_moffat_scale_radius_from_hlr(double hlr, double beta, double* out);
+/* MoffatCalculateScaleRadiusFromHLR with a truncation radius (src/SBMoffat.cpp:127-161: Brent
+ * from the untruncated value upwards). */
+int gsmi_moffat_scale_radius_from_hlr_trunc(double hlr, double trunc, double beta, double* out);
+/* SBMoffat truncated (trunc > 0; src/SBMoffat.cpp:163-244, 340-345, 497-626): out = {maxK,
+ * stepK, flux factor, half-light radius, number of table knots, knot spacing in k rD}; the
+ * table: kV_trunc on args = (k rD)^2 with its natural-spline second derivatives (cap >= knots).
+ * The Hankel transforms of the table are integrated here to ~1e-15 of the k = 0 value on the
+ * reference's split points; the reference's adaptive integrator stops at its own tolerance. */
+int gsmi_moffat_trunc_info(double beta, double scale_radius, double trunc, const double* gsp,
+                           double* out);
+int gsmi_moffat_trunc_table(double beta, double scale_radius, double trunc, const double* gsp,
+                            double* args, double* vals, double* y2, int cap);
 /* Exponential k-range clipping of the drawFFT k-image (bounds 0..Nk/2, -Nk/2..Nk/2, spacing dk)
  * for a leaf of scale radius r0 drawn with Jacobian jac (NULL = identity): out[2j], out[2j+1] =
  * [i1, i2) of row j (ky index j - Nk/2), as SBExponentialImpl::fillKImage computes them through

@@ -12,6 +12,7 @@ Coordinates are accumulated with ``np.cumsum`` which adds sequentially, i.e. wit
 rounding as the reference's ``kx += dkx`` loops.
 """
 import math
+import warnings
 from collections import deque
 
 import numpy as np
@@ -269,24 +270,91 @@ class Gaussian:
 # ----------------------------------------------------------------------------------------------
 # Moffat -- src/SBMoffat.cpp, galsim/moffat.py
 # ----------------------------------------------------------------------------------------------
+def _moffat_trunc_table(beta, maxRrD, gsp):
+    """SBMoffatImpl::setupFT (src/SBMoffat.cpp:565-626): kV_trunc tabulated on k^2 for
+    k = 0, dk, 2 dk, ... (a running sum; dk = table_spacing (kvalue_accuracy/10)^(1/4)) until 5
+    successive |values| <= kvalue_accuracy or k >= 50; value = 2 (beta-1) / fluxFactor times
+    int_0^maxRrD (1+r^2)^-beta J0(k r) r dr.  The integrals here are scipy's QUADPACK on the
+    reference's split points (the first 10 zeros of J0 below k maxRrD, src/math/Hankel.cpp:37-63)
+    to 1e-14; the last k with |value| > maxk_threshold is refined linearly towards the next knot.
+    Returns (SplineTable on k^2, maxk in units of 1/rD, fluxFactor).  Pinned against the
+    reference's own hankel_trunc (oracle/_ref) and the maxk/stepk known answers of
+    tests/test_moffat.py:130-160."""
+    import scipy.integrate as spi
+    key = (beta, maxRrD, gsp.key())
+    if key in _MOFFAT_TRUNC:
+        return _MOFFAT_TRUNC[key]
+    flux_factor = 1. - math.pow(1. + maxRrD * maxRrD, 1. - beta)
+    prefactor = 2. * (beta - 1.) / flux_factor
+    zeros = sps.jn_zeros(0, 10)
+
+    def hankel(k):
+        f = lambda r: r * math.exp(-beta * math.log1p(r * r)) * sps.j0(k * r)
+        cuts = [0.] + [z / k for z in zeros if k > 0 and z <= k * maxRrD] + [maxRrD]
+        with warnings.catch_warnings():
+            warnings.simplefilter('ignore')          # QUADPACK's roundoff notes at 1e-14
+            return sum(spi.quad(f, a, b, epsabs=1e-15, epsrel=1e-14, limit=400)[0]
+                       for a, b in zip(cuts[:-1], cuts[1:]))
+
+    dk = gsp.table_spacing * math.sqrt(math.sqrt(gsp.kvalue_accuracy / 10.))
+    args, vals = [], []
+    maxk = 0.
+    below = 0
+    k = 0.
+    while k < 50:
+        v = prefactor * hankel(k)
+        args.append(k * k)
+        vals.append(v)
+        if abs(v) > gsp.maxk_threshold:
+            maxk = k
+        below = 0 if abs(v) > gsp.kvalue_accuracy else below + 1
+        if below == 5:
+            break
+        k += dk
+    ft = nm.SplineTable(args, vals)
+    k1, k2 = maxk, maxk + dk
+    if k2 < 50:
+        f1, f2 = abs(float(ft(k1 * k1))), abs(float(ft(k2 * k2)))
+        t = gsp.maxk_threshold
+        maxk = (k1 * (f2 - t) + k2 * (t - f1)) / (f2 - f1)
+    out = (ft, maxk, flux_factor)
+    _MOFFAT_TRUNC[key] = out
+    return out
+
+
+_MOFFAT_TRUNC = {}
+
+
 class Moffat:
-    """SBMoffat, untruncated (src/SBMoffat.cpp:163-244 ctor, 286-345 kV_*, 419-479 fills,
-    497-549 maxK/stepK).  K_nu from scipy.special.kv.
+    """SBMoffat (src/SBMoffat.cpp:163-244 ctor, 286-345 kV_*, 419-479 fills, 497-549
+    maxK/stepK, 565-626 the truncated table).  K_nu from scipy.special.kv.
 
     Pinned by kValue(maxk)/flux == maxk_threshold (tests/test_moffat.py:183-214), the drawKImage
     size N=1174 for beta=1.5, maxk_threshold=1e-4 (tests/test_draw.py:1279) and the Hankel pair
-    with the Exponential (tests/test_draw.py:859).
+    with the Exponential (tests/test_draw.py:859); truncated, see _moffat_trunc_table.
     """
 
     def __init__(self, beta, scale_radius=None, half_light_radius=None, fwhm=None, trunc=0.,
                  flux=1., gsparams=None):
         self.beta = float(beta)
         self.trunc = float(trunc)
-        if self.trunc != 0.:
-            raise NotImplementedError("truncated Moffat is not restated in the oracle")
         self.flux = float(flux)
         self.gsparams = gsparams or DEFAULT_GSPARAMS
-        if half_light_radius is not None:
+        if half_light_radius is not None and self.trunc > 0.:
+            # MoffatCalculateScaleRadiusFromHLR with rm = trunc (src/SBMoffat.cpp:127-161)
+            re, rm, b = float(half_light_radius), self.trunc, self.beta
+            if rm <= math.sqrt(2.) * re:
+                raise ValueError("Moffat trunc must be > sqrt(2) * half_light_radius.")
+
+            def func(rd):
+                fre = 1. - fast_pow(1. + re * re / (rd * rd), 1. - b)
+                frm = 1. - fast_pow(1. + rm * rm / (rd * rd), 1. - b)
+                return 2. * fre - frm
+            r1 = re / math.sqrt(math.pow(0.5, 1. / (1. - b)) - 1.)
+            s = nm.Solve(func, r1, 2. * r1)
+            s.bracket_upper()
+            scale_radius = s.root()
+        elif half_light_radius is not None:
             # MoffatCalculateScaleRadiusFromHLR, rm == 0 branch (src/SBMoffat.cpp:116-126)
             rerd = math.sqrt(math.pow(0.5, 1. / (1. - self.beta)) - 1.)
             scale_radius = half_light_radius / rerd
@@ -296,6 +364,15 @@ class Moffat:
         self.rD = float(scale_radius)
         gsp = self.gsparams
         beta = self.beta
+        self._maxk = None
+        if self.trunc > 0.:
+            self.maxRrD = self.trunc / self.rD
+            self.maxR = self.trunc
+            self.ft, self._maxk, self.fluxFactor = _moffat_trunc_table(beta, self.maxRrD, gsp)
+            self.kind = 1
+            self.knorm2 = 1.
+            self.knorm = self.flux
+            return
         self.maxRrD = math.sqrt(math.pow(gsp.xvalue_accuracy, 1. / (1. - beta)) - 1.)
         self.maxR = self.maxRrD * self.rD
         self.fluxFactor = 1.
@@ -320,13 +397,14 @@ class Moffat:
             self.kind = 0
             self.knorm2 *= 4. / (math.gamma(beta - 1.) * math.pow(2., beta))
         self.knorm = self.flux * self.knorm2
-        self._maxk = None
 
     def kV(self, ksq):
         """kV_* of src/SBMoffat.cpp:286-345 (argument ksq = k^2 rD^2)."""
         ksq = np.asarray(ksq, dtype=float)
         k = np.sqrt(ksq)
         kind = self.kind
+        if kind == 1:                       # kV_trunc (340-345): 0 beyond the table
+            return np.where(ksq > self.ft.arg_max, 0., self.ft(ksq))
         with np.errstate(invalid='ignore', divide='ignore', over='ignore'):
             if kind == 15:
                 return nm.expd(-k)
@@ -361,7 +439,7 @@ class Moffat:
 
     @property
     def stepk(self):
-        # src/SBMoffat.cpp:522-549
+        # src/SBMoffat.cpp:522-549 (maxR is the truncation radius when truncated)
         gsp = self.gsparams
         if self.beta <= 1.1:
             return math.pi / self.maxR

@@ -37,6 +37,8 @@ def lib():
         L.gsref_bessel_root.argtypes = [d, ctypes.c_int]
         L.gsref_bessel_root.restype = d
         L.gsref_kolmogorov_info.argtypes = [dp, dp]
+        L.gsref_hankel_moffat_trunc.argtypes = [d] * 5
+        L.gsref_hankel_moffat_trunc.restype = d
         _lib = L
     return _lib
 

@@ -2,8 +2,9 @@
 // sources by oracle/Makefile into oracle/_ref/libgsref.so.
 //
 // TEST INFRASTRUCTURE ONLY: it exposes the reference's SersicInfo (src/SBSersic.cpp:223-537),
-// KolmogorovInfo (src/SBKolmogorov.cpp:337-419), the Sersic/Moffat radius helpers and the
-// special functions (src/math/*.cpp) so the oracle's restatement can be pinned against them.
+// KolmogorovInfo (src/SBKolmogorov.cpp:337-419), the Sersic/Moffat radius helpers, the Hankel
+// transforms and the special functions (src/math/*.cpp) so the oracle's restatement can be
+// pinned against them.
 // The reference's image code (src/Image.cpp) needs Eigen and FFTW, which this image lacks, so the
 // SBProfile draw path itself is not linked; --gc-sections drops the unreferenced code that would
 // need it.
@@ -73,6 +74,19 @@ int gsref_kolmogorov_info(const double* gsp, double* out)
 
 double gsref_moffat_scale_radius_from_hlr(double re, double rm, double beta)
 { return MoffatCalculateScaleRadiusFromHLR(re, rm, beta); }
+
+// The reference's hankel_trunc (src/math/Hankel.cpp: GKP integration over [0, rmax] split at
+// the first zeros of J0) of the truncated Moffat's radial profile (1 + r^2)^-beta, with the
+// reference's generic power fmath::expd(-beta log x) (src/SBMoffat.cpp:46-47, 272): the integral
+// that setupFT (src/SBMoffat.cpp:565-626) tabulates.
+double gsref_hankel_moffat_trunc(double beta, double k, double rmax, double relerr,
+                                 double abserr)
+{
+    std::function<double(double)> f = [beta](double r) {
+        return fmath::expd(-beta * std::log(1. + r * r));
+    };
+    return math::hankel_trunc(f, k, 0., rmax, relerr, abserr);
+}
 
 double gsref_bessel_k(double nu, double x) { return math::cyl_bessel_k(nu, x); }
 double gsref_bessel_j(double nu, double x) { return math::cyl_bessel_j(nu, x); }

@@ -129,7 +129,7 @@ def test_errors_are_loud():
     with pytest.raises(g.GalSimAMDRangeError):
         g.Moffat(1.05, scale_radius=1.)
     with pytest.raises(NotImplementedError):
-        g.Moffat(3., scale_radius=1., trunc=4.)
+        g.Sersic(3., half_light_radius=1., trunc=4.)
     out = np.zeros(10)
     rc = _lib.lib().gsmi_sersic_info(float('nan'), None, _lib.ptr(out), None)
     assert rc == -1 and b"bad argument" in _lib.lib().gsmi_last_error()

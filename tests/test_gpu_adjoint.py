@@ -97,6 +97,15 @@ def test_adjoint_fields_moffat_kinds_f64(beta):
     _check_fields_fd(profs, 40, 40, np.array([[0.2, 0.1]]))
 
 
+def test_adjoint_fields_truncated_moffat_f64():
+    # kV_trunc is a spline of its Hankel table on (k rD)^2: the adjoint takes its derivative;
+    # the table (a function of trunc / rD) is held fixed, as the finite differences of the
+    # fields hold it
+    profs = [g.Convolve([g.Gaussian(sigma=0.5, flux=50.).shear(g1=0.2, g2=-0.1),
+                         g.Moffat(2.5, fwhm=0.8, trunc=2.0)])]
+    _check_fields_fd(profs, 40, 40, np.array([[0.2, 0.1]]))
+
+
 def test_adjoint_fields_inprogram_shift_and_clip_f64():
     # a shift on one component stays in the program (its phase is not a field, but every leaf
     # gradient goes through it); a small Exponential is clipped by the reference's k-range rule

@@ -83,6 +83,12 @@ def test_exponential_moffat_seeded_sweep():
         (m.Exponential(half_light_radius=1.0, flux=600.) +
          m.Sersic(2.5, half_light_radius=0.5, flux=400.)).shear(g1=0.3, g2=-0.2),
         m.Kolmogorov(fwhm=0.75)]), (64, 64)),
+    ("moffat_trunc_2", lambda m: m.Moffat(2., fwhm=0.9, trunc=1.8, flux=3.), (48, 48)),
+    ("moffat_trunc_105_sheared", lambda m: m.Moffat(1.05, scale_radius=0.3, trunc=1.5)
+     .shear(g1=0.15, g2=-0.1), (48, 48)),
+    ("moffat_trunc_conv", lambda m: m.Convolve([m.Exponential(half_light_radius=0.6, flux=20.),
+                                                m.Moffat(3.2, half_light_radius=0.5, trunc=2.)]),
+     (56, 56)),
     ("odd_nonsquare", lambda m: _expmoffat(m, 1.1, 0.2, 0.3), (37, 52)),
     ("exp_small_clipped", lambda m: m.Exponential(scale_radius=0.02).shear(g1=0.4, g2=0.1),
      (24, 24)),
