@@ -78,9 +78,11 @@ _SIGS = {
     'gsmi_last_error': (ctypes.c_char_p, []),
     'gsmi_version': (_I, []),
     'gsmi_good_fft_size': (_I, [_I]),
-    'gsmi_sersic_info': (_I, [_D, _P, _P, _P]),
-    'gsmi_sersic_table': (_I, [_D, _P, _P, _P, _P, _I]),
+    'gsmi_sersic_info': (_I, [_D, _D, _P, _P, _P]),
+    'gsmi_sersic_table': (_I, [_D, _D, _P, _P, _P, _P, _I]),
     'gsmi_sersic_hlr': (_I, [_D, _D, _P]),
+    'gsmi_sersic_integrated_flux': (_I, [_D, _D, _P]),
+    'gsmi_sersic_truncated_scale': (_I, [_D, _D, _D, _P]),
     'gsmi_kolmogorov_info': (_I, [_P, _P]),
     'gsmi_moffat_info': (_I, [_D, _D, _D, _P, _P]),
     'gsmi_moffat_scale_radius_from_hlr': (_I, [_D, _D, _P]),
@@ -88,7 +90,7 @@ _SIGS = {
     'gsmi_moffat_trunc_info': (_I, [_D, _D, _D, _P, _P]),
     'gsmi_moffat_trunc_table': (_I, [_D, _D, _D, _P, _P, _P, _P, _I]),
     'gsmi_exponential_cliprows': (_I, [_D, _P, _P, _I, _D, _P]),
-    'gsmi_sersic_kvalue': (_I, [_D, _P, _P, _P, _I]),
+    'gsmi_sersic_kvalue': (_I, [_D, _D, _P, _P, _P, _I]),
     'gsmi_draw_fft_f64': (_I, [_P, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P]),
     'gsmi_draw_fft_f32': (_I, [_P, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P]),
     'gsmi_fold_colfft_f64': (_I, [_P, _I, _I, _P, _P, _P, _P, _P, _P]),

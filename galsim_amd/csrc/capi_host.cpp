@@ -34,27 +34,30 @@ extern "C" {
 
 int gsmi_good_fft_size(int n) { return good_fft_size(n); }
 
-int gsmi_sersic_info(double n, const double* gsp, double* out, int* ntab)
+int gsmi_sersic_info(double n, double trunc, const double* gsp, double* out, int* ntab)
 {
-    if (!out || !is_finite(n)) { set_error("gsmi_sersic_info: bad argument"); return GSMI_E_ARG; }
+    if (!out || !is_finite(n) || !(trunc >= 0.)) {
+        set_error("gsmi_sersic_info: bad argument");
+        return GSMI_E_ARG;
+    }
     return guarded("gsmi_sersic_info", [&] {
-        const SersicInfo& s = sersic_info(n, GSP::from(gsp));
-        double v[10] = {s.maxk, s.stepk, s.hlr, s.b, s.ksq_min, s.ksq_max, s.highk_a,
-                        s.highk_b, s.kderiv2, s.kderiv4};
-        for (int i = 0; i < 10; ++i) out[i] = v[i];
+        const SersicInfo& s = sersic_info(n, trunc, GSP::from(gsp));
+        double v[11] = {s.maxk, s.stepk, s.hlr, s.b, s.ksq_min, s.ksq_max, s.highk_a,
+                        s.highk_b, s.kderiv2, s.kderiv4, s.flux_fraction};
+        for (int i = 0; i < 11; ++i) out[i] = v[i];
         if (ntab) *ntab = int(s.ft.args.size());
     });
 }
 
-int gsmi_sersic_table(double n, const double* gsp, double* args, double* vals, double* y2,
-                      int cap)
+int gsmi_sersic_table(double n, double trunc, const double* gsp, double* args, double* vals,
+                      double* y2, int cap)
 {
-    if (!args || !vals || !y2 || !is_finite(n)) {
+    if (!args || !vals || !y2 || !is_finite(n) || !(trunc >= 0.)) {
         set_error("gsmi_sersic_table: bad argument");
         return GSMI_E_ARG;
     }
     return guarded("gsmi_sersic_table", [&] {
-        const SersicInfo& s = sersic_info(n, GSP::from(gsp));
+        const SersicInfo& s = sersic_info(n, trunc, GSP::from(gsp));
         int m = int(s.ft.args.size());
         if (m > cap) throw std::runtime_error("table capacity too small");
         if (!s.ft.equal_spaced) throw std::runtime_error("Sersic table knots not equally spaced");
@@ -66,11 +69,15 @@ int gsmi_sersic_table(double n, const double* gsp, double* args, double* vals, d
     });
 }
 
-int gsmi_sersic_kvalue(double n, const double* gsp, const double* ksq, double* out, int count)
+int gsmi_sersic_kvalue(double n, double trunc, const double* gsp, const double* ksq, double* out,
+                       int count)
 {
-    if (!ksq || !out || count < 0) { set_error("gsmi_sersic_kvalue: bad argument"); return GSMI_E_ARG; }
+    if (!ksq || !out || count < 0 || !(trunc >= 0.)) {
+        set_error("gsmi_sersic_kvalue: bad argument");
+        return GSMI_E_ARG;
+    }
     return guarded("gsmi_sersic_kvalue", [&] {
-        const SersicInfo& s = sersic_info(n, GSP::from(gsp));
+        const SersicInfo& s = sersic_info(n, trunc, GSP::from(gsp));
         for (int i = 0; i < count; ++i) {
             double q = ksq[i];
             if (q >= s.ksq_max) out[i] = (s.highk_a + s.highk_b / std::sqrt(q)) / q;
@@ -84,6 +91,22 @@ int gsmi_sersic_hlr(double n, double flux_fraction, double* out)
 {
     if (!out || !is_finite(n)) { set_error("gsmi_sersic_hlr: bad argument"); return GSMI_E_ARG; }
     return guarded("gsmi_sersic_hlr", [&] { *out = sersic_hlr(n, flux_fraction); });
+}
+
+int gsmi_sersic_integrated_flux(double n, double r, double* out)
+{
+    if (!out || !is_finite(n) || !(r >= 0.)) { set_error("bad argument"); return GSMI_E_ARG; }
+    return guarded("gsmi_sersic_integrated_flux", [&] { *out = sersic_integrated_flux(n, r); });
+}
+
+int gsmi_sersic_truncated_scale(double n, double hlr, double trunc, double* out)
+{
+    if (!out || !is_finite(n) || !(hlr > 0.) || !(trunc > 0.)) {
+        set_error("gsmi_sersic_truncated_scale: bad argument");
+        return GSMI_E_ARG;
+    }
+    return guarded("gsmi_sersic_truncated_scale",
+                   [&] { *out = sersic_truncated_scale(n, hlr, trunc); });
 }
 
 int gsmi_kolmogorov_info(const double* gsp, double* out)

@@ -452,7 +452,85 @@ double hankel_inf(RadialFn f, const void* ctx, double k, double relerr, double a
 }
 
 // ---------------------------------------------------------------------------------------------
-// Sersic (untruncated): src/SBSersic.cpp:223-669
+// Hankel transforms over a finite range (the truncated profiles' tables)
+// ---------------------------------------------------------------------------------------------
+namespace {
+// Gauss-Legendre rule of 16 points on [-1, 1] (nodes by Newton's method on P_16).
+struct GaussLegendre16 {
+    double x[16], w[16];
+    GaussLegendre16()
+    {
+        const int n = 16;
+        auto legendre = [n](double z, double* dp) {       // P_n(z) and P_n'(z)
+            double p0 = 1., p1 = z;
+            for (int j = 2; j <= n; ++j) {
+                double p2 = ((2. * j - 1.) * z * p1 - (j - 1.) * p0) / j;
+                p0 = p1;
+                p1 = p2;
+            }
+            *dp = n * (z * p1 - p0) / (z * z - 1.);
+            return p1;
+        };
+        for (int i = 0; i < n; ++i) {
+            double z = std::cos(M_PI * (i + 0.75) / (n + 0.5));
+            double dp = 1.;
+            for (int it = 0; it < 100; ++it) {
+                double dz = legendre(z, &dp) / dp;
+                z -= dz;
+                if (std::abs(dz) < 1e-16) break;
+            }
+            legendre(z, &dp);
+            x[i] = z;
+            w[i] = 2. / ((1. - z * z) * dp * dp);
+        }
+    }
+};
+
+template <class F>
+double gl16(const F& f, double a, double b)
+{
+    static const GaussLegendre16 g;
+    const double c = 0.5 * (a + b), h = 0.5 * (b - a);
+    double s = 0.;
+    for (int i = 0; i < 16; ++i) s += g.w[i] * f(c + h * g.x[i]);
+    return s * h;
+}
+
+// Adaptive 16-point Gauss-Legendre: a panel is accepted when its two halves agree with it to
+// tol; otherwise both halves recurse (depth-limited).
+template <class F>
+double gl_adapt(const F& f, double a, double b, double whole, double tol, int depth)
+{
+    const double m = 0.5 * (a + b);
+    const double l = gl16(f, a, m), r = gl16(f, m, b);
+    if (std::abs(l + r - whole) <= tol || depth >= 40) return l + r;
+    return gl_adapt(f, a, m, l, 0.5 * tol, depth + 1) + gl_adapt(f, m, b, r, 0.5 * tol, depth + 1);
+}
+
+// int_0^rmax f(r) J0(k r) r dr with the reference's split points (the first 10 zeros of J0
+// below k rmax, src/math/Hankel.cpp:37-63), each panel integrated to ~1e-15 of scale.
+template <class F>
+double hankel_trunc_gl(const F& f, double k, double rmax, double scale)
+{
+    auto g = [&](double r) { return r * f(r) * bessel_j0(k * r); };
+    std::vector<double> cuts{0.};
+    for (int s = 1; s <= 10; ++s) {
+        double root = bessel_j0_zero(s);
+        if (root > k * rmax) break;
+        cuts.push_back(root / k);
+    }
+    cuts.push_back(rmax);
+    double sum = 0.;
+    for (size_t i = 0; i + 1 < cuts.size(); ++i)
+        sum += gl_adapt(g, cuts[i], cuts[i + 1], gl16(g, cuts[i], cuts[i + 1]), 1e-15 * scale, 0);
+    return sum;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------
+// Sersic: src/SBSersic.cpp:223-775 (truncated: the flux fraction, the truncated gamma
+// functions of the low-k series and hankel_trunc in buildFT, CalculateTruncatedScale)
 // ---------------------------------------------------------------------------------------------
 namespace {
 struct MissingFlux {
@@ -482,9 +560,13 @@ double sersic_radial(double r, const void* ctx)
 void build_sersic(SersicInfo& info, const GSP& gsp)
 {
     const double n = info.n;
+    const bool truncated = info.trunc > 0.;
     info.invn = 1. / n;
     info.gamma2n = std::tgamma(2. * n);
-    info.b = calculate_b(n, 1.);
+    // getFluxFraction: SersicIntegratedFlux(n, trunc) = P(2n, trunc^(1/n))
+    info.flux_fraction = truncated ? gamma_p(2. * n, fast_pow(info.trunc, 1. / n)) : 1.;
+    const double ff = info.flux_fraction;
+    info.b = calculate_b(n, ff);
     info.hlr = std::pow(info.b, n);
 
     // stepK: radius enclosing all but folding_threshold of the flux (SBSersic.cpp:283-297,
@@ -508,18 +590,27 @@ void build_sersic(SersicInfo& info, const GSP& gsp)
             z = s.root();
         }
         double R = std::pow(z, n);
+        if (truncated && info.trunc < R) R = info.trunc;
         R = std::max(R, gsp.stepk_minimum_hlr);
         info.stepk = M_PI / R;
     }
 
     // buildFT (SBSersic.cpp:387-537)
     double gamma4n = std::tgamma(4. * n), gamma6n = std::tgamma(6. * n), gamma8n = std::tgamma(8. * n);
-    info.kderiv2 = -gamma4n / (4. * info.gamma2n) / 1.;
-    info.kderiv4 = gamma6n / (64. * info.gamma2n) / 1.;
-    double kderiv6 = gamma8n / (2304. * info.gamma2n) / 1.;
+    if (truncated) {
+        double z = std::pow(info.trunc, 1. / n);
+        gamma4n *= gamma_p(4. * n, z);
+        gamma6n *= gamma_p(6. * n, z);
+        gamma8n *= gamma_p(8. * n, z);
+    }
+    info.kderiv2 = -gamma4n / (4. * info.gamma2n) / ff;
+    info.kderiv4 = gamma6n / (64. * info.gamma2n) / ff;
+    double kderiv6 = gamma8n / (2304. * info.gamma2n) / ff;
     double kmin = std::pow(gsp.kvalue_accuracy / kderiv6, 1. / 6.);
     info.ksq_min = kmin * kmin;
-    double hankel_norm = 1. * n * info.gamma2n;
+    double hankel_norm = ff * n * info.gamma2n;
+    const double invn = info.invn;
+    auto radial = [invn](double r) { return expd(-fast_pow(r, invn)); };
     double dlogk = gsp.table_spacing * std::sqrt(std::sqrt(gsp.kvalue_accuracy / 10.));
     int n_correct = 0;
     const int n_fit = 10;
@@ -534,8 +625,12 @@ void build_sersic(SersicInfo& info, const GSP& gsp)
     for (double logk = std::log(kmin) - 0.001; logk < std::log(500.); logk += dlogk) {
         double k = expd(logk);
         double ksq = k * k;
-        double val = hankel_inf(sersic_radial, &info.invn, k, gsp.integration_relerr,
-                                gsp.integration_abserr * hankel_norm);
+        // the reference integrates to its own tolerances (hankel_trunc: adaptive GKP); the
+        // truncated integral here is taken to ~1e-15 of hankel_norm on the same split points
+        double val = truncated
+            ? hankel_trunc_gl(radial, k, info.trunc, hankel_norm)
+            : hankel_inf(sersic_radial, &info.invn, k, gsp.integration_relerr,
+                         gsp.integration_abserr * hankel_norm);
         val /= hankel_norm;
         double f0 = val * ksq;
         info.ft.args.push_back(logk);
@@ -591,16 +686,18 @@ void build_sersic(SersicInfo& info, const GSP& gsp)
 }
 }  // namespace
 
-const SersicInfo& sersic_info(double n, const GSP& gsp)
+const SersicInfo& sersic_info(double n, double trunc, const GSP& gsp)
 {
     if (!(n >= 0.3 && n <= 6.2)) throw std::runtime_error("Requested Sersic index out of range");
+    if (trunc < 0.) throw std::runtime_error("Sersic trunc must be >= 0");
     std::lock_guard<std::recursive_mutex> lock(table_mutex());
-    static std::map<std::pair<double, GSP>, std::unique_ptr<SersicInfo>> cache;
-    auto key = std::make_pair(n, gsp);
+    static std::map<std::tuple<double, double, GSP>, std::unique_ptr<SersicInfo>> cache;
+    auto key = std::make_tuple(n, trunc, gsp);
     auto it = cache.find(key);
     if (it != cache.end()) return *it->second;
     std::unique_ptr<SersicInfo> info(new SersicInfo);
     info->n = n;
+    info->trunc = trunc;
     build_sersic(*info, gsp);
     return *cache.emplace(key, std::move(info)).first->second;
 }
@@ -608,6 +705,36 @@ const SersicInfo& sersic_info(double n, const GSP& gsp)
 double sersic_hlr(double n, double flux_fraction)
 {
     return std::pow(calculate_b(n, flux_fraction), n);
+}
+
+double sersic_integrated_flux(double n, double r)
+{
+    return gamma_p(2. * n, fast_pow(r, 1. / n));
+}
+
+// SersicTruncatedScale / CalculateTruncatedScale (SBSersic.cpp:693-775): the scale radius, in
+// units of hlr, for which hlr encloses half of the flux within trunc; Brent on
+// 2 P(2n, b) - P(2n, x b) (x = (trunc / hlr)^(1/n)) from the asymptotic guess (A&S 6.5.32).
+double sersic_truncated_scale(double n, double hlr, double trunc)
+{
+    const double invn = 1. / n;
+    double b = calculate_b(n, 1.);
+    const double t = trunc / hlr;
+    if (t <= std::sqrt(2.))
+        throw std::runtime_error("Sersic truncation must be larger than sqrt(2)*half_light_radius.");
+    const double x = std::pow(t, invn);
+    double b1 = (std::log(0.5) + (2. * n - 1) * std::log(x)) / (x - 1.);
+    if (b1 < 1.e-3 * b) b1 = b / 2;
+    const double two_n = 2. * n;
+    auto func = [two_n, x](double bb) {
+        double f1 = gamma_p(two_n, bb);
+        double f2 = gamma_p(two_n, x * bb);
+        return (2. * f1 - f2) * std::tgamma(two_n);
+    };
+    auto s = make_brent(func, b1, b);
+    s.bracket_lower_with_limit(0.);
+    b = s.root();
+    return hlr * (1. / std::pow(b, n));
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -735,77 +862,6 @@ double moffat_pow_mbeta(double x, double beta, double xva)
     if (std::abs(beta - 3.5) < xva) return 1. / (x * x * x * std::sqrt(x));
     if (std::abs(beta - 4.) < xva) { double x2 = x * x; return 1. / (x2 * x2); }
     return fast_pow(x, -beta);
-}
-
-// Gauss-Legendre rule of 16 points on [-1, 1] (nodes by Newton's method on P_16).
-struct GaussLegendre16 {
-    double x[16], w[16];
-    GaussLegendre16()
-    {
-        const int n = 16;
-        auto legendre = [n](double z, double* dp) {       // P_n(z) and P_n'(z)
-            double p0 = 1., p1 = z;
-            for (int j = 2; j <= n; ++j) {
-                double p2 = ((2. * j - 1.) * z * p1 - (j - 1.) * p0) / j;
-                p0 = p1;
-                p1 = p2;
-            }
-            *dp = n * (z * p1 - p0) / (z * z - 1.);
-            return p1;
-        };
-        for (int i = 0; i < n; ++i) {
-            double z = std::cos(M_PI * (i + 0.75) / (n + 0.5));
-            double dp = 1.;
-            for (int it = 0; it < 100; ++it) {
-                double dz = legendre(z, &dp) / dp;
-                z -= dz;
-                if (std::abs(dz) < 1e-16) break;
-            }
-            legendre(z, &dp);
-            x[i] = z;
-            w[i] = 2. / ((1. - z * z) * dp * dp);
-        }
-    }
-};
-
-template <class F>
-double gl16(const F& f, double a, double b)
-{
-    static const GaussLegendre16 g;
-    const double c = 0.5 * (a + b), h = 0.5 * (b - a);
-    double s = 0.;
-    for (int i = 0; i < 16; ++i) s += g.w[i] * f(c + h * g.x[i]);
-    return s * h;
-}
-
-// Adaptive 16-point Gauss-Legendre: a panel is accepted when its two halves agree with it to
-// tol; otherwise both halves recurse (depth-limited).
-template <class F>
-double gl_adapt(const F& f, double a, double b, double whole, double tol, int depth)
-{
-    const double m = 0.5 * (a + b);
-    const double l = gl16(f, a, m), r = gl16(f, m, b);
-    if (std::abs(l + r - whole) <= tol || depth >= 40) return l + r;
-    return gl_adapt(f, a, m, l, 0.5 * tol, depth + 1) + gl_adapt(f, m, b, r, 0.5 * tol, depth + 1);
-}
-
-// int_0^rmax f(r) J0(k r) r dr with the reference's split points (the first 10 zeros of J0
-// below k rmax, src/math/Hankel.cpp:37-63), each panel integrated to ~1e-15 of scale.
-template <class F>
-double hankel_trunc_gl(const F& f, double k, double rmax, double scale)
-{
-    auto g = [&](double r) { return r * f(r) * bessel_j0(k * r); };
-    std::vector<double> cuts{0.};
-    for (int s = 1; s <= 10; ++s) {
-        double root = bessel_j0_zero(s);
-        if (root > k * rmax) break;
-        cuts.push_back(root / k);
-    }
-    cuts.push_back(rmax);
-    double sum = 0.;
-    for (size_t i = 0; i + 1 < cuts.size(); ++i)
-        sum += gl_adapt(g, cuts[i], cuts[i + 1], gl16(g, cuts[i], cuts[i + 1]), 1e-15 * scale, 0);
-    return sum;
 }
 
 struct MoffatTruncKey {

@@ -35,14 +35,17 @@ struct Spline {
 typedef double (*RadialFn)(double r, const void* ctx);
 double hankel_inf(RadialFn f, const void* ctx, double k, double relerr, double abserr);
 
+// SersicInfo in units of r0; trunc (in units of r0) > 0 for a truncated profile.
 struct SersicInfo {
-    double n = 0., invn = 0., gamma2n = 0.;
+    double n = 0., trunc = 0., invn = 0., gamma2n = 0., flux_fraction = 1.;
     double maxk = 0., stepk = 0., hlr = 0., b = 0.;
     double kderiv2 = 0., kderiv4 = 0., ksq_min = 0., ksq_max = 0., highk_a = 0., highk_b = 0.;
     Spline ft;    // f(k) k^2 as a function of log k
 };
-const SersicInfo& sersic_info(double n, const GSP& gsp);
+const SersicInfo& sersic_info(double n, double trunc, const GSP& gsp);
 double sersic_hlr(double n, double flux_fraction);
+double sersic_integrated_flux(double n, double r);
+double sersic_truncated_scale(double n, double hlr, double trunc);
 
 struct KolmogorovInfo { double maxk = 0., stepk = 0.; };
 const KolmogorovInfo& kolmogorov_info(const GSP& gsp);

@@ -1,4 +1,6 @@
-"""Sersic and DeVaucouleurs, untruncated (galsim/sersic.py; src/SBSersic.cpp)."""
+"""Sersic and DeVaucouleurs, untruncated or truncated (galsim/sersic.py; src/SBSersic.cpp)."""
+import math
+
 import numpy as np
 
 from . import _lib
@@ -8,19 +10,21 @@ from .gsparams import GSParams
 _INFO = {}
 
 
-def sersic_info(n, gsparams):
-    """(scalars, args, vals, y2) of the FP64 k-space table for index n (host, cached)."""
-    key = (float(n), gsparams)
+def sersic_info(n, gsparams, trunc=0.):
+    """(scalars, args, vals, y2) of the FP64 k-space table for index n and truncation radius
+    trunc in units of the scale radius (0: untruncated) (host, cached)."""
+    key = (float(n), float(trunc), gsparams)
     if key not in _INFO:
         g = _lib.gsp_array(gsparams)
-        out = np.zeros(10)
+        out = np.zeros(11)
         ntab = np.zeros(1, dtype=np.int32)
-        _lib.check(_lib.lib().gsmi_sersic_info(float(n), _lib.ptr(g), _lib.ptr(out),
-                                               _lib.ptr(ntab)), "Sersic info")
+        _lib.check(_lib.lib().gsmi_sersic_info(float(n), float(trunc), _lib.ptr(g),
+                                               _lib.ptr(out), _lib.ptr(ntab)), "Sersic info")
         m = int(ntab[0])
         args, vals, y2 = np.zeros(m), np.zeros(m), np.zeros(m)
-        _lib.check(_lib.lib().gsmi_sersic_table(float(n), _lib.ptr(g), _lib.ptr(args),
-                                                _lib.ptr(vals), _lib.ptr(y2), m), "Sersic table")
+        _lib.check(_lib.lib().gsmi_sersic_table(float(n), float(trunc), _lib.ptr(g),
+                                                _lib.ptr(args), _lib.ptr(vals), _lib.ptr(y2), m),
+                   "Sersic table")
         _INFO[key] = (out, args, vals, y2)
     return _INFO[key]
 
@@ -39,20 +43,46 @@ class Sersic(Leaf):
         self._gsparams = GSParams.check(gsparams)
         if self._n < Sersic._minimum_n or self._n > Sersic._maximum_n:
             raise _lib.GalSimAMDRangeError("Requested Sersic index %r out of range" % self._n)
-        if self._trunc != 0.:
-            raise NotImplementedError("truncated Sersic profiles are not part of galsim_amd yet")
+        if self._trunc < 0.:
+            raise _lib.GalSimAMDRangeError("Sersic trunc must be > 0")
+        out = np.zeros(1)
         if half_light_radius is not None:
             if scale_radius is not None:
                 raise ValueError("Only one of scale_radius or half_light_radius may be specified")
-            out = np.zeros(1)
-            _lib.check(_lib.lib().gsmi_sersic_hlr(self._n, 1., _lib.ptr(out)), "SersicHLR")
             self._hlr = float(half_light_radius)
-            self._r0 = self._hlr / float(out[0])
+            if self._trunc == 0. or flux_untruncated:
+                _lib.check(_lib.lib().gsmi_sersic_hlr(self._n, 1., _lib.ptr(out)), "SersicHLR")
+                self._r0 = self._hlr / float(out[0])
+            else:
+                # galsim/sersic.py: trunc must exceed sqrt(2) hlr; SersicTruncatedScale
+                if self._trunc <= math.sqrt(2.) * self._hlr:
+                    raise _lib.GalSimAMDRangeError(
+                        "Sersic trunc must be > sqrt(2) * half_light_radius")
+                _lib.check(_lib.lib().gsmi_sersic_truncated_scale(
+                    self._n, self._hlr, self._trunc, _lib.ptr(out)), "SersicTruncatedScale")
+                self._r0 = float(out[0])
         elif scale_radius is not None:
             self._r0 = float(scale_radius)
             self._hlr = 0.
         else:
             raise ValueError("Either scale_radius or half_light_radius must be specified")
+        if self._trunc > 0.:
+            _lib.check(_lib.lib().gsmi_sersic_integrated_flux(
+                self._n, self._trunc / self._r0, _lib.ptr(out)), "SersicIntegratedFlux")
+            self._flux_fraction = float(out[0])
+            if flux_untruncated:
+                # flux was the untruncated profile's; the hlr follows from the truncated one
+                self._flux *= self._flux_fraction
+                self._hlr = 0.
+        else:
+            self._flux_fraction = 1.
+
+    def _info(self):
+        return sersic_info(self._n, self.gsparams, self._trunc / self._r0)
+
+    @property
+    def trunc(self):
+        return self._trunc
 
     @property
     def n(self):
@@ -65,31 +95,33 @@ class Sersic(Leaf):
     @property
     def half_light_radius(self):
         if self._hlr == 0.:
-            self._hlr = self._r0 * float(sersic_info(self._n, self.gsparams)[0][2])
+            self._hlr = self._r0 * float(self._info()[0][2])
         return self._hlr
 
     @property
     def _maxk(self):
-        return float(sersic_info(self._n, self.gsparams)[0][0]) / self._r0
+        return float(self._info()[0][0]) / self._r0
 
     @property
     def _stepk(self):
-        return float(sersic_info(self._n, self.gsparams)[0][1]) / self._r0
+        return float(self._info()[0][1]) / self._r0
 
     def _fill(self, rec, ctx, jac):
-        sc, args, vals, y2 = sersic_info(self._n, self.gsparams)
+        sc, args, vals, y2 = self._info()
         rec['scale'] = self._r0
         rec['flux'] = self._flux
-        rec['table'] = ctx.add_table((self._n, self.gsparams), args, vals, y2)
+        rec['table'] = ctx.add_table((self._n, self._trunc / self._r0, self.gsparams), args,
+                                     vals, y2)
         rec['ntab'] = len(args)
         rec['p'][:] = [sc[4], sc[5], sc[6], sc[7], sc[8], sc[9]]
 
     def withFlux(self, flux):
-        return Sersic(self._n, scale_radius=self._r0, flux=flux, gsparams=self.gsparams)
+        return Sersic(self._n, scale_radius=self._r0, trunc=self._trunc, flux=flux,
+                      gsparams=self.gsparams)
 
     def __repr__(self):
-        return 'galsim_amd.Sersic(n=%r, scale_radius=%r, flux=%r)' % (self._n, self._r0,
-                                                                      self._flux)
+        return 'galsim_amd.Sersic(n=%r, scale_radius=%r, trunc=%r, flux=%r)' % (
+            self._n, self._r0, self._trunc, self._flux)
 
 
 class DeVaucouleurs(Sersic):

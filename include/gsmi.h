@@ -141,15 +141,24 @@ int gsmi_version(void);
 /* goodFFTSize (src/Image.cpp:1141-1152; pybind pysrc/Image.cpp:85). */
 int gsmi_good_fft_size(int n);
 
-/* SBSersic info (src/SBSersic.cpp:223-537 SersicInfo; pysrc/SBSersic.cpp SBSersic ctor):
- * out = {maxK, stepK, HLR, b, ksq_min, ksq_max, highk_a, highk_b, kderiv2, kderiv4} in units
- * of the scale radius; *ntab = number of table knots.  Untruncated profiles only. */
-int gsmi_sersic_info(double n, const double* gsp, double* out, int* ntab);
+/* SBSersic info (src/SBSersic.cpp:223-537 SersicInfo(n, trunc, gsparams); pysrc/SBSersic.cpp
+ * SBSersic ctor): out = {maxK, stepK, HLR, b, ksq_min, ksq_max, highk_a, highk_b, kderiv2,
+ * kderiv4, flux fraction} in units of the scale radius; trunc (units of the scale radius) = 0
+ * for an untruncated profile; *ntab = number of table knots.  The truncated table's Hankel
+ * transforms are integrated here to ~1e-15 of the k = 0 integral on the reference's split
+ * points; the reference's adaptive integrator stops at its own tolerance. */
+int gsmi_sersic_info(double n, double trunc, const double* gsp, double* out, int* ntab);
 /* The Sersic k-space table: knots (log k), values f(k) k^2 and spline second derivatives. */
-int gsmi_sersic_table(double n, const double* gsp, double* args, double* vals, double* y2,
-                      int cap);
+int gsmi_sersic_table(double n, double trunc, const double* gsp, double* args, double* vals,
+                      double* y2, int cap);
 /* SersicHLR (src/SBSersic.cpp:665-669; pybind pysrc/SBSersic.cpp). */
 int gsmi_sersic_hlr(double n, double flux_fraction, double* out);
+/* SersicIntegratedFlux (src/SBSersic.cpp:329-333): fraction of the flux within r (units of the
+ * scale radius). */
+int gsmi_sersic_integrated_flux(double n, double r, double* out);
+/* SersicTruncatedScale (src/SBSersic.cpp:693-775): the scale radius for which hlr encloses half
+ * of the flux within trunc. */
+int gsmi_sersic_truncated_scale(double n, double hlr, double trunc, double* out);
 /* KolmogorovInfo (src/SBKolmogorov.cpp:337-419): out = {maxK, stepK} in units of k0. */
 int gsmi_kolmogorov_info(const double* gsp, double* out);
 /* SBMoffat untruncated (src/SBMoffat.cpp:163-244, 497-549): out = {maxK, stepK, knorm,
@@ -177,7 +186,8 @@ int gsmi_moffat_trunc_table(double beta, double scale_radius, double trunc, cons
 int gsmi_exponential_cliprows(double r0, const double* gsp, const double* jac, int Nk, double dk,
                               int32_t* out);
 /* Host evaluation of a Sersic leaf k-value (SersicInfo::kValue, src/SBSersic.cpp:363-376). */
-int gsmi_sersic_kvalue(double n, const double* gsp, const double* ksq, double* out, int count);
+int gsmi_sersic_kvalue(double n, double trunc, const double* gsp, const double* ksq, double* out,
+                       int count);
 
 /* ---------------------------------------------------------------------------------------
  * Device rendering.  All buffers are device pointers; stamps/ops/leaves/tables are device

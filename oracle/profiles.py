@@ -583,39 +583,85 @@ def sersic_hlr(n, flux_fraction=1.):
     return math.pow(sersic_calculate_b(n, flux_fraction), n)
 
 
+def sersic_truncated_scale(n, hlr, trunc):
+    """SersicTruncatedScale / CalculateTruncatedScale (src/SBSersic.cpp:693-775)."""
+    b = sersic_calculate_b(n, 1.)
+    t = trunc / hlr
+    if t <= math.sqrt(2.):
+        raise ValueError("Sersic truncation must be larger than sqrt(2)*half_light_radius.")
+    x = math.pow(t, 1. / n)
+    b1 = (math.log(0.5) + (2. * n - 1) * math.log(x)) / (x - 1.)
+    if b1 < 1.e-3 * b:
+        b1 = b / 2
+    two_n = 2. * n
+
+    def func(bb):
+        return (2. * _gamma_p(two_n, bb) - _gamma_p(two_n, x * bb)) * math.gamma(two_n)
+    s = nm.Solve(func, b1, b)
+    s.bracket_lower_with_limit(0.)
+    return hlr / math.pow(s.root(), n)
+
+
+_GL24 = np.polynomial.legendre.leggauss(24)
+
+
+def _hankel_trunc_panels(radial, k, rmax):
+    """int_0^rmax radial(r) J0(k r) r dr by composite 24-point Gauss-Legendre (numpy): panels a
+    quarter period of J0 wide (at most rmax / 16), the first one refined geometrically towards
+    r = 0 for the r^(1/n) cusp of the Sersic profile.  Converges to ~1e-15 of the k = 0 value."""
+    w = min(0.5 * math.pi / k, rmax / 16.) if k > 0 else rmax / 16.
+    edges = np.arange(0., rmax, w)
+    edges = np.append(edges, rmax)
+    fine = edges[1] * np.logspace(-40, 0, 41, base=2.)
+    edges = np.concatenate([[0.], fine, edges[2:]])
+    a, b = edges[:-1], edges[1:]
+    x, wt = _GL24
+    r = 0.5 * (a + b)[:, None] + 0.5 * (b - a)[:, None] * x[None, :]
+    f = r * radial(r) * sps.j0(k * r)
+    return float(np.sum(0.5 * (b - a) * (f @ wt)))
+
+
 _SERSIC_INFO = {}
 
 
 class SersicInfo:
-    """SersicInfo for an untruncated profile (src/SBSersic.cpp:223-537).
+    """SersicInfo (src/SBSersic.cpp:223-537), in units of r0; trunc > 0 (units of r0) for a
+    truncated profile: the flux fraction P(2n, trunc^(1/n)), the truncated gamma functions of
+    the low-k series, the stepK limit at trunc, and the table's Hankel transforms over
+    [0, trunc] (here by QUADPACK to 1e-14; the reference's adaptive GKP stops at its own
+    tolerance, ~3e-9 of the k = 0 value).
 
     Pinned against oracle/_ref (the reference's own SersicInfo compiled from its sources):
-    table, ksq_min/ksq_max, high-k coefficients, maxK, stepK and HLR; and by the Sersic
-    maxk/stepk/r0 known answers of tests/test_sersic.py:497-558 (untruncated n=3.1 case).
+    table, ksq_min/ksq_max, high-k coefficients, maxK, stepK and HLR, truncated or not; and by
+    the Sersic maxk/stepk/r0 known answers of tests/test_sersic.py:497-558 (n=3.1, truncated
+    and untruncated).
     """
 
-    def __init__(self, n, gsp):
+    def __init__(self, n, gsp, trunc=0.):
         self.n = float(n)
         self.gsp = gsp
+        self.trunc = float(trunc)
         self.invn = 1. / self.n
         self.inv2n = 0.5 * self.invn
         self.gamma2n = math.gamma(2. * self.n)
-        self.flux_fraction = 1.
+        # getFluxFraction (src/SBSersic.cpp:335-350): SersicIntegratedFlux(n, trunc)
+        self.flux_fraction = (_gamma_p(2. * self.n, fast_pow(self.trunc, self.invn))
+                              if self.trunc > 0. else 1.)
         self._b = None
         self._build_ft()
         self._stepk = None
 
     @staticmethod
-    def get(n, gsp):
-        key = (float(n), gsp.key())
+    def get(n, gsp, trunc=0.):
+        key = (float(n), float(trunc), gsp.key())
         if key not in _SERSIC_INFO:
-            _SERSIC_INFO[key] = SersicInfo(n, gsp)
+            _SERSIC_INFO[key] = SersicInfo(n, gsp, trunc)
         return _SERSIC_INFO[key]
 
     @property
     def b(self):
         if self._b is None:
-            self._b = sersic_calculate_b(self.n, 1.)
+            self._b = sersic_calculate_b(self.n, self.flux_fraction)
         return self._b
 
     @property
@@ -646,20 +692,27 @@ class SersicInfo:
 
     @property
     def stepk(self):
-        # SersicInfo::stepK (src/SBSersic.cpp:283-297), untruncated
+        # SersicInfo::stepK (src/SBSersic.cpp:304-318)
         if self._stepk is None:
             R = self.missing_flux_radius(self.gsp.folding_threshold)
+            if self.trunc > 0. and self.trunc < R:
+                R = self.trunc
             R = max(R, self.gsp.stepk_minimum_hlr)
             self._stepk = math.pi / R
         return self._stepk
 
     def _build_ft(self):
-        # SersicInfo::buildFT (src/SBSersic.cpp:387-537), untruncated branch
+        # SersicInfo::buildFT (src/SBSersic.cpp:387-537)
         gsp = self.gsp
         n = self.n
         gamma4n = math.gamma(4. * n)
         gamma6n = math.gamma(6. * n)
         gamma8n = math.gamma(8. * n)
+        if self.trunc > 0.:
+            z = math.pow(self.trunc, 1. / n)
+            gamma4n *= _gamma_p(4. * n, z)
+            gamma6n *= _gamma_p(6. * n, z)
+            gamma8n *= _gamma_p(8. * n, z)
         ff = self.flux_fraction
         self.kderiv2 = -gamma4n / (4. * self.gamma2n) / ff
         self.kderiv4 = gamma6n / (64. * self.gamma2n) / ff
@@ -683,8 +736,11 @@ class SersicInfo:
         while logk < math.log(500.):
             k = nm.expd(logk)
             ksq = k * k
-            val = nm.hankel_inf(radial, k, gsp.integration_relerr,
-                                gsp.integration_abserr * hankel_norm)
+            if self.trunc > 0.:
+                val = _hankel_trunc_panels(radial, k, self.trunc)
+            else:
+                val = nm.hankel_inf(radial, k, gsp.integration_relerr,
+                                    gsp.integration_abserr * hankel_norm)
             val /= hankel_norm
             f0 = val * ksq
             args.append(logk)
@@ -747,19 +803,28 @@ class SersicInfo:
 
 
 class Sersic:
-    """SBSersic, untruncated (src/SBSersic.cpp:71-90, 237-281); galsim/sersic.py:201-250."""
+    """SBSersic (src/SBSersic.cpp:71-90, 237-281); galsim/sersic.py:201-250 (the truncation
+    radius from the half-light radius, flux_untruncated)."""
 
     def __init__(self, n, half_light_radius=None, scale_radius=None, flux=1., trunc=0.,
-                 gsparams=None):
-        if trunc != 0.:
-            raise NotImplementedError("truncated Sersic is not restated in the oracle")
+                 flux_untruncated=False, gsparams=None):
         self.n = float(n)
         self.flux = float(flux)
+        self.trunc = float(trunc)
         self.gsparams = gsparams or DEFAULT_GSPARAMS
         if half_light_radius is not None:
-            scale_radius = half_light_radius / sersic_hlr(self.n, 1.)
+            if self.trunc == 0. or flux_untruncated:
+                scale_radius = half_light_radius / sersic_hlr(self.n, 1.)
+            else:
+                scale_radius = sersic_truncated_scale(self.n, half_light_radius, self.trunc)
         self.r0 = float(scale_radius)
-        self.info = SersicInfo.get(self.n, self.gsparams)
+        self.info = SersicInfo.get(self.n, self.gsparams, self.trunc / self.r0)
+        if self.trunc > 0. and flux_untruncated:
+            self.flux *= self.info.flux_fraction
+
+    @property
+    def half_light_radius(self):
+        return self.r0 * self.info.hlr
 
     @property
     def maxk(self):

@@ -56,7 +56,8 @@ def test_sersic_tables_match_reference(n):
     assert sc[2] == pytest.approx(hlr, rel=1e-13)
     ksq = np.exp(np.linspace(np.log(1e-8), np.log(1e6), 4001))
     out = np.zeros_like(ksq)
-    _lib.check(_lib.lib().gsmi_sersic_kvalue(n, None, _lib.ptr(ksq), _lib.ptr(out), len(ksq)))
+    _lib.check(_lib.lib().gsmi_sersic_kvalue(n, 0., None, _lib.ptr(ksq), _lib.ptr(out),
+                                             len(ksq)))
     np.testing.assert_allclose(out, ref.sersic_kvalue(n, ksq), rtol=1e-10, atol=2e-15)
 
 
@@ -128,8 +129,8 @@ def test_errors_are_loud():
         g.Sersic(7.0, half_light_radius=1.)
     with pytest.raises(g.GalSimAMDRangeError):
         g.Moffat(1.05, scale_radius=1.)
-    with pytest.raises(NotImplementedError):
-        g.Sersic(3., half_light_radius=1., trunc=4.)
+    with pytest.raises(g.GalSimAMDRangeError):
+        g.Sersic(3., half_light_radius=1., trunc=1.4)         # trunc <= sqrt(2) hlr
     out = np.zeros(10)
-    rc = _lib.lib().gsmi_sersic_info(float('nan'), None, _lib.ptr(out), None)
+    rc = _lib.lib().gsmi_sersic_info(float('nan'), 0., None, _lib.ptr(out), None)
     assert rc == -1 and b"bad argument" in _lib.lib().gsmi_last_error()

@@ -89,6 +89,11 @@ def test_exponential_moffat_seeded_sweep():
     ("moffat_trunc_conv", lambda m: m.Convolve([m.Exponential(half_light_radius=0.6, flux=20.),
                                                 m.Moffat(3.2, half_light_radius=0.5, trunc=2.)]),
      (56, 56)),
+    ("sersic_trunc_conv", lambda m: m.Convolve([m.Sersic(2.5, half_light_radius=0.6,
+                                                         trunc=2.0, flux=7.),
+                                                m.Kolmogorov(fwhm=0.7)]), (56, 56)),
+    ("sersic_trunc_sheared", lambda m: m.Sersic(1.5, scale_radius=0.3, trunc=1.5)
+     .shear(g1=-0.2, g2=0.25), (48, 48)),
     ("odd_nonsquare", lambda m: _expmoffat(m, 1.1, 0.2, 0.3), (37, 52)),
     ("exp_small_clipped", lambda m: m.Exponential(scale_radius=0.02).shear(g1=0.4, g2=0.1),
      (24, 24)),
