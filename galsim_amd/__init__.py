@@ -2,7 +2,9 @@
 MI355X.  The public names mirror galsim's: profiles, transformations, Convolve/Add, Image,
 GSParams, Shear; drawing runs in hand-written HIP kernels through the C ABI of libgsmi.so
 (include/gsmi.h)."""
-from ._lib import GalSimAMDError, GalSimAMDRangeError
+from . import fft
+from ._lib import (GalSimAMDBoundsError, GalSimAMDError, GalSimAMDRangeError,
+                   GalSimAMDUndefinedBoundsError, GalSimAMDValueError)
 from .box import Box, Pixel
 from .convolve import Convolution, Convolve
 from .exponential import Exponential
@@ -21,7 +23,8 @@ from .transform import Transform, Transformation, _Transform
 GalSimError = GalSimAMDError
 
 __all__ = ['Add', 'BoundsI', 'Box', 'Convolution', 'Convolve', 'DeVaucouleurs', 'Exponential',
-           'GSObject', 'GSParams', 'GalSimAMDError', 'GalSimAMDRangeError', 'GalSimError',
+           'GSObject', 'GSParams', 'GalSimAMDBoundsError', 'GalSimAMDError', 'GalSimAMDRangeError',
+           'GalSimAMDUndefinedBoundsError', 'GalSimAMDValueError', 'GalSimError',
            'Gaussian', 'Image', 'ImageCD', 'ImageCF', 'ImageD', 'ImageF', 'Kolmogorov', 'Moffat',
            'Pixel', 'PositionD', 'PositionI', 'Sersic', 'Shear', 'Sum', 'Transform',
            'Transformation']

@@ -68,6 +68,18 @@ class GalSimAMDRangeError(GalSimAMDError, ValueError):
     """A parameter outside the range the reference accepts (mirrors GalSimRangeError)."""
 
 
+class GalSimAMDValueError(GalSimAMDError, ValueError):
+    """An argument of the wrong shape or value (mirrors GalSimValueError)."""
+
+
+class GalSimAMDBoundsError(GalSimAMDError, ValueError):
+    """Image bounds that do not include a required point (mirrors GalSimBoundsError)."""
+
+
+class GalSimAMDUndefinedBoundsError(GalSimAMDError):
+    """An operation on an image with undefined bounds (mirrors GalSimUndefinedBoundsError)."""
+
+
 _lib = None
 
 # name -> (restype, argtypes); every symbol declared in include/gsmi.h
@@ -100,6 +112,10 @@ _SIGS = {
     'gsmi_draw_kimage_f64': (_I, [_P, _I, _P, _P, _P, _I, _I, _I, _I, _I, _D, _P, _P]),
     'gsmi_irfft_f64': (_I, [_P, _I, _P, _P, _P]),
     'gsmi_wrap_hermx_f64': (_I, [_P, _I, _I, _P, _P]),
+    'gsmi_cfft2_f64': (_I, [_P, _I, _I, _I, _I, _I, _P, _P, _P]),
+    'gsmi_rfft2_f64': (_I, [_P, _I, _I, _I, _I, _P, _P, _P]),
+    'gsmi_irfft2_f64': (_I, [_P, _I, _I, _I, _I, _P, _P, _P]),
+    'gsmi_hermitian_fill_f64': (_I, [_P, _I, _I, _I, _P, _P]),
     'gsmi_draw_list_f64': (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P]),
     'gsmi_draw_list_f32': (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P]),
     'gsmi_fold_list_f64': (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P]),

@@ -817,6 +817,26 @@ __device__ inline void bfly3(cplx<T>* v)
     v[2] = csub(m, id);
 }
 
+// Radix 5 with w = e^{+2 pi i / 5}: Y1,4 = x0 + c1 (x1 + x4) + c2 (x2 + x3) +- i (s1 (x1 - x4) +
+// s2 (x2 - x3)), Y2,3 = x0 + c2 (x1 + x4) + c1 (x2 + x3) +- i (s2 (x1 - x4) - s1 (x2 - x3)).
+template <typename T>
+__device__ inline void bfly5(cplx<T>* v)
+{
+    const T c1 = T(0.30901699437494742410), c2 = T(-0.80901699437494742410);   // cos 2pi/5, 4pi/5
+    const T s1 = T(0.95105651629515357212), s2 = T(0.58778525229247312917);    // sin 2pi/5, 4pi/5
+    const cplx<T> t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+    const cplx<T> t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
+    const cplx<T> a1 = {v[0].x + c1 * t1.x + c2 * t2.x, v[0].y + c1 * t1.y + c2 * t2.y};
+    const cplx<T> a2 = {v[0].x + c2 * t1.x + c1 * t2.x, v[0].y + c2 * t1.y + c1 * t2.y};
+    const cplx<T> b1 = {s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y};
+    const cplx<T> b2 = {s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y};
+    v[0] = cadd(v[0], cadd(t1, t2));
+    v[1] = {a1.x - b1.y, a1.y + b1.x};
+    v[4] = {a1.x + b1.y, a1.y - b1.x};
+    v[2] = {a2.x - b2.y, a2.y + b2.x};
+    v[3] = {a2.x + b2.y, a2.y - b2.x};
+}
+
 template <typename T, int R>
 __device__ __forceinline__ void stockham_pass(const cplx<T>* src, cplx<T>* dst, int n, int nseg, int Ns,
                                      const cplx<T>* tw, int twstride)
@@ -841,6 +861,8 @@ __device__ __forceinline__ void stockham_pass(const cplx<T>* src, cplx<T>* dst, 
         if (R == 2) bfly2(v);
         if (R == 3) bfly3(v);
         if (R == 4) bfly4(v);
+        if (R == 5) bfly5(v);
+        if (R == 8) bfly8(v);
         cplx<T>* d = dst + seg * n + tq * Ns * R + jj;
 #pragma unroll
         for (int r = 0; r < R; ++r) d[r * Ns] = v[r];

@@ -1,0 +1,547 @@
+// Two-dimensional discrete Fourier transforms of images (galsim.fft and Image.calculate_fft /
+// calculate_inverse_fft): complex <-> complex, real -> half-complex and half-complex -> real, with
+// the reference's shift_in / shift_out conventions (src/Image.cpp:729-1000, whose FFTW calls these
+// replace), on gfx950.
+//
+// A transform is two passes over lines held in LDS, one workgroup per group of lines:
+//   * rows (x, contiguous) and columns (y, stride = row width: a workgroup takes adjacent columns,
+//     so each row of its tile is one contiguous run);
+//   * the real transforms use the half-length complex transform of z(m) = x(2m) + i x(2m+1) and
+//     split its result into the even / odd halves (forward), or build that sequence from the
+//     Hermitian half-plane (inverse);
+//   * a line length whose prime factors are 2, 3 and 5 runs as a mixed-radix Stockham transform
+//     (radix 5, 3, 8, 4, 2); any other length L as Bluestein's chirp-z convolution, in LDS, with a
+//     5-smooth transform length S >= 2L - 1;
+//   * fftshift of an even axis is a rotation of its index by half its length: an input shift is
+//     applied where the line is read, an output shift where it is written (the reference
+//     multiplies by the equivalent alternating signs).
+// The inverse transform is e^{+2 pi i} (the LDS transforms' own sign); the forward one conjugates
+// on the way in and out.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/gsmi.h"
+#include "device_program.h"
+#include "launch.h"
+
+namespace gsmi {
+namespace {
+
+// How one line length is transformed in LDS.
+template <typename T>
+struct LinePlan {
+    int L;                    // transform length
+    int S;                    // LDS elements per line: L, or the Bluestein length
+    const cplx<T>* tw;        // exp(+2 pi i m / S), m < S
+    const cplx<T>* chirp;     // Bluestein only (else nullptr): exp(+i pi m^2 / L), m < L
+    const cplx<T>* bh;        // Bluestein only: DFT_S(b) / S with b_m = conj(chirp_|m|), m mod S
+};
+
+// Mixed-radix transform of nseg segments of any 5-smooth length n.
+template <typename T>
+__device__ cplx<T>* fft_lds_any(cplx<T>* a, cplx<T>* b, int n, int nseg, const cplx<T>* tw)
+{
+    int Ns = 1;
+    int rem = n;
+    cplx<T>* src = a;
+    cplx<T>* dst = b;
+    while (rem > 1) {
+        int R;
+        if (rem % 5 == 0) { stockham_pass<T, 5>(src, dst, n, nseg, Ns, tw, 1); R = 5; }
+        else if (rem % 3 == 0) { stockham_pass<T, 3>(src, dst, n, nseg, Ns, tw, 1); R = 3; }
+        else if (rem % 8 == 0) { stockham_pass<T, 8>(src, dst, n, nseg, Ns, tw, 1); R = 8; }
+        else if (rem % 4 == 0) { stockham_pass<T, 4>(src, dst, n, nseg, Ns, tw, 1); R = 4; }
+        else { stockham_pass<T, 2>(src, dst, n, nseg, Ns, tw, 1); R = 2; }
+        Ns *= R;
+        rem /= R;
+        __syncthreads();
+        cplx<T>* t = src; src = dst; dst = t;
+    }
+    return src;
+}
+
+// Y(k) = sum_n a(li * S + n) e^{+2 pi i n k / L} for every line li < nl, k < L; the input occupies
+// the first L entries of each line of a.  Returns the buffer (a or b) holding Y.  Bluestein:
+// 2 n k = n^2 + k^2 - (k - n)^2, so Y(k) = c(k) sum_n (x(n) c(n)) conj(c(k - n)), c(m) =
+// e^{+i pi m^2 / L}: a circular convolution of length S >= 2L - 1, done as
+// conj(G(conj(G(x c) . G(b) / S))) with G the e^{+2 pi i} transform of length S.
+template <typename T>
+__device__ cplx<T>* line_dft(cplx<T>* a, cplx<T>* b, int nl, const LinePlan<T>& p)
+{
+    if (!p.chirp) return fft_lds_any<T>(a, b, p.L, nl, p.tw);
+    const int L = p.L, S = p.S;
+    const float inv_s = 1.f / float(S);
+    for (int e = threadIdx.x; e < nl * S; e += blockDim.x) {
+        const int n = e - S * fdiv(e, inv_s);
+        a[e] = n < L ? cmul(a[e], p.chirp[n]) : cplx<T>{T(0), T(0)};
+    }
+    __syncthreads();
+    cplx<T>* r = fft_lds_any<T>(a, b, S, nl, p.tw);
+    cplx<T>* o = (r == a) ? b : a;
+    for (int e = threadIdx.x; e < nl * S; e += blockDim.x) {
+        const int k = e - S * fdiv(e, inv_s);
+        r[e] = cconj(cmul(r[e], p.bh[k]));
+    }
+    __syncthreads();
+    cplx<T>* y = fft_lds_any<T>(r, o, S, nl, p.tw);
+    for (int e = threadIdx.x; e < nl * S; e += blockDim.x) {
+        const int k = e - S * fdiv(e, inv_s);
+        if (k < L) y[e] = cmul(cconj(y[e]), p.chirp[k]);
+    }
+    __syncthreads();
+    return y;
+}
+
+// Complex transform of `nlines` lines of length L: element n of line l at l * ls + n * es (one of
+// ls, es is 1).  Reads line (l + lrot) mod nlines at element (n + erot) mod L, writes line l at
+// element (k + krot) mod L, times scale; conj_io makes it the forward (e^{-2 pi i}) transform.
+// grid: ceil(nlines / nl); dynamic LDS: 2 * nl * S complex.
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+c2c_lines(const cplx<T>* __restrict__ src, cplx<T>* __restrict__ dst, int nlines, int nl,
+          long long ls, long long es, int lrot, int erot, int krot, int conj_io, T scale,
+          LinePlan<T> p)
+{
+    extern __shared__ unsigned char smem_raw[];
+    cplx<T>* bufa = reinterpret_cast<cplx<T>*>(smem_raw);
+    cplx<T>* bufb = bufa + size_t(nl) * p.S;
+    const int L = p.L, S = p.S;
+    const int l0 = blockIdx.x * nl;
+    const int nlc = min(nl, nlines - l0);
+    if (nlc <= 0) return;
+    const bool along = (es == 1);      // element index fastest in memory
+    const float inv_l = 1.f / float(L);
+    const float inv_nl = 1.f / float(nlc);
+    for (int e = threadIdx.x; e < nlc * L; e += blockDim.x) {
+        int li, n;
+        if (along) { li = fdiv(e, inv_l); n = e - li * L; }
+        else { n = fdiv(e, inv_nl); li = e - n * nlc; }
+        int sl = l0 + li + lrot;
+        if (sl >= nlines) sl -= nlines;
+        int sn = n + erot;
+        if (sn >= L) sn -= L;
+        cplx<T> v = src[sl * ls + sn * es];
+        if (conj_io) v.y = -v.y;
+        bufa[li * S + n] = v;
+    }
+    __syncthreads();
+    const cplx<T>* y = line_dft<T>(bufa, bufb, nlc, p);
+    const T ysign = conj_io ? -scale : scale;
+    for (int e = threadIdx.x; e < nlc * L; e += blockDim.x) {
+        int li, k;
+        if (along) { li = fdiv(e, inv_l); k = e - li * L; }
+        else { k = fdiv(e, inv_nl); li = e - k * nlc; }
+        // output position k holds Y((k + krot) mod L): the rotation by L/2 is its own inverse
+        int sk = k + krot;
+        if (sk >= L) sk -= L;
+        const cplx<T> v = y[li * S + sk];
+        dst[(l0 + li) * ls + k * es] = {v.x * scale, v.y * ysign};
+    }
+}
+
+// Forward real transform along x of an M x N real image into M x (N/2 + 1): line l of the output
+// is the transform of input row (l + lrot) mod M read from column erot on (cyclically).  p is the
+// plan of length N/2, twn = exp(+2 pi i m / N).  With z(m) = x(2m) + i x(2m+1) and Z = DFT(z):
+// X(k) = (Z(k) + conj Z(h-k)) / 2 - i e^{-2 pi i k / N} (Z(k) - conj Z(h-k)) / 2, h = N/2.
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+r2c_rows(const T* __restrict__ src, cplx<T>* __restrict__ dst, int M, int N, int nl, int lrot,
+         int erot, LinePlan<T> p, const cplx<T>* __restrict__ twn)
+{
+    extern __shared__ unsigned char smem_raw[];
+    cplx<T>* bufa = reinterpret_cast<cplx<T>*>(smem_raw);
+    cplx<T>* bufb = bufa + size_t(nl) * p.S;
+    const int h = N / 2, W = h + 1, S = p.S;
+    const int l0 = blockIdx.x * nl;
+    const int nlc = min(nl, M - l0);
+    if (nlc <= 0) return;
+    const float inv_h = 1.f / float(h);
+    for (int e = threadIdx.x; e < nlc * h; e += blockDim.x) {
+        const int li = fdiv(e, inv_h);
+        const int m = e - li * h;
+        int sl = l0 + li + lrot;
+        if (sl >= M) sl -= M;
+        const T* row = src + size_t(sl) * N;
+        int q = 2 * m + erot;               // x(2m) and x(2m+1) of the rotated row
+        if (q >= N) q -= N;
+        int q1 = q + 1;
+        if (q1 >= N) q1 -= N;
+        bufa[li * S + m] = {row[q], -row[q1]};   // conj(z): forward transform
+    }
+    __syncthreads();
+    const cplx<T>* y = line_dft<T>(bufa, bufb, nlc, p);   // conj(Z)
+    const float inv_w = 1.f / float(W);
+    for (int e = threadIdx.x; e < nlc * W; e += blockDim.x) {
+        const int li = fdiv(e, inv_w);
+        const int k = e - li * W;
+        const int ka = (k == h) ? 0 : k;
+        const int kb = (k == 0) ? 0 : h - k;
+        const cplx<T> zk = cconj(y[li * S + ka]);
+        const cplx<T> zm = cconj(y[li * S + kb]);
+        const cplx<T> E = cadd(zk, cconj(zm));
+        const cplx<T> O = csub(zk, cconj(zm));
+        const cplx<T> t = cmul(cconj(twn[k]), O);
+        dst[size_t(l0 + li) * W + k] = {T(0.5) * (E.x + t.y), T(0.5) * (E.y - t.x)};
+    }
+}
+
+// Inverse real transform along x of an M x (N/2 + 1) half-plane (already transformed along y)
+// into an M x N real image: output row l comes from input row (l + lrot) mod M and output
+// column j holds x((j + krot) mod N), times scale.  x(2m) = Re v(m), x(2m+1) = Im v(m) with
+// v = IDFT_{N/2}(E + i O), E(c) = Z(c) + conj Z(h-c), O(c) = (Z(c) - conj Z(h-c)) e^{2 pi i c / N};
+// the imaginary parts of Z(0) and Z(h) are dropped (as numpy.fft.irfft2 does).
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+c2r_rows(const cplx<T>* __restrict__ src, T* __restrict__ dst, int M, int N, int nl, int lrot,
+         int krot, T scale, LinePlan<T> p, const cplx<T>* __restrict__ twn)
+{
+    extern __shared__ unsigned char smem_raw[];
+    cplx<T>* bufa = reinterpret_cast<cplx<T>*>(smem_raw);
+    cplx<T>* bufb = bufa + size_t(nl) * p.S;
+    const int h = N / 2, W = h + 1, S = p.S;
+    const int l0 = blockIdx.x * nl;
+    const int nlc = min(nl, M - l0);
+    if (nlc <= 0) return;
+    const float inv_h = 1.f / float(h);
+    for (int e = threadIdx.x; e < nlc * h; e += blockDim.x) {
+        const int li = fdiv(e, inv_h);
+        const int c = e - li * h;
+        int sl = l0 + li + lrot;
+        if (sl >= M) sl -= M;
+        const cplx<T>* z = src + size_t(sl) * W;
+        cplx<T> zc = z[c];
+        cplx<T> zm = z[h - c];
+        if (c == 0) { zc.y = T(0); zm.y = T(0); }
+        const cplx<T> E = cadd(zc, cconj(zm));
+        const cplx<T> O = cmul(csub(zc, cconj(zm)), twn[c]);
+        bufa[li * S + c] = {E.x - O.y, E.y + O.x};
+    }
+    __syncthreads();
+    const cplx<T>* v = line_dft<T>(bufa, bufb, nlc, p);
+    const float inv_n = 1.f / float(N);
+    for (int e = threadIdx.x; e < nlc * N; e += blockDim.x) {
+        const int li = fdiv(e, inv_n);
+        const int j = e - li * N;
+        int q = j + krot;
+        if (q >= N) q -= N;
+        const cplx<T> w = v[li * S + (q >> 1)];
+        dst[size_t(l0 + li) * N + j] = ((q & 1) ? w.y : w.x) * scale;
+    }
+}
+
+// The full M x N transform of a real image from its M x (N/2 + 1) half (galsim/fft.py fft2's
+// Hermitian fill): column j >= N/2 (or j < N/2 with shift_out, whose columns are rotated by N/2)
+// is conj of the half-plane at row (M - i) mod M, column N - j (N/2 - j with shift_out).
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+hermitian_fill(const cplx<T>* __restrict__ rk, int M, int N, int shift_out,
+               cplx<T>* __restrict__ out)
+{
+    const size_t idx = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= size_t(M) * N) return;
+    const int i = int(idx / N);
+    const int j = int(idx - size_t(i) * N);
+    const int h = N / 2, W = h + 1;
+    const int im = (i == 0) ? 0 : M - i;
+    cplx<T> v;
+    if (!shift_out) {
+        v = (j < h) ? rk[size_t(i) * W + j] : cconj(rk[size_t(im) * W + (N - j)]);
+    } else {
+        v = (j >= h) ? rk[size_t(i) * W + (j - h)] : cconj(rk[size_t(im) * W + (h - j)]);
+    }
+    out[idx] = v;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------------------------
+bool smooth5(int n)
+{
+    if (n < 1) return false;
+    for (int f : {2, 3, 5})
+        while (n % f == 0) n /= f;
+    return n == 1;
+}
+
+int next_smooth5(int n)
+{
+    while (!smooth5(n)) ++n;
+    return n;
+}
+
+// LDS bytes a workgroup may use on the current device (the kernels' dynamic allocation).
+size_t lds_limit()
+{
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 65536;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess ||
+        v <= 0)
+        return 65536;
+    return size_t(v);
+}
+
+// Bluestein tables per (device, L): chirp c(m) = exp(+i pi (m^2 mod 2L) / L) and
+// bh = DFT_S(b) / S, b(m mod S) = conj c(|m|) for |m| < L, computed in long double.
+template <typename T>
+struct ChirpCache {
+    std::mutex m;
+    struct Entry { int device; int L; int S; cplx<T>* chirp; cplx<T>* bh; };
+    std::vector<Entry> entries;
+    bool get(int L, int S, const cplx<T>** chirp, const cplx<T>** bh)
+    {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess) return false;
+        std::lock_guard<std::mutex> lock(m);
+        for (auto& e : entries)
+            if (e.device == dev && e.L == L && e.S == S) {
+                *chirp = e.chirp;
+                *bh = e.bh;
+                return true;
+            }
+        const long double pi = 3.14159265358979323846264338327950288L;
+        std::vector<long double> cr(L), ci(L);
+        for (int k = 0; k < L; ++k) {
+            long long q = (static_cast<long long>(k) * k) % (2LL * L);
+            cr[k] = std::cos(pi * q / L);
+            ci[k] = std::sin(pi * q / L);
+        }
+        std::vector<long double> br(S, 0.L), bi(S, 0.L);
+        for (int k = 0; k < L; ++k) {
+            br[k] = cr[k];
+            bi[k] = -ci[k];
+            if (k > 0) {
+                br[S - k] = cr[k];
+                bi[S - k] = -ci[k];
+            }
+        }
+        std::vector<long double> rr(S), ri(S);
+        for (int k = 0; k < S; ++k) {
+            rr[k] = std::cos(2.L * pi * k / S);
+            ri[k] = std::sin(2.L * pi * k / S);
+        }
+        std::vector<cplx<T>> hc(L), hb(S);
+        for (int k = 0; k < L; ++k) hc[k] = {T(cr[k]), T(ci[k])};
+        for (int k = 0; k < S; ++k) {
+            long double sr = 0.L, si = 0.L;
+            long long idx = 0;
+            for (int m = 0; m < S; ++m, idx += k) {
+                if (idx >= S) idx %= S;
+                sr += br[m] * rr[idx] - bi[m] * ri[idx];
+                si += br[m] * ri[idx] + bi[m] * rr[idx];
+            }
+            hb[k] = {T(sr / S), T(si / S)};
+        }
+        cplx<T>* dc = nullptr;
+        cplx<T>* db = nullptr;
+        if (hipMalloc(&dc, L * sizeof(cplx<T>)) != hipSuccess) return false;
+        if (hipMalloc(&db, S * sizeof(cplx<T>)) != hipSuccess) {
+            (void)hipFree(dc);
+            return false;
+        }
+        if (hipMemcpy(dc, hc.data(), L * sizeof(cplx<T>), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(db, hb.data(), S * sizeof(cplx<T>), hipMemcpyHostToDevice) != hipSuccess) {
+            (void)hipFree(dc);
+            (void)hipFree(db);
+            return false;
+        }
+        entries.push_back({dev, L, S, dc, db});
+        *chirp = dc;
+        *bh = db;
+        return true;
+    }
+};
+ChirpCache<double>& chirps() { static ChirpCache<double> c; return c; }
+
+// Plan for length L, or GSMI_E_SIZE / GSMI_E_HIP with the error recorded.
+int make_plan(int L, size_t lds, LinePlan<double>* p)
+{
+    const size_t elem = sizeof(cplx<double>);
+    p->L = L;
+    p->chirp = nullptr;
+    p->bh = nullptr;
+    if (smooth5(L) && 2 * size_t(L) * elem <= lds) {
+        p->S = L;
+    } else {
+        p->S = next_smooth5(2 * L - 1);
+        if (2 * size_t(p->S) * elem > lds) {
+            set_error("FFT length " + std::to_string(L) + " exceeds the in-LDS limit (" +
+                      std::to_string(lds / (2 * elem)) + " complex per line; lengths with prime "
+                      "factors other than 2, 3, 5 need 2L-1 rounded up to such a length)");
+            return GSMI_E_SIZE;
+        }
+        if (!chirps().get(L, p->S, &p->chirp, &p->bh)) {
+            set_error("Bluestein table allocation failed");
+            return GSMI_E_HIP;
+        }
+    }
+    p->tw = twiddles<double>(p->S);
+    if (!p->tw) {
+        set_error("twiddle table allocation failed");
+        return GSMI_E_HIP;
+    }
+    return GSMI_OK;
+}
+
+// Lines per workgroup: about 2048 elements of work, at least one line, within the LDS limit.
+int lines_per_group(int S, int nlines, size_t lds)
+{
+    int nl = std::max(1, 2048 / S);
+    nl = std::min(nl, nlines);
+    while (nl > 1 && 2 * size_t(nl) * S * sizeof(cplx<double>) > lds) --nl;
+    return nl;
+}
+
+template <typename K>
+int prepare_lds(K kernel, size_t bytes)
+{
+    if (bytes <= 65536) return GSMI_OK;
+    return hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes)),
+                     "LDS attribute");
+}
+
+int launch_c2c(const cplx<double>* src, cplx<double>* dst, int nlines, long long ls, long long es,
+               int lrot, int erot, int krot, int conj_io, double scale, const LinePlan<double>& p,
+               size_t lds, hipStream_t s)
+{
+    const int nl = lines_per_group(p.S, nlines, lds);
+    const size_t bytes = 2 * size_t(nl) * p.S * sizeof(cplx<double>);
+    int rc = prepare_lds(c2c_lines<double>, bytes);
+    if (rc) return rc;
+    hipLaunchKernelGGL(c2c_lines<double>, dim3((nlines + nl - 1) / nl), dim3(kThreads), bytes, s,
+                       src, dst, nlines, nl, ls, es, lrot, erot, krot, conj_io, scale, p);
+    return hip_check(hipGetLastError(), "c2c_lines launch");
+}
+
+int check_sizes(const char* name, int M, int N)
+{
+    if (M < 2 || N < 2 || M % 2 || N % 2) {
+        set_error(std::string(name) + ": sizes must be even and >= 2");
+        return GSMI_E_ARG;
+    }
+    if (size_t(M) * size_t(N) > (size_t(1) << 31)) {
+        set_error(std::string(name) + ": image larger than 2^31 elements");
+        return GSMI_E_SIZE;
+    }
+    return GSMI_OK;
+}
+
+}  // namespace
+}  // namespace gsmi
+
+using namespace gsmi;
+
+extern "C" {
+
+int gsmi_cfft2_f64(const double* in, int M, int N, int inverse, int shift_in, int shift_out,
+                   double* out, void* scratch, void* stream)
+{
+    if (!in || !out || !scratch) {
+        set_error("gsmi_cfft2_f64: null buffer");
+        return GSMI_E_ARG;
+    }
+    int rc = check_sizes("gsmi_cfft2_f64", M, N);
+    if (rc) return rc;
+    const size_t lds = lds_limit();
+    LinePlan<double> px, py;
+    if ((rc = make_plan(N, lds, &px)) || (rc = make_plan(M, lds, &py))) return rc;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const auto* src = reinterpret_cast<const cplx<double>*>(in);
+    auto* tmp = reinterpret_cast<cplx<double>*>(scratch);
+    auto* dst = reinterpret_cast<cplx<double>*>(out);
+    const int conj_io = inverse ? 0 : 1;
+    const double scale = inverse ? 1. / (double(M) * double(N)) : 1.;
+    // rows: input rotated in both axes (shift_in), output x rotated (shift_out)
+    rc = launch_c2c(src, tmp, M, N, 1, shift_in ? M / 2 : 0, shift_in ? N / 2 : 0,
+                    shift_out ? N / 2 : 0, conj_io, 1., px, lds, s);
+    if (rc) return rc;
+    // columns: output y rotated (shift_out)
+    return launch_c2c(tmp, dst, N, 1, N, 0, 0, shift_out ? M / 2 : 0, conj_io, scale, py, lds, s);
+}
+
+int gsmi_rfft2_f64(const double* in, int M, int N, int shift_in, int shift_out, double* out,
+                   void* scratch, void* stream)
+{
+    if (!in || !out || !scratch) {
+        set_error("gsmi_rfft2_f64: null buffer");
+        return GSMI_E_ARG;
+    }
+    int rc = check_sizes("gsmi_rfft2_f64", M, N);
+    if (rc) return rc;
+    const size_t lds = lds_limit();
+    LinePlan<double> px, py;
+    if ((rc = make_plan(N / 2, lds, &px)) || (rc = make_plan(M, lds, &py))) return rc;
+    const cplx<double>* twn = twiddles<double>(N);
+    if (!twn) {
+        set_error("twiddle table allocation failed");
+        return GSMI_E_HIP;
+    }
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    auto* tmp = reinterpret_cast<cplx<double>*>(scratch);
+    const int W = N / 2 + 1;
+    const int nl = lines_per_group(px.S, M, lds);
+    const size_t bytes = 2 * size_t(nl) * px.S * sizeof(cplx<double>);
+    if ((rc = prepare_lds(r2c_rows<double>, bytes))) return rc;
+    hipLaunchKernelGGL(r2c_rows<double>, dim3((M + nl - 1) / nl), dim3(kThreads), bytes, s, in,
+                       tmp, M, N, nl, shift_in ? M / 2 : 0, shift_in ? N / 2 : 0, px, twn);
+    if ((rc = hip_check(hipGetLastError(), "r2c_rows launch"))) return rc;
+    return launch_c2c(tmp, reinterpret_cast<cplx<double>*>(out), W, 1, W, 0, 0,
+                      shift_out ? M / 2 : 0, 1, 1., py, lds, s);
+}
+
+int gsmi_irfft2_f64(const double* in, int M, int N, int shift_in, int shift_out, double* out,
+                    void* scratch, void* stream)
+{
+    if (!in || !out || !scratch) {
+        set_error("gsmi_irfft2_f64: null buffer");
+        return GSMI_E_ARG;
+    }
+    int rc = check_sizes("gsmi_irfft2_f64", M, N);
+    if (rc) return rc;
+    const size_t lds = lds_limit();
+    LinePlan<double> px, py;
+    if ((rc = make_plan(N / 2, lds, &px)) || (rc = make_plan(M, lds, &py))) return rc;
+    const cplx<double>* twn = twiddles<double>(N);
+    if (!twn) {
+        set_error("twiddle table allocation failed");
+        return GSMI_E_HIP;
+    }
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    auto* tmp = reinterpret_cast<cplx<double>*>(scratch);
+    const int W = N / 2 + 1;
+    // columns first (inverse, input rows rotated by shift_in), unscaled
+    rc = launch_c2c(reinterpret_cast<const cplx<double>*>(in), tmp, W, 1, W, 0,
+                    shift_in ? M / 2 : 0, 0, 0, 1., py, lds, s);
+    if (rc) return rc;
+    const int nl = lines_per_group(px.S, M, lds);
+    const size_t bytes = 2 * size_t(nl) * px.S * sizeof(cplx<double>);
+    if ((rc = prepare_lds(c2r_rows<double>, bytes))) return rc;
+    hipLaunchKernelGGL(c2r_rows<double>, dim3((M + nl - 1) / nl), dim3(kThreads), bytes, s, tmp,
+                       out, M, N, nl, shift_out ? M / 2 : 0, shift_out ? N / 2 : 0,
+                       1. / (double(M) * double(N)), px, twn);
+    return hip_check(hipGetLastError(), "c2r_rows launch");
+}
+
+int gsmi_hermitian_fill_f64(const double* half, int M, int N, int shift_out, double* out,
+                            void* stream)
+{
+    if (!half || !out) {
+        set_error("gsmi_hermitian_fill_f64: null buffer");
+        return GSMI_E_ARG;
+    }
+    int rc = check_sizes("gsmi_hermitian_fill_f64", M, N);
+    if (rc) return rc;
+    const size_t total = size_t(M) * N;
+    hipLaunchKernelGGL(hermitian_fill<double>, dim3(unsigned((total + kThreads - 1) / kThreads)),
+                       dim3(kThreads), 0, reinterpret_cast<hipStream_t>(stream),
+                       reinterpret_cast<const cplx<double>*>(half), M, N, shift_out,
+                       reinterpret_cast<cplx<double>*>(out));
+    return hip_check(hipGetLastError(), "hermitian_fill launch");
+}
+
+}  // extern "C"

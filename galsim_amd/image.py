@@ -103,9 +103,15 @@ class Image:
     def __call__(self, x, y):
         return self._array[y - self.bounds.ymin, x - self.bounds.xmin]
 
+    def calculate_fft(self):
+        """FFT of a real-space image into a k-space image on (0..N/2, -N/2..N/2-1), zero-padded
+        to (-N/2..N/2-1)^2 first (galsim/image.py:915-956), on the GPU."""
+        from . import render
+        return render.forward_fft(self)
+
     def calculate_inverse_fft(self):
-        """Inverse FFT of a k-space image on bounds (0..N/2, -N/2..N/2-1) or larger, wrapped
-        first (galsim/image.py:958-1016), on the GPU."""
+        """Inverse FFT of a k-space image on bounds (0..N/2, -N/2..N/2-1), padded and wrapped
+        first when its bounds differ (galsim/image.py:958-1016), on the GPU."""
         from . import render
         return render.inverse_fft(self)
 

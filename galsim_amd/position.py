@@ -51,6 +51,9 @@ class BoundsI:
     def isDefined(self):
         return self._defined
 
+    def includes(self, x, y):
+        return self._defined and self.xmin <= x <= self.xmax and self.ymin <= y <= self.ymax
+
     @property
     def center(self):
         # galsim/bounds.py:446-455

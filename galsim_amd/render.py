@@ -284,21 +284,82 @@ def wrap_hermx(karr, Nk, N):
     return h[..., 0] + 1j * h[..., 1]
 
 
+def _check_fft_image(image, what):
+    if image.scale is None:
+        raise _lib.GalSimAMDError("%s requires that the scale be set." % what)
+    if not image.bounds.isDefined():
+        raise _lib.GalSimAMDUndefinedBoundsError(
+            "%s requires that the image have defined bounds." % what)
+
+
+def forward_fft(image):
+    """Image.calculate_fft (galsim/image.py:915-956): the image zero-padded to
+    (-N/2..N/2-1)^2, N/2 = max(-xmin, xmax+1, -ymin, ymax+1), transformed by
+    rfft(shift_in=True, shift_out=True) on the GPU and scaled by dx^2; the result lies on
+    (0..N/2, -N/2..N/2-1) with scale 2 pi / (N dx).  A complex image contributes its real part, as
+    the reference's rfft reads it."""
+    from .image import Image
+    _check_fft_image(image, "calculate_fft")
+    b = image.bounds
+    No2 = max(-b.xmin, b.xmax + 1, -b.ymin, b.ymax + 1)
+    N = 2 * No2
+    arr = np.asarray(image.array)
+    if arr.dtype.kind == 'c':
+        arr = arr.real
+    full = np.zeros((N, N), dtype=np.float64)
+    full[b.ymin + No2: b.ymax + No2 + 1, b.xmin + No2: b.xmax + No2 + 1] = arr
+    torch = _torch()
+    x = torch.from_numpy(full).cuda()
+    out = torch.empty((N, No2 + 1), dtype=torch.complex128, device='cuda')
+    tmp = torch.empty((N, No2 + 1), dtype=torch.complex128, device='cuda')
+    _lib.check(_lib.lib().gsmi_rfft2_f64(x.data_ptr(), N, N, 1, 1, out.data_ptr(),
+                                         tmp.data_ptr(), torch.cuda.current_stream().cuda_stream),
+               "gsmi_rfft2_f64")
+    dx = image.scale
+    karr = out.cpu().numpy()
+    karr *= dx * dx
+    kim = Image(karr, xmin=0, ymin=-No2)
+    kim.scale = np.pi / (No2 * dx)
+    return kim
+
+
 def inverse_fft(kimage):
-    """Image.calculate_inverse_fft (galsim/image.py:958-1016)."""
+    """Image.calculate_inverse_fft (galsim/image.py:958-1016): the x >= 0 half of the k-image,
+    taken as Hermitian, zero-padded and wrapped (hermitian='x') onto (0..N/2, -N/2..N/2-1) unless
+    it already has exactly those bounds, N/2 = max(xmax, -ymin, ymax); then
+    irfft(shift_in=True, shift_out=True) on the GPU, scaled by (dk N / 2 pi)^2, on
+    (-N/2..N/2-1)^2 with scale 2 pi / (N dk)."""
     from .image import Image
     from .position import BoundsI
+    _check_fft_image(kimage, "calculate_inverse_fft")
     b = kimage.bounds
-    if b.xmin != 0:
-        raise _lib.GalSimAMDError("calculate_inverse_fft requires xmin == 0")
-    No2 = max(-b.ymin, b.ymax, b.xmax)
+    if not b.includes(0, 0):
+        raise _lib.GalSimAMDBoundsError(
+            "calculate_inverse_fft requires that the image includes (0,0)")
+    No2 = max(b.xmax, -b.ymin, b.ymax)
+    if No2 < 1:
+        raise _lib.GalSimAMDBoundsError("calculate_inverse_fft needs more than the (0,0) pixel")
     N = 2 * No2
-    full = np.zeros((2 * No2 + 1, No2 + 1), dtype=np.complex128)
-    arr = kimage.array
-    full[b.ymin + No2: b.ymax + No2 + 1, 0: b.xmax + 1] = arr
-    kw = wrap_hermx(full, 2 * No2, N)
-    real = irfft(kw, N)
-    dx = 2. * np.pi / (N * kimage.scale)
-    im = Image(real, xmin=-N // 2, ymin=-N // 2)
-    im.scale = dx
+    torch = _torch()
+    lib = _lib.lib()
+    stream = torch.cuda.current_stream().cuda_stream
+    arr = np.asarray(kimage.array).astype(np.complex128)
+    if b == BoundsI(0, No2, -No2, No2 - 1):
+        kw = torch.from_numpy(np.ascontiguousarray(arr)).cuda()
+    else:
+        full = np.zeros((N + 1, No2 + 1), dtype=np.complex128)
+        full[b.ymin + No2: b.ymax + No2 + 1, 0: b.xmax + 1] = arr[:, -b.xmin:]
+        kf = torch.from_numpy(full).cuda()
+        kw = torch.empty((N, No2 + 1), dtype=torch.complex128, device='cuda')
+        _lib.check(lib.gsmi_wrap_hermx_f64(kf.data_ptr(), N, N, kw.data_ptr(), stream),
+                   "gsmi_wrap_hermx_f64")
+    out = torch.empty((N, N), dtype=torch.float64, device='cuda')
+    tmp = torch.empty((N, No2 + 1), dtype=torch.complex128, device='cuda')
+    _lib.check(lib.gsmi_irfft2_f64(kw.data_ptr(), N, N, 1, 1, out.data_ptr(), tmp.data_ptr(),
+                                   stream), "gsmi_irfft2_f64")
+    dk = kimage.scale
+    real = out.cpu().numpy()
+    real *= (dk * No2 / np.pi) ** 2
+    im = Image(real, xmin=-No2, ymin=-No2)
+    im.scale = np.pi / (No2 * dk)
     return im

@@ -265,6 +265,41 @@ int gsmi_irfft_f64(const double* kin, int N, double* out, void* scratch, void* s
 int gsmi_wrap_hermx_f64(const double* kin, int Nk, int N, double* kout, void* stream);
 
 /* ---------------------------------------------------------------------------------------
+ * Two-dimensional DFTs of M x N images, M and N even (galsim.fft fft2 / ifft2 / rfft2 / irfft2,
+ * galsim/fft.py:29-300, and Image.calculate_fft / calculate_inverse_fft, galsim/image.py:915-1016).
+ * All buffers are device memory, row-major (x fastest), complex values as interleaved
+ * (re, im) float64.  Results equal numpy.fft's: with shift_in the input is fftshift-ed first,
+ * with shift_out the output is fftshift-ed after (only along y for the half-plane output of
+ * rfft2).  Line lengths with prime factors 2, 3, 5 only are transformed directly; other lengths
+ * L by Bluestein's algorithm with a 5-smooth length >= 2L - 1, which must fit the workgroup's
+ * LDS (GSMI_E_SIZE otherwise).  Work is queued on `stream`; no call allocates device memory
+ * except the per-length twiddle / chirp tables, made once per device.
+ * --------------------------------------------------------------------------------------- */
+
+/* cfft(in, out, inverse, shift_in, shift_out) (src/Image.cpp:913-990; pybind
+ * pysrc/Image.cpp:59-63): in and out are M x N complex; the inverse carries 1/(M N).  scratch
+ * holds M x N complex.  out may equal in. */
+int gsmi_cfft2_f64(const double* in, int M, int N, int inverse, int shift_in, int shift_out,
+                   double* out, void* scratch, void* stream);
+
+/* rfft(in, out, shift_in, shift_out) (src/Image.cpp:729-812; pybind pysrc/Image.cpp:56,61): in is
+ * M x N real, out M x (N/2+1) complex (kx = 0..N/2); scratch holds M x (N/2+1) complex. */
+int gsmi_rfft2_f64(const double* in, int M, int N, int shift_in, int shift_out, double* out,
+                   void* scratch, void* stream);
+
+/* irfft(in, out, shift_in, shift_out) (src/Image.cpp:814-911; pybind pysrc/Image.cpp:58,62) for
+ * any even M x N: in is M x (N/2+1) complex, out M x N real, with 1/(M N); the imaginary parts of
+ * the kx = 0 and kx = N/2 columns after the y transform are ignored.  scratch holds
+ * M x (N/2+1) complex. */
+int gsmi_irfft2_f64(const double* in, int M, int N, int shift_in, int shift_out, double* out,
+                    void* scratch, void* stream);
+
+/* The M x N transform of a real image from its M x (N/2+1) rfft2 half-plane (with the same
+ * shift_out), by Hermitian symmetry: the fill in galsim/fft.py:88-103. */
+int gsmi_hermitian_fill_f64(const double* half, int M, int N, int shift_out, double* out,
+                            void* stream);
+
+/* ---------------------------------------------------------------------------------------
  * Adjoint of gsmi_draw_fft_* (no counterpart in the reference, which has no derivatives: it is
  * the exact reverse-mode derivative of drawFFT, galsim/gsobject.py:2014-2045, as restated by
  * the two kernels above).

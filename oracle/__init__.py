@@ -16,6 +16,8 @@ Layout:
   planning, k-image fill through Transformation/Convolution/Sum, Hermitian wrap, inverse real
   FFT and crop (galsim/gsobject.py, galsim/transform.py, galsim/convolve.py, galsim/sum.py,
   src/SBProfile.cpp, src/SBTransform.cpp, src/Image.cpp).
+* ``fft``       -- galsim.fft (fft2 / ifft2 / rfft2 / irfft2 with shift_in / shift_out) and
+  Image.calculate_fft / calculate_inverse_fft over numpy.fft (galsim/fft.py, galsim/image.py).
 * ``ref``       -- ctypes loader for ``oracle/_ref/libgsref.so``, the reference's own profile
   numerics compiled from its sources by ``oracle/Makefile`` (used to pin this restatement).
 
