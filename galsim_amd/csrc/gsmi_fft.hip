@@ -274,17 +274,6 @@ int next_smooth5(int n)
     return n;
 }
 
-// LDS bytes a workgroup may use on the current device (the kernels' dynamic allocation).
-size_t lds_limit()
-{
-    int dev = 0, v = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return 65536;
-    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess ||
-        v <= 0)
-        return 65536;
-    return size_t(v);
-}
-
 // Bluestein tables per (device, L): chirp c(m) = exp(+i pi (m^2 mod 2L) / L) and
 // bh = DFT_S(b) / S, b(m mod S) = conj c(|m|) for |m| < L, computed in long double.
 template <typename T>
@@ -396,22 +385,13 @@ int lines_per_group(int S, int nlines, size_t lds)
     return nl;
 }
 
-template <typename K>
-int prepare_lds(K kernel, size_t bytes)
-{
-    if (bytes <= 65536) return GSMI_OK;
-    return hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes)),
-                     "LDS attribute");
-}
-
 int launch_c2c(const cplx<double>* src, cplx<double>* dst, int nlines, long long ls, long long es,
                int lrot, int erot, int krot, int conj_io, double scale, const LinePlan<double>& p,
                size_t lds, hipStream_t s)
 {
     const int nl = lines_per_group(p.S, nlines, lds);
     const size_t bytes = 2 * size_t(nl) * p.S * sizeof(cplx<double>);
-    int rc = prepare_lds(c2c_lines<double>, bytes);
+    int rc = allow_lds(reinterpret_cast<const void*>(c2c_lines<double>), bytes);
     if (rc) return rc;
     hipLaunchKernelGGL(c2c_lines<double>, dim3((nlines + nl - 1) / nl), dim3(kThreads), bytes, s,
                        src, dst, nlines, nl, ls, es, lrot, erot, krot, conj_io, scale, p);
@@ -486,7 +466,7 @@ int gsmi_rfft2_f64(const double* in, int M, int N, int shift_in, int shift_out, 
     const int W = N / 2 + 1;
     const int nl = lines_per_group(px.S, M, lds);
     const size_t bytes = 2 * size_t(nl) * px.S * sizeof(cplx<double>);
-    if ((rc = prepare_lds(r2c_rows<double>, bytes))) return rc;
+    if ((rc = allow_lds(reinterpret_cast<const void*>(r2c_rows<double>), bytes))) return rc;
     hipLaunchKernelGGL(r2c_rows<double>, dim3((M + nl - 1) / nl), dim3(kThreads), bytes, s, in,
                        tmp, M, N, nl, shift_in ? M / 2 : 0, shift_in ? N / 2 : 0, px, twn);
     if ((rc = hip_check(hipGetLastError(), "r2c_rows launch"))) return rc;
@@ -520,7 +500,7 @@ int gsmi_irfft2_f64(const double* in, int M, int N, int shift_in, int shift_out,
     if (rc) return rc;
     const int nl = lines_per_group(px.S, M, lds);
     const size_t bytes = 2 * size_t(nl) * px.S * sizeof(cplx<double>);
-    if ((rc = prepare_lds(c2r_rows<double>, bytes))) return rc;
+    if ((rc = allow_lds(reinterpret_cast<const void*>(c2r_rows<double>), bytes))) return rc;
     hipLaunchKernelGGL(c2r_rows<double>, dim3((M + nl - 1) / nl), dim3(kThreads), bytes, s, tmp,
                        out, M, N, nl, shift_out ? M / 2 : 0, shift_out ? N / 2 : 0,
                        1. / (double(M) * double(N)), px, twn);

@@ -599,15 +599,33 @@ const cplx<double>* twiddles<double>(int N) { return twd().get(N); }
 template <>
 const cplx<float>* twiddles<float>(int N) { return twf().get(N); }
 
-// 2^k or 3*2^k with both in-LDS transforms within 64 KiB per workgroup (N <= 2048 in FP64,
-// N <= 4096 in FP32).
-bool fft_size_ok(int N, size_t elem)
+// 2^k or 3*2^k with both in-LDS transforms within `lds` bytes per workgroup (64 KiB: N <= 2048
+// in FP64, N <= 4096 in FP32; 160 KiB: N <= 4096 in FP64, N <= 8192 in FP32).
+bool fft_size_ok(int N, size_t elem, size_t lds)
 {
     if (N < 4 || N % 2) return false;
     int m = N;
     while (m % 2 == 0) m /= 2;
     if (m != 1 && m != 3) return false;
-    return 2 * size_t(cols_per_group(N)) * N * elem <= 65536;
+    return 2 * size_t(cols_per_group(N)) * N * elem <= lds;
+}
+
+size_t lds_limit()
+{
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 65536;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess ||
+        v < 65536)
+        return 65536;
+    return size_t(v);
+}
+
+int allow_lds(const void* kernel, size_t bytes)
+{
+    if (bytes <= 65536) return GSMI_OK;
+    return hip_check(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         int(bytes)),
+                     "dynamic LDS attribute");
 }
 
 int hip_check(hipError_t e, const char* what)
@@ -638,9 +656,9 @@ template <typename T>
 int fold_impl(const gsmi_stamp* stamps, int nstamps, int N, const gsmi_op* ops,
               const gsmi_leaf* leaves, Tables<T> tb, void* scratch, void* stream)
 {
-    if (!fft_size_ok(N, sizeof(cplx<T>))) {
+    if (!fft_size_ok(N, sizeof(cplx<T>), lds_limit())) {
         set_error("unsupported FFT size " + std::to_string(N) +
-                  " (need 2^k or 3*2^k, <= 2048 in FP64, <= 4096 in FP32)");
+                  " (need 2^k or 3*2^k, <= 4096 in FP64, <= 8192 in FP32)");
         return GSMI_E_SIZE;
     }
     if (nstamps <= 0) return GSMI_OK;
@@ -654,6 +672,7 @@ int fold_impl(const gsmi_stamp* stamps, int nstamps, int N, const gsmi_op* ops,
     if (rc) return rc;
     dim3 g1((N / 2 + 1 + cpg - 1) / cpg, nstamps);
     size_t lds1 = 2 * size_t(cpg) * N * sizeof(cplx<T>);
+    if ((rc = allow_lds(reinterpret_cast<const void*>(FoldKernel<T>::fn), lds1))) return rc;
     hipLaunchKernelGGL(FoldKernel<T>::fn, g1, dim3(kThreads), lds1,
                        reinterpret_cast<hipStream_t>(stream), stamps, ops, leaves, tb, tw,
                        reinterpret_cast<cplx<T>*>(scratch));
@@ -664,7 +683,7 @@ template <typename T>
 int rows_impl(const gsmi_stamp* stamps, int nstamps, int N, int max_ny, const void* scratch,
               T* out, void* stream)
 {
-    if (!fft_size_ok(N, sizeof(cplx<T>))) {
+    if (!fft_size_ok(N, sizeof(cplx<T>), lds_limit())) {
         set_error("unsupported FFT size " + std::to_string(N));
         return GSMI_E_SIZE;
     }
@@ -677,6 +696,8 @@ int rows_impl(const gsmi_stamp* stamps, int nstamps, int N, int max_ny, const vo
     const int rpg = rows_per_group(N);
     dim3 g2((max_ny + rpg - 1) / rpg, nstamps);
     size_t lds2 = 2 * size_t(rpg) * (N / 2) * sizeof(cplx<T>);
+    int rc = allow_lds(reinterpret_cast<const void*>(row_c2r<T>), lds2);
+    if (rc) return rc;
     hipLaunchKernelGGL(row_c2r<T>, g2, dim3(kThreads), lds2, reinterpret_cast<hipStream_t>(stream),
                        stamps, reinterpret_cast<const cplx<T>*>(scratch), tw, out);
     return hip_check(hipGetLastError(), "row_c2r launch");

@@ -18,8 +18,15 @@ void set_error(const std::string& msg);
 // GSMI_OK, or GSMI_E_HIP with the HIP error text recorded.
 int hip_check(hipError_t e, const char* what);
 
-// N is 2^k or 3*2^k and the two in-LDS transforms of a workgroup fit 64 KiB.
-bool fft_size_ok(int N, size_t elem);
+// N is 2^k or 3*2^k and the two in-LDS transforms of a workgroup fit `lds` bytes.
+bool fft_size_ok(int N, size_t elem, size_t lds = 65536);
+
+// LDS bytes one workgroup may allocate on the current device (160 KiB on gfx950; 64 KiB if the
+// query fails).  Launches above 64 KiB set the kernel's dynamic-LDS attribute first.
+size_t lds_limit();
+
+// hipFuncSetAttribute(MaxDynamicSharedMemorySize) for launches above 64 KiB.
+int allow_lds(const void* kernel, size_t bytes);
 
 // Device table exp(+2 pi i m / N), m = 0..N-1, cached per device and N (nullptr on failure).
 template <typename T>
