@@ -271,6 +271,36 @@ def large_bench(args, torch):
                         "0.8 arcsec x Pixel, drawImage's own stamp sizes, FP32"}
 
 
+def fft_module_bench(args, torch):
+    """galsim_amd.fft on device-resident float64 images: rfft2 then irfft2 (shift_in and
+    shift_out, as Image.calculate_fft / calculate_inverse_fft use them) for square sizes, timed
+    with HIP events on the current stream.  Algorithmic bytes per pair: rfft2 reads 8 M N and
+    writes 16 M (N/2 + 1); irfft2 reads 16 M (N/2 + 1) and writes 8 M N."""
+    from galsim_amd import fft as gfft
+    res = {}
+    for N in (1174, 2048, 4096):
+        x = torch.randn((N, N), dtype=torch.float64, device='cuda')
+        for _ in range(2):
+            gfft.irfft2(gfft.rfft2(x, True, True), True, True)
+        k = 10
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(k):
+            y = gfft.irfft2(gfft.rfft2(x, True, True), True, True)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / k
+        err = float((y - x).abs().max() / x.abs().max())
+        W = N // 2 + 1
+        nbytes = 2 * (8 * N * N + 16 * N * W)
+        res[str(N)] = {"ms_per_pair": round(ms, 4), "algorithmic_GB_per_s":
+                       round(nbytes / (ms * 1e-3) / 1e9, 1), "round_trip_err": err}
+        log("fft module %dx%d: rfft2+irfft2 %.3f ms (%.0f GB/s algorithmic), round trip %.1e"
+            % (N, N, ms, nbytes / (ms * 1e-3) / 1e9, err))
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -288,6 +318,7 @@ def main():
     ap.add_argument('--no-batch-fit', action='store_true')
     ap.add_argument('--large-n', type=int, default=64)
     ap.add_argument('--no-large', action='store_true')
+    ap.add_argument('--no-fft-module', action='store_true')
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='MIB',
                     help='only run a plain device copy of MIB MiB (3 times) and exit; used to '
                          'scale the FETCH_SIZE/WRITE_SIZE counters')
@@ -452,6 +483,7 @@ def main():
     fwd_bwd = None if args.no_fit else fwd_bwd_bench(args, torch)
     batch_fit = None if (args.no_fit or args.no_batch_fit) else fit_bench(args, torch)
     large = None if args.no_large else large_bench(args, torch)
+    fft_mod = None if args.no_fft_module else fft_module_bench(args, torch)
 
     # --- CPU baseline: the oracle (numpy FP64 restatement), one core ------------------------
     cpu = None
@@ -497,6 +529,7 @@ def main():
         "fwd_bwd": fwd_bwd,
         "batch_fit": batch_fit,
         "large_stamps": large,
+        "fft_module_f64": fft_mod,
         "plain_fp64_gpu_stamps_per_s": round(plain_rate, 1),
         "speedup_vs_plain_fp64_gpu": round(value / plain_rate, 2),
         "drawimage_fp64_end_to_end_stamps_per_s": round(drawimage_rate, 1),

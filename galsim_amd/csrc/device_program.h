@@ -964,6 +964,16 @@ __device__ __forceinline__ cplx<T>* fft_lds_c(cplx<T>* a, cplx<T>* b, int n, int
     }
 }
 
+// Workgroups of a launch are dispatched round-robin over the 8 XCDs, each with its own L2.  This
+// bijection of [0, G) gives each XCD one contiguous range of logical indices, so neighbouring
+// tiles (adjacent columns, adjacent column chunks of one stamp) share an L2 and whole cache lines.
+__device__ __forceinline__ int xcd_contiguous(int b, int G)
+{
+    const int q = G >> 3, r = G & 7;
+    const int x = b & 7, s = b >> 3;
+    return (x < r) ? x * (q + 1) + s : r * (q + 1) + (x - r) * q + s;
+}
+
 __host__ __device__ inline int cols_per_group(int N)
 {
     return N >= 1024 ? 1 : 1024 / N;

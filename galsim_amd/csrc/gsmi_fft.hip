@@ -111,10 +111,12 @@ c2c_lines(const cplx<T>* __restrict__ src, cplx<T>* __restrict__ dst, int nlines
     cplx<T>* bufa = reinterpret_cast<cplx<T>*>(smem_raw);
     cplx<T>* bufb = bufa + size_t(nl) * p.S;
     const int L = p.L, S = p.S;
-    const int l0 = blockIdx.x * nl;
+    const bool along = (es == 1);      // element index fastest in memory
+    // a column tile reads nl adjacent values per row: neighbouring tiles go to one XCD so the
+    // rest of each cache line is read from its L2
+    const int l0 = (along ? int(blockIdx.x) : xcd_contiguous(blockIdx.x, gridDim.x)) * nl;
     const int nlc = min(nl, nlines - l0);
     if (nlc <= 0) return;
-    const bool along = (es == 1);      // element index fastest in memory
     const float inv_l = 1.f / float(L);
     const float inv_nl = 1.f / float(nlc);
     for (int e = threadIdx.x; e < nlc * L; e += blockDim.x) {
