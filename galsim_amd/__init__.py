@@ -6,7 +6,8 @@ from . import fft
 from ._lib import (GalSimAMDBoundsError, GalSimAMDError, GalSimAMDRangeError,
                    GalSimAMDUndefinedBoundsError, GalSimAMDValueError)
 from .box import Box, Pixel
-from .convolve import Convolution, Convolve
+from .convolve import (AutoConvolution, AutoConvolve, AutoCorrelate, AutoCorrelation, Convolution,
+                       Convolve, Deconvolution, Deconvolve)
 from .exponential import Exponential
 from .gaussian import Gaussian
 from .gsobject import GSObject
@@ -22,7 +23,9 @@ from .transform import Transform, Transformation, _Transform
 
 GalSimError = GalSimAMDError
 
-__all__ = ['Add', 'BoundsI', 'Box', 'Convolution', 'Convolve', 'DeVaucouleurs', 'Exponential',
+__all__ = ['Add', 'AutoConvolution', 'AutoConvolve', 'AutoCorrelate', 'AutoCorrelation', 'BoundsI',
+           'Box', 'Convolution', 'Convolve', 'DeVaucouleurs', 'Deconvolution', 'Deconvolve',
+           'Exponential',
            'GSObject', 'GSParams', 'GalSimAMDBoundsError', 'GalSimAMDError', 'GalSimAMDRangeError',
            'GalSimAMDUndefinedBoundsError', 'GalSimAMDValueError', 'GalSimError',
            'Gaussian', 'Image', 'ImageCD', 'ImageCF', 'ImageD', 'ImageF', 'Kolmogorov', 'Moffat',

@@ -30,8 +30,9 @@ GRAD_FIELDS = 31
 ADJ_COMPLEX = 1
 ADJ_SHAPE = 2
 
-OP_LEAF, OP_MUL, OP_ADD, OP_SCALE, OP_PHASE = 1, 2, 3, 4, 5
+OP_LEAF, OP_MUL, OP_ADD, OP_SCALE, OP_PHASE, OP_INV = 1, 2, 3, 4, 5, 6
 LEAF_EXPONENTIAL, LEAF_GAUSSIAN, LEAF_MOFFAT, LEAF_KOLMOGOROV, LEAF_SERSIC, LEAF_BOX = range(1, 7)
+LEAF_KCLIP = 7
 
 ERRORS = {-1: "invalid argument", -2: "parameter out of range", -3: "solve/table failure",
           -4: "HIP runtime failure", -5: "unsupported FFT size"}

@@ -53,6 +53,8 @@ def _prepare_differentiable(ctx, op0, nops, naxis, leaf0, moffat_generic=False):
     of ops appended.  Exponential clip rows (FP64) stay those of the lowered Jacobian."""
     from .moffat import temme_constants
     for code, arg, *_ in ctx.ops[op0:op0 + nops]:
+        if code == _lib.OP_INV:
+            raise NotImplementedError("a Deconvolution is not differentiable here")
         if code == _lib.OP_LEAF:
             rec = ctx.leaves[leaf0 + arg]
             if rec['type'] in _RADIAL:

@@ -93,3 +93,160 @@ class Convolution(GSObject):
 
     def __repr__(self):
         return 'galsim_amd.Convolution(%r)' % (self.obj_list,)
+
+
+def Deconvolve(obj, gsparams=None, propagate_gsparams=True):
+    """galsim/convolve.py:429-461 for a GSObject."""
+    if not isinstance(obj, GSObject):
+        raise TypeError("Argument to Deconvolve must be a GSObject.")
+    return Deconvolution(obj, gsparams=gsparams, propagate_gsparams=propagate_gsparams)
+
+
+class Deconvolution(GSObject):
+    """galsim/convolve.py:464-612: the k-image is 1 / F(k) of the original where
+    |F| > flux * kvalue_accuracy (else the inverse of that threshold), and 0 beyond the original's
+    maxk.  Lowered as the original's program, an INV op and a KCLIP leaf (maxk cut)."""
+
+    def __init__(self, obj, gsparams=None, propagate_gsparams=True):
+        if not isinstance(obj, GSObject):
+            raise TypeError("Argument to Deconvolution must be a GSObject.")
+        self._gsparams = GSParams.check(gsparams, obj.gsparams)
+        self._min_acc_kvalue = obj.flux * self.gsparams.kvalue_accuracy
+        self._inv_min_acc_kvalue = 1. / self._min_acc_kvalue
+        self._propagate_gsparams = propagate_gsparams
+        self._orig_obj = obj.withGSParams(self._gsparams) if propagate_gsparams else obj
+
+    @property
+    def orig_obj(self):
+        return self._orig_obj
+
+    def withGSParams(self, gsparams=None, **kwargs):
+        if gsparams == self.gsparams and not kwargs:
+            return self
+        import copy
+        ret = copy.copy(self)
+        ret._gsparams = GSParams.check(gsparams, self.gsparams, **kwargs)
+        if self._propagate_gsparams:
+            ret._orig_obj = self._orig_obj.withGSParams(ret._gsparams)
+        return ret
+
+    @property
+    def _maxk(self):
+        return self.orig_obj.maxk
+
+    @property
+    def _stepk(self):
+        return self.orig_obj.stepk
+
+    @property
+    def _flux(self):
+        return 1. / self.orig_obj.flux
+
+    def _lower(self, ctx, jac):
+        from ._leaf import jac_record
+        self.orig_obj._lower(ctx, jac)
+        ctx.emit(_lib.OP_INV, 0, self._min_acc_kvalue, self._inv_min_acc_kvalue)
+        if getattr(ctx, 'kvalue_mode', False):
+            return                      # Deconvolution._kValue has no maxk cut
+        if ctx.dk is None:
+            raise _lib.GalSimAMDError("a Deconvolution needs the k-image spacing when lowered")
+        rec = np.zeros((), dtype=_lib.LEAF_DTYPE)
+        sep, j = jac_record(jac)
+        rec['type'] = _lib.LEAF_KCLIP
+        rec['separable'] = sep
+        rec['jac'] = j
+        rec['cliprows'] = -1
+        rec['scale'] = float(ctx.dk)
+        rec['p'][0] = self.maxk ** 2
+        rec['flux'] = 1.
+        ctx.emit(_lib.OP_LEAF, ctx.add_leaf(rec))
+        ctx.emit(_lib.OP_MUL)
+
+    def __repr__(self):
+        return 'galsim_amd.Deconvolution(%r)' % (self.orig_obj,)
+
+
+def _auto_checks(obj, real_space, what):
+    if not isinstance(obj, GSObject):
+        raise TypeError("Argument to %s must be a GSObject." % what)
+    if real_space is None:
+        real_space = obj.has_hard_edges
+    elif bool(real_space) != real_space:
+        raise TypeError("real_space must be a boolean")
+    if real_space and obj.is_analytic_x:
+        raise NotImplementedError("real-space convolution is not part of galsim_amd")
+
+
+def AutoConvolve(obj, real_space=None, gsparams=None, propagate_gsparams=True):
+    """galsim/convolve.py:615-644 for a GSObject."""
+    if not isinstance(obj, GSObject):
+        raise TypeError("Argument to AutoConvolve must be a GSObject.")
+    return AutoConvolution(obj, real_space, gsparams, propagate_gsparams)
+
+
+class AutoConvolution(Convolution):
+    """galsim/convolve.py:646-771: Convolution([obj, obj]) (the factor drawn twice and
+    multiplied, as Convolution._drawKImage does)."""
+
+    def __init__(self, obj, real_space=None, gsparams=None, propagate_gsparams=True):
+        _auto_checks(obj, real_space, "AutoConvolution")
+        self._real_space = False
+        self._gsparams = GSParams.check(gsparams, obj.gsparams)
+        self._propagate_gsparams = propagate_gsparams
+        self._orig_obj = obj.withGSParams(self._gsparams) if propagate_gsparams else obj
+        self._obj_list = [self._orig_obj, self._orig_obj]
+
+    @property
+    def orig_obj(self):
+        return self._orig_obj
+
+    def withGSParams(self, gsparams=None, **kwargs):
+        if gsparams == self.gsparams and not kwargs:
+            return self
+        import copy
+        ret = copy.copy(self)
+        ret._gsparams = GSParams.check(gsparams, self.gsparams, **kwargs)
+        if self._propagate_gsparams:
+            ret._orig_obj = self._orig_obj.withGSParams(ret._gsparams)
+        ret._obj_list = [ret._orig_obj, ret._orig_obj]
+        return ret
+
+    def __repr__(self):
+        return 'galsim_amd.AutoConvolution(%r)' % (self.orig_obj,)
+
+
+def AutoCorrelate(obj, real_space=None, gsparams=None, propagate_gsparams=True):
+    """galsim/convolve.py:774-803 for a GSObject."""
+    if not isinstance(obj, GSObject):
+        raise TypeError("Argument to AutoCorrelate must be a GSObject.")
+    return AutoCorrelation(obj, real_space, gsparams, propagate_gsparams)
+
+
+class AutoCorrelation(Convolution):
+    """galsim/convolve.py:805-935: Convolution([obj, obj.transform(-1, 0, 0, -1)])."""
+
+    def __init__(self, obj, real_space=None, gsparams=None, propagate_gsparams=True):
+        _auto_checks(obj, real_space, "AutoCorrelation")
+        self._real_space = False
+        self._gsparams = GSParams.check(gsparams, obj.gsparams)
+        self._propagate_gsparams = propagate_gsparams
+        self._orig_obj = obj.withGSParams(self._gsparams) if propagate_gsparams else obj
+        self._obj_list = [self._orig_obj, self._orig_obj.transform(-1, 0, 0, -1)]
+
+    @property
+    def orig_obj(self):
+        return self._orig_obj
+
+    def withGSParams(self, gsparams=None, **kwargs):
+        if gsparams == self.gsparams and not kwargs:
+            return self
+        import copy
+        ret = copy.copy(self)
+        ret._gsparams = GSParams.check(gsparams, self.gsparams, **kwargs)
+        if self._propagate_gsparams:
+            ret._orig_obj = self._orig_obj.withGSParams(ret._gsparams)
+        ret._obj_list = [ret._orig_obj, ret._orig_obj.transform(-1, 0, 0, -1)]
+        return ret
+
+    def __repr__(self):
+        return 'galsim_amd.AutoCorrelation(%r)' % (self.orig_obj,)

@@ -290,6 +290,35 @@ __device__ inline void moffat_generic_f32(const LeafC<T>& C, float k, float* g, 
     *dg = -0.5f * pk * km1;
 }
 
+// Deconvolution._drawKImage's cut (galsim/convolve.py:602-612) at k-image pixel (x, y) = (i, j):
+// k' = (J00 x + J10 y, J01 x + J11 y), cut where |k'|^2 dk^2 > maxk^2, in FP64 with the
+// reference's order of operations (no fused multiply-adds) so the boundary samples agree.
+__device__ inline bool kclip(const gsmi_leaf* __restrict__ rec, int i, int j)
+{
+    const double x = double(i), y = double(j);
+    const double kxp = __dadd_rn(__dmul_rn(rec->jac[0], x), __dmul_rn(rec->jac[2], y));
+    const double kyp = __dadd_rn(__dmul_rn(rec->jac[1], x), __dmul_rn(rec->jac[3], y));
+    const double ksq = __dmul_rn(__dadd_rn(__dmul_rn(kxp, kxp), __dmul_rn(kyp, kyp)),
+                                 __dmul_rn(rec->scale, rec->scale));
+    return ksq > rec->p[0];
+}
+
+// 1 / z where |z| > a, else b: Deconvolution._drawKImage's inversion (galsim/convolve.py:
+// 602-606), with numpy's complex reciprocal (ratio of the smaller to the larger component).
+template <typename T>
+__device__ inline cplx<T> deconv_inv(cplx<T> z, T a, T b)
+{
+    if (!(hypot(z.x, z.y) > a)) return {b, T(0)};
+    if (fabs(z.x) >= fabs(z.y)) {
+        const T rat = z.y / z.x;
+        const T scl = T(1) / (z.x + z.y * rat);
+        return {scl, -rat * scl};
+    }
+    const T rat = z.x / z.y;
+    const T scl = T(1) / (z.y + z.x * rat);
+    return {rat * scl, -scl};
+}
+
 // Value of a decoded leaf at k-image sample (i, j) (kx = i dk, ky = j dk).  rec is the leaf's
 // record, read only by the FP64 Bessel-function Moffat kinds.
 template <typename T>
@@ -387,6 +416,8 @@ __device__ inline T eval_leafc(const LeafC<T>& C, const LeafI& I, const gsmi_lea
                                   : ddiv(dsinpi(Yb), T(M_PI) * Yb);
         return flux * sx * sy;
       }
+      case GSMI_LEAF_KCLIP:
+        return kclip(rec, i, j) ? T(0) : T(1);
     }
     return T(0);
 }
@@ -515,6 +546,9 @@ __device__ inline cplx<T> run_ops(const ProgramD& d, const gsmi_op* __restrict__
                 st.s[0] = cmul(st.s[0], cplx<T>{fs * cs, -fs * sn});
                 break;
               }
+              case GSMI_OP_INV:
+                st.s[0] = deconv_inv(st.s[0], T(ops[o].a), T(ops[o].b));
+                break;
             }
         }
     }
@@ -551,6 +585,8 @@ __device__ inline T run_ops_real(const ProgramD& d, const gsmi_op* __restrict__ 
                 st[0] = (code == GSMI_OP_MUL) ? st[0] * b : st[0] + b;
             } else if (code == GSMI_OP_SCALE) {
                 st[0] *= T(ops[o].a);
+            } else if (code == GSMI_OP_INV) {
+                st[0] = fabs(st[0]) > T(ops[o].a) ? T(1) / st[0] : T(ops[o].b);
             }
         }
     }
@@ -696,6 +732,9 @@ __device__ inline cplx<T> eval_program(const gsmi_op* __restrict__ ops, int nops
             st.s[0] = cmul(st.s[0], cplx<T>{fs * cs, -fs * sn});
             break;
           }
+          case GSMI_OP_INV:
+            st.s[0] = deconv_inv(st.s[0], T(op->a), T(op->b));
+            break;
         }
     }
     return st.s[0];

@@ -36,6 +36,7 @@ class Lowering:
         self.ntab = 0
         self.clip = []
         self.nclip = 0
+        self.kvalue_mode = False     # lowering for kValue: no k-image cuts
 
     def emit(self, code, arg=0, a=0., b=0., c=0.):
         self.ops.append((code, arg, a, b, c))
@@ -213,7 +214,7 @@ def draw_fft(prof, bounds, precision='f64', return_device=False):
 def draw_kimage(prof, bounds, dk):
     """k-image of ``prof`` on ``bounds`` with spacing dk (complex128)."""
     lib = _lib.lib()
-    ctx = Lowering('f64')
+    ctx = Lowering('f64', dk=dk)
     prof._lower(ctx, None)
     check_stack(ctx.ops, precision='f64')
     ny, nx = bounds.numpyShape()
@@ -239,6 +240,7 @@ def kvalue(prof, kx, ky):
     grid point (1, 1) with dk = 1."""
     lib = _lib.lib()
     ctx = Lowering('f64')
+    ctx.kvalue_mode = True
     prof._lower(ctx, np.array([[float(kx), 0.], [0., float(ky)]]))
     check_stack(ctx.ops, precision='f64')
     torch = _torch()

@@ -37,13 +37,17 @@ extern "C" {
  *   ADD       pop b, a; push a+b                                        (Sum)
  *   SCALE s   top *= s                                                  (Transformation flux)
  *   PHASE     top *= fs * exp(-i (ux kx + uy ky))                       (ApplyKImagePhases)
+ *   INV a b   top = 1 / top where |top| > a, else b (b = 1 / a)         (Deconvolution)
+ * A Deconvolution lowers to its original's program, INV, then a KCLIP leaf and MUL: the leaf is
+ * 1 where the reference keeps the inverse and 0 where it zeroes it (beyond the original's maxk).
  * ------------------------------------------------------------------------------------- */
 enum {
     GSMI_OP_LEAF = 1,
     GSMI_OP_MUL = 2,
     GSMI_OP_ADD = 3,
     GSMI_OP_SCALE = 4,
-    GSMI_OP_PHASE = 5
+    GSMI_OP_PHASE = 5,
+    GSMI_OP_INV = 6
 };
 
 enum {
@@ -52,13 +56,16 @@ enum {
     GSMI_LEAF_MOFFAT = 3,
     GSMI_LEAF_KOLMOGOROV = 4,
     GSMI_LEAF_SERSIC = 5,
-    GSMI_LEAF_BOX = 6
+    GSMI_LEAF_BOX = 6,
+    GSMI_LEAF_KCLIP = 7   /* Deconvolution's maxk cut (galsim/convolve.py:602-612): 0 where
+                             ((J00 x + J10 y)^2 + (J01 x + J11 y)^2) dk^2 > maxk^2 at k-image
+                             pixel (x, y), else 1; dk in scale, maxk^2 in p[0], in FP64 */
 };
 
 typedef struct gsmi_op {
     int32_t code;
     int32_t arg;          /* leaf index for LEAF */
-    double a, b, c;       /* SCALE: a; PHASE: ux, uy, fs */
+    double a, b, c;       /* SCALE: a; PHASE: ux, uy, fs; INV: threshold, value below it */
 } gsmi_op;
 
 typedef struct gsmi_leaf {

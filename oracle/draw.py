@@ -6,6 +6,7 @@ Restates, in numpy FP64:
 * Transformation / _Transform composition and k-image drawing (galsim/transform.py:192-230,
   359-392, 490-517, 592-640), Shear.getMatrix (galsim/shear.py:365-386);
 * Convolution and Sum (galsim/convolve.py:296-305, 420-426; galsim/sum.py:242-250, 341-347);
+  Deconvolution, AutoConvolution, AutoCorrelation (galsim/convolve.py:464-935);
 * SBProfile::drawK + FillQuadrant (src/SBProfile.cpp:383-506);
 * ApplyKImagePhases with its renormalised phase recurrence (src/SBTransform.cpp:740-883);
 * drawImage -> drawFFT -> drawFFT_makeKImage / drawFFT_finish (galsim/gsobject.py:1115-1253,
@@ -462,6 +463,56 @@ def Convolve(*objs, **kw):
     if len(objs) == 1 and isinstance(objs[0], (list, tuple)):
         objs = objs[0]
     return Convolution(objs, **kw)
+
+
+class Deconvolution(Obj):
+    """galsim/convolve.py:464-612: the original's k-image inverted where its modulus exceeds
+    flux * kvalue_accuracy (else the inverse of that threshold), zeroed beyond its maxk."""
+
+    def __init__(self, obj):
+        self.orig = obj
+        self.gsparams = obj.gsparams
+        self.min_acc = obj.flux * self.gsparams.kvalue_accuracy
+        self.inv_min_acc = 1. / self.min_acc
+
+    @property
+    def maxk(self):
+        return self.orig.maxk
+
+    @property
+    def stepk(self):
+        return self.orig.stepk
+
+    @property
+    def flux(self):
+        return 1. / self.orig.flux
+
+    def _drawKImage(self, image, jac):
+        self.orig._drawKImage(image, jac)
+        a = image.array
+        do_inverse = np.abs(a) > self.min_acc
+        a[do_inverse] = 1. / a[do_inverse]
+        a[~do_inverse] = self.inv_min_acc
+        kx, ky = np.meshgrid(np.arange(a.shape[1], dtype=float) + image.xmin,
+                             np.arange(a.shape[0], dtype=float) + image.ymin)
+        if jac is not None:
+            kx, ky = (jac[0, 0] * kx + jac[1, 0] * ky), (jac[0, 1] * kx + jac[1, 1] * ky)
+        ksq = (kx ** 2 + ky ** 2) * image.dk ** 2
+        a[ksq > self.maxk ** 2] = 0.
+
+
+def Deconvolve(obj):
+    return Deconvolution(obj)
+
+
+def AutoConvolve(obj):
+    """galsim/convolve.py:646-771: a Convolution of [obj, obj]."""
+    return Convolution([obj, obj], gsparams=obj.gsparams)
+
+
+def AutoCorrelate(obj):
+    """galsim/convolve.py:805-935: a Convolution of [obj, obj.transform(-1, 0, 0, -1)]."""
+    return Convolution([obj, obj.transform(-1, 0, 0, -1)], gsparams=obj.gsparams)
 
 
 class Sum(Obj):
