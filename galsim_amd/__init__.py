@@ -17,6 +17,7 @@ from .kolmogorov import Kolmogorov
 from .moffat import Moffat
 from .position import BoundsI, PositionD, PositionI
 from .sersic import DeVaucouleurs, Sersic
+from .spergel import Spergel
 from .shear import Shear
 from .sum import Add, Sum
 from .transform import Transform, Transformation, _Transform
@@ -29,5 +30,5 @@ __all__ = ['Add', 'AutoConvolution', 'AutoConvolve', 'AutoCorrelate', 'AutoCorre
            'GSObject', 'GSParams', 'GalSimAMDBoundsError', 'GalSimAMDError', 'GalSimAMDRangeError',
            'GalSimAMDUndefinedBoundsError', 'GalSimAMDValueError', 'GalSimError',
            'Gaussian', 'Image', 'ImageCD', 'ImageCF', 'ImageD', 'ImageF', 'Kolmogorov', 'Moffat',
-           'Pixel', 'PositionD', 'PositionI', 'Sersic', 'Shear', 'Sum', 'Transform',
+           'Pixel', 'PositionD', 'PositionI', 'Sersic', 'Shear', 'Spergel', 'Sum', 'Transform',
            'Transformation']

@@ -33,6 +33,7 @@ ADJ_SHAPE = 2
 OP_LEAF, OP_MUL, OP_ADD, OP_SCALE, OP_PHASE, OP_INV = 1, 2, 3, 4, 5, 6
 LEAF_EXPONENTIAL, LEAF_GAUSSIAN, LEAF_MOFFAT, LEAF_KOLMOGOROV, LEAF_SERSIC, LEAF_BOX = range(1, 7)
 LEAF_KCLIP = 7
+LEAF_SPERGEL = 8
 
 ERRORS = {-1: "invalid argument", -2: "parameter out of range", -3: "solve/table failure",
           -4: "HIP runtime failure", -5: "unsupported FFT size"}
@@ -103,6 +104,10 @@ _SIGS = {
     'gsmi_moffat_trunc_info': (_I, [_D, _D, _D, _P, _P]),
     'gsmi_moffat_trunc_table': (_I, [_D, _D, _D, _P, _P, _P, _P, _I]),
     'gsmi_exponential_cliprows': (_I, [_D, _P, _P, _I, _D, _P]),
+    'gsmi_kclip_rows': (_I, [_D, _D, _P, _I, _D, _P]),
+    'gsmi_spergel_info': (_I, [_D, _P, _P]),
+    'gsmi_spergel_flux_radius': (_I, [_D, _D, _P]),
+    'gsmi_spergel_integrated_flux': (_I, [_D, _D, _P]),
     'gsmi_sersic_kvalue': (_I, [_D, _D, _P, _P, _P, _I]),
     'gsmi_draw_fft_f64': (_I, [_P, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P]),
     'gsmi_draw_fft_f32': (_I, [_P, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P]),

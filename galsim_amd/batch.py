@@ -57,6 +57,8 @@ def _prepare_differentiable(ctx, op0, nops, naxis, leaf0, moffat_generic=False):
             raise NotImplementedError("a Deconvolution is not differentiable here")
         if code == _lib.OP_LEAF:
             rec = ctx.leaves[leaf0 + arg]
+            if rec['type'] not in _RADIAL + (_lib.LEAF_BOX,):
+                raise NotImplementedError("leaf type %d has no adjoint here" % rec['type'])
             if rec['type'] in _RADIAL:
                 rec['separable'] = 0
             if moffat_generic and rec['type'] == _lib.LEAF_MOFFAT and rec['kind'] == 1:

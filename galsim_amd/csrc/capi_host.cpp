@@ -145,83 +145,57 @@ int gsmi_exponential_cliprows(double r0, const double* gsp, const double* jac, i
     GSP g = GSP::from(gsp);
     // SBExponentialImpl ctor (src/SBExponential.cpp:71-76)
     const double ksq_max = std::pow(g.kvalue_accuracy, -1. / 1.5) - 1.;
-    const double kmax = std::sqrt(ksq_max);
-    const int m = Nk / 2 + 1;
-    const int half = Nk / 2;
-    // SBProfile::drawK on bounds (0..Nk/2, -Nk/2..Nk/2) (src/SBProfile.cpp:383-425)
-    const double x0 = 0. * dk;
-    const double y0 = -half * dk;
-    if (!jac || (jac[1] == 0. && jac[2] == 0.)) {
-        // 1-d fill through FillQuadrant: quadrant rows ky = 0, dky, ... (src/SBProfile.cpp:447-506,
-        // GetKValueRange1d 508-531)
-        double A = jac ? jac[0] : 1., D = jac ? jac[3] : 1.;
-        double dkx = (jac ? dk * A : dk) * r0;
-        double dky = (jac ? dk * D : dk) * r0;
-        double kx0 = (jac ? x0 * A : x0) * r0;
-        double ky = 0. * r0;
-        for (int q = 0; q <= half; ++q, ky += dky) {
-            int i1, i2;
-            if (std::abs(ky) >= kmax) {
-                i1 = i2 = m;
-            } else {
-                double kysq = ky * ky;
-                double dsq = ksq_max - kysq;
-                if (kx0 * kx0 > dsq || (kx0 + m * dkx) * (kx0 + m * dkx) > dsq) {
-                    double d = std::sqrt(dsq);
-                    i1 = int(std::ceil((-kx0 - d) / dkx));
-                    i2 = int(std::floor((-kx0 + d) / dkx));
-                    if (i1 > i2) std::swap(i1, i2);
-                    ++i2;
-                    if (i2 <= 0 || i1 >= m) i1 = i2 = m;
-                    else { if (i1 < 0) i1 = 0; if (i2 > m) i2 = m; }
-                } else {
-                    i1 = 0;
-                    i2 = m;
-                }
-            }
-            for (int j : {half - q, half + q}) {
-                out[2 * j] = i1;
-                out[2 * j + 1] = i2;
-            }
-        }
-    } else {
-        // general Jacobian: 2-d fill, GetKValueRange2d (src/SBProfile.cpp:533-588) including its
-        // test of |ky0 + m + dky| (line 544)
-        double A = jac[0], B = jac[1], C = jac[2], D = jac[3];
-        double kx0 = (A * x0 + C * y0) * r0;
-        double dkx = (A * dk) * r0;
-        double dkxy = (C * dk) * r0;
-        double ky0 = (B * x0 + D * y0) * r0;
-        double dky = (D * dk) * r0;
-        double dkyx = (B * dk) * r0;
-        for (int j = 0; j <= Nk; ++j, kx0 += dkxy, ky0 += dky) {
-            int i1, i2;
-            if (std::abs(kx0) > kmax || std::abs(kx0 + m * dkx) > kmax ||
-                std::abs(ky0) > kmax || std::abs(ky0 + m + dkyx) > kmax) {
-                double a = dkx * dkx + dkyx * dkyx;
-                double b = dkx * kx0 + dkyx * ky0;
-                double c = kx0 * kx0 + ky0 * ky0 - ksq_max;
-                double d = b * b - a * c;
-                if (d <= 0.) {
-                    i1 = i2 = m;
-                } else {
-                    d = std::sqrt(d);
-                    i1 = int(std::ceil((-b - d) / a));
-                    i2 = int(std::floor((-b + d) / a));
-                    if (i1 > i2) std::swap(i1, i2);
-                    ++i2;
-                    if (i2 <= 0 || i1 >= m) i1 = i2 = m;
-                    else { if (i1 < 0) i1 = 0; if (i2 > m) i2 = m; }
-                }
-            } else {
-                i1 = 0;
-                i2 = m;
-            }
-            out[2 * j] = i1;
-            out[2 * j + 1] = i2;
-        }
-    }
+    kclip_rows(ksq_max, r0, jac, Nk, dk, out);
     return GSMI_OK;
+}
+
+int gsmi_kclip_rows(double ksq_max, double r0, const double* jac, int Nk, double dk,
+                    int32_t* out)
+{
+    if (!out || Nk < 2 || Nk % 2 || !(r0 > 0) || !(dk > 0) || !(ksq_max > 0)) {
+        set_error("gsmi_kclip_rows: bad argument");
+        return GSMI_E_ARG;
+    }
+    kclip_rows(ksq_max, r0, jac, Nk, dk, out);
+    return GSMI_OK;
+}
+
+int gsmi_spergel_info(double nu, const double* gsp, double* out)
+{
+    if (!out || !is_finite(nu)) {
+        set_error("gsmi_spergel_info: bad argument");
+        return GSMI_E_ARG;
+    }
+    if (!(nu >= -0.85 && nu <= 4.0)) {
+        set_error("gsmi_spergel_info: Spergel index out of range [-0.85, 4]");
+        return GSMI_E_RANGE;
+    }
+    return guarded("gsmi_spergel_info", [&] {
+        const SpergelInfo& s = spergel_info(nu, GSP::from(gsp));
+        out[0] = s.maxk;
+        out[1] = s.stepk;
+        out[2] = s.hlr;
+        out[3] = s.ksq_max;
+    });
+}
+
+int gsmi_spergel_flux_radius(double nu, double f, double* out)
+{
+    if (!out || !is_finite(nu) || !(f > 0. && f < 1.)) {
+        set_error("gsmi_spergel_flux_radius: bad argument");
+        return GSMI_E_ARG;
+    }
+    return guarded("gsmi_spergel_flux_radius", [&] { *out = spergel_flux_radius(nu, f); });
+}
+
+int gsmi_spergel_integrated_flux(double nu, double u, double* out)
+{
+    if (!out || !is_finite(nu) || !(u >= 0.)) {
+        set_error("gsmi_spergel_integrated_flux: bad argument");
+        return GSMI_E_ARG;
+    }
+    return guarded("gsmi_spergel_integrated_flux",
+                   [&] { *out = spergel_integrated_flux(nu, u); });
 }
 
 int gsmi_moffat_scale_radius_from_hlr(double hlr, double beta, double* out)

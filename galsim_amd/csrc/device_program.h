@@ -418,6 +418,14 @@ __device__ inline T eval_leafc(const LeafC<T>& C, const LeafI& I, const gsmi_lea
       }
       case GSMI_LEAF_KCLIP:
         return kclip(rec, i, j) ? T(0) : T(1);
+      case GSMI_LEAF_SPERGEL: {
+        // SBSpergelImpl::fillKImage, kloop_1d / kloop_2d (src/SBSpergel.cpp:121-139, 456-527)
+        T ksq = X * X + Y * Y;
+        if (exp_clip<T>(C, I, tb, i, j, Nk, ksq)) return T(0);
+        if constexpr (sizeof(T) == 4)
+            return flux * __builtin_amdgcn_exp2f(C.p[1] * __builtin_amdgcn_logf(1.f + ksq));
+        return flux * pow(T(1) + ksq, C.p[1]);
+      }
     }
     return T(0);
 }

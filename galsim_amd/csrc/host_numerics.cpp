@@ -943,4 +943,131 @@ double moffat_scale_radius_from_hlr_trunc(double re, double rm, double beta)
     return s.root();
 }
 
+// ---------------------------------------------------------------------------------------------
+// Spergel: src/SBSpergel.cpp:529-653
+// ---------------------------------------------------------------------------------------------
+double spergel_integrated_flux(double nu, double u)
+{
+    // Spergel (2010) eq. 8, as SpergelIntegratedFlux (src/SBSpergel.cpp:545-580)
+    const double gamma_nup2 = std::tgamma(nu + 2.);
+    const double fnup1 = std::pow(u / 2., nu + 1.) * std::cyl_bessel_k(nu + 1., u) / gamma_nup2;
+    return 1.0 - 2.0 * (1. + nu) * fnup1;
+}
+
+double spergel_flux_radius(double nu, double f)
+{
+    // CalculateFluxRadius (src/SBSpergel.cpp:582-598): bracket [0.1, 3.5], x tolerance 1e-25
+    auto func = [nu, f](double u) { return spergel_integrated_flux(nu, u) - f; };
+    auto s = make_brent(func, 0.1, 3.5);
+    s.xtol = 1.e-25;
+    if (f < 0.5) s.bracket_lower_with_limit(0.0);
+    else s.bracket_upper();
+    return s.root();
+}
+
+const SpergelInfo& spergel_info(double nu, const GSP& gsp)
+{
+    if (!(nu >= -0.85 && nu <= 4.0)) throw std::runtime_error("Spergel index out of range");
+    std::lock_guard<std::recursive_mutex> lock(table_mutex());
+    static std::map<std::pair<double, GSP>, SpergelInfo> cache;
+    auto key = std::make_pair(nu, gsp);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    SpergelInfo info;
+    info.hlr = spergel_flux_radius(nu, 0.5);
+    // stepK (src/SBSpergel.cpp:616-627)
+    double R = spergel_flux_radius(nu, 1.0 - gsp.folding_threshold);
+    R = std::max(R, gsp.stepk_minimum_hlr * info.hlr);
+    info.stepk = M_PI / R;
+    // maxK (629-636): (1 + k^2)^(-1-nu) = maxk_threshold
+    info.maxk = std::sqrt(std::pow(gsp.maxk_threshold, -1. / (1 + nu)) - 1.0);
+    // SBSpergelImpl ctor (src/SBSpergel.cpp:83-86)
+    info.ksq_max = std::pow(gsp.kvalue_accuracy, -1. / (nu + 1.)) - 1.;
+    return cache.emplace(key, info).first->second;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Row ranges of a k-space fill clipped at (k r0)^2 > ksq_max: SBProfile::drawK on bounds
+// (0..Nk/2, -Nk/2..Nk/2) (src/SBProfile.cpp:383-425) with GetKValueRange1d / 2d (508-588).
+// ---------------------------------------------------------------------------------------------
+void kclip_rows(double ksq_max, double r0, const double* jac, int Nk, double dk, int32_t* out)
+{
+    const double kmax = std::sqrt(ksq_max);
+    const int m = Nk / 2 + 1;
+    const int half = Nk / 2;
+    // SBProfile::drawK on bounds (0..Nk/2, -Nk/2..Nk/2) (src/SBProfile.cpp:383-425)
+    const double x0 = 0. * dk;
+    const double y0 = -half * dk;
+    if (!jac || (jac[1] == 0. && jac[2] == 0.)) {
+        // 1-d fill through FillQuadrant: quadrant rows ky = 0, dky, ... (src/SBProfile.cpp:447-506,
+        // GetKValueRange1d 508-531)
+        double A = jac ? jac[0] : 1., D = jac ? jac[3] : 1.;
+        double dkx = (jac ? dk * A : dk) * r0;
+        double dky = (jac ? dk * D : dk) * r0;
+        double kx0 = (jac ? x0 * A : x0) * r0;
+        double ky = 0. * r0;
+        for (int q = 0; q <= half; ++q, ky += dky) {
+            int i1, i2;
+            if (std::abs(ky) >= kmax) {
+                i1 = i2 = m;
+            } else {
+                double kysq = ky * ky;
+                double dsq = ksq_max - kysq;
+                if (kx0 * kx0 > dsq || (kx0 + m * dkx) * (kx0 + m * dkx) > dsq) {
+                    double d = std::sqrt(dsq);
+                    i1 = int(std::ceil((-kx0 - d) / dkx));
+                    i2 = int(std::floor((-kx0 + d) / dkx));
+                    if (i1 > i2) std::swap(i1, i2);
+                    ++i2;
+                    if (i2 <= 0 || i1 >= m) i1 = i2 = m;
+                    else { if (i1 < 0) i1 = 0; if (i2 > m) i2 = m; }
+                } else {
+                    i1 = 0;
+                    i2 = m;
+                }
+            }
+            for (int j : {half - q, half + q}) {
+                out[2 * j] = i1;
+                out[2 * j + 1] = i2;
+            }
+        }
+    } else {
+        // general Jacobian: 2-d fill, GetKValueRange2d (src/SBProfile.cpp:533-588) including its
+        // test of |ky0 + m + dky| (line 544)
+        double A = jac[0], B = jac[1], C = jac[2], D = jac[3];
+        double kx0 = (A * x0 + C * y0) * r0;
+        double dkx = (A * dk) * r0;
+        double dkxy = (C * dk) * r0;
+        double ky0 = (B * x0 + D * y0) * r0;
+        double dky = (D * dk) * r0;
+        double dkyx = (B * dk) * r0;
+        for (int j = 0; j <= Nk; ++j, kx0 += dkxy, ky0 += dky) {
+            int i1, i2;
+            if (std::abs(kx0) > kmax || std::abs(kx0 + m * dkx) > kmax ||
+                std::abs(ky0) > kmax || std::abs(ky0 + m + dkyx) > kmax) {
+                double a = dkx * dkx + dkyx * dkyx;
+                double b = dkx * kx0 + dkyx * ky0;
+                double c = kx0 * kx0 + ky0 * ky0 - ksq_max;
+                double d = b * b - a * c;
+                if (d <= 0.) {
+                    i1 = i2 = m;
+                } else {
+                    d = std::sqrt(d);
+                    i1 = int(std::ceil((-b - d) / a));
+                    i2 = int(std::floor((-b + d) / a));
+                    if (i1 > i2) std::swap(i1, i2);
+                    ++i2;
+                    if (i2 <= 0 || i1 >= m) i1 = i2 = m;
+                    else { if (i1 < 0) i1 = 0; if (i2 > m) i2 = m; }
+                }
+            } else {
+                i1 = 0;
+                i2 = m;
+            }
+            out[2 * j] = i1;
+            out[2 * j + 1] = i2;
+        }
+    }
+}
+
 }  // namespace gsmi

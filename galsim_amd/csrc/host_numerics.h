@@ -70,4 +70,21 @@ double moffat_scale_radius_from_hlr_trunc(double re, double rm, double beta);
 
 double bessel_k(double nu, double x);
 
+// Spergel (src/SBSpergel.cpp:529-653): sizes in units of the scale radius r0, and the clipping
+// radius of the k-space fill, ksq_max = kvalue_accuracy^(-1/(nu+1)) - 1 (in (k r0)^2); cached
+// per (nu, gsparams).
+struct SpergelInfo {
+    double maxk = 0., stepk = 0., hlr = 0., ksq_max = 0.;
+};
+const SpergelInfo& spergel_info(double nu, const GSP& gsp);
+// Radius (in r0) enclosing the fraction f of the flux: Brent on the closed-form enclosed flux.
+double spergel_flux_radius(double nu, double f);
+// Enclosed flux fraction within u r0: 1 - 2 (1+nu) (u/2)^(nu+1) K_{nu+1}(u) / Gamma(nu+2).
+double spergel_integrated_flux(double nu, double u);
+
+// GetKValueRange1d / 2d (src/SBProfile.cpp:508-588) of a radial profile clipped at
+// (k r0)^2 > ksq_max, on the stored rows of an (Nk/2+1) x (Nk+1) k-image with spacing dk: the
+// [i1, i2) range of each row, out[2 j], out[2 j + 1] for j = 0..Nk.
+void kclip_rows(double ksq_max, double r0, const double* jac, int Nk, double dk, int32_t* out);
+
 }  // namespace gsmi

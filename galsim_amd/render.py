@@ -66,6 +66,20 @@ class Lowering:
         self.nclip += len(out)
         return off
 
+    def kclip_rows(self, ksq_max, r0, jac):
+        """Host row ranges of a radial leaf clipped at (k r0)^2 > ksq_max (FP64 path only)."""
+        if self.precision != 'f64' or self.Nk is None:
+            return -1
+        out = np.zeros(2 * (self.Nk + 1), dtype=np.int32)
+        j = None if jac is None else np.ascontiguousarray(np.asarray(jac, float).reshape(4))
+        _lib.check(_lib.lib().gsmi_kclip_rows(float(ksq_max), float(r0), _lib.ptr(j),
+                                              int(self.Nk), float(self.dk), _lib.ptr(out)),
+                   "k clip rows")
+        off = self.nclip
+        self.clip.append(out)
+        self.nclip += len(out)
+        return off
+
     def ops_array(self):
         a = np.zeros(len(self.ops), dtype=_lib.OP_DTYPE)
         for i, (code, arg, x, y, z) in enumerate(self.ops):

@@ -57,9 +57,11 @@ enum {
     GSMI_LEAF_KOLMOGOROV = 4,
     GSMI_LEAF_SERSIC = 5,
     GSMI_LEAF_BOX = 6,
-    GSMI_LEAF_KCLIP = 7   /* Deconvolution's maxk cut (galsim/convolve.py:602-612): 0 where
+    GSMI_LEAF_KCLIP = 7,  /* Deconvolution's maxk cut (galsim/convolve.py:602-612): 0 where
                              ((J00 x + J10 y)^2 + (J01 x + J11 y)^2) dk^2 > maxk^2 at k-image
                              pixel (x, y), else 1; dk in scale, maxk^2 in p[0], in FP64 */
+    GSMI_LEAF_SPERGEL = 8 /* flux (1 + (k r0)^2)^(-1-nu), 0 where (k r0)^2 > ksq_max:
+                             scale = r0, p[0] = ksq_max, p[1] = -1-nu; cliprows as Exponential */
 };
 
 typedef struct gsmi_op {
@@ -192,6 +194,19 @@ int gsmi_moffat_trunc_table(double beta, double scale_radius, double trunc, cons
  * GetKValueRange1d/2d (src/SBProfile.cpp:508-588, src/SBExponential.cpp:442-510). */
 int gsmi_exponential_cliprows(double r0, const double* gsp, const double* jac, int Nk, double dk,
                               int32_t* out);
+
+/* The same row ranges for any radial leaf clipped at (k r0)^2 > ksq_max (Spergel: ksq_max =
+ * kvalue_accuracy^(-1/(nu+1)) - 1, SBSpergelImpl::fillKImage src/SBSpergel.cpp:456-527). */
+int gsmi_kclip_rows(double ksq_max, double r0, const double* jac, int Nk, double dk,
+                    int32_t* out);
+
+/* Spergel profile scalars (SpergelInfo, src/SBSpergel.cpp:529-653; pybind SBSpergel and
+ * SpergelCalculateHLR, pysrc/SBSpergel.cpp): out = {maxk, stepk, hlr, ksq_max} in units of the
+ * scale radius.  -0.85 <= nu <= 4 (GSMI_E_RANGE otherwise). */
+int gsmi_spergel_info(double nu, const double* gsp, double* out);
+/* SBSpergel::calculateFluxRadius / calculateIntegratedFlux in units of the scale radius. */
+int gsmi_spergel_flux_radius(double nu, double f, double* out);
+int gsmi_spergel_integrated_flux(double nu, double u, double* out);
 /* Host evaluation of a Sersic leaf k-value (SersicInfo::kValue, src/SBSersic.cpp:363-376). */
 int gsmi_sersic_kvalue(double n, double trunc, const double* gsp, const double* ksq, double* out,
                        int count);

@@ -308,6 +308,10 @@ def Sersic(n, **kw):
     return Leaf(pr.Sersic(n, **kw))
 
 
+def Spergel(nu, **kw):
+    return Leaf(pr.Spergel(nu, **kw))
+
+
 def Box(width, height, flux=1., gsparams=None):
     return Leaf(pr.Box(width, height, flux, gsparams))
 

@@ -218,6 +218,94 @@ class Exponential:
 
 
 # ----------------------------------------------------------------------------------------------
+# Spergel -- src/SBSpergel.cpp, galsim/spergel.py
+# ----------------------------------------------------------------------------------------------
+def spergel_integrated_flux(nu, u):
+    """Enclosed flux fraction within u r0 (SpergelIntegratedFlux, src/SBSpergel.cpp:545-580;
+    Spergel 2010 eq. 8): 1 - 2 (1+nu) (u/2)^(nu+1) K_{nu+1}(u) / Gamma(nu+2)."""
+    return 1. - 2. * (1. + nu) * (u / 2.) ** (nu + 1.) * sps.kv(nu + 1., u) / math.gamma(nu + 2.)
+
+
+def spergel_flux_radius(nu, f):
+    """CalculateFluxRadius (src/SBSpergel.cpp:582-598), solved to machine precision (the
+    reference asks its Brent solver for an x tolerance of 1e-25)."""
+    import scipy.optimize as spo
+    func = lambda u: spergel_integrated_flux(nu, u) - f
+    lo, hi = 0.1, 3.5
+    while func(lo) > 0.:
+        lo *= 0.5
+    while func(hi) < 0.:
+        hi *= 2.
+    return spo.brentq(func, lo, hi, xtol=1e-300, rtol=4 * np.finfo(float).eps, maxiter=500)
+
+
+class Spergel:
+    """SBSpergel (src/SBSpergel.cpp:72-99 ctor, 456-527 fills with kloop_1d/2d 121-139,
+    529-653 SpergelInfo); galsim/spergel.py:28-239.
+
+    Pinned by the known answers of tests/test_spergel.py:106-190 (scale radius, maxk, stepk at
+    nu = 0.2 and -0.2), the Mathematica flux radii of tests/test_spergel.py:30-83, and
+    nu = 0.5 being the Exponential (k-image and half-light radius).
+    """
+
+    def __init__(self, nu, half_light_radius=None, scale_radius=None, flux=1., gsparams=None):
+        self.nu = float(nu)
+        if not -0.85 <= self.nu <= 4.0:
+            raise ValueError("Spergel index out of range")
+        if half_light_radius is not None:
+            scale_radius = half_light_radius / spergel_flux_radius(self.nu, 0.5)
+        self.r0 = float(scale_radius)
+        self.flux = float(flux)
+        self.gsparams = gsparams or DEFAULT_GSPARAMS
+        self.ksq_max = self.gsparams.kvalue_accuracy ** (-1. / (self.nu + 1.)) - 1.
+        self.k_max = math.sqrt(self.ksq_max)
+
+    @property
+    def maxk(self):
+        # SpergelInfo::maxK (src/SBSpergel.cpp:629-636)
+        thr = self.gsparams.maxk_threshold
+        return math.sqrt(thr ** (-1. / (1 + self.nu)) - 1.0) / self.r0
+
+    @property
+    def stepk(self):
+        # SpergelInfo::stepK (src/SBSpergel.cpp:616-627)
+        gsp = self.gsparams
+        R = spergel_flux_radius(self.nu, 1. - gsp.folding_threshold)
+        R = max(R, gsp.stepk_minimum_hlr * spergel_flux_radius(self.nu, 0.5))
+        return (math.pi / R) / self.r0
+
+    def fill_1d(self, kx0, dkx, m, ky0, dky, n):
+        r0 = self.r0
+        kx0 *= r0; dkx *= r0; ky0 *= r0; dky *= r0
+        out = np.zeros((n, m))
+        kys = _axis(ky0, dky, n)
+        for j in range(n):
+            i1, i2, kysq = _kvalue_range_1d(m, self.k_max, self.ksq_max, kx0, dkx, kys[j])
+            if i1 == m:
+                continue
+            kx = _axis(kx0 + i1 * dkx, dkx, i2 - i1)
+            ksqp1 = kx * kx + (kysq + 1.)
+            out[j, i1:i2] = self.flux * ksqp1 ** (-(self.nu + 1.))
+        return out
+
+    def fill_2d(self, kx0, dkx, dkxy, ky0, dky, dkyx, m, n):
+        r0 = self.r0
+        kx0 *= r0; dkx *= r0; dkxy *= r0; ky0 *= r0; dky *= r0; dkyx *= r0
+        out = np.zeros((n, m))
+        rx = _axis(kx0, dkxy, n)
+        ry = _axis(ky0, dky, n)
+        for j in range(n):
+            i1, i2 = _kvalue_range_2d(m, self.k_max, self.ksq_max, rx[j], dkx, ry[j], dkyx)
+            if i1 == m:
+                continue
+            kx = _axis(rx[j] + i1 * dkx, dkx, i2 - i1)
+            ky = _axis(ry[j] + i1 * dkyx, dkyx, i2 - i1)
+            ksqp1 = 1. + kx * kx + ky * ky
+            out[j, i1:i2] = self.flux * ksqp1 ** (-(self.nu + 1.))
+        return out
+
+
+# ----------------------------------------------------------------------------------------------
 # Gaussian -- src/SBGaussian.cpp, galsim/gaussian.py
 # ----------------------------------------------------------------------------------------------
 class Gaussian:

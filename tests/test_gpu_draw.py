@@ -97,6 +97,14 @@ def test_exponential_moffat_seeded_sweep():
     ("odd_nonsquare", lambda m: _expmoffat(m, 1.1, 0.2, 0.3), (37, 52)),
     ("exp_small_clipped", lambda m: m.Exponential(scale_radius=0.02).shear(g1=0.4, g2=0.1),
      (24, 24)),
+    ("spergel_02", lambda m: m.Spergel(0.2, half_light_radius=0.8, flux=17.9), (48, 48)),
+    ("spergel_m06_conv", lambda m: m.Convolve([m.Spergel(-0.6, half_light_radius=0.6, flux=3.)
+                                               .shear(g1=0.2, g2=-0.1),
+                                               m.Kolmogorov(fwhm=0.7)]), (56, 56)),
+    ("spergel_3_shift", lambda m: m.Spergel(3., scale_radius=0.15).shift(0.1, 0.05),
+     (40, 40)),
+    ("spergel_small_clipped", lambda m: m.Spergel(0.85, scale_radius=0.03)
+     .shear(g1=0.3, g2=0.2), (24, 24)),
 ])
 @pytest.mark.parametrize("precision", ["f64", "f32"])
 def test_profiles_parity(name, make, shape, precision):
