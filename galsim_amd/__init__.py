@@ -3,6 +3,8 @@ MI355X.  The public names mirror galsim's: profiles, transformations, Convolve/A
 GSParams, Shear; drawing runs in hand-written HIP kernels through the C ABI of libgsmi.so
 (include/gsmi.h)."""
 from . import fft
+from .airy import Airy
+from .angle import Angle, AngleUnit, arcmin, arcsec, degrees, hours, radians
 from ._lib import (GalSimAMDBoundsError, GalSimAMDError, GalSimAMDRangeError,
                    GalSimAMDUndefinedBoundsError, GalSimAMDValueError)
 from .box import Box, Pixel
@@ -13,6 +15,7 @@ from .gaussian import Gaussian
 from .gsobject import GSObject
 from .gsparams import GSParams
 from .image import Image, ImageCD, ImageCF, ImageD, ImageF
+from .inclined import InclinedExponential
 from .kolmogorov import Kolmogorov
 from .moffat import Moffat
 from .position import BoundsI, PositionD, PositionI
@@ -24,11 +27,11 @@ from .transform import Transform, Transformation, _Transform
 
 GalSimError = GalSimAMDError
 
-__all__ = ['Add', 'AutoConvolution', 'AutoConvolve', 'AutoCorrelate', 'AutoCorrelation', 'BoundsI',
+__all__ = ['Add', 'Airy', 'Angle', 'AngleUnit', 'AutoConvolution', 'AutoConvolve', 'AutoCorrelate', 'AutoCorrelation', 'BoundsI',
            'Box', 'Convolution', 'Convolve', 'DeVaucouleurs', 'Deconvolution', 'Deconvolve',
            'Exponential',
            'GSObject', 'GSParams', 'GalSimAMDBoundsError', 'GalSimAMDError', 'GalSimAMDRangeError',
            'GalSimAMDUndefinedBoundsError', 'GalSimAMDValueError', 'GalSimError',
-           'Gaussian', 'Image', 'ImageCD', 'ImageCF', 'ImageD', 'ImageF', 'Kolmogorov', 'Moffat',
+           'Gaussian', 'Image', 'InclinedExponential', 'ImageCD', 'ImageCF', 'ImageD', 'ImageF', 'Kolmogorov', 'Moffat',
            'Pixel', 'PositionD', 'PositionI', 'Sersic', 'Shear', 'Spergel', 'Sum', 'Transform',
-           'Transformation']
+           'Transformation', 'arcmin', 'arcsec', 'degrees', 'fft', 'hours', 'radians']

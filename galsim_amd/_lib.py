@@ -34,6 +34,8 @@ OP_LEAF, OP_MUL, OP_ADD, OP_SCALE, OP_PHASE, OP_INV = 1, 2, 3, 4, 5, 6
 LEAF_EXPONENTIAL, LEAF_GAUSSIAN, LEAF_MOFFAT, LEAF_KOLMOGOROV, LEAF_SERSIC, LEAF_BOX = range(1, 7)
 LEAF_KCLIP = 7
 LEAF_SPERGEL = 8
+LEAF_AIRY = 9
+LEAF_INCLINED_EXP = 10
 
 ERRORS = {-1: "invalid argument", -2: "parameter out of range", -3: "solve/table failure",
           -4: "HIP runtime failure", -5: "unsupported FFT size"}
@@ -106,6 +108,8 @@ _SIGS = {
     'gsmi_exponential_cliprows': (_I, [_D, _P, _P, _I, _D, _P]),
     'gsmi_kclip_rows': (_I, [_D, _D, _P, _I, _D, _P]),
     'gsmi_spergel_info': (_I, [_D, _P, _P]),
+    'gsmi_inclined_exp_info': (_I, [_D, _D, _P, _P]),
+    'gsmi_inclined_exp_kvalue': (_I, [_D, _D, _D, _D, _P, _P, _P, _I]),
     'gsmi_spergel_flux_radius': (_I, [_D, _D, _P]),
     'gsmi_spergel_integrated_flux': (_I, [_D, _D, _P]),
     'gsmi_sersic_kvalue': (_I, [_D, _D, _P, _P, _P, _I]),

@@ -179,6 +179,33 @@ int gsmi_spergel_info(double nu, const double* gsp, double* out)
     });
 }
 
+int gsmi_inclined_exp_info(double cosi, double hs, const double* gsp, double* out)
+{
+    if (!out || !(cosi >= 0. && cosi <= 1.) || !(hs >= 0.) || !is_finite(hs)) {
+        set_error("gsmi_inclined_exp_info: bad argument");
+        return GSMI_E_ARG;
+    }
+    return guarded("gsmi_inclined_exp_info", [&] {
+        InclinedExpInfo s = inclined_exp_info(cosi, hs, GSP::from(gsp));
+        out[0] = s.maxk;
+        out[1] = s.stepk;
+        out[2] = s.ksq_min;
+        out[3] = s.ksq_max;
+    });
+}
+
+int gsmi_inclined_exp_kvalue(double cosi, double hs, double ksq_min, double ksq_max,
+                             const double* kx, const double* ky, double* out, int count)
+{
+    if (!kx || !ky || !out || count < 0) {
+        set_error("gsmi_inclined_exp_kvalue: bad argument");
+        return GSMI_E_ARG;
+    }
+    for (int i = 0; i < count; ++i)
+        out[i] = inclined_exp_kvalue(kx[i], ky[i], cosi, hs, ksq_min, ksq_max);
+    return GSMI_OK;
+}
+
 int gsmi_spergel_flux_radius(double nu, double f, double* out)
 {
     if (!out || !is_finite(nu) || !(f > 0. && f < 1.)) {

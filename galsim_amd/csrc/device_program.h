@@ -290,6 +290,56 @@ __device__ inline void moffat_generic_f32(const LeafC<T>& C, float k, float* g, 
     *dg = -0.5f * pk * km1;
 }
 
+// Airy k-space: the overlap area of two annuli (outer radius 1, inner radius obs) whose centres
+// are sqrt(t) apart (AiryInfoObs::kValue and its helpers, src/SBAiry.cpp:259-329; the
+// unobscured form AiryInfoNoObs::kValue, 409-420).
+// (Clamped where rounding could take an argument just outside the domain; in the reference's
+// FP64 arithmetic the clamps do not bind.)
+template <typename T>
+__device__ inline T airy_chord(T r, T h, T rsq, T hsq)
+{
+    if (r == T(0)) return T(0);
+    return rsq * asin(fmin(ddiv(h, r), T(1))) - h * dsqrt(fmax(rsq - hsq, T(0)));
+}
+
+template <typename T>
+__device__ inline T airy_circles(T r, T s, T rsq, T ssq, T tsq)     // radii r >= s
+{
+    const T rps_sq = (r + s) * (r + s);
+    if (tsq >= rps_sq) return T(0);
+    const T rms_sq = (r - s) * (r - s);
+    if (tsq <= rms_sq) return T(M_PI) * ssq;
+    const T hsq = fmax(T(0.5) * (rsq + ssq) - ddiv(tsq * tsq + rps_sq * rms_sq, T(4) * tsq),
+                       T(0));
+    const T h = dsqrt(hsq);
+    if (tsq < rsq - ssq)
+        return T(M_PI) * ssq - airy_chord(s, h, ssq, hsq) + airy_chord(r, h, rsq, hsq);
+    return airy_chord(s, h, ssq, hsq) + airy_chord(r, h, rsq, hsq);
+}
+
+template <typename T>
+__device__ inline T airy_circle_pair(T r, T rsq, T tsq)            // equal radii
+{
+    if (tsq >= T(4) * rsq) return T(0);
+    if (tsq == T(0)) return T(M_PI) * rsq;
+    const T hsq = rsq - tsq / T(4);
+    return T(2) * airy_chord(r, dsqrt(hsq), rsq, hsq);
+}
+
+template <typename T>
+__device__ inline T airy_kvalue(T t, T obs, T obssq)
+{
+    if (obs == T(0)) {
+        if (t >= T(4)) return T(0);
+        if (t == T(0)) return T(M_PI);
+        const T hsq = T(1) - t / T(4);
+        const T h = dsqrt(hsq);
+        return T(2) * (asin(h) - h * dsqrt(T(1) - hsq));
+    }
+    return airy_circle_pair(T(1), T(1), t) - T(2) * airy_circles(T(1), obs, T(1), obssq, t) +
+           airy_circle_pair(obs, obssq, t);
+}
+
 // Deconvolution._drawKImage's cut (galsim/convolve.py:602-612) at k-image pixel (x, y) = (i, j):
 // k' = (J00 x + J10 y, J01 x + J11 y), cut where |k'|^2 dk^2 > maxk^2, in FP64 with the
 // reference's order of operations (no fused multiply-adds) so the boundary samples agree.
@@ -418,6 +468,29 @@ __device__ inline T eval_leafc(const LeafC<T>& C, const LeafI& I, const gsmi_lea
       }
       case GSMI_LEAF_KCLIP:
         return kclip(rec, i, j) ? T(0) : T(1);
+      case GSMI_LEAF_AIRY:
+        // SBAiryImpl::fillKImage (src/SBAiry.cpp:189-248): knorm A((k lam/D / pi)^2)
+        return flux * airy_kvalue<T>(X * X + Y * Y, C.p[0], C.p[1]);
+      case GSMI_LEAF_INCLINED_EXP: {
+        // SBInclinedExponentialImpl::kValueHelper (src/SBInclinedExponential.cpp:277-336):
+        // a face-on Exponential at (kx, ky cos i) times the sech^2 disk's vertical factor
+        const T kyc = Y * C.p[0];
+        const T ksq = X * X + kyc * kyc;
+        if (ksq > C.p[3]) return T(0);
+        T base;
+        if (ksq < C.p[2]) {
+            base = T(1) - T(1.5) * ksq * (T(1) - T(1.25) * ksq);
+        } else {
+            const T t = T(1) + ksq;
+            base = ddiv(T(1), t * dsqrt(t));
+        }
+        const T sky = C.p[1] * Y;
+        const T sky2 = sky * sky;
+        const T conv = sky2 < C.p[2] ? T(1) - T(0.16666666667) * sky2 *
+                                                   (T(1) - T(0.116666666667) * sky2)
+                                     : ddiv(sky, sinh(sky));
+        return flux * base * conv;
+      }
       case GSMI_LEAF_SPERGEL: {
         // SBSpergelImpl::fillKImage, kloop_1d / kloop_2d (src/SBSpergel.cpp:121-139, 456-527)
         T ksq = X * X + Y * Y;

@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstring>
 #include <deque>
+#include <limits>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -178,6 +179,25 @@ struct Brent {
         }
         throw std::runtime_error("Too many iterations in bracketLowerWithLimit()");
     }
+    // Bisection (Solve.h's default method when setMethod(Brent) is not called).
+    double bisect()
+    {
+        eval();
+        double f = flo, fmid = fhi;
+        if (f * fmid > 0.) throw std::runtime_error("Root is not bracketed");
+        double dx, rtb;
+        if (f < 0.) { dx = ub - lb; rtb = lb; }
+        else { dx = lb - ub; rtb = ub; }
+        for (int j = 1; j <= max_steps; ++j) {
+            dx *= 0.5;
+            const double xmid = rtb + dx;
+            fmid = f_eval(xmid);
+            if (fmid <= 0.) rtb = xmid;
+            if (std::abs(dx) < xtol || fmid == 0.) return rtb;
+        }
+        throw std::runtime_error("Too many bisections");
+    }
+    double f_eval(double x) { return f(x); }
     double root()
     {
         double a = lb, b = ub, c = ub, d = b - a, e = b - a;
@@ -1068,6 +1088,83 @@ void kclip_rows(double ksq_max, double r0, const double* jac, int Nk, double dk,
             out[2 * j + 1] = i2;
         }
     }
+}
+
+// ---------------------------------------------------------------------------------------------
+// InclinedExponential: src/SBInclinedExponential.cpp:64-160 (ctor), 277-336 (kValueHelper)
+// ---------------------------------------------------------------------------------------------
+double inclined_exp_kvalue(double kx, double ky, double cosi, double hs, double ksq_min,
+                           double ksq_max)
+{
+    const double ky_cosi = ky * cosi;
+    const double ksq = kx * kx + ky_cosi * ky_cosi;
+    double res_base;
+    if (ksq > ksq_max) return 0.;
+    if (ksq < ksq_min) {
+        res_base = (1. - 1.5 * ksq * (1. - 1.25 * ksq));
+    } else {
+        const double temp = 1. + ksq;
+        res_base = 1. / (temp * std::sqrt(temp));
+    }
+    const double sky = hs * ky;
+    const double sky2 = sky * sky;
+    double res_conv;
+    if (sky2 < ksq_min) res_conv = (1. - 0.16666666667 * sky2 * (1. - 0.116666666667 * sky2));
+    else res_conv = sky / std::sinh(sky);
+    return res_base * res_conv;
+}
+
+InclinedExpInfo inclined_exp_info(double cosi, double hs, const GSP& gsp)
+{
+    InclinedExpInfo info;
+    double logx = std::log(gsp.folding_threshold);
+    double R = -logx;
+    for (int i = 0; i < 3; i++) R = std::log(1. + R) - logx;
+    const double exp_hlr = 1.6783469900166605;
+    R = std::max(R, gsp.stepk_minimum_hlr * exp_hlr);
+    info.stepk = M_PI / R;
+    info.ksq_min = std::pow(gsp.kvalue_accuracy / (35. / 16. + 31. / 15120. * hs), 1. / 3.);
+    const double maxk_min = std::pow(gsp.maxk_threshold, -1. / 3.);
+    const double clipk_min = std::pow(gsp.kvalue_accuracy, -1. / 3.);
+    if (cosi == 1) {
+        info.maxk = maxk_min;
+        info.ksq_max = clipk_min * clipk_min;
+        return info;
+    }
+    double maxk_max, clipk_max;
+    if (cosi > 0.01) {
+        if (cosi < 0.96) {
+            maxk_max = maxk_min / cosi;
+            clipk_max = clipk_min / cosi;
+        } else {
+            maxk_max = 1.05 * maxk_min;
+            clipk_max = 1.05 * clipk_min;
+        }
+    } else {
+        maxk_max = 100 * maxk_min;
+        clipk_max = 100 * clipk_min;
+    }
+    // the solves use the k-value without the clipping radius (still infinite then)
+    const double kmin = info.ksq_min;
+    const double inf = std::numeric_limits<double>::infinity();
+    auto kfunc = [=](double target) {
+        return [=](double k) {
+            return std::max(inclined_exp_kvalue(0., k, cosi, hs, kmin, inf),
+                            inclined_exp_kvalue(k, 0., cosi, hs, kmin, inf)) - target;
+        };
+    };
+    auto mf = kfunc(gsp.maxk_threshold);
+    auto ms = make_brent(mf, maxk_min, maxk_max);
+    if (mf(maxk_min) <= 0) ms.bracket_lower_with_limit(0.);
+    else ms.bracket_upper();
+    info.maxk = ms.root() + ms.xtol;
+    auto cf = kfunc(gsp.kvalue_accuracy);
+    auto cs = make_brent(cf, clipk_min, clipk_max);
+    if (cf(clipk_min) <= 0) cs.bracket_lower_with_limit(0.);
+    else cs.bracket_upper();
+    const double clipk = cs.bisect() + cs.xtol;
+    info.ksq_max = clipk * clipk;
+    return info;
 }
 
 }  // namespace gsmi

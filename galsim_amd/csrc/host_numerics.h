@@ -87,4 +87,14 @@ double spergel_integrated_flux(double nu, double u);
 // [i1, i2) range of each row, out[2 j], out[2 j + 1] for j = 0..Nk.
 void kclip_rows(double ksq_max, double r0, const double* jac, int Nk, double dk, int32_t* out);
 
+// InclinedExponential (src/SBInclinedExponential.cpp): sizes in units of the scale radius and the
+// k-value thresholds, for |cos i| and hs = pi/2 h0 |sin i| / r0.  The maximum k and the clipping
+// radius come from the reference's solves (Brent, then bisection) on the k-value along the axes.
+struct InclinedExpInfo {
+    double maxk = 0., stepk = 0., ksq_min = 0., ksq_max = 0.;
+};
+InclinedExpInfo inclined_exp_info(double cosi, double hs, const GSP& gsp);
+double inclined_exp_kvalue(double kx, double ky, double cosi, double hs, double ksq_min,
+                           double ksq_max);
+
 }  // namespace gsmi

@@ -60,8 +60,16 @@ enum {
     GSMI_LEAF_KCLIP = 7,  /* Deconvolution's maxk cut (galsim/convolve.py:602-612): 0 where
                              ((J00 x + J10 y)^2 + (J01 x + J11 y)^2) dk^2 > maxk^2 at k-image
                              pixel (x, y), else 1; dk in scale, maxk^2 in p[0], in FP64 */
-    GSMI_LEAF_SPERGEL = 8 /* flux (1 + (k r0)^2)^(-1-nu), 0 where (k r0)^2 > ksq_max:
+    GSMI_LEAF_SPERGEL = 8, /* flux (1 + (k r0)^2)^(-1-nu), 0 where (k r0)^2 > ksq_max:
                              scale = r0, p[0] = ksq_max, p[1] = -1-nu; cliprows as Exponential */
+    GSMI_LEAF_AIRY = 9,   /* knorm A(t), t = (k lam_over_diam / pi)^2, A the overlap area of two
+                             unit annuli (inner radius obscuration) at separation sqrt(t)
+                             (AiryInfoObs / AiryInfoNoObs::kValue, src/SBAiry.cpp:259-329,
+                             409-420): scale = lam_over_diam / pi, flux = knorm =
+                             flux / (pi (1 - obs^2)), p[0] = obs, p[1] = obs^2 */
+    GSMI_LEAF_INCLINED_EXP = 10 /* kValueHelper of src/SBInclinedExponential.cpp:277-336 at
+                             (kx, ky) r0: scale = r0, p[0] = |cos i|,
+                             p[1] = pi/2 h0 |sin i| / r0, p[2] = ksq_min, p[3] = ksq_max */
 };
 
 typedef struct gsmi_op {
@@ -204,6 +212,16 @@ int gsmi_kclip_rows(double ksq_max, double r0, const double* jac, int Nk, double
  * SpergelCalculateHLR, pysrc/SBSpergel.cpp): out = {maxk, stepk, hlr, ksq_max} in units of the
  * scale radius.  -0.85 <= nu <= 4 (GSMI_E_RANGE otherwise). */
 int gsmi_spergel_info(double nu, const double* gsp, double* out);
+/* InclinedExponential scalars (SBInclinedExponentialImpl ctor,
+ * src/SBInclinedExponential.cpp:64-160; pybind pysrc/SBInclinedExponential.cpp) for
+ * cosi = |cos i| and hs = pi/2 h0 |sin i| / r0: out = {maxk, stepk, ksq_min, ksq_max}, sizes in
+ * units of the scale radius. */
+int gsmi_inclined_exp_info(double cosi, double hs, const double* gsp, double* out);
+/* kValueHelper (src/SBInclinedExponential.cpp:277-336) at (kx, ky) in units of 1/r0, on the
+ * host: out[i] = value / flux. */
+int gsmi_inclined_exp_kvalue(double cosi, double hs, double ksq_min, double ksq_max,
+                             const double* kx, const double* ky, double* out, int count);
+
 /* SBSpergel::calculateFluxRadius / calculateIntegratedFlux in units of the scale radius. */
 int gsmi_spergel_flux_radius(double nu, double f, double* out);
 int gsmi_spergel_integrated_flux(double nu, double u, double* out);

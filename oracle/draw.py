@@ -312,6 +312,14 @@ def Spergel(nu, **kw):
     return Leaf(pr.Spergel(nu, **kw))
 
 
+def Airy(lam_over_diam, **kw):
+    return Leaf(pr.Airy(lam_over_diam, **kw))
+
+
+def InclinedExponential(inclination, scale_radius, scale_height, **kw):
+    return Leaf(pr.InclinedExponential(inclination, scale_radius, scale_height, **kw))
+
+
 def Box(width, height, flux=1., gsparams=None):
     return Leaf(pr.Box(width, height, flux, gsparams))
 

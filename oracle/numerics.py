@@ -140,6 +140,26 @@ class Solve:
             self.flower = self.func(self.lb)
         raise SolveError("Too many iterations in bracketLowerWithLimit()")
 
+    def bisect(self):
+        # bisect, Solve.h:376-393 (the method a Solve uses unless setMethod(Brent) is called)
+        self._evaluate_bounds()
+        f, fmid = self.flower, self.fupper
+        if f * fmid > 0.:
+            raise SolveError("Root is not bracketed")
+        if f < 0.:
+            dx, rtb = self.ub - self.lb, self.lb
+        else:
+            dx, rtb = self.lb - self.ub, self.ub
+        for _ in range(self.max_steps):
+            dx *= 0.5
+            xmid = rtb + dx
+            fmid = self.func(xmid)
+            if fmid <= 0.:
+                rtb = xmid
+            if abs(dx) < self.xtol or fmid == 0.:
+                return rtb
+        raise SolveError("Too many bisections")
+
     def root(self):
         # zbrent, Solve.h:396-462
         a = self.lb

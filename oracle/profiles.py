@@ -306,6 +306,221 @@ class Spergel:
 
 
 # ----------------------------------------------------------------------------------------------
+# Airy -- src/SBAiry.cpp, galsim/airy.py
+# ----------------------------------------------------------------------------------------------
+def _airy_chord(r, h, rsq, hsq):
+    """AiryInfoObs::chord (src/SBAiry.cpp:259-272)."""
+    if r == 0.:
+        return np.zeros_like(h)
+    return rsq * np.arcsin(h / r) - h * np.sqrt(rsq - hsq)
+
+
+def _airy_circles(r, s, rsq, ssq, tsq):
+    """Area of the intersection of circles of radii r >= s, centres sqrt(tsq) apart
+    (src/SBAiry.cpp:274-296)."""
+    rps_sq = (r + s) * (r + s)
+    rms_sq = (r - s) * (r - s)
+    out = np.zeros_like(tsq)
+    inner = tsq <= rms_sq
+    out[inner] = math.pi * ssq
+    mid = (tsq < rps_sq) & ~inner
+    t = tsq[mid]
+    hsq = 0.5 * (rsq + ssq) - (t * t + rps_sq * rms_sq) / (4. * t)
+    h = np.sqrt(hsq)
+    a = _airy_chord(s, h, ssq, hsq) + _airy_chord(r, h, rsq, hsq)
+    b = math.pi * ssq - _airy_chord(s, h, ssq, hsq) + _airy_chord(r, h, rsq, hsq)
+    out[mid] = np.where(t < rsq - ssq, b, a)
+    return out
+
+
+def _airy_circle_pair(r, rsq, tsq):
+    """Equal radii (src/SBAiry.cpp:298-313)."""
+    out = np.zeros_like(tsq)
+    out[tsq == 0.] = math.pi * rsq
+    mid = (tsq < 4. * rsq) & (tsq != 0.)
+    hsq = rsq - tsq[mid] / 4.
+    out[mid] = 2. * _airy_chord(r, np.sqrt(hsq), rsq, hsq)
+    return out
+
+
+def airy_kvalue(t, obs):
+    """Overlap area of two annuli (outer radius 1, inner obs) at separation sqrt(t)
+    (AiryInfoObs::kValue src/SBAiry.cpp:315-329; AiryInfoNoObs::kValue 409-420)."""
+    t = np.asarray(t, dtype=float)
+    if obs == 0.:
+        out = np.zeros_like(t)
+        out[t == 0.] = math.pi
+        mid = (t < 4.) & (t != 0.)
+        hsq = 1. - t[mid] / 4.
+        h = np.sqrt(hsq)
+        out[mid] = 2. * (np.arcsin(h) - h * np.sqrt(1. - hsq))
+        return out
+    obssq = obs * obs
+    return (_airy_circle_pair(1., 1., t) - 2. * _airy_circles(1., obs, 1., obssq, t) +
+            _airy_circle_pair(obs, obssq, t))
+
+
+class Airy:
+    """SBAiry (src/SBAiry.cpp:66-80 ctor, 189-248 fills, 250-257 sizes, 332-348 and 445-455
+    stepK); galsim/airy.py:28-249.
+
+    Pinned by the reference's known maxk / stepk (tests/test_airy.py:226-240), kValue(0) = flux,
+    the textbook lens area of two unit circles (obscuration 0) and a brute-force area integral of
+    two overlapping annuli (tests/test_airy_cpu.py).
+    """
+
+    def __init__(self, lam_over_diam, obscuration=0., flux=1., gsparams=None):
+        self.lod = float(lam_over_diam)
+        self.obs = float(obscuration)
+        self.flux = float(flux)
+        self.gsparams = gsparams or DEFAULT_GSPARAMS
+        self.knorm = self.flux / (math.pi * (1. - self.obs * self.obs))
+        self.inv_D_pi = self.lod / math.pi
+
+    @property
+    def maxk(self):
+        return 2. * math.pi / self.lod
+
+    @property
+    def stepk(self):
+        gsp = self.gsparams
+        R = 1. / (gsp.folding_threshold * 0.5 * math.pi * math.pi * (1. - self.obs))
+        R = max(R, gsp.stepk_minimum_hlr * 0.5348321477)
+        return (math.pi / R) / self.lod
+
+    def fill_1d(self, kx0, dkx, m, ky0, dky, n):
+        f = self.inv_D_pi
+        kx0 *= f; dkx *= f; ky0 *= f; dky *= f
+        kx = _axis(kx0, dkx, m)
+        ky = _axis(ky0, dky, n)
+        t = kx[None, :] * kx[None, :] + (ky * ky)[:, None]
+        return self.knorm * airy_kvalue(t, self.obs)
+
+    def fill_2d(self, kx0, dkx, dkxy, ky0, dky, dkyx, m, n):
+        f = self.inv_D_pi
+        kx0 *= f; dkx *= f; dkxy *= f; ky0 *= f; dky *= f; dkyx *= f
+        out = np.zeros((n, m))
+        rx = _axis(kx0, dkxy, n)
+        ry = _axis(ky0, dky, n)
+        for j in range(n):
+            kx = _axis(rx[j], dkx, m)
+            ky = _axis(ry[j], dkyx, m)
+            out[j] = self.knorm * airy_kvalue(kx * kx + ky * ky, self.obs)
+        return out
+
+
+# ----------------------------------------------------------------------------------------------
+# InclinedExponential -- src/SBInclinedExponential.cpp, galsim/inclined.py
+# ----------------------------------------------------------------------------------------------
+def inclined_exp_kvalue(kx, ky, cosi, hs, ksq_min, ksq_max):
+    """kValueHelper (src/SBInclinedExponential.cpp:277-336) at (kx, ky) in units of 1/r0."""
+    kx = np.asarray(kx, dtype=float)
+    ky = np.asarray(ky, dtype=float)
+    ky_cosi = ky * cosi
+    ksq = kx * kx + ky_cosi * ky_cosi
+    temp = 1. + ksq
+    base = np.where(ksq < ksq_min, 1. - 1.5 * ksq * (1. - 1.25 * ksq),
+                    1. / (temp * np.sqrt(temp)))
+    sky = hs * ky
+    sky2 = sky * sky
+    with np.errstate(invalid='ignore', divide='ignore', over='ignore'):
+        conv = np.where(sky2 < ksq_min, 1. - 0.16666666667 * sky2 * (1. - 0.116666666667 * sky2),
+                        sky / np.sinh(sky))
+    return np.where(ksq > ksq_max, 0., base * conv)
+
+
+class InclinedExponential:
+    """SBInclinedExponential (src/SBInclinedExponential.cpp:64-160 ctor with its maxk (Brent) and
+    clipping-radius (bisection) solves, 207-275 fills, 277-336 kValueHelper); galsim/inclined.py.
+
+    Pinned by Lance Miller's independent images (tests/inclined_exponential_images, compared in
+    tests/test_inclined.py:101-160), by inclination 0 reducing to the Exponential and by the
+    maxk / clipping roots satisfying their equations.
+    """
+
+    def __init__(self, inclination, scale_radius, scale_height, flux=1., gsparams=None):
+        self.r0 = float(scale_radius)
+        self.h0 = float(scale_height)
+        self.flux = float(flux)
+        self.gsparams = gsparams or DEFAULT_GSPARAMS
+        gsp = self.gsparams
+        self.cosi = abs(math.cos(inclination))
+        self.hs = 0.5 * math.pi * self.h0 * abs(math.sin(inclination)) / self.r0
+        logx = math.log(gsp.folding_threshold)
+        R = -logx
+        for _ in range(3):
+            R = math.log(1. + R) - logx
+        R = max(R, gsp.stepk_minimum_hlr * 1.6783469900166605)
+        self._stepk = math.pi / R
+        self.ksq_min = (gsp.kvalue_accuracy / (35. / 16. + 31. / 15120. * self.hs)) ** (1. / 3.)
+        maxk_min = gsp.maxk_threshold ** (-1. / 3.)
+        clipk_min = gsp.kvalue_accuracy ** (-1. / 3.)
+        if self.cosi == 1:
+            self._maxk = maxk_min
+            self.ksq_max = clipk_min * clipk_min
+            return
+        if self.cosi > 0.01:
+            if self.cosi < 0.96:
+                maxk_max, clipk_max = maxk_min / self.cosi, clipk_min / self.cosi
+            else:
+                maxk_max, clipk_max = 1.05 * maxk_min, 1.05 * clipk_min
+        else:
+            maxk_max, clipk_max = 100 * maxk_min, 100 * clipk_min
+
+        def func(target):
+            def f(k):
+                a = float(inclined_exp_kvalue(0., k, self.cosi, self.hs, self.ksq_min, math.inf))
+                b = float(inclined_exp_kvalue(k, 0., self.cosi, self.hs, self.ksq_min, math.inf))
+                return max(a, b) - target
+            return f
+
+        f = func(gsp.maxk_threshold)
+        s = nm.Solve(f, maxk_min, maxk_max)
+        if f(maxk_min) <= 0:
+            s.bracket_lower_with_limit(0.)
+        else:
+            s.bracket_upper()
+        self._maxk = s.root() + s.xtol
+        f = func(gsp.kvalue_accuracy)
+        s = nm.Solve(f, clipk_min, clipk_max)
+        if f(clipk_min) <= 0:
+            s.bracket_lower_with_limit(0.)
+        else:
+            s.bracket_upper()
+        clipk = s.bisect() + s.xtol
+        self.ksq_max = clipk * clipk
+
+    @property
+    def maxk(self):
+        return self._maxk / self.r0
+
+    @property
+    def stepk(self):
+        return self._stepk / self.r0
+
+    def _kv(self, kx, ky):
+        return self.flux * inclined_exp_kvalue(kx, ky, self.cosi, self.hs, self.ksq_min,
+                                               self.ksq_max)
+
+    def fill_1d(self, kx0, dkx, m, ky0, dky, n):
+        r0 = self.r0
+        kx0 *= r0; dkx *= r0; ky0 *= r0; dky *= r0
+        kx = _axis(kx0, dkx, m)
+        ky = _axis(ky0, dky, n)
+        return self._kv(kx[None, :], ky[:, None] + 0. * kx[None, :])
+
+    def fill_2d(self, kx0, dkx, dkxy, ky0, dky, dkyx, m, n):
+        r0 = self.r0
+        kx0 *= r0; dkx *= r0; dkxy *= r0; ky0 *= r0; dky *= r0; dkyx *= r0
+        out = np.zeros((n, m))
+        rx = _axis(kx0, dkxy, n)
+        ry = _axis(ky0, dky, n)
+        for j in range(n):
+            out[j] = self._kv(_axis(rx[j], dkx, m), _axis(ry[j], dkyx, m))
+        return out
+
+
+# ----------------------------------------------------------------------------------------------
 # Gaussian -- src/SBGaussian.cpp, galsim/gaussian.py
 # ----------------------------------------------------------------------------------------------
 class Gaussian:

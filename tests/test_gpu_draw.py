@@ -32,6 +32,12 @@ def _draw_pair(make, nx, ny, scale=0.2, precision='f64', **kw):
     return im, ref, info
 
 
+def _incl(m, inc, r0, h0, flux=1.):
+    if m is g:
+        return g.InclinedExponential(inc * g.radians, scale_radius=r0, scale_height=h0, flux=flux)
+    return od.InclinedExponential(inc, r0, h0, flux=flux)
+
+
 def _expmoffat(m, hlr=1.5, g1=0.1, g2=-0.05, beta=3, fwhm=0.7):
     gal = m.Exponential(half_light_radius=hlr, flux=1e4).shear(g1=g1, g2=g2)
     return m.Convolve([gal, m.Moffat(beta=beta, fwhm=fwhm)])
@@ -105,6 +111,18 @@ def test_exponential_moffat_seeded_sweep():
      (40, 40)),
     ("spergel_small_clipped", lambda m: m.Spergel(0.85, scale_radius=0.03)
      .shear(g1=0.3, g2=0.2), (24, 24)),
+    ("airy", lambda m: m.Airy(lam_over_diam=0.6, flux=2.), (40, 40)),
+    ("airy_obs_sheared", lambda m: m.Airy(lam_over_diam=0.8, obscuration=0.32)
+     .shear(g1=0.2, g2=0.1), (48, 48)),
+    ("airy_obs_conv", lambda m: m.Convolve([m.Exponential(half_light_radius=0.7, flux=9.),
+                                            m.Airy(lam_over_diam=0.5, obscuration=0.1)]),
+     (48, 48)),
+    ("inclined_13", lambda m: _incl(m, 1.3, 0.6, 0.1, 10.), (48, 48)),
+    ("inclined_edge_on_rotated", lambda m: _incl(m, 1.57, 0.5, 0.04, 2.1).rotate(7.4),
+     (48, 48)),
+    ("inclined_conv_sheared", lambda m: m.Convolve([_incl(m, 0.78, 0.4, 0.1, 3.)
+                                                    .shear(g1=0.1, g2=0.2),
+                                                    m.Kolmogorov(fwhm=0.7)]), (48, 48)),
 ])
 @pytest.mark.parametrize("precision", ["f64", "f32"])
 def test_profiles_parity(name, make, shape, precision):
