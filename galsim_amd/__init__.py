@@ -15,7 +15,7 @@ from .gaussian import Gaussian
 from .gsobject import GSObject
 from .gsparams import GSParams
 from .image import Image, ImageCD, ImageCF, ImageD, ImageF
-from .inclined import InclinedExponential
+from .inclined import InclinedExponential, InclinedSersic
 from .kolmogorov import Kolmogorov
 from .moffat import Moffat
 from .position import BoundsI, PositionD, PositionI
@@ -32,6 +32,6 @@ __all__ = ['Add', 'Airy', 'Angle', 'AngleUnit', 'AutoConvolution', 'AutoConvolve
            'Exponential',
            'GSObject', 'GSParams', 'GalSimAMDBoundsError', 'GalSimAMDError', 'GalSimAMDRangeError',
            'GalSimAMDUndefinedBoundsError', 'GalSimAMDValueError', 'GalSimError',
-           'Gaussian', 'Image', 'InclinedExponential', 'ImageCD', 'ImageCF', 'ImageD', 'ImageF', 'Kolmogorov', 'Moffat',
+           'Gaussian', 'Image', 'InclinedExponential', 'InclinedSersic', 'ImageCD', 'ImageCF', 'ImageD', 'ImageF', 'Kolmogorov', 'Moffat',
            'Pixel', 'PositionD', 'PositionI', 'Sersic', 'Shear', 'Spergel', 'Sum', 'Transform',
            'Transformation', 'arcmin', 'arcsec', 'degrees', 'fft', 'hours', 'radians']

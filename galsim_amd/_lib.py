@@ -36,6 +36,7 @@ LEAF_KCLIP = 7
 LEAF_SPERGEL = 8
 LEAF_AIRY = 9
 LEAF_INCLINED_EXP = 10
+LEAF_INCLINED_VERT = 11
 
 ERRORS = {-1: "invalid argument", -2: "parameter out of range", -3: "solve/table failure",
           -4: "HIP runtime failure", -5: "unsupported FFT size"}
@@ -109,6 +110,7 @@ _SIGS = {
     'gsmi_kclip_rows': (_I, [_D, _D, _P, _I, _D, _P]),
     'gsmi_spergel_info': (_I, [_D, _P, _P]),
     'gsmi_inclined_exp_info': (_I, [_D, _D, _P, _P]),
+    'gsmi_inclined_sersic_info': (_I, [_D, _D, _D, _D, _P, _P]),
     'gsmi_inclined_exp_kvalue': (_I, [_D, _D, _D, _D, _P, _P, _P, _I]),
     'gsmi_spergel_flux_radius': (_I, [_D, _D, _P]),
     'gsmi_spergel_integrated_flux': (_I, [_D, _D, _P]),

@@ -206,6 +206,22 @@ int gsmi_inclined_exp_kvalue(double cosi, double hs, double ksq_min, double ksq_
     return GSMI_OK;
 }
 
+int gsmi_inclined_sersic_info(double n, double trunc, double cosi, double hs, const double* gsp,
+                              double* out)
+{
+    if (!out || !is_finite(n) || !(trunc >= 0.) || !(cosi >= 0. && cosi <= 1.) || !(hs >= 0.) ||
+        !is_finite(hs)) {
+        set_error("gsmi_inclined_sersic_info: bad argument");
+        return GSMI_E_ARG;
+    }
+    return guarded("gsmi_inclined_sersic_info", [&] {
+        InclinedSersicInfo s = inclined_sersic_info(n, trunc, cosi, hs, GSP::from(gsp));
+        out[0] = s.maxk;
+        out[1] = s.ksq_min;
+        out[2] = s.ksq_max;
+    });
+}
+
 int gsmi_spergel_flux_radius(double nu, double f, double* out)
 {
     if (!out || !is_finite(nu) || !(f > 0. && f < 1.)) {

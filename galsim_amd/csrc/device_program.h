@@ -491,6 +491,16 @@ __device__ inline T eval_leafc(const LeafC<T>& C, const LeafI& I, const gsmi_lea
                                      : ddiv(sky, sinh(sky));
         return flux * base * conv;
       }
+      case GSMI_LEAF_INCLINED_VERT: {
+        // the clip and vertical factor of SBInclinedSersicImpl::kValueHelper
+        // (src/SBInclinedSersic.cpp:321-370); the Sersic base is a separate leaf
+        const T kyc = Y * C.p[0];
+        if (X * X + kyc * kyc > C.p[3]) return T(0);
+        const T sky = C.p[1] * Y;
+        const T sky2 = sky * sky;
+        return sky2 < C.p[2] ? T(1) - T(0.16666666667) * sky2 * (T(1) - T(0.116666666667) * sky2)
+                             : ddiv(sky, sinh(sky));
+      }
       case GSMI_LEAF_SPERGEL: {
         // SBSpergelImpl::fillKImage, kloop_1d / kloop_2d (src/SBSpergel.cpp:121-139, 456-527)
         T ksq = X * X + Y * Y;

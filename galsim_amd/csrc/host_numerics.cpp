@@ -1167,4 +1167,74 @@ InclinedExpInfo inclined_exp_info(double cosi, double hs, const GSP& gsp)
     return info;
 }
 
+// ---------------------------------------------------------------------------------------------
+// InclinedSersic: src/SBInclinedSersic.cpp:86-209 (ctor), 321-370 (kValueHelper)
+// ---------------------------------------------------------------------------------------------
+double sersic_kvalue(const SersicInfo& s, double ksq)
+{
+    // SersicInfo::kValue (src/SBSersic.cpp:363-376)
+    if (ksq >= s.ksq_max) return (s.highk_a + s.highk_b / std::sqrt(ksq)) / ksq;
+    if (ksq < s.ksq_min) return 1. + ksq * (s.kderiv2 + ksq * s.kderiv4);
+    return s.ft(0.5 * std::log(ksq)) / ksq;
+}
+
+double inclined_sersic_kvalue(const SersicInfo& s, double kx, double ky, double cosi, double hs,
+                              double ksq_min, double ksq_max)
+{
+    const double ky_cosi = ky * cosi;
+    const double ksq = kx * kx + ky_cosi * ky_cosi;
+    if (ksq > ksq_max) return 0.;
+    const double res_base = sersic_kvalue(s, ksq);
+    const double sky = hs * ky;
+    const double sky2 = sky * sky;
+    double res_conv;
+    if (sky2 < ksq_min) res_conv = (1. - 0.16666666667 * sky2 * (1. - 0.116666666667 * sky2));
+    else res_conv = sky / std::sinh(sky);
+    return res_base * res_conv;
+}
+
+InclinedSersicInfo inclined_sersic_info(double n, double trunc, double cosi, double hs,
+                                        const GSP& gsp)
+{
+    const SersicInfo& s = sersic_info(n, trunc, gsp);
+    InclinedSersicInfo info;
+    const double kderiv6 = 31. / 15120. * hs;
+    info.ksq_min = std::pow(gsp.kvalue_accuracy / kderiv6, 1. / 3.);
+    const double maxk_min = std::pow(gsp.maxk_threshold, -1. / 3.);
+    const double clipk_min = std::pow(gsp.kvalue_accuracy, -1. / 3.);
+    double maxk_max, clipk_max;
+    if (cosi > 0.01) {
+        if (cosi < 0.96) {
+            maxk_max = maxk_min / cosi;
+            clipk_max = clipk_min / cosi;
+        } else {
+            maxk_max = 1.05 * maxk_min;
+            clipk_max = 1.05 * clipk_min;
+        }
+    } else {
+        maxk_max = 100 * maxk_min;
+        clipk_max = 100 * clipk_min;
+    }
+    const double kmin = info.ksq_min;
+    const double inf = std::numeric_limits<double>::infinity();
+    auto kfunc = [&s, cosi, hs, kmin, inf](double target) {
+        return [&s, cosi, hs, kmin, inf, target](double k) {
+            return std::max(inclined_sersic_kvalue(s, k, 0., cosi, hs, kmin, inf),
+                            inclined_sersic_kvalue(s, 0., k, cosi, hs, kmin, inf)) - target;
+        };
+    };
+    auto mf = kfunc(gsp.maxk_threshold);
+    auto ms = make_brent(mf, maxk_min, maxk_max);
+    if (mf(maxk_min) <= 0) ms.bracket_lower_with_limit(0.);
+    else ms.bracket_upper();
+    info.maxk = ms.root() + ms.xtol;
+    auto cf = kfunc(gsp.kvalue_accuracy);
+    auto cs = make_brent(cf, clipk_min, clipk_max);
+    if (cf(clipk_min) <= 0) cs.bracket_lower_with_limit(0.);
+    else cs.bracket_upper();
+    const double clipk = cs.bisect() + cs.xtol;
+    info.ksq_max = clipk * clipk;
+    return info;
+}
+
 }  // namespace gsmi

@@ -97,4 +97,14 @@ InclinedExpInfo inclined_exp_info(double cosi, double hs, const GSP& gsp);
 double inclined_exp_kvalue(double kx, double ky, double cosi, double hs, double ksq_min,
                            double ksq_max);
 
+// InclinedSersic (src/SBInclinedSersic.cpp): the Sersic k-value (SersicInfo::kValue, k in 1/r0)
+// at (kx, ky cos i) times the same vertical factor; maxk (in 1/r0) and the clipping radius from
+// the reference's solves, ksq_min of the vertical factor's Taylor branch.
+struct InclinedSersicInfo {
+    double maxk = 0., ksq_min = 0., ksq_max = 0.;
+};
+double sersic_kvalue(const SersicInfo& s, double ksq);
+InclinedSersicInfo inclined_sersic_info(double n, double trunc, double cosi, double hs,
+                                        const GSP& gsp);
+
 }  // namespace gsmi

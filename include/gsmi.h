@@ -67,9 +67,13 @@ enum {
                              (AiryInfoObs / AiryInfoNoObs::kValue, src/SBAiry.cpp:259-329,
                              409-420): scale = lam_over_diam / pi, flux = knorm =
                              flux / (pi (1 - obs^2)), p[0] = obs, p[1] = obs^2 */
-    GSMI_LEAF_INCLINED_EXP = 10 /* kValueHelper of src/SBInclinedExponential.cpp:277-336 at
+    GSMI_LEAF_INCLINED_EXP = 10, /* kValueHelper of src/SBInclinedExponential.cpp:277-336 at
                              (kx, ky) r0: scale = r0, p[0] = |cos i|,
                              p[1] = pi/2 h0 |sin i| / r0, p[2] = ksq_min, p[3] = ksq_max */
+    GSMI_LEAF_INCLINED_VERT = 11 /* InclinedSersic's vertical factor and clip
+                             (src/SBInclinedSersic.cpp:321-370): 0 where
+                             (kx r0)^2 + (ky r0 cos i)^2 > ksq_max, else x / sinh(x) (Taylor
+                             below ksq_min), x = hs ky r0; same fields as INCLINED_EXP */
 };
 
 typedef struct gsmi_op {
@@ -221,6 +225,13 @@ int gsmi_inclined_exp_info(double cosi, double hs, const double* gsp, double* ou
  * host: out[i] = value / flux. */
 int gsmi_inclined_exp_kvalue(double cosi, double hs, double ksq_min, double ksq_max,
                              const double* kx, const double* ky, double* out, int count);
+
+/* InclinedSersic scalars (SBInclinedSersicImpl ctor, src/SBInclinedSersic.cpp:86-209) for
+ * Sersic index n, truncation radius trunc (units of r0; 0 = none), cosi = |cos i| and
+ * hs = pi/2 h0 |sin i| / r0: out = {maxk (1/r0), ksq_min, ksq_max}.  The profile is lowered as a
+ * Sersic leaf whose Jacobian scales ky' by cos i times a GSMI_LEAF_INCLINED_VERT leaf. */
+int gsmi_inclined_sersic_info(double n, double trunc, double cosi, double hs, const double* gsp,
+                              double* out);
 
 /* SBSpergel::calculateFluxRadius / calculateIntegratedFlux in units of the scale radius. */
 int gsmi_spergel_flux_radius(double nu, double f, double* out);

@@ -320,6 +320,10 @@ def InclinedExponential(inclination, scale_radius, scale_height, **kw):
     return Leaf(pr.InclinedExponential(inclination, scale_radius, scale_height, **kw))
 
 
+def InclinedSersic(n, inclination, scale_radius, scale_height, **kw):
+    return Leaf(pr.InclinedSersic(n, inclination, scale_radius, scale_height, **kw))
+
+
 def Box(width, height, flux=1., gsparams=None):
     return Leaf(pr.Box(width, height, flux, gsparams))
 

@@ -38,6 +38,13 @@ def _incl(m, inc, r0, h0, flux=1.):
     return od.InclinedExponential(inc, r0, h0, flux=flux)
 
 
+def _incls(m, n, inc, r0, h0, flux=1., trunc=0.):
+    if m is g:
+        return g.InclinedSersic(n, inc * g.radians, scale_radius=r0, scale_height=h0, flux=flux,
+                                trunc=trunc)
+    return od.InclinedSersic(n, inc, r0, h0, flux=flux, trunc=trunc)
+
+
 def _expmoffat(m, hlr=1.5, g1=0.1, g2=-0.05, beta=3, fwhm=0.7):
     gal = m.Exponential(half_light_radius=hlr, flux=1e4).shear(g1=g1, g2=g2)
     return m.Convolve([gal, m.Moffat(beta=beta, fwhm=fwhm)])
@@ -123,6 +130,11 @@ def test_exponential_moffat_seeded_sweep():
     ("inclined_conv_sheared", lambda m: m.Convolve([_incl(m, 0.78, 0.4, 0.1, 3.)
                                                     .shear(g1=0.1, g2=0.2),
                                                     m.Kolmogorov(fwhm=0.7)]), (48, 48)),
+    ("inclined_sersic_15", lambda m: _incls(m, 1.5, 1.1, 0.1, 0.03, 4.), (48, 48)),
+    ("inclined_sersic_trunc_rot", lambda m: _incls(m, 0.8, 1.4, 0.3, 0.03, 2., 1.2)
+     .rotate(1.1), (48, 48)),
+    ("inclined_sersic_conv", lambda m: m.Convolve([_incls(m, 2.5, 0.6, 0.05, 0.01, 5.),
+                                                   m.Kolmogorov(fwhm=0.7)]), (48, 48)),
 ])
 @pytest.mark.parametrize("precision", ["f64", "f32"])
 def test_profiles_parity(name, make, shape, precision):
