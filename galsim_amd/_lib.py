@@ -128,6 +128,7 @@ _SIGS = {
     'gsmi_rfft2_f64': (_I, [_P, _I, _I, _I, _I, _P, _P, _P]),
     'gsmi_irfft2_f64': (_I, [_P, _I, _I, _I, _I, _P, _P, _P]),
     'gsmi_hermitian_fill_f64': (_I, [_P, _I, _I, _I, _P, _P]),
+    'gsmi_fft_column_passes': (_I, [_I]),
     'gsmi_draw_list_f64': (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P]),
     'gsmi_draw_list_f32': (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P, _P]),
     'gsmi_fold_list_f64': (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P]),

@@ -97,52 +97,74 @@ __device__ cplx<T>* line_dft(cplx<T>* a, cplx<T>* b, int nl, const LinePlan<T>& 
     return y;
 }
 
-// Complex transform of `nlines` lines of length L: element n of line l at l * ls + n * es (one of
-// ls, es is 1).  Reads line (l + lrot) mod nlines at element (n + erot) mod L, writes line l at
-// element (k + krot) mod L, times scale; conj_io makes it the forward (e^{-2 pi i}) transform.
-// grid: ceil(nlines / nl); dynamic LDS: 2 * nl * S complex.
+// Where element n of line l lives: (l / D) A + (l % D) B + n es.  A plain set of lines with line
+// stride ls is D >= nlines, B = ls; the four-step passes below use the two-level form.
+struct LineMap {
+    long long D, A, B, es;
+    __device__ __forceinline__ long long base(int l, int nlines) const
+    {
+        return (D >= nlines) ? l * B : (l / D) * A + (l % D) * B;
+    }
+};
+
+// Twiddles applied to a four-step first pass: output k of line l is multiplied by
+// tw[k * n2], n2 = (l / div) % mod, tw = exp(+2 pi i m / L) of the whole length (tw == nullptr:
+// none).
+template <typename T>
+struct PassTwiddle {
+    const cplx<T>* tw;
+    int div, mod;
+};
+
+// Complex transform of `nlines` lines of length L (see LineMap).  Reads line (l + lrot) mod
+// nlines at element (n + erot) mod L, writes line l at element k the value Y((k + krot) mod L),
+// times scale; conj_io makes it the forward (e^{-2 pi i}) transform (conjugated input, and output
+// conjugated after the optional twiddle).  grid: ceil(nlines / nl); dynamic LDS: 2 nl S complex.
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
 c2c_lines(const cplx<T>* __restrict__ src, cplx<T>* __restrict__ dst, int nlines, int nl,
-          long long ls, long long es, int lrot, int erot, int krot, int conj_io, T scale,
-          LinePlan<T> p)
+          LineMap in, int lrot, int erot, LineMap out, int krot, int conj_in, int conj_out,
+          T scale, LinePlan<T> p, PassTwiddle<T> ptw)
 {
     extern __shared__ unsigned char smem_raw[];
     cplx<T>* bufa = reinterpret_cast<cplx<T>*>(smem_raw);
     cplx<T>* bufb = bufa + size_t(nl) * p.S;
     const int L = p.L, S = p.S;
-    const bool along = (es == 1);      // element index fastest in memory
-    // a column tile reads nl adjacent values per row: neighbouring tiles go to one XCD so the
-    // rest of each cache line is read from its L2
-    const int l0 = (along ? int(blockIdx.x) : xcd_contiguous(blockIdx.x, gridDim.x)) * nl;
+    const bool along_in = (in.es == 1);     // element index fastest in memory
+    const bool along_out = (out.es == 1);
+    // a tile of lines adjacent in memory reads nl neighbouring values per element: neighbouring
+    // tiles go to one XCD so the rest of each cache line is read from its L2
+    const int l0 = (along_in ? int(blockIdx.x) : xcd_contiguous(blockIdx.x, gridDim.x)) * nl;
     const int nlc = min(nl, nlines - l0);
     if (nlc <= 0) return;
     const float inv_l = 1.f / float(L);
     const float inv_nl = 1.f / float(nlc);
     for (int e = threadIdx.x; e < nlc * L; e += blockDim.x) {
         int li, n;
-        if (along) { li = fdiv(e, inv_l); n = e - li * L; }
+        if (along_in) { li = fdiv(e, inv_l); n = e - li * L; }
         else { n = fdiv(e, inv_nl); li = e - n * nlc; }
         int sl = l0 + li + lrot;
         if (sl >= nlines) sl -= nlines;
         int sn = n + erot;
         if (sn >= L) sn -= L;
-        cplx<T> v = src[sl * ls + sn * es];
-        if (conj_io) v.y = -v.y;
+        cplx<T> v = src[in.base(sl, nlines) + sn * in.es];
+        if (conj_in) v.y = -v.y;
         bufa[li * S + n] = v;
     }
     __syncthreads();
     const cplx<T>* y = line_dft<T>(bufa, bufb, nlc, p);
-    const T ysign = conj_io ? -scale : scale;
     for (int e = threadIdx.x; e < nlc * L; e += blockDim.x) {
         int li, k;
-        if (along) { li = fdiv(e, inv_l); k = e - li * L; }
+        if (along_out) { li = fdiv(e, inv_l); k = e - li * L; }
         else { k = fdiv(e, inv_nl); li = e - k * nlc; }
         // output position k holds Y((k + krot) mod L): the rotation by L/2 is its own inverse
         int sk = k + krot;
         if (sk >= L) sk -= L;
-        const cplx<T> v = y[li * S + sk];
-        dst[(l0 + li) * ls + k * es] = {v.x * scale, v.y * ysign};
+        cplx<T> v = y[li * S + sk];
+        if (ptw.tw) v = cmul(v, ptw.tw[sk * (((l0 + li) / ptw.div) % ptw.mod)]);
+        v.x *= scale;
+        v.y *= conj_out ? -scale : scale;
+        dst[out.base(l0 + li, nlines) + k * out.es] = v;
     }
 }
 
@@ -387,17 +409,101 @@ int lines_per_group(int S, int nlines, size_t lds)
     return nl;
 }
 
-int launch_c2c(const cplx<double>* src, cplx<double>* dst, int nlines, long long ls, long long es,
-               int lrot, int erot, int krot, int conj_io, double scale, const LinePlan<double>& p,
-               size_t lds, hipStream_t s)
+int launch_lines(const cplx<double>* src, cplx<double>* dst, int nlines, LineMap in, int lrot,
+                 int erot, LineMap out, int krot, int conj_in, int conj_out, double scale,
+                 const LinePlan<double>& p, PassTwiddle<double> ptw, size_t lds, hipStream_t s)
 {
     const int nl = lines_per_group(p.S, nlines, lds);
     const size_t bytes = 2 * size_t(nl) * p.S * sizeof(cplx<double>);
     int rc = allow_lds(reinterpret_cast<const void*>(c2c_lines<double>), bytes);
     if (rc) return rc;
     hipLaunchKernelGGL(c2c_lines<double>, dim3((nlines + nl - 1) / nl), dim3(kThreads), bytes, s,
-                       src, dst, nlines, nl, ls, es, lrot, erot, krot, conj_io, scale, p);
+                       src, dst, nlines, nl, in, lrot, erot, out, krot, conj_in, conj_out, scale,
+                       p, ptw);
     return hip_check(hipGetLastError(), "c2c_lines launch");
+}
+
+// One pass over plain lines (line stride ls, element stride es) that fit the LDS.
+int launch_c2c(const cplx<double>* src, cplx<double>* dst, int nlines, long long ls, long long es,
+               int lrot, int erot, int krot, int conj_io, double scale, const LinePlan<double>& p,
+               size_t lds, hipStream_t s)
+{
+    const LineMap m{nlines, 0, ls, es};
+    return launch_lines(src, dst, nlines, m, lrot, erot, m, krot, conj_io, conj_io, scale, p,
+                        PassTwiddle<double>{nullptr, 1, 1}, lds, s);
+}
+
+// Columns too long for one workgroup's LDS: the four-step DFT of length L = L1 L2 (both factors
+// fitting): with n = n1 L2 + n2 and k = k1 + L1 k2, first L2 W transforms of length L1 over n1
+// (the data read as the n1 x (n2, c) matrix, output times exp(+2 pi i k1 n2 / L), written back in
+// place), then L1 W transforms of length L2 over n2 written to k = k1 + L1 k2 in dst (which must
+// differ from tmp).  Columns are W adjacent values per row (row stride W); the input rotation
+// erot = L/2 (shift) rotates n1 by L1/2 and the output rotation L/2 rotates k2 by L2/2, so L1 and
+// L2 are even.  src -> tmp -> dst; tmp may equal src.
+struct LongPlan {
+    int L1 = 0, L2 = 0;
+    LinePlan<double> p1, p2;
+    const cplx<double>* tw = nullptr;
+};
+
+int make_long_plan(int L, size_t lds, LongPlan* lp)
+{
+    // the most balanced even factorisation with both factors transformable in LDS
+    int best = 0;
+    for (int a = 2; a * a <= L; a += 2) {
+        if (L % a) continue;
+        const int b = L / a;
+        if (b % 2) continue;
+        best = a;
+    }
+    if (!best) {
+        set_error("FFT length " + std::to_string(L) + " has no even factorisation for the "
+                  "four-step transform");
+        return GSMI_E_SIZE;
+    }
+    lp->L1 = best;
+    lp->L2 = L / best;
+    int rc;
+    if ((rc = make_plan(lp->L1, lds, &lp->p1)) || (rc = make_plan(lp->L2, lds, &lp->p2)))
+        return rc;
+    lp->tw = twiddles<double>(L);
+    if (!lp->tw) {
+        set_error("twiddle table allocation failed");
+        return GSMI_E_HIP;
+    }
+    return GSMI_OK;
+}
+
+int launch_columns_long(const cplx<double>* src, cplx<double>* tmp, cplx<double>* dst, int W,
+                        const LongPlan& lp, int erot, int krot, int conj_io, double scale,
+                        size_t lds, hipStream_t s)
+{
+    const long long L1 = lp.L1, L2 = lp.L2;
+    // pass 1: line l = n2 W + c, element n1 at n1 (L2 W) + l
+    const int n1lines = int(L2 * W);
+    const LineMap m1{n1lines, 0, 1, L2 * W};
+    int rc = launch_lines(src, tmp, n1lines, m1, 0, erot ? int(L1 / 2) : 0, m1, 0, conj_io, 0,
+                          1., lp.p1, PassTwiddle<double>{lp.tw, W, int(L2)}, lds, s);
+    if (rc) return rc;
+    // pass 2: line l = k1 W + c, element n2 at k1 (L2 W) + c + n2 W; output k2 at
+    // (k1 + L1 k2) W + c = l + k2 (L1 W)
+    const int n2lines = int(L1 * W);
+    const LineMap in2{W, L2 * W, 1, W};
+    const LineMap out2{n2lines, 0, 1, L1 * W};
+    return launch_lines(tmp, dst, n2lines, in2, 0, 0, out2, krot ? int(L2 / 2) : 0, 0, conj_io,
+                        scale, lp.p2, PassTwiddle<double>{nullptr, 1, 1}, lds, s);
+}
+
+// A column length transformable in one pass, or the four-step plan for it.
+int column_plan(int L, size_t lds, LinePlan<double>* p, LongPlan* lp, bool* longp)
+{
+    const size_t elem = sizeof(cplx<double>);
+    const size_t cap = lds / (2 * elem);
+    *longp = false;
+    if (smooth5(L) && size_t(L) <= cap) return make_plan(L, lds, p);
+    if (!smooth5(L) && size_t(next_smooth5(2 * L - 1)) <= cap) return make_plan(L, lds, p);
+    *longp = true;
+    return make_long_plan(L, lds, lp);
 }
 
 int check_sizes(const char* name, int M, int N)
@@ -431,7 +537,9 @@ int gsmi_cfft2_f64(const double* in, int M, int N, int inverse, int shift_in, in
     if (rc) return rc;
     const size_t lds = lds_limit();
     LinePlan<double> px, py;
-    if ((rc = make_plan(N, lds, &px)) || (rc = make_plan(M, lds, &py))) return rc;
+    LongPlan lpy;
+    bool ly = false;
+    if ((rc = make_plan(N, lds, &px)) || (rc = column_plan(M, lds, &py, &lpy, &ly))) return rc;
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     const auto* src = reinterpret_cast<const cplx<double>*>(in);
     auto* tmp = reinterpret_cast<cplx<double>*>(scratch);
@@ -443,6 +551,7 @@ int gsmi_cfft2_f64(const double* in, int M, int N, int inverse, int shift_in, in
                     shift_out ? N / 2 : 0, conj_io, 1., px, lds, s);
     if (rc) return rc;
     // columns: output y rotated (shift_out)
+    if (ly) return launch_columns_long(tmp, tmp, dst, N, lpy, 0, shift_out, conj_io, scale, lds, s);
     return launch_c2c(tmp, dst, N, 1, N, 0, 0, shift_out ? M / 2 : 0, conj_io, scale, py, lds, s);
 }
 
@@ -457,7 +566,10 @@ int gsmi_rfft2_f64(const double* in, int M, int N, int shift_in, int shift_out, 
     if (rc) return rc;
     const size_t lds = lds_limit();
     LinePlan<double> px, py;
-    if ((rc = make_plan(N / 2, lds, &px)) || (rc = make_plan(M, lds, &py))) return rc;
+    LongPlan lpy;
+    bool ly = false;
+    if ((rc = make_plan(N / 2, lds, &px)) || (rc = column_plan(M, lds, &py, &lpy, &ly)))
+        return rc;
     const cplx<double>* twn = twiddles<double>(N);
     if (!twn) {
         set_error("twiddle table allocation failed");
@@ -472,6 +584,9 @@ int gsmi_rfft2_f64(const double* in, int M, int N, int shift_in, int shift_out, 
     hipLaunchKernelGGL(r2c_rows<double>, dim3((M + nl - 1) / nl), dim3(kThreads), bytes, s, in,
                        tmp, M, N, nl, shift_in ? M / 2 : 0, shift_in ? N / 2 : 0, px, twn);
     if ((rc = hip_check(hipGetLastError(), "r2c_rows launch"))) return rc;
+    if (ly)
+        return launch_columns_long(tmp, tmp, reinterpret_cast<cplx<double>*>(out), W, lpy, 0,
+                                   shift_out, 1, 1., lds, s);
     return launch_c2c(tmp, reinterpret_cast<cplx<double>*>(out), W, 1, W, 0, 0,
                       shift_out ? M / 2 : 0, 1, 1., py, lds, s);
 }
@@ -487,7 +602,10 @@ int gsmi_irfft2_f64(const double* in, int M, int N, int shift_in, int shift_out,
     if (rc) return rc;
     const size_t lds = lds_limit();
     LinePlan<double> px, py;
-    if ((rc = make_plan(N / 2, lds, &px)) || (rc = make_plan(M, lds, &py))) return rc;
+    LongPlan lpy;
+    bool ly = false;
+    if ((rc = make_plan(N / 2, lds, &px)) || (rc = column_plan(M, lds, &py, &lpy, &ly)))
+        return rc;
     const cplx<double>* twn = twiddles<double>(N);
     if (!twn) {
         set_error("twiddle table allocation failed");
@@ -496,9 +614,16 @@ int gsmi_irfft2_f64(const double* in, int M, int N, int shift_in, int shift_out,
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     auto* tmp = reinterpret_cast<cplx<double>*>(scratch);
     const int W = N / 2 + 1;
-    // columns first (inverse, input rows rotated by shift_in), unscaled
-    rc = launch_c2c(reinterpret_cast<const cplx<double>*>(in), tmp, W, 1, W, 0,
-                    shift_in ? M / 2 : 0, 0, 0, 1., py, lds, s);
+    // columns first (inverse, input rows rotated by shift_in), unscaled; a four-step column
+    // transform leaves its result in the second half of scratch
+    if (ly) {
+        rc = launch_columns_long(reinterpret_cast<const cplx<double>*>(in), tmp,
+                                 tmp + size_t(M) * W, W, lpy, shift_in, 0, 0, 1., lds, s);
+        tmp += size_t(M) * W;
+    } else {
+        rc = launch_c2c(reinterpret_cast<const cplx<double>*>(in), tmp, W, 1, W, 0,
+                        shift_in ? M / 2 : 0, 0, 0, 1., py, lds, s);
+    }
     if (rc) return rc;
     const int nl = lines_per_group(px.S, M, lds);
     const size_t bytes = 2 * size_t(nl) * px.S * sizeof(cplx<double>);
@@ -507,6 +632,20 @@ int gsmi_irfft2_f64(const double* in, int M, int N, int shift_in, int shift_out,
                        out, M, N, nl, shift_out ? M / 2 : 0, shift_out ? N / 2 : 0,
                        1. / (double(M) * double(N)), px, twn);
     return hip_check(hipGetLastError(), "c2r_rows launch");
+}
+
+int gsmi_fft_column_passes(int M)
+{
+    if (M < 2 || M % 2) {
+        set_error("gsmi_fft_column_passes: M must be even and >= 2");
+        return GSMI_E_ARG;
+    }
+    LinePlan<double> p;
+    LongPlan lp;
+    bool longp = false;
+    const int rc = column_plan(M, lds_limit(), &p, &lp, &longp);
+    if (rc) return rc;
+    return longp ? 2 : 1;
 }
 
 int gsmi_hermitian_fill_f64(const double* half, int M, int N, int shift_out, double* out,

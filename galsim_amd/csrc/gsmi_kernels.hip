@@ -12,6 +12,7 @@
 // The k-image of the reference never exists in memory; nor does the full N x N real image.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -612,11 +613,16 @@ bool fft_size_ok(int N, size_t elem, size_t lds)
 
 size_t lds_limit()
 {
+    // queried once per device (the per-stamp drawing path launches several kernels per stamp)
+    static std::atomic<int> cached[64] = {};
     int dev = 0, v = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 65536;
+    if (dev >= 0 && dev < 64 && cached[dev].load(std::memory_order_relaxed) > 0)
+        return size_t(cached[dev].load(std::memory_order_relaxed));
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess ||
         v < 65536)
-        return 65536;
+        v = 65536;
+    if (dev >= 0 && dev < 64) cached[dev].store(v, std::memory_order_relaxed);
     return size_t(v);
 }
 

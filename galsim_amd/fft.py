@@ -95,6 +95,15 @@ def _rfft(x, M, N, shift_in, shift_out):
     return out
 
 
+def irfft_scratch_rows(M):
+    """Rows of (N/2+1) complex scratch gsmi_irfft2_f64 needs: M, or 2 M when columns of length
+    M take the four-step transform on this device."""
+    passes = _lib.lib().gsmi_fft_column_passes(int(M))
+    if passes < 0:
+        _lib.check(passes, "gsmi_fft_column_passes")
+    return 2 * M if passes == 2 else M
+
+
 def fft2(a, shift_in=False, shift_out=False):
     """numpy.fft.fft2(a) for a 2-d array with even sizes (galsim/fft.py:29-106): complex input is
     transformed as complex128; real input through rfft2 and the Hermitian fill of the kx < 0 half,
@@ -142,7 +151,7 @@ def irfft2(a, shift_in=False, shift_out=False):
     k, dev = _as_device(a, True)
     torch = _torch()
     out = torch.empty((M, N), dtype=torch.float64, device='cuda')
-    tmp = torch.empty((M, No2 + 1), dtype=torch.complex128, device='cuda')
+    tmp = torch.empty((irfft_scratch_rows(M), No2 + 1), dtype=torch.complex128, device='cuda')
     _lib.check(_lib.lib().gsmi_irfft2_f64(k.data_ptr(), M, N, int(bool(shift_in)),
                                           int(bool(shift_out)), out.data_ptr(), tmp.data_ptr(),
                                           _stream(torch)), "gsmi_irfft2_f64")

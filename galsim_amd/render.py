@@ -214,6 +214,9 @@ def draw_fft(prof, bounds, precision='f64', return_device=False):
         rc = lib.gsmi_draw_fft_f64(base + o_stamp, 1, N, ny, base + o_ops, base + o_leaves,
                                    base + o_tab, base + o_clip, scratch.data_ptr(),
                                    out.data_ptr(), stream)
+        if rc == -5:
+            # an FFT size beyond the fused kernels' LDS: the same steps unfused on the GPU
+            return _draw_fft_unfused(prof, bounds, N, Nk, dk, return_device)
     else:
         rc = lib.gsmi_draw_fft_f32(base + o_stamp, 1, N, ny, base + o_ops, base + o_leaves,
                                    base + o_tab, base + o_tab32, scratch.data_ptr(),
@@ -221,6 +224,50 @@ def draw_fft(prof, bounds, precision='f64', return_device=False):
     _lib.check(rc, "gsmi_draw_fft_%s" % precision)
     if return_device:
         return out, (dev, scratch)
+    host = out.cpu().numpy()
+    return host, float(host.sum(dtype=float))
+
+
+def _draw_fft_unfused(prof, bounds, N, Nk, dk, return_device=False):
+    """drawFFT in FP64 as the reference's separate steps, for FFT sizes whose column transform
+    does not fit one workgroup's LDS (N = 6144, 8192): the k-image on (0..Nk/2, -Nk/2..Nk/2)
+    (gsmi_draw_kimage_f64, with the reference's clip rows), wrapImage(hermx) onto the N-grid
+    (gsmi_wrap_hermx_f64), irfft(shift_in, shift_out) (gsmi_irfft2_f64, whose column transform
+    is then four-step) and the crop (galsim/gsobject.py:1927-2045)."""
+    from .fft import irfft_scratch_rows
+    lib = _lib.lib()
+    ctx = Lowering('f64', Nk, dk)
+    prof._lower(ctx, None)
+    check_stack(ctx.ops, precision='f64')
+    torch = _torch()
+    pk = _Packer()
+    o_ops = pk.add(ctx.ops_array())
+    o_leaves = pk.add(ctx.leaves_array())
+    tables = np.concatenate(ctx.tables) if ctx.tables else np.zeros(1)
+    o_tab = pk.add(tables)
+    clip = np.concatenate(ctx.clip) if ctx.clip else np.zeros(2, dtype=np.int32)
+    o_clip = pk.add(clip)
+    dev = _device_buffer(torch, pk.host())
+    base = dev.data_ptr()
+    stream = torch.cuda.current_stream().cuda_stream
+    h = Nk // 2
+    kimg = torch.empty((Nk + 1, h + 1), dtype=torch.complex128, device='cuda')
+    _lib.check(lib.gsmi_draw_kimage_f64(base + o_ops, len(ctx.ops), base + o_leaves, base + o_tab,
+                                        base + o_clip, 0, -h, h + 1, Nk + 1, Nk, float(dk),
+                                        kimg.data_ptr(), stream), "gsmi_draw_kimage_f64")
+    kw = torch.empty((N, N // 2 + 1), dtype=torch.complex128, device='cuda')
+    _lib.check(lib.gsmi_wrap_hermx_f64(kimg.data_ptr(), Nk, N, kw.data_ptr(), stream),
+               "gsmi_wrap_hermx_f64")
+    del kimg
+    real = torch.empty((N, N), dtype=torch.float64, device='cuda')
+    tmp = torch.empty((irfft_scratch_rows(N), N // 2 + 1), dtype=torch.complex128, device='cuda')
+    _lib.check(lib.gsmi_irfft2_f64(kw.data_ptr(), N, N, 1, 1, real.data_ptr(), tmp.data_ptr(),
+                                   stream), "gsmi_irfft2_f64")
+    del kw, tmp
+    out = real[bounds.ymin + N // 2: bounds.ymax + N // 2 + 1,
+               bounds.xmin + N // 2: bounds.xmax + N // 2 + 1].contiguous()
+    if return_device:
+        return out, (dev,)
     host = out.cpu().numpy()
     return host, float(host.sum(dtype=float))
 
@@ -369,8 +416,9 @@ def inverse_fft(kimage):
         kw = torch.empty((N, No2 + 1), dtype=torch.complex128, device='cuda')
         _lib.check(lib.gsmi_wrap_hermx_f64(kf.data_ptr(), N, N, kw.data_ptr(), stream),
                    "gsmi_wrap_hermx_f64")
+    from .fft import irfft_scratch_rows
     out = torch.empty((N, N), dtype=torch.float64, device='cuda')
-    tmp = torch.empty((N, No2 + 1), dtype=torch.complex128, device='cuda')
+    tmp = torch.empty((irfft_scratch_rows(N), No2 + 1), dtype=torch.complex128, device='cuda')
     _lib.check(lib.gsmi_irfft2_f64(kw.data_ptr(), N, N, 1, 1, out.data_ptr(), tmp.data_ptr(),
                                    stream), "gsmi_irfft2_f64")
     dk = kimage.scale

@@ -323,7 +323,8 @@ int gsmi_wrap_hermx_f64(const double* kin, int Nk, int N, double* kout, void* st
  * with shift_out the output is fftshift-ed after (only along y for the half-plane output of
  * rfft2).  Line lengths with prime factors 2, 3, 5 only are transformed directly; other lengths
  * L by Bluestein's algorithm with a 5-smooth length >= 2L - 1, which must fit the workgroup's
- * LDS (GSMI_E_SIZE otherwise).  Work is queued on `stream`; no call allocates device memory
+ * LDS; a column length M beyond that takes a four-step transform (M = M1 M2, both even and
+ * transformable in LDS); GSMI_E_SIZE otherwise.  Work is queued on `stream`; no call allocates device memory
  * except the per-length twiddle / chirp tables, made once per device.
  * --------------------------------------------------------------------------------------- */
 
@@ -341,9 +342,15 @@ int gsmi_rfft2_f64(const double* in, int M, int N, int shift_in, int shift_out, 
 /* irfft(in, out, shift_in, shift_out) (src/Image.cpp:814-911; pybind pysrc/Image.cpp:58,62) for
  * any even M x N: in is M x (N/2+1) complex, out M x N real, with 1/(M N); the imaginary parts of
  * the kx = 0 and kx = N/2 columns after the y transform are ignored.  scratch holds
- * M x (N/2+1) complex. */
+ * M x (N/2+1) complex, twice that when M takes the four-step column transform
+ * (gsmi_fft_column_passes(M) == 2). */
 int gsmi_irfft2_f64(const double* in, int M, int N, int shift_in, int shift_out, double* out,
                     void* scratch, void* stream);
+
+/* 1 if columns of length M transform in one LDS pass on the current device, 2 if they take the
+ * four-step transform, or a negative GSMI_E_* code if M is not supported.  No counterpart in the
+ * reference (FFTW plans any length). */
+int gsmi_fft_column_passes(int M);
 
 /* The M x N transform of a real image from its M x (N/2+1) rfft2 half-plane (with the same
  * shift_out), by Hermitian symmetry: the fill in galsim/fft.py:88-103. */

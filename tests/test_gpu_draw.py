@@ -229,9 +229,15 @@ def test_fft_sizes_beyond_64k_lds(precision, r0, moffat, N):
     assert _peak_rel(im.array, ref) < (F64_TOL if precision == 'f64' else F32_TOL)
 
 
-def test_fft_size_beyond_lds_is_an_error():
-    # N = 6144 in FP64 needs 192 KiB for one column transform: refused before any launch
-    gal = g.Exponential(scale_radius=40., flux=1e4).shear(g1=0.2, g2=0.1)
-    prof = g.Convolve([gal, g.Moffat(beta=3, fwhm=0.7)])
-    with pytest.raises(g.GalSimAMDError, match="unsupported FFT size 6144"):
-        prof.drawImage(nx=64, ny=64, scale=0.2, dtype=np.float64, precision='f64')
+@pytest.mark.parametrize("r0,moffat,N", [(40., True, 6144), (75., False, 8192)])
+def test_fp64_fft_sizes_beyond_the_fused_kernels(r0, moffat, N):
+    # N = 6144 and 8192 in FP64 need 192-256 KiB for one column transform: drawn as the
+    # reference's separate steps (k-image, wrap, irfft with a four-step column transform)
+    def make(m):
+        gal = m.Exponential(scale_radius=r0, flux=1e4).shear(g1=0.2, g2=0.1)
+        return m.Convolve([gal, m.Moffat(beta=3, fwhm=0.7)]) if moffat else gal
+    print("expected N", N, flush=True)
+    im, ref, info = _draw_pair(make, 64, 64, precision='f64')
+    print("N", info['N'], "peak-rel", _peak_rel(im.array, ref))
+    assert info['N'] == N
+    assert _peak_rel(im.array, ref) < F64_TOL

@@ -58,7 +58,7 @@ def test_matches_numpy_on_reference_inputs(si, so):
 # direct mixed-radix lengths (2, 3, 5 factors) and Bluestein lengths (prime factors 7, 11, 23,
 # 131, 587)
 SIZES = [(64, 96), (128, 128), (250, 384), (14, 22), (46, 262), (1174, 1174), (2048, 1536),
-         (300, 2048)]
+         (300, 2048), (6144, 40), (8192, 64)]
 
 
 @pytest.mark.parametrize("shape", SIZES)
@@ -74,6 +74,14 @@ def test_sizes_against_numpy(shape):
         _close(rk, of.rfft2(x, si, so))
         _close(g.fft.fft2(x, si, so), of.fft2(x, si, so))
         _close(g.fft.irfft2(rk, so, si), x)
+
+
+def test_column_passes_query():
+    from galsim_amd import _lib
+    L = _lib.lib()
+    assert L.gsmi_fft_column_passes(2048) == 1
+    assert L.gsmi_fft_column_passes(8192) == 2
+    assert L.gsmi_fft_column_passes(7) < 0
 
 
 def test_device_tensors_stay_on_device():
