@@ -170,3 +170,38 @@ def test_plain_fp64_sweep_256_against_oracle():
         worst = max(worst, np.max(np.abs(b - ref)) / np.max(np.abs(ref)))
     print("256-stamp FP64 sweep: worst peak-relative error %.2e" % worst)
     assert worst < 1e-10
+
+
+def test_batch_of_added_profiles_matches_drawimage():
+    # the work-list kernels run every leaf and operator through the program interpreter: a
+    # batch mixing Spergel, Airy, the inclined disks and a deconvolution equals drawImage
+    def make(i):
+        k = i % 5
+        if k == 0:
+            return g.Convolve([g.Spergel(0.3 + 0.1 * i, half_light_radius=0.6, flux=50.),
+                               g.Kolmogorov(fwhm=0.7)])
+        if k == 1:
+            return g.Airy(lam_over_diam=0.4 + 0.02 * i, obscuration=0.2, flux=20.)
+        if k == 2:
+            return g.Convolve([g.InclinedExponential((0.3 + 0.1 * i) * g.radians,
+                                                     half_light_radius=0.8, flux=30.),
+                               g.Moffat(3., fwhm=0.7)])
+        if k == 3:
+            return g.Convolve([g.InclinedSersic(2.0, 1.0 * g.radians, half_light_radius=0.5,
+                                                flux=30.), g.Kolmogorov(fwhm=0.6)])
+        psf = g.Gaussian(sigma=0.3)
+        gal = g.Exponential(half_light_radius=0.7, flux=40.).shear(g1=0.1, g2=0.2)
+        return g.Convolve([gal, psf, g.Moffat(3., fwhm=0.8), g.Deconvolve(psf)])
+    profs = [make(i) for i in range(10)]
+    nx = np.array([40 + 4 * (i % 3) for i in range(10)])
+    ny = np.array([44 - 4 * (i % 2) for i in range(10)])
+    offs = np.zeros((10, 2))
+    for prec, tol in (('f64', 1e-12), ('f32', 1e-5)):
+        plan = BatchPlan(profs, nx, ny, 0.2, offs, precision=prec)
+        plan.run()
+        for i in range(10):
+            im = profs[i].drawImage(nx=int(nx[i]), ny=int(ny[i]), scale=0.2, dtype=np.float64)
+            b = plan.stamp(i).double().cpu().numpy()
+            err = np.max(np.abs(b - im.array)) / np.max(np.abs(im.array))
+            print(prec, i, err)
+            assert err < tol, (prec, i, err)
