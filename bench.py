@@ -271,6 +271,94 @@ def large_bench(args, torch):
                         "0.8 arcsec x Pixel, drawImage's own stamp sizes, FP32"}
 
 
+def _config0_cpu_worker(job):
+    """One CPU process of the multi-core baseline: oracle drawImage of stamps i0, i0 + step, ...
+    of the 256-stamp Exponential x Moffat workload until the time budget is spent."""
+    i0, step, n, seed, seconds = job
+    os.environ['OMP_NUM_THREADS'] = '1'
+    from galsim_amd import workloads
+    from oracle import draw as od
+    p = workloads.exp_moffat_params(n, seed)
+    workloads.exp_moffat_profile(od, p, i0).drawImage(64, 64, scale=0.2)    # warm caches
+    done = 0
+    t0 = time.perf_counter()
+    i = i0
+    while time.perf_counter() - t0 < seconds:
+        workloads.exp_moffat_profile(od, p, i % n).drawImage(
+            64, 64, scale=0.2, offset=tuple(p['offset'][i % n]))
+        done += 1
+        i += step
+    return done, time.perf_counter() - t0
+
+
+def config0_bench(args, torch):
+    """The reference's CPU-runnable case: 256 seeded Exponential x Moffat(beta=3, fwhm=0.7")
+    stamps of 64x64 at 0.2"/px, FP64.  The plain per-stamp GPU path (one stamp per launch
+    sequence, pre-planned), the batched FP64 path (work lists replayed from a HIP graph), the
+    error of both against the oracle, and the oracle on 1 core and on the host's cores (one
+    process per core, up to 16)."""
+    from galsim_amd import workloads
+    from galsim_amd.batch import BatchPlan
+    from oracle import draw as od
+    n = 256
+    profs, nx, ny, offs, p = workloads.build('exp_moffat', n, seed=args.seed + 5)
+    singles = [BatchPlan([profs[i]], nx[i:i + 1], ny[i:i + 1], 0.2, offs[i:i + 1],
+                         precision='f64') for i in range(n)]
+    for s_ in singles[:8]:
+        s_.run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s_ in singles:
+        s_.run()
+    torch.cuda.synchronize()
+    plain = n / (time.perf_counter() - t0)
+    plan = BatchPlan(profs, nx, ny, 0.2, offs, precision='f64')
+    plan.autotune(reps=2)
+    plan.capture()
+    for _ in range(2):
+        plan.replay()
+    torch.cuda.synchronize()
+    k = 20
+    t0 = time.perf_counter()
+    for _ in range(k):
+        plan.replay()
+    torch.cuda.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / k
+    worst = 0.
+    for i in range(0, n, 16):
+        ref = workloads.exp_moffat_profile(od, p, i).drawImage(64, 64, scale=0.2,
+                                                               offset=tuple(offs[i]))
+        a = plan.stamp(i).cpu().numpy()
+        b = singles[i].stamp(0).cpu().numpy()
+        worst = max(worst, float(np.max(np.abs(a - ref)) / np.max(np.abs(ref))),
+                    float(np.max(np.abs(b - ref)) / np.max(np.abs(ref))))
+    log("config0 (256 Exp x Moffat 64x64 FP64): plain per-stamp %.0f stamps/s, batched %.3f ms "
+        "(%.0f stamps/s); max peak-rel error vs oracle %.2e" % (plain, ms, n / (ms * 1e-3),
+                                                               worst))
+    out = {"workload": "256 Exponential x Moffat(beta=3, fwhm=0.7) 64x64 stamps, 0.2 arcsec/px, "
+                       "FP64, seeded hlr/shear/offset",
+           "plain_fp64_gpu_stamps_per_s": round(plain, 1),
+           "batched_fp64_gpu_stamps_per_s": round(n / (ms * 1e-3), 1),
+           "batched_fp64_ms_per_step": round(ms, 4),
+           "max_peak_rel_err_vs_oracle": worst}
+    if not args.no_cpu:
+        import multiprocessing as mp
+        secs = max(args.cpu_seconds / 2., 2.)
+        d1, t1 = _config0_cpu_worker((0, 1, n, args.seed + 5, secs))
+        cores = max(1, min(16, os.cpu_count() or 1))
+        ctx = mp.get_context('spawn')
+        with ctx.Pool(cores) as pool:
+            res = pool.map(_config0_cpu_worker,
+                           [(c, cores, n, args.seed + 5, secs) for c in range(cores)])
+        multi = sum(d for d, _ in res) / max(t for _, t in res)
+        log("config0 CPU oracle: %.1f stamps/s on 1 core, %.1f on %d processes"
+            % (d1 / t1, multi, cores))
+        out["cpu_oracle_1_core_stamps_per_s"] = round(d1 / t1, 2)
+        out["cpu_oracle_multicore_stamps_per_s"] = round(multi, 2)
+        out["cpu_oracle_processes"] = cores
+    return out
+
+
 def fft_module_bench(args, torch):
     """galsim_amd.fft on device-resident float64 images: rfft2 then irfft2 (shift_in and
     shift_out, as Image.calculate_fft / calculate_inverse_fft use them) for square sizes, timed
@@ -319,6 +407,7 @@ def main():
     ap.add_argument('--large-n', type=int, default=64)
     ap.add_argument('--no-large', action='store_true')
     ap.add_argument('--no-fft-module', action='store_true')
+    ap.add_argument('--no-config0', action='store_true')
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='MIB',
                     help='only run a plain device copy of MIB MiB (3 times) and exit; used to '
                          'scale the FETCH_SIZE/WRITE_SIZE counters')
@@ -484,6 +573,7 @@ def main():
     batch_fit = None if (args.no_fit or args.no_batch_fit) else fit_bench(args, torch)
     large = None if args.no_large else large_bench(args, torch)
     fft_mod = None if args.no_fft_module else fft_module_bench(args, torch)
+    config0 = None if args.no_config0 else config0_bench(args, torch)
 
     # --- CPU baseline: the oracle (numpy FP64 restatement), one core ------------------------
     cpu = None
@@ -530,6 +620,7 @@ def main():
         "batch_fit": batch_fit,
         "large_stamps": large,
         "fft_module_f64": fft_mod,
+        "config0_exp_moffat_fp64": config0,
         "plain_fp64_gpu_stamps_per_s": round(plain_rate, 1),
         "speedup_vs_plain_fp64_gpu": round(value / plain_rate, 2),
         "drawimage_fp64_end_to_end_stamps_per_s": round(drawimage_rate, 1),
