@@ -7,7 +7,7 @@ from .airy import Airy
 from .angle import Angle, AngleUnit, arcmin, arcsec, degrees, hours, radians
 from ._lib import (GalSimAMDBoundsError, GalSimAMDError, GalSimAMDRangeError,
                    GalSimAMDUndefinedBoundsError, GalSimAMDValueError)
-from .box import Box, Pixel
+from .box import Box, DeltaFunction, Pixel, TopHat
 from .convolve import (AutoConvolution, AutoConvolve, AutoCorrelate, AutoCorrelation, Convolution,
                        Convolve, Deconvolution, Deconvolve)
 from .exponential import Exponential
@@ -28,10 +28,10 @@ from .transform import Transform, Transformation, _Transform
 GalSimError = GalSimAMDError
 
 __all__ = ['Add', 'Airy', 'Angle', 'AngleUnit', 'AutoConvolution', 'AutoConvolve', 'AutoCorrelate', 'AutoCorrelation', 'BoundsI',
-           'Box', 'Convolution', 'Convolve', 'DeVaucouleurs', 'Deconvolution', 'Deconvolve',
+           'Box', 'Convolution', 'Convolve', 'DeVaucouleurs', 'Deconvolution', 'Deconvolve', 'DeltaFunction',
            'Exponential',
            'GSObject', 'GSParams', 'GalSimAMDBoundsError', 'GalSimAMDError', 'GalSimAMDRangeError',
            'GalSimAMDUndefinedBoundsError', 'GalSimAMDValueError', 'GalSimError',
            'Gaussian', 'Image', 'InclinedExponential', 'InclinedSersic', 'ImageCD', 'ImageCF', 'ImageD', 'ImageF', 'Kolmogorov', 'Moffat',
-           'Pixel', 'PositionD', 'PositionI', 'Sersic', 'Shear', 'Spergel', 'Sum', 'Transform',
+           'Pixel', 'PositionD', 'PositionI', 'Sersic', 'Shear', 'Spergel', 'Sum', 'TopHat', 'Transform',
            'Transformation', 'arcmin', 'arcsec', 'degrees', 'fft', 'hours', 'radians']

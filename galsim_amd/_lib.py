@@ -37,6 +37,8 @@ LEAF_SPERGEL = 8
 LEAF_AIRY = 9
 LEAF_INCLINED_EXP = 10
 LEAF_INCLINED_VERT = 11
+LEAF_TOPHAT = 12
+LEAF_DELTA = 13
 
 ERRORS = {-1: "invalid argument", -2: "parameter out of range", -3: "solve/table failure",
           -4: "HIP runtime failure", -5: "unsupported FFT size"}

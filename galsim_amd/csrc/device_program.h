@@ -501,6 +501,15 @@ __device__ inline T eval_leafc(const LeafC<T>& C, const LeafI& I, const gsmi_lea
         return sky2 < C.p[2] ? T(1) - T(0.16666666667) * sky2 * (T(1) - T(0.116666666667) * sky2)
                              : ddiv(sky, sinh(sky));
       }
+      case GSMI_LEAF_TOPHAT: {
+        // SBTopHatImpl::kValue2 (src/SBBox.cpp:289-299)
+        const T kr0sq = X * X + Y * Y;
+        if (kr0sq < T(1.e-4)) return flux * (T(1) - kr0sq * (T(1. / 8.) + T(1. / 192.) * kr0sq));
+        const T kr0 = dsqrt(kr0sq);
+        return T(2) * flux * ddiv(T(j1(kr0)), kr0);
+      }
+      case GSMI_LEAF_DELTA:
+        return flux;
       case GSMI_LEAF_SPERGEL: {
         // SBSpergelImpl::fillKImage, kloop_1d / kloop_2d (src/SBSpergel.cpp:121-139, 456-527)
         T ksq = X * X + Y * Y;

@@ -70,10 +70,13 @@ enum {
     GSMI_LEAF_INCLINED_EXP = 10, /* kValueHelper of src/SBInclinedExponential.cpp:277-336 at
                              (kx, ky) r0: scale = r0, p[0] = |cos i|,
                              p[1] = pi/2 h0 |sin i| / r0, p[2] = ksq_min, p[3] = ksq_max */
-    GSMI_LEAF_INCLINED_VERT = 11 /* InclinedSersic's vertical factor and clip
+    GSMI_LEAF_INCLINED_VERT = 11, /* InclinedSersic's vertical factor and clip
                              (src/SBInclinedSersic.cpp:321-370): 0 where
                              (kx r0)^2 + (ky r0 cos i)^2 > ksq_max, else x / sinh(x) (Taylor
                              below ksq_min), x = hs ky r0; same fields as INCLINED_EXP */
+    GSMI_LEAF_TOPHAT = 12, /* flux 2 J1(k r) / (k r), Taylor below (k r)^2 = 1e-4
+                             (SBTopHatImpl::kValue2, src/SBBox.cpp:289-299): scale = radius */
+    GSMI_LEAF_DELTA = 13  /* flux everywhere (DeltaFunction, galsim/deltafunction.py) */
 };
 
 typedef struct gsmi_op {

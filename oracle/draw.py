@@ -316,6 +316,14 @@ def Airy(lam_over_diam, **kw):
     return Leaf(pr.Airy(lam_over_diam, **kw))
 
 
+def TopHat(radius, **kw):
+    return Leaf(pr.TopHat(radius, **kw))
+
+
+def DeltaFunction(**kw):
+    return Leaf(pr.DeltaFunction(**kw))
+
+
 def InclinedExponential(inclination, scale_radius, scale_height, **kw):
     return Leaf(pr.InclinedExponential(inclination, scale_radius, scale_height, **kw))
 

@@ -1286,3 +1286,66 @@ class InclinedSersic:
             out[j] = self.flux * self.helper(_axis(rx[j], dkx, m), _axis(ry[j], dkyx, m))
         return out
 
+
+# ----------------------------------------------------------------------------------------------
+# TopHat and DeltaFunction -- src/SBBox.cpp:283-445, galsim/box.py, galsim/deltafunction.py
+# ----------------------------------------------------------------------------------------------
+class TopHat:
+    """SBTopHat (src/SBBox.cpp:283-445): flux 2 J1(k r)/(k r) with its Taylor branch.  Pinned by
+    the reference's known maxk / stepk (tests/test_box.py:275-297) and by the closed-form 2-d
+    transform of a uniform disk integrated numerically (tests/test_tophat_cpu.py)."""
+
+    def __init__(self, radius, flux=1., gsparams=None):
+        self.r0 = float(radius)
+        self.flux = float(flux)
+        self.gsparams = gsparams or DEFAULT_GSPARAMS
+
+    @property
+    def maxk(self):
+        return (2. * math.sqrt(2. / math.pi) / self.gsparams.maxk_threshold) ** (2. / 3.) / self.r0
+
+    @property
+    def stepk(self):
+        return math.pi / self.r0
+
+    def kvalue2(self, kr0sq):
+        kr0sq = np.asarray(kr0sq, dtype=float)
+        kr0 = np.sqrt(kr0sq)
+        with np.errstate(divide='ignore', invalid='ignore'):
+            big = 2. * self.flux * sps.j1(kr0) / kr0
+        small = self.flux * (1. - kr0sq * ((1. / 8.) + (1. / 192.) * kr0sq))
+        return np.where(kr0sq < 1.e-4, small, big)
+
+    def fill_1d(self, kx0, dkx, m, ky0, dky, n):
+        r0 = self.r0
+        kx = _axis(kx0 * r0, dkx * r0, m)
+        ky = _axis(ky0 * r0, dky * r0, n)
+        return self.kvalue2(kx[None, :] * kx[None, :] + (ky * ky)[:, None])
+
+    def fill_2d(self, kx0, dkx, dkxy, ky0, dky, dkyx, m, n):
+        r0 = self.r0
+        kx0 *= r0; dkx *= r0; dkxy *= r0; ky0 *= r0; dky *= r0; dkyx *= r0
+        out = np.zeros((n, m))
+        rx = _axis(kx0, dkxy, n)
+        ry = _axis(ky0, dky, n)
+        for j in range(n):
+            kx = _axis(rx[j], dkx, m)
+            ky = _axis(ry[j], dkyx, m)
+            out[j] = self.kvalue2(kx * kx + ky * ky)
+        return out
+
+
+class DeltaFunction:
+    """galsim/deltafunction.py: the constant flux; maxk = stepk = 1e300."""
+
+    def __init__(self, flux=1., gsparams=None):
+        self.flux = float(flux)
+        self.gsparams = gsparams or DEFAULT_GSPARAMS
+        self.maxk = self.stepk = 1.e300
+
+    def fill_1d(self, kx0, dkx, m, ky0, dky, n):
+        return np.full((n, m), self.flux)
+
+    def fill_2d(self, kx0, dkx, dkxy, ky0, dky, dkyx, m, n):
+        return np.full((n, m), self.flux)
+

@@ -130,6 +130,12 @@ def test_exponential_moffat_seeded_sweep():
     ("inclined_conv_sheared", lambda m: m.Convolve([_incl(m, 0.78, 0.4, 0.1, 3.)
                                                     .shear(g1=0.1, g2=0.2),
                                                     m.Kolmogorov(fwhm=0.7)]), (48, 48)),
+    ("tophat", lambda m: m.TopHat(radius=0.5, flux=3.), (40, 40)),
+    ("tophat_sheared_conv", lambda m: m.Convolve([m.TopHat(radius=0.3, flux=2.)
+                                                  .shear(g1=0.2, g2=0.1),
+                                                  m.Moffat(3., fwhm=0.6)]), (40, 40)),
+    ("delta_shifted_conv", lambda m: m.Convolve([m.DeltaFunction(flux=5.).shift(0.1, -0.15),
+                                                 m.Moffat(2.5, fwhm=0.7)]), (40, 40)),
     ("inclined_sersic_15", lambda m: _incls(m, 1.5, 1.1, 0.1, 0.03, 4.), (48, 48)),
     ("inclined_sersic_trunc_rot", lambda m: _incls(m, 0.8, 1.4, 0.3, 0.03, 2., 1.2)
      .rotate(1.1), (48, 48)),
