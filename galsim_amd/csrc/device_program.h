@@ -31,6 +31,11 @@ struct Tables<float> {
     const float* tab32;
 };
 
+// TopHat's 2 J1(x) / x (x^2 >= 1e-4), out of line: the Bessel function's code would otherwise
+// be inlined into every fold kernel's interpreter path and raise its register count.
+__device__ __noinline__ double tophat_j1_ratio(double x) { return 2. * j1(x) / x; }
+__device__ __noinline__ float tophat_j1_ratio(float x) { return 2.f * j1f(x) / x; }
+
 // Moffat kV for the Bessel-function cases, kept out of line so the common path stays small.
 __device__ __noinline__ double moffat_kv_bessel(int kind, double nu, double k, double ksq,
                                                 double at_zero)
@@ -505,8 +510,7 @@ __device__ inline T eval_leafc(const LeafC<T>& C, const LeafI& I, const gsmi_lea
         // SBTopHatImpl::kValue2 (src/SBBox.cpp:289-299)
         const T kr0sq = X * X + Y * Y;
         if (kr0sq < T(1.e-4)) return flux * (T(1) - kr0sq * (T(1. / 8.) + T(1. / 192.) * kr0sq));
-        const T kr0 = dsqrt(kr0sq);
-        return T(2) * flux * ddiv(T(j1(kr0)), kr0);
+        return flux * tophat_j1_ratio(dsqrt(kr0sq));
       }
       case GSMI_LEAF_DELTA:
         return flux;

@@ -131,6 +131,19 @@ __device__ __forceinline__ void fold_cells_static(const LeafC<T>* __restrict__ l
 // of the output window stored.  STATIC: the program has the compile-time shape SIG (leaf
 // constants lc); otherwise the interpreter runs.  X holds the axis factors X(kx), kx in [0, hk],
 // then Y(ky), ky in [-hk, hk].
+// The interpreter's cell evaluation, out of line for the specialised kernels: there it only runs
+// for stamps that do not match the compiled program, and inlined it would set the register
+// budget of the whole kernel.
+template <typename T>
+__device__ __noinline__ void fold_cells_interp(const gsmi_op* __restrict__ sops, int nops,
+                                               const gsmi_leaf* __restrict__ sleaves,
+                                               Tables<T> tb, const cplx<T>* X, const cplx<T>* Y,
+                                               T dk, int N, int Nk, int c0, int ncol,
+                                               cplx<T>* bufa)
+{
+    fold_cells<T>(sops, nops, sleaves, tb, X, Y, dk, N, Nk, c0, ncol, bufa);
+}
+
 template <typename T, int SIG, bool STATIC>
 __device__ __forceinline__ void fold_chunks(const gsmi_stamp* __restrict__ st, int first,
                                             int count, int cpg, const LeafC<T>* lc,
@@ -156,6 +169,9 @@ __device__ __forceinline__ void fold_chunks(const gsmi_stamp* __restrict__ st, i
         if constexpr (STATIC)
             fold_cells_static<T, SIG>(lc, li, sleaves, tb, X, X + 2 * hk + 1, dk, N, Nk, c0,
                                       ncol, bufa);
+        else if constexpr (SIG > 0)
+            fold_cells_interp<T>(sops, nops, sleaves, tb, X, X + 2 * hk + 1, dk, N, Nk, c0, ncol,
+                                 bufa);
         else
             fold_cells<T>(sops, nops, sleaves, tb, X, X + 2 * hk + 1, dk, N, Nk, c0, ncol, bufa);
         __syncthreads();
