@@ -191,4 +191,4 @@ def ptr(a):
 def gsp_array(gsparams):
     if gsparams is None:
         return None
-    return np.ascontiguousarray(gsparams._getinitargs_f(), dtype=np.float64)
+    return gsparams._f64_array()

@@ -274,12 +274,11 @@ __device__ __forceinline__ void fold_colfft_body(const gsmi_stamp* __restrict__ 
                                                  const gsmi_leaf* __restrict__ leaves,
                                                  const Tables<T>& tb,
                                                  const cplx<T>* __restrict__ tw,
-                                                 cplx<T>* __restrict__ scratch)
+                                                 cplx<T>* __restrict__ scratch, int cpg)
 {
     extern __shared__ unsigned char smem_raw[];
     const gsmi_stamp* st = stamps + blockIdx.y;
     const int N = st->N;
-    const int cpg = cols_per_group(N);
     const int c0 = blockIdx.x * cpg;
     const int ncol = min(cpg, N / 2 + 1 - c0);
     if (ncol <= 0) return;
@@ -330,17 +329,17 @@ __device__ __forceinline__ void fold_list_body(const gsmi_stamp* __restrict__ st
 __global__ void __launch_bounds__(kThreads, 5)
 fold_colfft_f32(const gsmi_stamp* __restrict__ stamps, const gsmi_op* __restrict__ ops,
                 const gsmi_leaf* __restrict__ leaves, Tables<float> tb,
-                const cplx<float>* __restrict__ tw, cplx<float>* __restrict__ scratch)
+                const cplx<float>* __restrict__ tw, cplx<float>* __restrict__ scratch, int cpg)
 {
-    fold_colfft_body<float>(stamps, ops, leaves, tb, tw, scratch);
+    fold_colfft_body<float>(stamps, ops, leaves, tb, tw, scratch, cpg);
 }
 
 __global__ void __launch_bounds__(kThreads)
 fold_colfft_f64(const gsmi_stamp* __restrict__ stamps, const gsmi_op* __restrict__ ops,
                 const gsmi_leaf* __restrict__ leaves, Tables<double> tb,
-                const cplx<double>* __restrict__ tw, cplx<double>* __restrict__ scratch)
+                const cplx<double>* __restrict__ tw, cplx<double>* __restrict__ scratch, int cpg)
 {
-    fold_colfft_body<double>(stamps, ops, leaves, tb, tw, scratch);
+    fold_colfft_body<double>(stamps, ops, leaves, tb, tw, scratch, cpg);
 }
 
 template <int SIG>
@@ -426,13 +425,12 @@ __device__ __forceinline__ void row_c2r_body(cplx<T>* lds, int rpg, const cplx<T
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
 row_c2r(const gsmi_stamp* __restrict__ stamps, const cplx<T>* __restrict__ scratch,
-        const cplx<T>* __restrict__ tw, T* __restrict__ out)
+        const cplx<T>* __restrict__ tw, T* __restrict__ out, int rpg)
 {
     extern __shared__ unsigned char smem_raw[];
     cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
     const gsmi_stamp* st = stamps + blockIdx.y;
     const int N = st->N;
-    const int rpg = rows_per_group(N);
     const int r0 = blockIdx.x * rpg;
     const int nrow = min(rpg, st->ny - r0);
     if (nrow <= 0) return;
@@ -674,6 +672,17 @@ template int launch_axis_factors<float>(const gsmi_stamp*, int, const gsmi_op*,
                                         const gsmi_leaf*, void*, void*);
 
 namespace {
+// Columns (rows) per workgroup of a per-size launch over `nstamps` stamps of `lines` columns
+// (rows) each: the LDS-filling tile `full`, halved while the launch would have fewer than
+// 1024 workgroups, so that a launch of one or a few stamps spreads over the 256 CUs instead of
+// running on a handful of them.  Any tile from 1 to `full` is valid for both kernels.
+int small_launch_tile(int full, int lines, int nstamps)
+{
+    int t = full;
+    while (t > 1 && size_t(nstamps) * size_t((lines + t - 1) / t) < 1024) t /= 2;
+    return t;
+}
+
 template <typename T>
 int fold_impl(const gsmi_stamp* stamps, int nstamps, int N, const gsmi_op* ops,
               const gsmi_leaf* leaves, Tables<T> tb, void* scratch, void* stream)
@@ -689,7 +698,7 @@ int fold_impl(const gsmi_stamp* stamps, int nstamps, int N, const gsmi_op* ops,
         set_error("twiddle table allocation failed");
         return GSMI_E_HIP;
     }
-    const int cpg = cols_per_group(N);
+    const int cpg = small_launch_tile(cols_per_group(N), N / 2 + 1, nstamps);
     int rc = launch_axis_factors<T>(stamps, nstamps, ops, leaves, scratch, stream);
     if (rc) return rc;
     dim3 g1((N / 2 + 1 + cpg - 1) / cpg, nstamps);
@@ -697,7 +706,7 @@ int fold_impl(const gsmi_stamp* stamps, int nstamps, int N, const gsmi_op* ops,
     if ((rc = allow_lds(reinterpret_cast<const void*>(FoldKernel<T>::fn), lds1))) return rc;
     hipLaunchKernelGGL(FoldKernel<T>::fn, g1, dim3(kThreads), lds1,
                        reinterpret_cast<hipStream_t>(stream), stamps, ops, leaves, tb, tw,
-                       reinterpret_cast<cplx<T>*>(scratch));
+                       reinterpret_cast<cplx<T>*>(scratch), cpg);
     return hip_check(hipGetLastError(), "fold_colfft launch");
 }
 
@@ -715,13 +724,13 @@ int rows_impl(const gsmi_stamp* stamps, int nstamps, int N, int max_ny, const vo
         set_error("twiddle table allocation failed");
         return GSMI_E_HIP;
     }
-    const int rpg = rows_per_group(N);
+    const int rpg = small_launch_tile(rows_per_group(N), max_ny, nstamps);
     dim3 g2((max_ny + rpg - 1) / rpg, nstamps);
     size_t lds2 = 2 * size_t(rpg) * (N / 2) * sizeof(cplx<T>);
     int rc = allow_lds(reinterpret_cast<const void*>(row_c2r<T>), lds2);
     if (rc) return rc;
     hipLaunchKernelGGL(row_c2r<T>, g2, dim3(kThreads), lds2, reinterpret_cast<hipStream_t>(stream),
-                       stamps, reinterpret_cast<const cplx<T>*>(scratch), tw, out);
+                       stamps, reinterpret_cast<const cplx<T>*>(scratch), tw, out, rpg);
     return hip_check(hipGetLastError(), "row_c2r launch");
 }
 

@@ -1,4 +1,5 @@
 """GSParams: the accuracy knobs of the FFT path (galsim/gsparams.py:126-269)."""
+import numpy as np
 
 
 class GSParams:
@@ -27,6 +28,11 @@ class GSParams:
         self._integration_relerr = float(integration_relerr)
         self._integration_abserr = float(integration_abserr)
         self._shoot_accuracy = float(shoot_accuracy)
+        # GSParams are immutable: the argument tuple and the float64 array handed to the native
+        # library are built once (every profile constructor and drawing call asks for them)
+        self._args = tuple(self.__dict__['_' + n] for n in GSParams._names)
+        self._f64 = np.array(self._args, dtype=np.float64)
+        self._f64.flags.writeable = False
 
     def __getattr__(self, name):
         if name in GSParams._names:
@@ -34,10 +40,14 @@ class GSParams:
         raise AttributeError(name)
 
     def _getinitargs(self):
-        return tuple(getattr(self, n) for n in GSParams._names)
+        return self._args
 
     def _getinitargs_f(self):
-        return [float(v) for v in self._getinitargs()]
+        return [float(v) for v in self._args]
+
+    def _f64_array(self):
+        """The 13 parameters as a read-only contiguous float64 array, in the order above."""
+        return self._f64
 
     def withParams(self, **kwargs):
         if not kwargs:
