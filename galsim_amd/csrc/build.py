@@ -13,7 +13,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("GSMI_ARCH", "gfx950")
 
 HOST_SRCS = ["host_numerics.cpp", "capi_host.cpp"]
-HIP_SRCS = ["gsmi_kernels.hip", "gsmi_adjoint.hip", "gsmi_fields.hip", "gsmi_fft.hip"]
+HIP_SRCS = ["gsmi_kernels.hip", "gsmi_adjoint.hip", "gsmi_fields.hip", "gsmi_fft.hip",
+            "gsmi_real.hip"]
 
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
              # loaded by torch's bundled HIP runtime: keep the code object version it reads

@@ -10,6 +10,14 @@ from .image import Image
 from .position import BoundsI, PositionD, PositionI
 
 
+def _real_added(pixels, dtype):
+    """drawReal's return value: the sum of the drawn values as stored in a float32 / float64
+    image, or in the float64 temporary the reference uses for other image types
+    (galsim/gsobject.py:1875-1889)."""
+    dt = dtype if dtype in (np.float32, np.float64) else np.float64
+    return float(pixels.astype(dt).sum(dtype=float))
+
+
 class GSObject:
     """Base class (galsim/gsobject.py:37-2609).  Subclasses define ``_maxk``, ``_stepk``,
     ``_flux`` and ``_lower(ctx, jac)`` (their contribution to the k-space program)."""
@@ -240,10 +248,11 @@ class GSObject:
         """drawImage (galsim/gsobject.py:1283-1844) for the FFT methods.
 
         method 'auto' and 'fft' convolve by the pixel; 'no_pixel' and 'sb' draw the profile as
-        is ('sb' in surface-brightness units).  Profiles the reference would draw in real space
-        ('real_space', or 'no_pixel'/'sb' on an analytic profile) and photon shooting are not part
-        of this library and raise NotImplementedError.  ``precision`` selects the FP64 kernels
-        (default, the reference's arithmetic) or the FP32 ones.
+        is ('sb' in surface-brightness units): by direct evaluation at the pixel centres when the
+        profile is analytic in real space (drawReal, as the reference does), else by FFT.
+        Real-space convolution ('real_space') and photon shooting are not part of this library and
+        raise NotImplementedError.  ``precision`` selects the FP64 kernels (default, the
+        reference's arithmetic) or the FP32 ones of the FFT path; drawReal is FP64.
         """
         from .box import Pixel
         from .convolve import Convolve
@@ -279,9 +288,7 @@ class GSObject:
         if method in ('auto', 'fft'):
             prof = Convolve(prof, Pixel(scale=1.0, gsparams=self.gsparams), real_space=False,
                             gsparams=self.gsparams)
-        elif prof.is_analytic_x:
-            raise NotImplementedError("method=%r on an analytic profile draws in real space in "
-                                      "the reference; not part of galsim_amd" % method)
+        use_real = method in ('no_pixel', 'sb') and prof.is_analytic_x
         image = prof._setup_image(image, nx, ny, bounds, add_to_image, dtype, center)
         image.scale = scale
         if setup_only:
@@ -289,13 +296,32 @@ class GSObject:
             return image
         c = image.center
         draw_bounds = image.bounds.shift(PositionI(-c.x, -c.y))
-        pixels, added = render.draw_fft(prof, draw_bounds, precision=precision)
+        if use_real:
+            from . import real
+            pixels = real.draw_real(prof, draw_bounds, 1.0)
+            added = _real_added(pixels, image.array.dtype)
+        else:
+            pixels, added = render.draw_fft(prof, draw_bounds, precision=precision)
         if add_to_image:
             image.array[...] += pixels.astype(image.array.dtype)
         else:
             image.array[...] = pixels.astype(image.array.dtype)
         image.added_flux = added / flux_scale
         return image
+
+    def drawReal(self, image, add_to_image=False):
+        """drawReal (galsim/gsobject.py:1846-1889): the profile sampled at the centre of each
+        pixel times the pixel area, its origin at the image's (0, 0), evaluated on the GPU.
+        Returns the flux drawn (the sum of the drawn pixel values)."""
+        from . import real
+        if image.scale is None or not image.scale > 0:
+            raise ValueError("drawReal requires an image with a PixelScale wcs", image)
+        pixels = real.draw_real(self, image.bounds, float(image.scale))
+        if add_to_image:
+            image.array[...] += pixels.astype(image.array.dtype)
+        else:
+            image.array[...] = pixels.astype(image.array.dtype)
+        return _real_added(pixels, image.array.dtype)
 
     def drawFFT(self, image, add_to_image=False, precision='f64'):
         """drawFFT (galsim/gsobject.py:2014-2045): image bounds centred on the profile origin."""

@@ -423,6 +423,51 @@ int gsmi_exp_moffat_grad(const double* theta, int n, int fit_beta, const double*
                          const double* gfields, const int64_t* inv, double inv_scale,
                          double* dtheta, void* stream);
 
+/* -------------------------------------------------------------------------------------------
+ * Real-space drawing: drawImage(method='no_pixel' / 'sb') of analytic profiles, which the
+ * reference draws by direct evaluation at pixel centres (GSObject.drawReal / _drawReal,
+ * galsim/gsobject.py:1846-1898, galsim/transform.py:571-583, galsim/sum.py:304-310, into
+ * SBProfile::draw, src/SBProfile.cpp:304-381, and each profile's fillXImage).  One component per
+ * transformed leaf; the host has applied draw()'s offset / Jacobian set-up and the leaf's own
+ * radius scaling, so pixel (i, j) of an m x n image sits at
+ *   form 0: x = x0 + i dx, y = y0 + j dy, or |i - izero| dx, |j - jzero| dy when izero / jzero
+ *           are nonzero (the quadrant fill, src/SBProfile.cpp:447-488, for the radial leaves);
+ *   form 1: x = x0 + i dx + j dxy, y = y0 + i dyx + j dy.
+ * The pixel value is sum_c amp_c * (norm_c f_c(x, y)), components added in order, with f_c:
+ *   GAUSSIAN   exp(-(x^2 + y^2) / 2) (form 0: exp(-x^2/2) exp(-y^2/2), src/SBGaussian.cpp:120-190)
+ *   EXPONENTIAL exp(-sqrt(x^2 + y^2))                           (src/SBExponential.cpp:381-440)
+ *   MOFFAT     (1 + r^2)^-beta for r^2 <= p[1], else 0; kind selects 1/x, 1/x^1.5, ... 1/x^4
+ *              (1, 15, 2, 25, 3, 35, 4) or exp(-beta log x) (0), p[0] = beta
+ *                                                         (src/SBMoffat.cpp:210-272, 340-415)
+ *   SERSIC     exp(-exp(p[0] log r^2)) (p[0] = 1 / 2n), 0 where p[1] > 0 and r^2 > p[1]; pixel
+ *              (ic, jc) >= 0 takes r = 0                  (src/SBSersic.cpp:104-222, 354-360)
+ *   BOX        1 inside; form 0: i1 <= i < i2 and j1 <= j < j2; form 1: |x| < p[0], |y| < p[1]
+ *                                                                (src/SBBox.cpp:80-148)
+ *   TOPHAT     1 inside; form 0: rows j1 <= j < j2, columns from xmax = sqrt(p[0] - y^2);
+ *              form 1: x^2 + y^2 < p[0]                          (src/SBBox.cpp:301-370)
+ * ------------------------------------------------------------------------------------- */
+#define GSMI_XCOMP_MAX 16
+
+typedef struct gsmi_xcomp {
+    int32_t type;         /* GSMI_LEAF_GAUSSIAN, _EXPONENTIAL, _MOFFAT, _SERSIC, _BOX, _TOPHAT */
+    int32_t form;         /* 0 or 1, as above */
+    int32_t kind;         /* Moffat power form */
+    int32_t izero, jzero; /* form 0 quadrant origin */
+    int32_t ic, jc;       /* Sersic centre pixel (-1: none) */
+    int32_t i1, i2, j1, j2; /* Box (form 0) index ranges; TopHat rows j1..j2 */
+    int32_t pad;
+    double x0, dx, dxy, y0, dy, dyx;
+    double norm;          /* the leaf's amplitude */
+    double amp;           /* draw()'s flux ratio, applied after the fill */
+    double p[2];
+} gsmi_xcomp;
+
+/* Fill the m x n FP64 image `out` (device, row-major, row j = y) with the components comps[0..
+ * ncomp) (a host array, ncomp <= GSMI_XCOMP_MAX), adding to its contents when accumulate is 1.
+ * Replaces the fillXImage loops of SBProfile::draw (src/SBProfile.cpp:304-381). */
+int gsmi_draw_real_f64(const gsmi_xcomp* comps, int ncomp, int m, int n, int accumulate,
+                       double* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
