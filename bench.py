@@ -389,6 +389,79 @@ def fft_module_bench(args, torch):
     return res
 
 
+def real_space_bench(args, torch):
+    """Real-space drawing (drawImage method='no_pixel' of analytic profiles): the pixel fill
+    kernel on a device-resident 4096 x 4096 FP64 image of a sheared bulge + disk (Sersic n = 4
+    + Exponential), timed with HIP events on the current stream; algorithmic bytes 8 per pixel
+    (written once, nothing read).  Also the per-call time of drawImage for a 64 x 64 stamp and
+    the oracle's fill of a 256 x 256 image on one core."""
+    import galsim_amd as g
+    from galsim_amd import real, _lib
+    from galsim_amd.position import BoundsI
+    from galsim_amd.transform import _Transform
+    from oracle import draw as od, xdraw as ox
+
+    def bulge_disk(m):
+        return m.Add(m.Sersic(4., half_light_radius=0.6, flux=0.4).shear(g1=0.05, g2=0.1),
+                     m.Exponential(half_light_radius=1.2, flux=0.6).shear(g1=0.3, g2=-0.1))
+    scale = 0.2
+    N = 4096
+    prof = _Transform(bulge_disk(g), np.array([[1. / scale, 0.], [0., 1. / scale]]),
+                      (0.3, -0.2), scale ** 2)
+    b = BoundsI(-N // 2, N // 2 - 1, -N // 2, N // 2 - 1)
+    recs = np.ascontiguousarray(real.components(prof, b, 1.))
+    out = torch.empty((N, N), dtype=torch.float64, device='cuda')
+    stream = torch.cuda.current_stream().cuda_stream
+    L = _lib.lib()
+
+    def fill():
+        _lib.check(L.gsmi_draw_real_f64(recs.ctypes.data, len(recs), N, N, 0, out.data_ptr(),
+                                        stream), "gsmi_draw_real_f64")
+    for _ in range(3):
+        fill()
+    k = 20
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(k):
+        fill()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / k
+    gbps = 8. * N * N / (ms * 1e-3) / 1e9
+    flux = float(out.sum()) * 1.
+    # drawImage per call, 64 x 64 stamps
+    obj = bulge_disk(g)
+    for _ in range(5):
+        obj.drawImage(nx=64, ny=64, scale=scale, method='no_pixel')
+    t0 = time.perf_counter()
+    calls = 200
+    for _ in range(calls):
+        obj.drawImage(nx=64, ny=64, scale=scale, method='no_pixel')
+    per_call_us = (time.perf_counter() - t0) / calls * 1e6
+    # the oracle on one core
+    oobj = bulge_disk(od)
+    t0 = time.perf_counter()
+    reps = 0
+    while reps < 3 or time.perf_counter() - t0 < 2.:
+        ox.draw_image_sb(oobj, -128, 127, -128, 127, scale, method='no_pixel',
+                         offset=(0.3, -0.2))
+        reps += 1
+    cpu_px = reps * 256 * 256 / (time.perf_counter() - t0)
+    log("real-space fill 4096^2 bulge+disk FP64: %.3f ms (%.2e pixels/s, %.0f GB/s written, "
+        "flux %.6f); drawImage no_pixel 64^2 %.0f us/call; oracle %.2e pixels/s on 1 core"
+        % (ms, N * N / (ms * 1e-3), gbps, flux, per_call_us, cpu_px))
+    return {"workload": "sheared Sersic(n=4) + Exponential, 4096 x 4096 FP64 image, 0.2 "
+                        "arcsec/px, method='no_pixel'",
+            "fill_ms": round(ms, 4), "pixels_per_s": round(N * N / (ms * 1e-3), 1),
+            "roofline": {"bound": "hbm", "achieved": round(gbps, 1), "peak": 8000.,
+                         "unit": "GB/s", "frac": round(gbps / 8000., 4),
+                         "algorithmic_bytes_per_launch": 8 * N * N},
+            "image_flux": flux,
+            "drawimage_64x64_us_per_call": round(per_call_us, 1),
+            "cpu_oracle_pixels_per_s_1_core": round(cpu_px, 1)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -407,6 +480,7 @@ def main():
     ap.add_argument('--large-n', type=int, default=64)
     ap.add_argument('--no-large', action='store_true')
     ap.add_argument('--no-fft-module', action='store_true')
+    ap.add_argument('--no-real-space', action='store_true')
     ap.add_argument('--no-config0', action='store_true')
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='MIB',
                     help='only run a plain device copy of MIB MiB (3 times) and exit; used to '
@@ -574,6 +648,7 @@ def main():
     large = None if args.no_large else large_bench(args, torch)
     fft_mod = None if args.no_fft_module else fft_module_bench(args, torch)
     config0 = None if args.no_config0 else config0_bench(args, torch)
+    real_space = None if args.no_real_space else real_space_bench(args, torch)
 
     # --- CPU baseline: the oracle (numpy FP64 restatement), one core ------------------------
     cpu = None
@@ -620,6 +695,7 @@ def main():
         "batch_fit": batch_fit,
         "large_stamps": large,
         "fft_module_f64": fft_mod,
+        "real_space_f64": real_space,
         "config0_exp_moffat_fp64": config0,
         "plain_fp64_gpu_stamps_per_s": round(plain_rate, 1),
         "speedup_vs_plain_fp64_gpu": round(value / plain_rate, 2),
