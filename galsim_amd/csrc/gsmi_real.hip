@@ -11,6 +11,7 @@
 #include <string>
 
 #include "../../include/gsmi.h"
+#include "device_math.h"
 #include "launch.h"
 
 namespace gsmi {
@@ -73,6 +74,22 @@ __device__ double xcomp_value(const gsmi_xcomp& c, int i, int j, int m)
         v = (c.p[1] > 0. && rsq > c.p[1]) ? 0. : c.norm * exp(-exp(c.p[0] * log(rsq)));
         break;
     }
+    case GSMI_LEAF_AIRY: {
+        // AiryInfoObs / AiryInfoNoObs radial function (src/SBAiry.cpp:88-108, 410-429) at
+        // r = |(x, y)| in units of lambda / D; p[0] = obscuration, p[1] = the Taylor threshold
+        const double nu = sqrt(x * x + y * y) * M_PI;
+        const double eps = c.p[0];
+        double xv = (nu < c.p[1]) ? 0.5 * (1. - eps * eps) : (j1(nu) - eps * j1(eps * nu)) / nu;
+        v = c.norm * ((xv * xv) * (M_PI / (1. - eps * eps)));
+        break;
+    }
+    case GSMI_LEAF_SPERGEL: {
+        // SpergelInfo::xValue (src/SBSpergel.cpp:644-651): K_nu(r) r^nu, p[1] at r = 0
+        const double r = sqrt(x * x + y * y);
+        const double nu = c.p[0];
+        v = c.norm * ((r == 0.) ? c.p[1] : bessel_k(fabs(nu), r) * exp(nu * log(r)));
+        break;
+    }
     case GSMI_LEAF_BOX:
         if (c.form == 0)
             v = (i >= c.i1 && i < c.i2 && j >= c.j1 && j < c.j2) ? c.norm : 0.;
@@ -124,7 +141,8 @@ bool xcomp_ok(const gsmi_xcomp& c)
 {
     switch (c.type) {
     case GSMI_LEAF_GAUSSIAN: case GSMI_LEAF_EXPONENTIAL: case GSMI_LEAF_MOFFAT:
-    case GSMI_LEAF_SERSIC: case GSMI_LEAF_BOX: case GSMI_LEAF_TOPHAT:
+    case GSMI_LEAF_SERSIC: case GSMI_LEAF_BOX: case GSMI_LEAF_TOPHAT: case GSMI_LEAF_AIRY:
+    case GSMI_LEAF_SPERGEL:
         break;
     default:
         return false;

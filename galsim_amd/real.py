@@ -8,7 +8,7 @@ galsim/convolve.py:395-397), ending at each leaf's SBProfile::draw (src/SBProfil
 every leaf into one gsmi_xcomp record: draw()'s offset / Jacobian set-up and the leaf's own
 normalisation and radius scaling, done here on the host in the reference's order; the device
 evaluates the records at every pixel.  Leaves covered: Gaussian, Exponential, Moffat, Sersic
-(DeVaucouleurs), Box / Pixel and TopHat.
+(DeVaucouleurs), Airy, Spergel, Box / Pixel and TopHat.
 """
 import math
 
@@ -84,11 +84,13 @@ def _moffat_kind(beta, xacc):
 
 def _leaf_component(leaf, m, n, setup):
     """The gsmi_xcomp record of one leaf drawn with draw()'s set-up ``setup``."""
+    from .airy import Airy
     from .box import Box, TopHat
     from .exponential import Exponential
     from .gaussian import Gaussian
     from .moffat import Moffat
     from .sersic import Sersic
+    from .spergel import Spergel
     form, x0, dx, dxy, y0, dy, dyx, izero, jzero, amp = setup
     rec = np.zeros((), dtype=XCOMP_DTYPE)
     rec['form'] = form
@@ -132,6 +134,21 @@ def _leaf_component(leaf, m, n, setup):
         rec['norm'] = (leaf._flux * xnorm) * (inv * inv)
         rec['p'] = (0.5 / nn, t * t if t > 0. else 0.)
         radial = inv
+    elif isinstance(leaf, Airy):
+        # src/SBAiry.cpp:66-80 (D = 1 / lam_over_diam, xnorm = flux D^2), 88-108, 410-429
+        D = 1. / leaf._lod
+        rec['type'] = _lib.LEAF_AIRY
+        rec['norm'] = leaf._flux * (D * D)
+        rec['p'] = (leaf._obscuration, math.sqrt(8. * leaf.gsparams.xvalue_accuracy))
+        radial = D
+    elif isinstance(leaf, Spergel):
+        # src/SBSpergel.cpp:80-95 (xnorm = flux getXNorm / r0^2), 525-535, 644-651
+        nu, r0 = leaf._nu, leaf._r0
+        g1 = math.gamma(nu + 1.)
+        rec['type'] = _lib.LEAF_SPERGEL
+        rec['norm'] = (leaf._flux * (math.pow(2., -nu) / g1 / (2.0 * math.pi))) / (r0 * r0)
+        rec['p'] = (nu, g1 / (2. * nu) * math.pow(2., nu) if nu > 0. else math.inf)
+        radial = 1. / r0
     elif isinstance(leaf, Box):
         # src/SBBox.cpp:55-148
         rec['type'] = _lib.LEAF_BOX
@@ -158,7 +175,8 @@ def _leaf_component(leaf, m, n, setup):
         izero = jzero = 0
     else:
         raise NotImplementedError("drawReal of %s is not part of galsim_amd (Gaussian, "
-                                  "Exponential, Moffat, Sersic, Box, Pixel and TopHat are)"
+                                  "Exponential, Moffat, Sersic, Airy, Spergel, Box, Pixel and TopHat "
+                                  "are)"
                                   % type(leaf).__name__)
     if radial is not None:
         x0, dx, dxy, y0, dy, dyx = (x0 * radial, dx * radial, dxy * radial, y0 * radial,

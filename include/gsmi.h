@@ -441,6 +441,10 @@ int gsmi_exp_moffat_grad(const double* theta, int n, int fit_beta, const double*
  *                                                         (src/SBMoffat.cpp:210-272, 340-415)
  *   SERSIC     exp(-exp(p[0] log r^2)) (p[0] = 1 / 2n), 0 where p[1] > 0 and r^2 > p[1]; pixel
  *              (ic, jc) >= 0 takes r = 0                  (src/SBSersic.cpp:104-222, 354-360)
+ *   AIRY       nu = pi r: (pi / (1 - e^2)) ((J1(nu) - e J1(e nu)) / nu)^2, or with
+ *              (1 - e^2) / 2 inside the bracket for nu < p[1]; e = p[0]   (src/SBAiry.cpp:88-190)
+ *   SPERGEL    K_nu(r) r^nu (nu = p[0]), p[1] at r = 0 (infinite for nu <= 0)
+ *                                                         (src/SBSpergel.cpp:80-108, 394-452, 644-651)
  *   BOX        1 inside; form 0: i1 <= i < i2 and j1 <= j < j2; form 1: |x| < p[0], |y| < p[1]
  *                                                                (src/SBBox.cpp:80-148)
  *   TOPHAT     1 inside; form 0: rows j1 <= j < j2, columns from xmax = sqrt(p[0] - y^2);
@@ -449,7 +453,8 @@ int gsmi_exp_moffat_grad(const double* theta, int n, int fit_beta, const double*
 #define GSMI_XCOMP_MAX 16
 
 typedef struct gsmi_xcomp {
-    int32_t type;         /* GSMI_LEAF_GAUSSIAN, _EXPONENTIAL, _MOFFAT, _SERSIC, _BOX, _TOPHAT */
+    int32_t type;         /* GSMI_LEAF_GAUSSIAN, _EXPONENTIAL, _MOFFAT, _SERSIC, _AIRY, _SPERGEL,
+                             _BOX, _TOPHAT */
     int32_t form;         /* 0 or 1, as above */
     int32_t kind;         /* Moffat power form */
     int32_t izero, jzero; /* form 0 quadrant origin */

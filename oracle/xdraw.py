@@ -15,10 +15,12 @@ through shear / rotation matrices).
 import math
 
 import numpy as np
+import scipy.special as sps
 
 from . import numerics as nm
 from .draw import Convolution, Leaf, Sum, Transformation
-from .profiles import Box, Exponential, Gaussian, Moffat, Sersic, TopHat, fast_pow
+from .profiles import (Airy, Box, Exponential, Gaussian, Moffat, Sersic, Spergel, TopHat,
+                       fast_pow)
 
 
 def _acc(x0, dx, m):
@@ -91,6 +93,23 @@ class _X:
             self.inv2n = info.inv2n
             self.trunc_sq = info.trunc * info.trunc if info.trunc > 0. else 0.
             self.kind = 'sersic'
+        elif isinstance(prof, Airy):
+            # src/SBAiry.cpp:66-80, AiryInfoObs / AiryInfoNoObs radial functions 88-108, 410-429
+            self.inv = 1. / prof.lod
+            self.norm = prof.flux * (self.inv * self.inv)
+            self.obs = prof.obs
+            self.thresh = math.sqrt(8. * prof.gsparams.xvalue_accuracy)
+            self.kind = 'airy'
+        elif isinstance(prof, Spergel):
+            # src/SBSpergel.cpp:80-95, 525-535, SpergelInfo::getXNorm / xValue 644-651
+            nu = prof.nu
+            g1 = math.gamma(nu + 1.)
+            self.inv = 1. / prof.r0
+            self.norm = (prof.flux * (math.pow(2., -nu) / g1 / (2.0 * math.pi))) / (
+                prof.r0 * prof.r0)
+            self.nu = nu
+            self.xnorm0 = g1 / (2. * nu) * math.pow(2., nu) if nu > 0. else math.inf
+            self.kind = 'spergel'
         elif isinstance(prof, Box):
             # src/SBBox.cpp:55-148
             self.norm = prof.flux / (prof.width * prof.height)
@@ -118,6 +137,19 @@ class _X:
                 return v
             if self.kind == 'gauss':
                 return self.norm * nm.expd(-0.5 * rsq)
+            if self.kind == 'spergel':
+                r = np.sqrt(rsq)
+                v = sps.kv(self.nu, r) * fast_pow(r, self.nu)
+                return self.norm * np.where(r == 0., self.xnorm0, v)
+            if self.kind == 'airy':
+                e = self.obs
+                nu = np.sqrt(rsq) * math.pi
+                xv = np.where(nu < self.thresh, 0.5 * (1. - e * e),
+                              (sps.j1(nu) - e * sps.j1(e * nu)) / np.where(nu > 0., nu, 1.))
+                if e == 0.:
+                    xv = np.where(nu < self.thresh, 0.5, sps.j1(nu) / np.where(nu > 0., nu, 1.))
+                    return self.norm * ((xv * xv) * math.pi)
+                return self.norm * ((xv * xv) * (math.pi / (1. - e * e)))
         raise AssertionError(self.kind)
 
     def fill_plain(self, x0, dx, m, y0, dy, n):

@@ -27,7 +27,7 @@ def _oracle_sb(obj, N, scale=0.2, method='sb'):
 
 
 # The reference's known-result tests (tests/test_exponential.py:29-45, test_moffat.py:30-48,
-# test_sersic.py:30-95, test_box.py:28-40, 140-155, test_airy.py is Airy (not drawn here),
+# test_sersic.py:30-95, test_box.py:28-40, 140-155, test_airy.py:29-45, test_spergel.py:29-55,
 # test_gaussian.py:30-40, test_transforms.py:45-383, test_sum.py:28-45): method='sb',
 # use_true_center=False, scale 0.2, 5 decimals.
 def _golden_cases():
@@ -56,7 +56,9 @@ def _golden_cases():
          golden),
         ("double_gaussian.fits", lambda: od.Add(od.Gaussian(flux=0.75, sigma=1.),
                                                 od.Gaussian(flux=0.25, sigma=3.)), golden),
-    ]
+        ("airy_.8_.1.fits", lambda: od.Airy(1. / 0.8, obscuration=0.1, flux=1.), golden),
+    ] + [("spergel_nu%.2f.fits" % nu, (lambda nu=nu: od.Spergel(nu, half_light_radius=1.)),
+          golden) for nu in (-0.85, -0.5, 0.0, 0.85, 4.0)]
 
 
 @pytest.mark.parametrize("name,make,where", _golden_cases(), ids=lambda v: v
@@ -133,6 +135,18 @@ def _eval_records(recs, m, n):
                 v = c['norm'] * np.exp(-np.exp(c['p'][0] * np.log(rsq)))
                 if c['p'][1] > 0:
                     v = np.where(rsq > c['p'][1], 0., v)
+            elif t == _lib.LEAF_SPERGEL:
+                import scipy.special as sps
+                r = np.sqrt(x * x + y * y)
+                nu = c['p'][0]
+                v = c['norm'] * np.where(r == 0., c['p'][1], sps.kv(nu, r) * np.exp(nu * np.log(r)))
+            elif t == _lib.LEAF_AIRY:
+                import scipy.special as sps
+                e = c['p'][0]
+                nu = np.sqrt(x * x + y * y) * math.pi
+                xv = np.where(nu < c['p'][1], 0.5 * (1. - e * e),
+                              (sps.j1(nu) - e * sps.j1(e * nu)) / np.where(nu > 0, nu, 1.))
+                v = c['norm'] * ((xv * xv) * (math.pi / (1. - e * e)))
             elif t == _lib.LEAF_BOX:
                 if c['form'] == 0:
                     inside = ((ii >= c['i1']) & (ii < c['i2']) & (jj >= c['j1']) & (jj < c['j2']))
@@ -169,6 +183,12 @@ def _host_cases():
         ("box_shift", lambda m: m.Box(0.9, 0.5).shift(0.05, 0.03)),
         ("box_rot", lambda m: m.Box(1.1, 0.7).rotate(0.3)),
         ("tophat", lambda m: m.TopHat(0.93, flux=2.)),
+        ("airy_obs_shear", lambda m: m.Airy(lam_over_diam=0.45, obscuration=0.3, flux=2.)
+         .shear(g1=0.1, g2=0.05)),
+        ("airy_unobs", lambda m: m.Airy(lam_over_diam=0.6)),
+        ("spergel_shear", lambda m: m.Spergel(0.6, half_light_radius=0.9, flux=1.5)
+         .shear(g1=0.2, g2=-0.1)),
+        ("spergel_neg", lambda m: m.Spergel(-0.5, scale_radius=0.4)),
         ("tophat_shear", lambda m: m.TopHat(0.93).shear(g1=-0.2, g2=0.15)),
         ("sum", lambda m: m.Add(m.Gaussian(sigma=0.5, flux=0.4),
                                 m.Exponential(half_light_radius=1.1, flux=0.6).shear(g1=0.3,
