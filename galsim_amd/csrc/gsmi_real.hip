@@ -36,6 +36,25 @@ __device__ __forceinline__ double moffat_pow(double x, int kind, double beta)
     }
 }
 
+// The Bessel-function leaves are called out of line: inlined, their register demand would be
+// the whole kernel's, for every profile.
+
+// AiryInfoObs / AiryInfoNoObs radial function (src/SBAiry.cpp:88-108, 410-429) at r in units of
+// lambda / D; eps = obscuration, thresh = the Taylor threshold on pi r
+__device__ __noinline__ double airy_radial(double r, double eps, double thresh)
+{
+    const double nu = r * M_PI;
+    const double xv = (nu < thresh) ? 0.5 * (1. - eps * eps)
+                                    : (j1(nu) - eps * j1(eps * nu)) / nu;
+    return (xv * xv) * (M_PI / (1. - eps * eps));
+}
+
+// SpergelInfo::xValue (src/SBSpergel.cpp:644-651): K_nu(r) r^nu, xnorm0 at r = 0
+__device__ __noinline__ double spergel_radial(double r, double nu, double xnorm0)
+{
+    return (r == 0.) ? xnorm0 : bessel_k(fabs(nu), r) * exp(nu * log(r));
+}
+
 // ceil(v) as an index clamped to [lo, hi] (v may be far outside the image, or NaN: empty range)
 __device__ __forceinline__ int clamp_ceil(double v, int lo, int hi)
 {
@@ -74,22 +93,12 @@ __device__ double xcomp_value(const gsmi_xcomp& c, int i, int j, int m)
         v = (c.p[1] > 0. && rsq > c.p[1]) ? 0. : c.norm * exp(-exp(c.p[0] * log(rsq)));
         break;
     }
-    case GSMI_LEAF_AIRY: {
-        // AiryInfoObs / AiryInfoNoObs radial function (src/SBAiry.cpp:88-108, 410-429) at
-        // r = |(x, y)| in units of lambda / D; p[0] = obscuration, p[1] = the Taylor threshold
-        const double nu = sqrt(x * x + y * y) * M_PI;
-        const double eps = c.p[0];
-        double xv = (nu < c.p[1]) ? 0.5 * (1. - eps * eps) : (j1(nu) - eps * j1(eps * nu)) / nu;
-        v = c.norm * ((xv * xv) * (M_PI / (1. - eps * eps)));
+    case GSMI_LEAF_AIRY:
+        v = c.norm * airy_radial(sqrt(x * x + y * y), c.p[0], c.p[1]);
         break;
-    }
-    case GSMI_LEAF_SPERGEL: {
-        // SpergelInfo::xValue (src/SBSpergel.cpp:644-651): K_nu(r) r^nu, p[1] at r = 0
-        const double r = sqrt(x * x + y * y);
-        const double nu = c.p[0];
-        v = c.norm * ((r == 0.) ? c.p[1] : bessel_k(fabs(nu), r) * exp(nu * log(r)));
+    case GSMI_LEAF_SPERGEL:
+        v = c.norm * spergel_radial(sqrt(x * x + y * y), c.p[0], c.p[1]);
         break;
-    }
     case GSMI_LEAF_BOX:
         if (c.form == 0)
             v = (i >= c.i1 && i < c.i2 && j >= c.j1 && j < c.j2) ? c.norm : 0.;
