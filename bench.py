@@ -415,8 +415,8 @@ def real_space_bench(args, torch):
     L = _lib.lib()
 
     def fill():
-        _lib.check(L.gsmi_draw_real_f64(recs.ctypes.data, len(recs), N, N, 0, out.data_ptr(),
-                                        stream), "gsmi_draw_real_f64")
+        _lib.check(L.gsmi_draw_real_f64(recs.ctypes.data, len(recs), None, N, N, 0,
+                                        out.data_ptr(), stream), "gsmi_draw_real_f64")
     for _ in range(3):
         fill()
     k = 20

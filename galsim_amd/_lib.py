@@ -142,7 +142,8 @@ _SIGS = {
     'gsmi_adjoint_f32': (_I, [_P, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     'gsmi_exp_moffat_fields': (_I, [_P, _I, _I, _P, _P, _D, _P, _P, _P, _P, _P]),
     'gsmi_exp_moffat_grad': (_I, [_P, _I, _I, _P, _P, _P, _D, _P, _P]),
-    'gsmi_draw_real_f64': (_I, [_P, _I, _I, _I, _I, _P, _P]),
+    'gsmi_draw_real_f64': (_I, [_P, _I, _P, _I, _I, _I, _P, _P]),
+    'gsmi_kolmogorov_xtable': (_I, [_P, _P, _P, _P, _I]),
 }
 
 

@@ -109,6 +109,22 @@ int gsmi_sersic_truncated_scale(double n, double hlr, double trunc, double* out)
                    [&] { *out = sersic_truncated_scale(n, hlr, trunc); });
 }
 
+int gsmi_kolmogorov_xtable(const double* gsp, double* args, double* vals, double* y2, int n)
+{
+    int count = -1;
+    int rc = guarded("gsmi_kolmogorov_xtable", [&] {
+        const KolmogorovInfo& k = kolmogorov_info(GSP::from(gsp));
+        count = int(k.xargs.size());
+        if (args && vals && y2 && n >= count)
+            for (int i = 0; i < count; ++i) {
+                args[i] = k.xargs[i];
+                vals[i] = k.xvals[i];
+                y2[i] = k.xy2[i];
+            }
+    });
+    return rc ? rc : count;
+}
+
 int gsmi_kolmogorov_info(const double* gsp, double* out)
 {
     if (!out) { set_error("gsmi_kolmogorov_info: bad argument"); return GSMI_E_ARG; }

@@ -55,6 +55,28 @@ __device__ __noinline__ double spergel_radial(double r, double nu, double xnorm0
     return (r == 0.) ? xnorm0 : bessel_k(fabs(nu), r) * exp(nu * log(r));
 }
 
+// KolmogorovInfo::xValue (src/SBKolmogorov.cpp:114-115): the natural cubic spline of the radial
+// table below its last knot, 0 beyond; the knot interval by lower_bound, as Table's lookup of
+// unequally spaced arguments
+__device__ __noinline__ double kolmogorov_radial(double r, const double* __restrict__ t, int n)
+{
+    const double* a = t;
+    const double* v = t + n;
+    const double* y2 = t + 2 * n;
+    if (!(r < a[n - 1])) return 0.;
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (a[mid] < r) lo = mid + 1; else hi = mid;
+    }
+    const int i = lo < 1 ? 1 : (lo > n - 1 ? n - 1 : lo);
+    const double h = a[i] - a[i - 1];
+    const double aa = a[i] - r;
+    const double bb = h - aa;
+    return (aa * v[i - 1] + bb * v[i] -
+            (1. / 6.) * aa * bb * ((aa + h) * y2[i - 1] + (bb + h) * y2[i])) / h;
+}
+
 // ceil(v) as an index clamped to [lo, hi] (v may be far outside the image, or NaN: empty range)
 __device__ __forceinline__ int clamp_ceil(double v, int lo, int hi)
 {
@@ -63,7 +85,8 @@ __device__ __forceinline__ int clamp_ceil(double v, int lo, int hi)
     return v < double(lo) ? lo : (v > double(hi) ? hi : int(v));
 }
 
-__device__ double xcomp_value(const gsmi_xcomp& c, int i, int j, int m)
+__device__ double xcomp_value(const gsmi_xcomp& c, int i, int j, int m,
+                              const double* __restrict__ tables)
 {
     double x, y;
     if (c.form == 0) {
@@ -95,6 +118,9 @@ __device__ double xcomp_value(const gsmi_xcomp& c, int i, int j, int m)
     }
     case GSMI_LEAF_AIRY:
         v = c.norm * airy_radial(sqrt(x * x + y * y), c.p[0], c.p[1]);
+        break;
+    case GSMI_LEAF_KOLMOGOROV:
+        v = c.norm * kolmogorov_radial(sqrt(x * x + y * y), tables + c.i1, c.i2);
         break;
     case GSMI_LEAF_SPERGEL:
         v = c.norm * spergel_radial(sqrt(x * x + y * y), c.p[0], c.p[1]);
@@ -130,7 +156,8 @@ __device__ double xcomp_value(const gsmi_xcomp& c, int i, int j, int m)
 
 // grid: ceil(m n / 256) workgroups (capped), grid-stride over the pixels
 __global__ void __launch_bounds__(256)
-draw_real_f64(XComps comps, int ncomp, int m, int n, int accumulate, double* __restrict__ out)
+draw_real_f64(XComps comps, int ncomp, const double* __restrict__ tables, int m, int n,
+              int accumulate, double* __restrict__ out)
 {
     const size_t total = size_t(m) * size_t(n);
     for (size_t p = size_t(blockIdx.x) * blockDim.x + threadIdx.x; p < total;
@@ -139,19 +166,22 @@ draw_real_f64(XComps comps, int ncomp, int m, int n, int accumulate, double* __r
         const int i = int(p - size_t(j) * size_t(m));
         double acc = 0.;
         for (int k = 0; k < ncomp; ++k) {
-            const double v = xcomp_value(comps.c[k], i, j, m);
+            const double v = xcomp_value(comps.c[k], i, j, m, tables);
             acc = (k == 0) ? v : acc + v;
         }
         out[p] = accumulate ? out[p] + acc : acc;
     }
 }
 
-bool xcomp_ok(const gsmi_xcomp& c)
+bool xcomp_ok(const gsmi_xcomp& c, const double* tables)
 {
     switch (c.type) {
     case GSMI_LEAF_GAUSSIAN: case GSMI_LEAF_EXPONENTIAL: case GSMI_LEAF_MOFFAT:
     case GSMI_LEAF_SERSIC: case GSMI_LEAF_BOX: case GSMI_LEAF_TOPHAT: case GSMI_LEAF_AIRY:
     case GSMI_LEAF_SPERGEL:
+        break;
+    case GSMI_LEAF_KOLMOGOROV:
+        if (!tables || c.i1 < 0 || c.i2 < 2) return false;
         break;
     default:
         return false;
@@ -162,8 +192,8 @@ bool xcomp_ok(const gsmi_xcomp& c)
 }  // namespace
 }  // namespace gsmi
 
-extern "C" int gsmi_draw_real_f64(const gsmi_xcomp* comps, int ncomp, int m, int n,
-                                  int accumulate, double* out, void* stream)
+extern "C" int gsmi_draw_real_f64(const gsmi_xcomp* comps, int ncomp, const double* tables,
+                                  int m, int n, int accumulate, double* out, void* stream)
 {
     using namespace gsmi;
     if (!out || m <= 0 || n <= 0 || ncomp < 0 || ncomp > GSMI_XCOMP_MAX ||
@@ -174,9 +204,9 @@ extern "C" int gsmi_draw_real_f64(const gsmi_xcomp* comps, int ncomp, int m, int
     }
     XComps xc = {};
     for (int k = 0; k < ncomp; ++k) {
-        if (!xcomp_ok(comps[k])) {
+        if (!xcomp_ok(comps[k], tables)) {
             set_error("gsmi_draw_real_f64: component " + std::to_string(k) +
-                      " has an unsupported type or form");
+                      " has an unsupported type or form (or a Kolmogorov table is missing)");
             return GSMI_E_ARG;
         }
         xc.c[k] = comps[k];
@@ -185,7 +215,7 @@ extern "C" int gsmi_draw_real_f64(const gsmi_xcomp* comps, int ncomp, int m, int
     const size_t want = (total + 255) / 256;
     const int grid = int(want < 8192 ? want : 8192);
     hipLaunchKernelGGL(draw_real_f64, dim3(grid), dim3(256), 0,
-                       reinterpret_cast<hipStream_t>(stream), xc, ncomp, m, n,
+                       reinterpret_cast<hipStream_t>(stream), xc, ncomp, tables, m, n,
                        accumulate ? 1 : 0, out);
     return hip_check(hipGetLastError(), "draw_real launch");
 }

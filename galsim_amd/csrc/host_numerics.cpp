@@ -794,6 +794,9 @@ const KolmogorovInfo& kolmogorov_info(const GSP& gsp)
     double hlr = 2.992939911888651 * 0.5548101137;
     R = std::max(R, gsp.stepk_minimum_hlr * hlr);
     info.stepk = M_PI / R;
+    info.xargs = radial.args;
+    info.xvals = radial.vals;
+    info.xy2 = radial.y2;
     return cache.emplace(gsp, info).first->second;
 }
 

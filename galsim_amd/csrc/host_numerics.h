@@ -47,7 +47,10 @@ double sersic_hlr(double n, double flux_fraction);
 double sersic_integrated_flux(double n, double r);
 double sersic_truncated_scale(double n, double hlr, double trunc);
 
-struct KolmogorovInfo { double maxk = 0., stepk = 0.; };
+struct KolmogorovInfo {
+    double maxk = 0., stepk = 0.;
+    std::vector<double> xargs, xvals, xy2;   // the x-space radial table (natural cubic spline)
+};
 const KolmogorovInfo& kolmogorov_info(const GSP& gsp);
 
 // Moffat (untruncated): kind code of the k-space formula, normalisations and sizes.

@@ -8,7 +8,7 @@ galsim/convolve.py:395-397), ending at each leaf's SBProfile::draw (src/SBProfil
 every leaf into one gsmi_xcomp record: draw()'s offset / Jacobian set-up and the leaf's own
 normalisation and radius scaling, done here on the host in the reference's order; the device
 evaluates the records at every pixel.  Leaves covered: Gaussian, Exponential, Moffat, Sersic
-(DeVaucouleurs), Airy, Spergel, Box / Pixel and TopHat.
+(DeVaucouleurs), Kolmogorov, Airy, Spergel, Box / Pixel and TopHat.
 """
 import math
 
@@ -82,9 +82,32 @@ def _moffat_kind(beta, xacc):
     return 0
 
 
-def _leaf_component(leaf, m, n, setup):
-    """The gsmi_xcomp record of one leaf drawn with draw()'s set-up ``setup``."""
+_KOLM_XTABLE = {}
+
+
+def kolmogorov_xtable(gsparams):
+    """KolmogorovInfo's x-space radial table (gsmi_kolmogorov_xtable) as one float64 array:
+    knots, values, second derivatives (cached per GSParams)."""
+    if gsparams not in _KOLM_XTABLE:
+        g = _lib.gsp_array(gsparams)
+        L = _lib.lib()
+        n = L.gsmi_kolmogorov_xtable(_lib.ptr(g), None, None, None, 0)
+        if n < 0:
+            _lib.check(n, "gsmi_kolmogorov_xtable")
+        t = np.zeros(3 * n)
+        got = L.gsmi_kolmogorov_xtable(_lib.ptr(g), _lib.ptr(t[:n]), _lib.ptr(t[n:2 * n]),
+                                       _lib.ptr(t[2 * n:]), n)
+        if got != n:
+            _lib.check(got if got < 0 else -3, "gsmi_kolmogorov_xtable")
+        _KOLM_XTABLE[gsparams] = t
+    return _KOLM_XTABLE[gsparams]
+
+
+def _leaf_component(leaf, m, n, setup, tables):
+    """The gsmi_xcomp record of one leaf drawn with draw()'s set-up ``setup``; a Kolmogorov
+    leaf appends its radial table to the list ``tables``."""
     from .airy import Airy
+    from .kolmogorov import Kolmogorov
     from .box import Box, TopHat
     from .exponential import Exponential
     from .gaussian import Gaussian
@@ -141,6 +164,16 @@ def _leaf_component(leaf, m, n, setup):
         rec['norm'] = leaf._flux * (D * D)
         rec['p'] = (leaf._obscuration, math.sqrt(8. * leaf.gsparams.xvalue_accuracy))
         radial = D
+    elif isinstance(leaf, Kolmogorov):
+        # src/SBKolmogorov.cpp:83-115 (k0 = K0_FACTOR / lam_over_r0, xnorm = flux k0^2)
+        k0 = leaf._k0
+        t = kolmogorov_xtable(leaf.gsparams)
+        rec['type'] = _lib.LEAF_KOLMOGOROV
+        rec['norm'] = leaf._flux * (k0 * k0)
+        rec['i1'] = sum(len(a) for a in tables)
+        rec['i2'] = len(t) // 3
+        tables.append(t)
+        radial = k0
     elif isinstance(leaf, Spergel):
         # src/SBSpergel.cpp:80-95 (xnorm = flux getXNorm / r0^2), 525-535, 644-651
         nu, r0 = leaf._nu, leaf._r0
@@ -175,8 +208,8 @@ def _leaf_component(leaf, m, n, setup):
         izero = jzero = 0
     else:
         raise NotImplementedError("drawReal of %s is not part of galsim_amd (Gaussian, "
-                                  "Exponential, Moffat, Sersic, Airy, Spergel, Box, Pixel and TopHat "
-                                  "are)"
+                                  "Exponential, Moffat, Kolmogorov, Sersic, Airy, Spergel, Box, Pixel "
+                                  "and TopHat are)"
                                   % type(leaf).__name__)
     if radial is not None:
         x0, dx, dxy, y0, dy, dyx = (x0 * radial, dx * radial, dxy * radial, y0 * radial,
@@ -195,9 +228,12 @@ def _leaf_component(leaf, m, n, setup):
     return rec
 
 
-def components(prof, bounds, imscale=1.):
+def components(prof, bounds, imscale=1., tables=None):
     """gsmi_xcomp records (in drawing order) of ``prof`` drawn into an image with ``bounds``
-    (its profile origin at (0, 0)) and pixel scale ``imscale``."""
+    (its profile origin at (0, 0)) and pixel scale ``imscale``; the Kolmogorov radial tables the
+    records index are appended to the list ``tables``."""
+    if tables is None:
+        tables = []
     from .convolve import Convolution
     from .sum import Sum
     from .transform import Transformation
@@ -230,7 +266,7 @@ def components(prof, bounds, imscale=1.):
         else:
             setup = _draw_setup(xmin, ymin, m, n, float(imscale), jac, float(offset[0]),
                                 float(offset[1]), float(flux_scaling))
-            out.append(_leaf_component(obj, m, n, setup))
+            out.append(_leaf_component(obj, m, n, setup, tables))
 
     walk(prof, None, (0., 0.), 1.)
     return np.array(out, dtype=XCOMP_DTYPE) if out else np.zeros(0, dtype=XCOMP_DTYPE)
@@ -243,15 +279,19 @@ def draw_real(prof, bounds, imscale=1., device_out=False):
     if not torch.cuda.is_available():
         raise _lib.GalSimAMDError("galsim_amd drawReal runs on the GPU; no HIP device is "
                                   "available")
-    recs = np.ascontiguousarray(components(prof, bounds, imscale))
+    tables = []
+    recs = np.ascontiguousarray(components(prof, bounds, imscale, tables))
     m, n = bounds.xmax - bounds.xmin + 1, bounds.ymax - bounds.ymin + 1
     out = torch.empty((n, m), dtype=torch.float64, device='cuda')
+    dtab = (torch.from_numpy(np.concatenate(tables)).to('cuda') if tables else None)
     stream = torch.cuda.current_stream().cuda_stream
     L = _lib.lib()
     if len(recs) == 0:
         out.zero_()
     for c0 in range(0, len(recs), XCOMP_MAX):
         chunk = np.ascontiguousarray(recs[c0:c0 + XCOMP_MAX])
-        _lib.check(L.gsmi_draw_real_f64(chunk.ctypes.data, len(chunk), m, n, int(c0 > 0),
-                                        out.data_ptr(), stream), "gsmi_draw_real_f64")
+        _lib.check(L.gsmi_draw_real_f64(chunk.ctypes.data, len(chunk),
+                                        None if dtab is None else dtab.data_ptr(), m, n,
+                                        int(c0 > 0), out.data_ptr(), stream),
+                   "gsmi_draw_real_f64")
     return out if device_out else out.cpu().numpy()

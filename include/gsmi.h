@@ -443,6 +443,8 @@ int gsmi_exp_moffat_grad(const double* theta, int n, int fit_beta, const double*
  *              (ic, jc) >= 0 takes r = 0                  (src/SBSersic.cpp:104-222, 354-360)
  *   AIRY       nu = pi r: (pi / (1 - e^2)) ((J1(nu) - e J1(e nu)) / nu)^2, or with
  *              (1 - e^2) / 2 inside the bracket for nu < p[1]; e = p[0]   (src/SBAiry.cpp:88-190)
+ *   KOLMOGOROV the spline of gsmi_kolmogorov_xtable at r = |(x, y)| below its last knot, else 0
+ *                                                  (src/SBKolmogorov.cpp:104-115, 127-185)
  *   SPERGEL    K_nu(r) r^nu (nu = p[0]), p[1] at r = 0 (infinite for nu <= 0)
  *                                                         (src/SBSpergel.cpp:80-108, 394-452, 644-651)
  *   BOX        1 inside; form 0: i1 <= i < i2 and j1 <= j < j2; form 1: |x| < p[0], |y| < p[1]
@@ -453,13 +455,14 @@ int gsmi_exp_moffat_grad(const double* theta, int n, int fit_beta, const double*
 #define GSMI_XCOMP_MAX 16
 
 typedef struct gsmi_xcomp {
-    int32_t type;         /* GSMI_LEAF_GAUSSIAN, _EXPONENTIAL, _MOFFAT, _SERSIC, _AIRY, _SPERGEL,
-                             _BOX, _TOPHAT */
+    int32_t type;         /* GSMI_LEAF_GAUSSIAN, _EXPONENTIAL, _MOFFAT, _KOLMOGOROV, _SERSIC,
+                             _AIRY, _SPERGEL, _BOX, _TOPHAT */
     int32_t form;         /* 0 or 1, as above */
     int32_t kind;         /* Moffat power form */
     int32_t izero, jzero; /* form 0 quadrant origin */
     int32_t ic, jc;       /* Sersic centre pixel (-1: none) */
-    int32_t i1, i2, j1, j2; /* Box (form 0) index ranges; TopHat rows j1..j2 */
+    int32_t i1, i2, j1, j2; /* Box (form 0) index ranges; TopHat rows j1..j2; Kolmogorov: table
+                               offset i1 and knots i2 */
     int32_t pad;
     double x0, dx, dxy, y0, dy, dyx;
     double norm;          /* the leaf's amplitude */
@@ -469,9 +472,16 @@ typedef struct gsmi_xcomp {
 
 /* Fill the m x n FP64 image `out` (device, row-major, row j = y) with the components comps[0..
  * ncomp) (a host array, ncomp <= GSMI_XCOMP_MAX), adding to its contents when accumulate is 1.
+ * `tables` (device, may be NULL without Kolmogorov components) holds each Kolmogorov component's
+ * radial table at tables + i1: i2 knots, then i2 values, then i2 second derivatives.
  * Replaces the fillXImage loops of SBProfile::draw (src/SBProfile.cpp:304-381). */
-int gsmi_draw_real_f64(const gsmi_xcomp* comps, int ncomp, int m, int n, int accumulate,
-                       double* out, void* stream);
+int gsmi_draw_real_f64(const gsmi_xcomp* comps, int ncomp, const double* tables, int m, int n,
+                       int accumulate, double* out, void* stream);
+
+/* KolmogorovInfo's x-space radial table (src/SBKolmogorov.cpp:337-384: f(0) then the Hankel
+ * transform of exp(-k^(5/3)) / 2 pi on log r from -3, natural cubic spline): returns the number
+ * of knots, and fills args / vals / y2 (second derivatives) when n is at least that. */
+int gsmi_kolmogorov_xtable(const double* gsp, double* args, double* vals, double* y2, int n);
 
 #ifdef __cplusplus
 }

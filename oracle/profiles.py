@@ -808,6 +808,32 @@ def kolmogorov_info(gsp):
     return res
 
 
+def kolmogorov_xtable(gsp):
+    """KolmogorovInfo's x-space radial table (src/SBKolmogorov.cpp:337-384): knots (0, then
+    exp(log r) from -3 by dlogr until val r^2 / 1.67 < shoot_accuracy / 2 pi) and values (f(0),
+    then the Hankel transform of exp(-k^(5/3)) / 2 pi), the table kolmogorov_info walks."""
+    key = ('xtable',) + tuple(gsp.key())
+    if key in _KOLM_INFO:
+        return _KOLM_INFO[key]
+    dlogr = gsp.table_spacing * math.sqrt(math.sqrt(gsp.xvalue_accuracy / 10.))
+    thresh = gsp.shoot_accuracy / (2. * math.pi)
+    kfunc = lambda k: nm.expd(-fast_pow(k, 5. / 3.))
+    args, vals = [0.], [0.0876786563672346]
+    logr = -3.
+    while logr < math.log(1.e4):
+        r = math.exp(logr)
+        val = nm.hankel_inf(kfunc, r, gsp.integration_relerr, gsp.integration_abserr)
+        val /= 2. * math.pi
+        args.append(r)
+        vals.append(val)
+        if val * r * r / 1.67 < thresh:
+            break
+        logr += dlogr
+    res = (np.array(args), np.array(vals))
+    _KOLM_INFO[key] = res
+    return res
+
+
 class Kolmogorov:
     """SBKolmogorov (src/SBKolmogorov.cpp:83-100 ctor, 185-245 fills, 258-259 kValue);
     maxk galsim/kolmogorov.py:281-284."""

@@ -19,8 +19,8 @@ import scipy.special as sps
 
 from . import numerics as nm
 from .draw import Convolution, Leaf, Sum, Transformation
-from .profiles import (Airy, Box, Exponential, Gaussian, Moffat, Sersic, Spergel, TopHat,
-                       fast_pow)
+from .profiles import (Airy, Box, Exponential, Gaussian, Kolmogorov, Moffat, Sersic, Spergel,
+                       TopHat, fast_pow, kolmogorov_xtable)
 
 
 def _acc(x0, dx, m):
@@ -100,6 +100,14 @@ class _X:
             self.obs = prof.obs
             self.thresh = math.sqrt(8. * prof.gsparams.xvalue_accuracy)
             self.kind = 'airy'
+        elif isinstance(prof, Kolmogorov):
+            # src/SBKolmogorov.cpp:83-115, KolmogorovInfo::xValue 114-115
+            self.inv = prof.k0
+            self.norm = prof.flux * (prof.k0 * prof.k0)
+            args, vals = kolmogorov_xtable(prof.gsparams)
+            self.table = nm.SplineTable(args, vals)
+            self.rmax = args[-1]
+            self.kind = 'kolm'
         elif isinstance(prof, Spergel):
             # src/SBSpergel.cpp:80-95, 525-535, SpergelInfo::getXNorm / xValue 644-651
             nu = prof.nu
@@ -137,6 +145,12 @@ class _X:
                 return v
             if self.kind == 'gauss':
                 return self.norm * nm.expd(-0.5 * rsq)
+            if self.kind == 'kolm':
+                r = np.sqrt(rsq)
+                flat = r.ravel()
+                v = np.array([self.table.lookup(float(q)) if q < self.rmax else 0.
+                              for q in flat]).reshape(r.shape)
+                return self.norm * v
             if self.kind == 'spergel':
                 r = np.sqrt(rsq)
                 v = sps.kv(self.nu, r) * fast_pow(r, self.nu)

@@ -52,6 +52,7 @@ def _golden_for_product(name):
         "double_gaussian.fits": lambda: g.Add(g.Gaussian(flux=0.75, sigma=1),
                                               g.Gaussian(flux=0.25, sigma=3)),
         "airy_.8_.1.fits": lambda: g.Airy(lam_over_diam=1. / 0.8, obscuration=0.1, flux=1),
+        "kolmogorov.fits": lambda: g.Kolmogorov(lam_over_r0=1.5, flux=1.8),
         "spergel_nu-0.85.fits": lambda: g.Spergel(nu=-0.85, half_light_radius=1.0),
         "spergel_nu-0.50.fits": lambda: g.Spergel(nu=-0.5, half_light_radius=1.0),
         "spergel_nu0.00.fits": lambda: g.Spergel(nu=0.0, half_light_radius=1.0),
@@ -133,12 +134,10 @@ def test_large_image_flux():
     assert im.added_flux == pytest.approx(7., rel=1e-9)
 
 
-def test_float32_image_and_unsupported():
+def test_float32_image():
     obj = g.Sersic(n=4, half_light_radius=0.7).shear(g1=0.1, g2=0.2)
     im = obj.drawImage(nx=40, ny=40, scale=0.2, method='sb')
     assert im.array.dtype == np.float32
     ref = ox.draw_image_sb(od.Sersic(4., half_light_radius=0.7).shear(g1=0.1, g2=0.2),
                            *_centred(40), 0.2, offset=(-0.5, -0.5))
     assert _peak_rel(im.array, ref) < 3e-7
-    with pytest.raises(NotImplementedError):
-        g.Kolmogorov(fwhm=0.7).drawImage(nx=16, ny=16, scale=0.2, method='no_pixel')

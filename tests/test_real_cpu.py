@@ -28,6 +28,7 @@ def _oracle_sb(obj, N, scale=0.2, method='sb'):
 
 # The reference's known-result tests (tests/test_exponential.py:29-45, test_moffat.py:30-48,
 # test_sersic.py:30-95, test_box.py:28-40, 140-155, test_airy.py:29-45, test_spergel.py:29-55,
+# test_kolmogorov.py:29-52,
 # test_gaussian.py:30-40, test_transforms.py:45-383, test_sum.py:28-45): method='sb',
 # use_true_center=False, scale 0.2, 5 decimals.
 def _golden_cases():
@@ -57,6 +58,7 @@ def _golden_cases():
         ("double_gaussian.fits", lambda: od.Add(od.Gaussian(flux=0.75, sigma=1.),
                                                 od.Gaussian(flux=0.25, sigma=3.)), golden),
         ("airy_.8_.1.fits", lambda: od.Airy(1. / 0.8, obscuration=0.1, flux=1.), golden),
+        ("kolmogorov.fits", lambda: od.Kolmogorov(lam_over_r0=1.5, flux=1.8), golden_top),
     ] + [("spergel_nu%.2f.fits" % nu, (lambda nu=nu: od.Spergel(nu, half_light_radius=1.)),
           golden) for nu in (-0.85, -0.5, 0.0, 0.85, 4.0)]
 
@@ -104,7 +106,7 @@ def test_oracle_closed_forms():
         np.testing.assert_allclose(img, norm * (1. + rsq) ** -beta, rtol=1e-13)
 
 
-def _eval_records(recs, m, n):
+def _eval_records(recs, m, n, tables=()):
     """numpy evaluation of gsmi_xcomp records (the device formula restated for the host check)."""
     from galsim_amd import _lib
     ii = np.arange(m)[None, :].astype(float)
@@ -135,6 +137,14 @@ def _eval_records(recs, m, n):
                 v = c['norm'] * np.exp(-np.exp(c['p'][0] * np.log(rsq)))
                 if c['p'][1] > 0:
                     v = np.where(rsq > c['p'][1], 0., v)
+            elif t == _lib.LEAF_KOLMOGOROV:
+                from oracle.numerics import SplineTable
+                tab = np.concatenate(tables)[c['i1']:c['i1'] + 3 * c['i2']]
+                k = int(c['i2'])
+                sp = SplineTable(tab[:k], tab[k:2 * k])
+                r = np.sqrt(x * x + y * y)
+                v = c['norm'] * np.array([sp.lookup(float(q)) if q < tab[k - 1] else 0.
+                                          for q in r.ravel()]).reshape(r.shape)
             elif t == _lib.LEAF_SPERGEL:
                 import scipy.special as sps
                 r = np.sqrt(x * x + y * y)
@@ -189,6 +199,9 @@ def _host_cases():
         ("spergel_shear", lambda m: m.Spergel(0.6, half_light_radius=0.9, flux=1.5)
          .shear(g1=0.2, g2=-0.1)),
         ("spergel_neg", lambda m: m.Spergel(-0.5, scale_radius=0.4)),
+        ("kolm_shear", lambda m: m.Kolmogorov(fwhm=0.7, flux=1.3).shear(g1=0.1, g2=0.2)),
+        ("kolm_sum", lambda m: m.Add(m.Kolmogorov(lam_over_r0=0.5),
+                                     m.Gaussian(sigma=0.3, flux=0.2))),
         ("tophat_shear", lambda m: m.TopHat(0.93).shear(g1=-0.2, g2=0.15)),
         ("sum", lambda m: m.Add(m.Gaussian(sigma=0.5, flux=0.4),
                                 m.Exponential(half_light_radius=1.1, flux=0.6).shear(g1=0.3,
@@ -207,8 +220,9 @@ def test_host_records_match_oracle(name, make):
     prof = _Transform(obj, np.array([[1. / scale, 0.], [0., 1. / scale]]), (0.21, -0.37),
                       scale ** 2 / scale ** 2)
     xmin, xmax, ymin, ymax = _centred(N)
-    recs = real.components(prof, BoundsI(xmin, xmax, ymin, ymax), 1.)
-    got = _eval_records(recs, N, N)
+    tables = []
+    recs = real.components(prof, BoundsI(xmin, xmax, ymin, ymax), 1., tables)
+    got = _eval_records(recs, N, N, tables)
     want = ox.draw_image_sb(make(od), xmin, xmax, ymin, ymax, scale, offset=(0.21, -0.37))
     np.testing.assert_allclose(got, want, rtol=0, atol=1e-11 * np.abs(want).max())
 
@@ -218,7 +232,21 @@ def test_records_reject_unsupported_profiles():
     from galsim_amd import real
     from galsim_amd.position import BoundsI
     with pytest.raises(NotImplementedError):
-        real.components(g.Kolmogorov(fwhm=0.7), BoundsI(-4, 3, -4, 3))
+        real.components(g.InclinedExponential(0.3 * g.radians, scale_radius=1., scale_height=0.1),
+                        BoundsI(-4, 3, -4, 3))
     with pytest.raises(NotImplementedError):
         real.components(g.Convolve(g.Gaussian(sigma=1.), g.Box(1., 1.), real_space=True),
                         BoundsI(-4, 3, -4, 3))
+
+
+def test_kolmogorov_xtable_matches_oracle():
+    # the host table (libgsmi) against the oracle's restatement of the same loop
+    import galsim_amd as g
+    from galsim_amd import real
+    import oracle.profiles as pr
+    t = real.kolmogorov_xtable(g.GSParams())
+    args, vals = pr.kolmogorov_xtable(pr.DEFAULT_GSPARAMS)
+    n = len(t) // 3
+    assert n == len(args)
+    np.testing.assert_array_equal(t[:n], args)
+    np.testing.assert_allclose(t[n:2 * n], vals, rtol=0, atol=1e-13 * vals[0])
