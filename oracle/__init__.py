@@ -11,13 +11,18 @@ Layout:
   (src/Table.cpp), goodFFTSize (src/Image.cpp:1141-1152), Ogata Hankel transform
   (src/math/Hankel.cpp:79-247).
 * ``profiles``  -- leaf profile scalars and k-space fills: Exponential, Gaussian, Moffat,
-  Kolmogorov, Sersic, Box/Pixel (src/SB*.cpp and galsim/*.py).
+  Kolmogorov (and its x-space radial table), Sersic, Spergel, Airy, InclinedExponential,
+  InclinedSersic, Box/Pixel, TopHat, DeltaFunction (src/SB*.cpp and galsim/*.py).
 * ``draw``      -- the drawImage(method='fft') orchestration: world->image transform, size
   planning, k-image fill through Transformation/Convolution/Sum, Hermitian wrap, inverse real
   FFT and crop (galsim/gsobject.py, galsim/transform.py, galsim/convolve.py, galsim/sum.py,
   src/SBProfile.cpp, src/SBTransform.cpp, src/Image.cpp).
 * ``fft``       -- galsim.fft (fft2 / ifft2 / rfft2 / irfft2 with shift_in / shift_out) and
   Image.calculate_fft / calculate_inverse_fft over numpy.fft (galsim/fft.py, galsim/image.py).
+* ``xdraw``     -- real-space drawing (drawImage method='no_pixel' / 'sb' of analytic
+  profiles): SBProfile::draw's set-up and each leaf's fillXImage with the reference's
+  accumulated coordinates, quadrant fill and per-component sum (src/SBProfile.cpp:304-488,
+  src/SB*.cpp fillXImage, galsim/gsobject.py:1846-1898).
 * ``ref``       -- ctypes loader for ``oracle/_ref/libgsref.so``, the reference's own profile
   numerics compiled from its sources by ``oracle/Makefile`` (used to pin this restatement).
 
