@@ -28,8 +28,8 @@ def _image_profile(prof, nx, ny, scale, offset, use_true_center, method):
     flux_scale = 1. / (scale * scale) if method == 'sb' else 1.
     p = _Transform(prof, jac, (off.x, off.y), scale ** 2 * flux_scale)
     if method in ('auto', 'fft'):
-        p = Convolve(p, Pixel(scale=1.0, gsparams=prof.gsparams), real_space=False,
-                     gsparams=prof.gsparams)
+        p = Convolve(p, Pixel(scale=1.0, gsparams=prof.gsparams),
+                     real_space=None if method == 'auto' else False, gsparams=prof.gsparams)
     b = BoundsI(1, nx, 1, ny)
     c = b.center
     return p, b.shift(PositionI(-c.x, -c.y)), flux_scale

@@ -1,4 +1,6 @@
 """Convolution (galsim/convolve.py), Fourier-space convolution only."""
+import warnings
+
 import numpy as np
 
 from . import _lib
@@ -36,6 +38,10 @@ class Convolution(GSObject):
             real_space = hard_edge if len(args) <= 2 else False
         if real_space:
             raise NotImplementedError("real-space convolution is not part of galsim_amd")
+        if hard_edge and len(args) == 2:
+            # galsim/convolve.py:171-175
+            warnings.warn("Doing convolution of 2 objects, both with hard edges. "
+                          "This might be more accurate and/or faster using real_space=True")
         self._real_space = False
         if gsparams is None:
             self._gsparams = GSParams.combine([obj.gsparams for obj in args])

@@ -85,6 +85,38 @@ __device__ __forceinline__ int clamp_ceil(double v, int lo, int hi)
     return v < double(lo) ? lo : (v > double(hi) ? hi : int(v));
 }
 
+// Form-1 Box / TopHat membership of pixel (i, j): 2 strictly inside (|x| < w/2 and |y| < h/2;
+// x^2 + y^2 < r^2), 1 on the closed edge, 0 outside.
+__device__ __forceinline__ int edge_class(const gsmi_xcomp& c, int i, int j)
+{
+    const double x = (c.x0 + double(j) * c.dxy) + double(i) * c.dx;
+    const double y = (c.y0 + double(j) * c.dy) + double(i) * c.dyx;
+    if (c.type == GSMI_LEAF_BOX) {
+        if (fabs(x) > c.p[0] || fabs(y) > c.p[1]) return 0;
+        return (fabs(x) < c.p[0] && fabs(y) < c.p[1]) ? 2 : 1;
+    }
+    const double rsq = x * x + y * y;
+    if (rsq > c.p[0]) return 0;
+    return rsq < c.p[0] ? 2 : 1;
+}
+
+// The reference fills a row of a sheared Box or TopHat as zeros while outside, then the norm
+// while strictly inside, then zeros (src/SBBox.cpp:136-143, 358-362): if the first pixel of the
+// row that is not outside lies exactly on the edge, the whole row stays zero.  Pixel i is
+// therefore filled iff it is strictly inside and the pixel before its strictly-inside segment
+// (found by bisection: the segment of a convex region along a row is one run) is outside, or the
+// segment starts the row.
+__device__ bool edge_row_filled(const gsmi_xcomp& c, int i, int j)
+{
+    if (edge_class(c, i, j) != 2) return false;
+    int lo = 0, hi = i;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (edge_class(c, mid, j) == 2) hi = mid; else lo = mid + 1;
+    }
+    return lo == 0 || edge_class(c, lo - 1, j) == 0;
+}
+
 __device__ double xcomp_value(const gsmi_xcomp& c, int i, int j, int m,
                               const double* __restrict__ tables)
 {
@@ -129,7 +161,7 @@ __device__ double xcomp_value(const gsmi_xcomp& c, int i, int j, int m,
         if (c.form == 0)
             v = (i >= c.i1 && i < c.i2 && j >= c.j1 && j < c.j2) ? c.norm : 0.;
         else
-            v = (fabs(x) < c.p[0] && fabs(y) < c.p[1]) ? c.norm : 0.;
+            v = edge_row_filled(c, i, j) ? c.norm : 0.;
         break;
     case GSMI_LEAF_TOPHAT:
         if (c.form == 0) {
@@ -145,7 +177,7 @@ __device__ double xcomp_value(const gsmi_xcomp& c, int i, int j, int m,
                 }
             }
         } else {
-            v = (x * x + y * y < c.p[0]) ? c.norm : 0.;
+            v = edge_row_filled(c, i, j) ? c.norm : 0.;
         }
         break;
     default:

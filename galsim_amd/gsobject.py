@@ -10,12 +10,19 @@ from .image import Image
 from .position import BoundsI, PositionD, PositionI
 
 
-def _real_added(pixels, dtype):
-    """drawReal's return value: the sum of the drawn values as stored in a float32 / float64
-    image, or in the float64 temporary the reference uses for other image types
-    (galsim/gsobject.py:1875-1889)."""
-    dt = dtype if dtype in (np.float32, np.float64) else np.float64
-    return float(pixels.astype(dt).sum(dtype=float))
+def _real_values(pixels, dtype, add_to_image):
+    """drawReal's drawn values and its return value (galsim/gsobject.py:1875-1889): a float32 or
+    float64 image that is overwritten receives the values directly; otherwise they are drawn
+    into a temporary, float64 for complex128 / int32 / uint32 images and float32 for every other
+    type (complex64, and float64 when adding), and the temporary's sum is returned."""
+    if dtype in (np.float32, np.float64) and not add_to_image:
+        dt = dtype
+    elif dtype in (np.complex128, np.int32, np.uint32):
+        dt = np.float64
+    else:
+        dt = np.float32
+    vals = pixels.astype(dt)
+    return vals, float(vals.sum(dtype=float))
 
 
 class GSObject:
@@ -286,7 +293,11 @@ class GSObject:
         jac = None if scale == 1. else np.array([[1. / scale, 0.], [0., 1. / scale]])
         prof = _Transform(self, jac, (offset.x, offset.y), scale ** 2 * flux_scale)
         if method in ('auto', 'fft'):
-            prof = Convolve(prof, Pixel(scale=1.0, gsparams=self.gsparams), real_space=False,
+            # 'auto' leaves the choice to Convolution (real space for two hard-edged profiles,
+            # galsim/gsobject.py:1754-1764), which raises here: real-space convolution is not
+            # part of this library
+            prof = Convolve(prof, Pixel(scale=1.0, gsparams=self.gsparams),
+                            real_space=None if method == 'auto' else False,
                             gsparams=self.gsparams)
         use_real = method in ('no_pixel', 'sb') and prof.is_analytic_x
         image = prof._setup_image(image, nx, ny, bounds, add_to_image, dtype, center)
@@ -298,8 +309,8 @@ class GSObject:
         draw_bounds = image.bounds.shift(PositionI(-c.x, -c.y))
         if use_real:
             from . import real
-            pixels = real.draw_real(prof, draw_bounds, 1.0)
-            added = _real_added(pixels, image.array.dtype)
+            pixels, added = _real_values(real.draw_real(prof, draw_bounds, 1.0),
+                                         image.array.dtype, add_to_image)
         else:
             pixels, added = render.draw_fft(prof, draw_bounds, precision=precision)
         if add_to_image:
@@ -316,12 +327,13 @@ class GSObject:
         from . import real
         if image.scale is None or not image.scale > 0:
             raise ValueError("drawReal requires an image with a PixelScale wcs", image)
-        pixels = real.draw_real(self, image.bounds, float(image.scale))
+        pixels, added = _real_values(real.draw_real(self, image.bounds, float(image.scale)),
+                                     image.array.dtype, add_to_image)
         if add_to_image:
             image.array[...] += pixels.astype(image.array.dtype)
         else:
             image.array[...] = pixels.astype(image.array.dtype)
-        return _real_added(pixels, image.array.dtype)
+        return added
 
     def drawFFT(self, image, add_to_image=False, precision='f64'):
         """drawFFT (galsim/gsobject.py:2014-2045): image bounds centred on the profile origin."""

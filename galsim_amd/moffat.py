@@ -36,6 +36,11 @@ class Moffat(Leaf):
     _leaf_type = _lib.LEAF_MOFFAT
     _is_analytic_x = True
 
+    @property
+    def _has_hard_edges(self):
+        # a truncated Moffat ends at trunc (galsim/moffat.py:203-204)
+        return self._trunc != 0.
+
     def __init__(self, beta, scale_radius=None, half_light_radius=None, fwhm=None, trunc=0.,
                  flux=1., gsparams=None):
         self._beta = float(beta)

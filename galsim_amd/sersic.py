@@ -35,6 +35,11 @@ class Sersic(Leaf):
     _minimum_n = 0.3
     _maximum_n = 6.2
 
+    @property
+    def _has_hard_edges(self):
+        # a truncated Sersic ends at trunc (galsim/sersic.py:339-340)
+        return self._trunc != 0.
+
     def __init__(self, n, half_light_radius=None, scale_radius=None, flux=1., trunc=0.,
                  flux_untruncated=False, gsparams=None):
         self._n = float(n)

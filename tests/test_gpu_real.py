@@ -109,9 +109,11 @@ def test_drawreal_direct_scale_and_add():
                        -20, 19, -15, 16, 0.25)
     assert _peak_rel(img.array, ref) < F64_TOL
     assert added == pytest.approx(float(ref.sum()), rel=1e-12)
+    # adding goes through a float32 temporary, as in the reference (galsim/gsobject.py:1879-1888)
     added2 = obj.drawReal(img, add_to_image=True)
-    assert _peak_rel(img.array, 2. * ref) < F64_TOL
-    assert added2 == pytest.approx(float(ref.sum()), rel=1e-12)
+    ref32 = ref.astype(np.float32)
+    assert _peak_rel(img.array, ref + ref32) < F64_TOL
+    assert added2 == pytest.approx(float(ref32.sum(dtype=float)), rel=1e-12)
 
 
 def test_sum_of_many_components_chunks():
@@ -141,3 +143,21 @@ def test_float32_image():
     ref = ox.draw_image_sb(od.Sersic(4., half_light_radius=0.7).shear(g1=0.1, g2=0.2),
                            *_centred(40), 0.2, offset=(-0.5, -0.5))
     assert _peak_rel(im.array, ref) < 3e-7
+
+
+def test_sheared_box_edge_rows_follow_reference_scan():
+    # the box edges fall on pixel centres: every row's first non-outside pixel is an edge pixel,
+    # so the reference's row scan (src/SBBox.cpp:136-143) leaves the image zero
+    im = g.Pixel(scale=1.).transform(0., 1., 1., 0.).drawImage(
+        nx=8, ny=8, scale=0.5, method='no_pixel', dtype=np.float64, use_true_center=False)
+    assert np.all(im.array == 0.)
+    for name, obj_g, obj_o in [
+            ("box", g.Box(1.5, 1.).transform(0.5, 0.25, -0.25, 0.5),
+             od.Box(1.5, 1.).transform(0.5, 0.25, -0.25, 0.5)),
+            ("tophat", g.TopHat(1.5).transform(0.5, 0.25, 0.25, 1.),
+             od.TopHat(1.5).transform(0.5, 0.25, 0.25, 1.))]:
+        im = obj_g.drawImage(nx=24, ny=24, scale=0.25, method='no_pixel', dtype=np.float64,
+                             use_true_center=False)
+        ref = ox.draw_image_sb(obj_o, *_centred(24), 0.25, method='no_pixel')
+        assert np.array_equal(im.array != 0., ref != 0.), name
+        assert _peak_rel(im.array, ref) < F64_TOL, name

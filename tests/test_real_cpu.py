@@ -250,3 +250,70 @@ def test_kolmogorov_xtable_matches_oracle():
     assert n == len(args)
     np.testing.assert_array_equal(t[:n], args)
     np.testing.assert_allclose(t[n:2 * n], vals, rtol=0, atol=1e-13 * vals[0])
+
+
+# -- sheared Box / TopHat rows whose first pixel in the closed region lies on the edge ----------
+# The reference fills such a row as zeros while outside, the norm while strictly inside, zeros
+# after (src/SBBox.cpp:136-143, 358-362); a row whose first non-outside pixel is exactly on the
+# edge stays zero.  Pixel(1) with x and y swapped, drawn at scale 0.5, puts the box edges on pixel
+# centres: every row's first non-outside pixel is an edge pixel, so the reference draws zeros.
+def test_box_edge_rows_stay_zero_in_oracle():
+    obj = od.Pixel(1.).transform(0., 1., 1., 0.)
+    img = ox.draw_image_sb(obj, *_centred(8), 0.5, method='no_pixel')
+    assert np.all(img == 0.)
+    # a per-pixel inside test would fill the centre pixel: the row rule is what empties it
+    assert ox.draw_image_sb(od.Pixel(1.).transform(0., 1., 1., 0.), *_centred(8), 0.499,
+                            method='no_pixel').max() > 0.
+
+
+def _row_rule(cls):
+    """The GPU kernel's per-pixel form of the row scan (gsmi_real.hip edge_row_filled): pixel i
+    is filled iff it is strictly inside (class 2) and the pixel before its strictly-inside run
+    (found by bisection) is outside (class 0), or the run starts the row."""
+    out = np.zeros(len(cls), dtype=bool)
+    for i in range(len(cls)):
+        if cls[i] != 2:
+            continue
+        lo, hi = 0, i
+        while lo < hi:
+            mid = (lo + hi) // 2
+            if cls[mid] == 2:
+                hi = mid
+            else:
+                lo = mid + 1
+        out[i] = lo == 0 or cls[lo - 1] == 0
+    return out
+
+
+@pytest.mark.parametrize("kind", ["box", "tophat"])
+def test_kernel_row_rule_matches_reference_scan(kind):
+    rng = np.random.default_rng(7)
+    m = n = 24
+    ncheck = 0
+    for trial in range(60):
+        # dyadic steps and origins, so that edges fall exactly on pixel centres in many rows
+        d = rng.choice([0.25, 0.5, 0.125], 4) * rng.choice([-1., 1.], 4)
+        if abs(d[0] * d[3] - d[1] * d[2]) < 1e-3:
+            continue
+        x0, y0 = -m / 2 * d[0] - n / 2 * d[1], -m / 2 * d[2] - n / 2 * d[3]
+        X, Y = ox._grid(x0, d[0], d[1], y0, d[3], d[2], m, n)
+        if kind == 'box':
+            w, h = rng.choice([0.5, 1., 1.5, 0.75], 2)
+            outside = (np.abs(X) > w) | (np.abs(Y) > h)
+            inside = (np.abs(X) < w) & (np.abs(Y) < h)
+        else:
+            r2 = rng.choice([1., 2.25, 0.5625])
+            outside = X * X + Y * Y > r2
+            inside = X * X + Y * Y < r2
+        cls = np.where(outside, 0, np.where(inside, 2, 1))
+        for j in range(n):
+            ref = np.zeros(m, dtype=bool)
+            i = 0
+            while i < m and outside[j, i]:
+                i += 1
+            while i < m and inside[j, i]:
+                ref[i] = True
+                i += 1
+            assert np.array_equal(_row_rule(cls[j]), ref), (trial, j)
+            ncheck += int(np.any(cls[j] == 1))
+    assert ncheck > 50      # rows with exact-edge pixels were exercised
