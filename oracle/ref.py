@@ -39,6 +39,11 @@ def lib():
         L.gsref_kolmogorov_info.argtypes = [dp, dp]
         L.gsref_hankel_moffat_trunc.argtypes = [d] * 5
         L.gsref_hankel_moffat_trunc.restype = d
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.gsref_kvalue_range_1d.argtypes = [ctypes.c_int, d, d, d, d, d, ip, dp]
+        L.gsref_kvalue_range_1d.restype = None
+        L.gsref_kvalue_range_2d.argtypes = [ctypes.c_int, d, d, d, d, d, d, ip]
+        L.gsref_kvalue_range_2d.restype = None
         _lib = L
     return _lib
 
@@ -91,3 +96,20 @@ def __getattr__(name):
     if name.startswith('__'):
         raise AttributeError(name)
     return getattr(lib(), 'gsref_' + name)
+
+
+def kvalue_range_1d(m, kmax, ksqmax, kx0, dkx, ky):
+    """The reference's GetKValueRange1d (src/SBProfile.cpp:508-530): (i1, i2)."""
+    out = np.zeros(2, dtype=np.int32)
+    kysq = np.zeros(1)
+    lib().gsref_kvalue_range_1d(int(m), kmax, ksqmax, kx0, dkx, ky,
+                                out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _ptr(kysq))
+    return int(out[0]), int(out[1])
+
+
+def kvalue_range_2d(m, kmax, ksqmax, kx0, dkx, ky0, dky):
+    """The reference's GetKValueRange2d (src/SBProfile.cpp:532-588): (i1, i2)."""
+    out = np.zeros(2, dtype=np.int32)
+    lib().gsref_kvalue_range_2d(int(m), kmax, ksqmax, kx0, dkx, ky0, dky,
+                                out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    return int(out[0]), int(out[1])

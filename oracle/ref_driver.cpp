@@ -15,6 +15,7 @@
 #include "SBSersicImpl.h"
 #include "SBKolmogorovImpl.h"
 #include "SBMoffat.h"
+#include "SBProfileImpl.h"
 #include "math/Bessel.h"
 #include "math/Gamma.h"
 #include "math/Hankel.h"
@@ -110,5 +111,22 @@ double gsref_hankel_kolmogorov(double r, double relerr, double abserr)
 }
 
 double gsref_expd(double x) { return fmath::expd(x); }
+
+// GetKValueRange1d / GetKValueRange2d (src/SBProfile.cpp:508-588): the row range [i1, i2) of a
+// k-image row that the clipped fills (SBExponential, SBSpergel) evaluate; out = {i1, i2}, and
+// for the 1-d form out_kysq = ky^2 when the reference sets it.
+void gsref_kvalue_range_1d(int m, double kmax, double ksqmax, double kx0, double dkx, double ky,
+                           int* out, double* out_kysq)
+{
+    double kysq = -1.;
+    GetKValueRange1d(out[0], out[1], m, kmax, ksqmax, kx0, dkx, ky, kysq);
+    *out_kysq = kysq;
+}
+
+void gsref_kvalue_range_2d(int m, double kmax, double ksqmax, double kx0, double dkx, double ky0,
+                           double dky, int* out)
+{
+    GetKValueRange2d(out[0], out[1], m, kmax, ksqmax, kx0, dkx, ky0, dky);
+}
 
 }

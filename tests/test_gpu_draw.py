@@ -250,3 +250,42 @@ def test_fp64_fft_sizes_beyond_the_fused_kernels(r0, moffat, N):
     print("N", info['N'], "peak-rel", _peak_rel(im.array, ref))
     assert info['N'] == N
     assert _peak_rel(im.array, ref) < F64_TOL
+
+
+@pytest.mark.parametrize("idx", range(4))
+def test_drawkimage_phases_match_closed_form(idx):
+    # the closed-form transformed Gaussians of tests/test_oracle_phases.py (ApplyKImagePhases'
+    # separable and 2-d branches, offsets composed through outer Jacobians, flux scalings)
+    # through the GPU's k-image program
+    from tests.test_oracle_phases import _cases, closed_form
+    name, obj, terms = _cases(g)[idx]
+    nx, ny, dk = 41, 37, 0.21
+    kim = g.ImageCD(nx, ny, scale=dk)
+    obj.drawKImage(kim)
+    assert kim.bounds.xmin == -(nx // 2) and kim.bounds.ymin == -(ny // 2)
+    ref = closed_form(terms, nx, ny, dk)
+    err = float(np.max(np.abs(kim.array - ref)) / np.max(np.abs(ref)))
+    print(name, err)
+    assert err < 1e-12, (name, err)
+
+
+def test_golden_moffat_pixel(golden_dir):
+    # the reference's FFT-path golden against an exact Maple calculation
+    # (tests/test_convolve.py:29-60), 4 decimals, and the oracle at 1e-10
+    saved = read_fits(os.path.join(golden_dir, "moffat_pixel.fits"))
+    fw = 1.0927449310213702
+    psf = g.Moffat(beta=1.5, fwhm=fw, trunc=4 * fw, flux=1)
+    pixel = g.Pixel(scale=0.2, flux=1.)
+    with pytest.warns(UserWarning, match="hard edges"):
+        conv = g.Convolve([psf, pixel], real_space=False)
+    img = g.ImageF(saved.shape[1], saved.shape[0], scale=0.2)
+    img.setCenter(0, 0)
+    conv.drawImage(img, scale=0.2, method="sb", use_true_center=False)
+    np.testing.assert_array_almost_equal(img.array, saved, 4)
+    ref = od.Convolution([od.Moffat(1.5, fwhm=fw, trunc=4 * fw, flux=1.),
+                          od.Pixel(0.2, flux=1.)]).drawImage(
+        saved.shape[1], saved.shape[0], scale=0.2, method='sb', use_true_center=False)
+    img64 = g.ImageD(saved.shape[1], saved.shape[0], scale=0.2)
+    img64.setCenter(0, 0)
+    conv.drawImage(img64, scale=0.2, method="sb", use_true_center=False)
+    assert _peak_rel(img64.array, ref) < F64_TOL

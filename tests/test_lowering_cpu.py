@@ -97,3 +97,21 @@ def test_drawreal_temporary_dtype():
         vals, added = _real_values(px, np.dtype(dt), add)
         assert vals.dtype == want, (dt, add)
         assert added == float(px.astype(want).sum(dtype=float))
+
+
+@pytest.mark.parametrize("idx", range(4))
+def test_lowered_phases_match_closed_form(idx):
+    # the product's lowering of the closed-form transformed Gaussians (both ApplyKImagePhases
+    # branches, offsets composed through outer Jacobians), evaluated by the program emulator
+    from galsim_amd import render
+    from tests.program_emulator import eval_program
+    from tests.test_oracle_phases import _cases, closed_form
+    name, obj, terms = _cases(g)[idx]
+    nx, ny, dk = 41, 37, 0.21
+    ctx = render.Lowering('f64', dk=dk)
+    obj._lower(ctx, None)
+    ii, jj = np.meshgrid(np.arange(nx) - nx // 2, np.arange(ny) - ny // 2)
+    got = eval_program(ctx, ii, jj, dk, 0)
+    ref = closed_form(terms, nx, ny, dk)
+    err = float(np.max(np.abs(got - ref)) / np.max(np.abs(ref)))
+    assert err < 1e-12, (name, err)

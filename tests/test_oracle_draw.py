@@ -103,3 +103,16 @@ def test_aliasing_fold_when_maxk_beyond_nyquist():
     assert info['Nk'] > info['N']
     # folding preserves the total flux in the periodic image (the k=0 sample is untouched)
     assert abs(img.sum() - 1.) < 5e-3
+
+
+def test_golden_moffat_pixel(golden_dir):
+    # the reference's FFT-path golden against an exact Maple calculation
+    # (tests/test_convolve.py:29-60): a truncated Moffat(beta=1.5) convolved with a Pixel by FFT,
+    # drawn method='sb', use_true_center=False, at 4 decimals
+    saved = read_fits(os.path.join(golden_dir, "moffat_pixel.fits"))
+    fw = 1.0927449310213702
+    conv = od.Convolution([od.Moffat(1.5, fwhm=fw, trunc=4 * fw, flux=1.),
+                           od.Pixel(0.2, flux=1.)])
+    ny, nx = saved.shape
+    img = conv.drawImage(nx, ny, scale=0.2, method='sb', use_true_center=False)
+    np.testing.assert_array_almost_equal(img.astype(np.float32), saved, 4)
