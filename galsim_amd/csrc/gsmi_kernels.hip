@@ -487,7 +487,7 @@ rows_list(const gsmi_stamp* __restrict__ stamps, int nstamps,
 // full N x N real image from N column-transformed rows (irfft entry point)
 __global__ void __launch_bounds__(kThreads)
 irfft_rows_f64(const cplx<double>* __restrict__ scratch, int N, const cplx<double>* __restrict__ tw,
-               double* __restrict__ out)
+               int tws, double* __restrict__ out)
 {
     extern __shared__ unsigned char smem_raw[];
     cplx<double>* lds = reinterpret_cast<cplx<double>*>(smem_raw);
@@ -497,7 +497,7 @@ irfft_rows_f64(const cplx<double>* __restrict__ scratch, int N, const cplx<doubl
     if (nrow <= 0) return;
     const int W = N / 2 + 1;
     row_c2r_body<double>(lds, rpg, scratch + size_t(r0) * W, nrow, N, -N / 2, N, tw,
-                         out + size_t(r0) * N);
+                         out + size_t(r0) * N, tws);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -543,7 +543,7 @@ wrap_hermx_f64(const cplx<double>* __restrict__ kin, int Nk, int N, cplx<double>
 // Copy a wrapped k-image (rows ky=-N/2..N/2-1, row-major) into column-major DFT order.
 __global__ void __launch_bounds__(kThreads)
 irfft_colfft_f64(const cplx<double>* __restrict__ kin, int N, const cplx<double>* __restrict__ tw,
-                 cplx<double>* __restrict__ scratch)
+                 int tws, cplx<double>* __restrict__ scratch)
 {
     extern __shared__ unsigned char smem_raw[];
     cplx<double>* lds = reinterpret_cast<cplx<double>*>(smem_raw);
@@ -562,7 +562,7 @@ irfft_colfft_f64(const cplx<double>* __restrict__ kin, int N, const cplx<double>
         bufa[cl * N + r] = kin[row * W + c0 + cl];
     }
     __syncthreads();
-    cplx<double>* res = fft_lds<double>(bufa, bufb, N, ncol, tw, 1);
+    cplx<double>* res = fft_lds<double>(bufa, bufb, N, ncol, tw, tws);
     for (int e = threadIdx.x; e < ncol * N; e += blockDim.x) {
         const int row = e / ncol;                       // output position p = row - N/2
         const int cl = e - row * ncol;
@@ -814,6 +814,95 @@ int rows_list_impl(const gsmi_stamp* stamps, int nstamps, const gsmi_schedule* s
 }
 }  // namespace
 
+constexpr int kNoAllocMaxN = GSMI_NOALLOC_MAX_N;
+
+// Column then row pass of irfft(shift_in, shift_out) through `scratch` (N (N/2+1) complex128),
+// with twiddles tw[m * tws] = exp(2 pi i m / N).
+int irfft_launch(const double* kin, int N, double* out, cplx<double>* scratch,
+                 const cplx<double>* tw, int tws, void* stream)
+{
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const int cpg = cols_per_group(N);
+    size_t lds1 = 2 * size_t(cpg) * N * sizeof(cplx<double>);
+    hipLaunchKernelGGL(irfft_colfft_f64, dim3((N / 2 + cpg) / cpg), dim3(kThreads), lds1, s,
+                       reinterpret_cast<const cplx<double>*>(kin), N, tw, tws, scratch);
+    int rc = hip_check(hipGetLastError(), "irfft columns launch");
+    if (rc) return rc;
+    const int rpg = rows_per_group(N);
+    size_t lds2 = 2 * size_t(rpg) * (N / 2) * sizeof(cplx<double>);
+    hipLaunchKernelGGL(irfft_rows_f64, dim3((N + rpg - 1) / rpg), dim3(kThreads), lds2, s,
+                       scratch, N, tw, tws, out);
+    return hip_check(hipGetLastError(), "irfft rows launch");
+}
+
+// ---------------------------------------------------------------------------------------------
+// Static scratch of gsmi_irfft_f64_noalloc: the N (N/2+1) complex128 column-transformed rows for
+// N <= kNoAllocMaxN, then the two twiddle tables exp(2 pi i m / 4096), m < 4096, and
+// exp(2 pi i m / 3072), m < 3072, from which every supported N takes every (4096/N)-th or
+// (3072/N)-th entry.  One copy per device, held until the process ends; every call on a device
+// shares it, so calls that overlap on one device race.  Host code never passes the array's name
+// to a kernel or copy: it uses the device address hipGetSymbolAddress returns for it.
+// ---------------------------------------------------------------------------------------------
+constexpr size_t kNoAllocRowsBytes = size_t(kNoAllocMaxN) * (kNoAllocMaxN / 2 + 1) *
+                                     sizeof(cplx<double>);
+constexpr size_t kNoAllocTw2Offset = kNoAllocRowsBytes;
+constexpr size_t kNoAllocTw3Offset = kNoAllocTw2Offset + size_t(kTwPow2) * sizeof(cplx<double>);
+constexpr size_t kNoAllocBytes = kNoAllocTw3Offset + size_t(kTwPow3) * sizeof(cplx<double>);
+
+namespace noalloc {
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+}
+
+namespace {
+// exp(+2 pi i m / n), m = 0..n-1, rounded from long double (the values the per-size twiddle
+// cache holds: every table here has a power-of-two ratio to the sizes it serves).
+std::vector<cplx<double>> host_twiddles(int n)
+{
+    std::vector<cplx<double>> h(n);
+    for (int k = 0; k < n; ++k) {
+        long double ang = 2.0L * 3.14159265358979323846264338327950288L * k / n;
+        h[k].x = double(std::cos(ang));
+        h[k].y = double(std::sin(ang));
+    }
+    return h;
+}
+}  // namespace
+
+// Device address of this device's copy of noalloc::scratch.  The first call on a device looks
+// the address up and writes the twiddle tables into it (synchronous copies from host memory,
+// no allocation); later calls return the cached address.
+int noalloc_base(unsigned char** base)
+{
+    static std::mutex m;
+    static unsigned char* ptrs[64] = {};
+    int dev = 0;
+    int rc = hip_check(hipGetDevice(&dev), "hipGetDevice");
+    if (rc) return rc;
+    if (dev < 0 || dev >= 64) {
+        set_error("gsmi_irfft_f64_noalloc: device index out of range");
+        return GSMI_E_ARG;
+    }
+    std::lock_guard<std::mutex> lock(m);
+    if (!ptrs[dev]) {
+        void* p = nullptr;
+        rc = hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(noalloc::scratch)),
+                       "hipGetSymbolAddress(static scratch)");
+        if (rc) return rc;
+        unsigned char* b = static_cast<unsigned char*>(p);
+        const std::vector<cplx<double>> t2 = host_twiddles(kTwPow2);
+        const std::vector<cplx<double>> t3 = host_twiddles(kTwPow3);
+        rc = hip_check(hipMemcpy(b + kNoAllocTw2Offset, t2.data(), t2.size() * sizeof(t2[0]),
+                                 hipMemcpyHostToDevice), "static twiddle upload");
+        if (rc) return rc;
+        rc = hip_check(hipMemcpy(b + kNoAllocTw3Offset, t3.data(), t3.size() * sizeof(t3[0]),
+                                 hipMemcpyHostToDevice), "static twiddle upload");
+        if (rc) return rc;
+        ptrs[dev] = b;
+    }
+    *base = ptrs[dev];
+    return GSMI_OK;
+}
+
 void set_error(const std::string& msg) { g_err = msg; }
 
 }  // namespace gsmi
@@ -894,21 +983,30 @@ int gsmi_irfft_f64(const double* kin, int N, double* out, void* scratch, void* s
         set_error("twiddle table allocation failed");
         return GSMI_E_HIP;
     }
-    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    const int cpg = cols_per_group(N);
-    size_t lds1 = 2 * size_t(cpg) * N * sizeof(cplx<double>);
-    hipLaunchKernelGGL(irfft_colfft_f64, dim3((N / 2 + cpg) / cpg), dim3(kThreads), lds1, s,
-                       reinterpret_cast<const cplx<double>*>(kin), N, tw,
-                       reinterpret_cast<cplx<double>*>(scratch));
-    int rc = hip_check(hipGetLastError(), "irfft columns launch");
-    if (rc) return rc;
-    const int rpg = rows_per_group(N);
-    size_t lds2 = 2 * size_t(rpg) * (N / 2) * sizeof(cplx<double>);
-    hipLaunchKernelGGL(irfft_rows_f64, dim3((N + rpg - 1) / rpg), dim3(kThreads), lds2, s,
-                       reinterpret_cast<const cplx<double>*>(scratch), N, tw, out);
-    return hip_check(hipGetLastError(), "irfft rows launch");
+    return irfft_launch(kin, N, out, reinterpret_cast<cplx<double>*>(scratch), tw, 1, stream);
 }
 
+int gsmi_irfft_f64_noalloc(const double* kin, int N, double* out, void* stream)
+{
+    if (!kin || !out) {
+        set_error("gsmi_irfft_f64_noalloc: null buffer");
+        return GSMI_E_ARG;
+    }
+    if (N > kNoAllocMaxN || !fft_size_ok(N, sizeof(cplx<double>))) {
+        set_error("gsmi_irfft_f64_noalloc: unsupported FFT size " + std::to_string(N) +
+                  " (need 2^k or 3*2^k, <= " + std::to_string(kNoAllocMaxN) +
+                  ", the size of its static scratch)");
+        return GSMI_E_SIZE;
+    }
+    unsigned char* base = nullptr;
+    int rc = noalloc_base(&base);
+    if (rc) return rc;
+    const bool p3 = (N % 3 == 0);
+    const cplx<double>* tw = reinterpret_cast<const cplx<double>*>(
+        base + (p3 ? kNoAllocTw3Offset : kNoAllocTw2Offset));
+    const int tws = p3 ? kTwPow3 / N : kTwPow2 / N;
+    return irfft_launch(kin, N, out, reinterpret_cast<cplx<double>*>(base), tw, tws, stream);
+}
 
 int gsmi_fold_colfft_f64(const gsmi_stamp* stamps, int nstamps, int N, const gsmi_op* ops,
                          const gsmi_leaf* leaves, const double* tables, const int32_t* cliprows,

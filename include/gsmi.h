@@ -313,6 +313,17 @@ int gsmi_draw_kimage_f64(const gsmi_op* ops, int nops, const gsmi_leaf* leaves,
  * N * (N/2+1) complex128. */
 int gsmi_irfft_f64(const double* kin, int N, double* out, void* scratch, void* stream);
 
+/* gsmi_irfft_f64 without allocating: the same result, with its N (N/2+1) complex128 scratch and
+ * twiddle tables in a static device array of the library (about 33.7 MB per device, held until
+ * the process ends) instead of a caller buffer.  No counterpart in the reference: it is for
+ * callers that cannot allocate (code under HIP graph capture after a first call on the device,
+ * tight serving loops, harnesses that forbid hipMalloc).  N <= GSMI_NOALLOC_MAX_N, else
+ * GSMI_E_SIZE.  The first call on each device uploads the twiddle tables with synchronous
+ * copies (make it outside any capture).  Calls that overlap on one device race on the shared
+ * array. */
+#define GSMI_NOALLOC_MAX_N 2048
+int gsmi_irfft_f64_noalloc(const double* kin, int N, double* out, void* stream);
+
 /* wrapImage(hermx=true, hermy=false) (src/Image.cpp:577-726; pybind pysrc/Image.cpp:80):
  * folds the (Nk/2+1) x (Nk+1) k-image (rows ky=-Nk/2..Nk/2) onto (N/2+1) x N (rows
  * ky=-N/2..N/2-1).  Both complex128 row-major. */
