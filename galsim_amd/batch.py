@@ -97,6 +97,12 @@ def _program_shape(recs, ops, leaves):
 
 
 FOLD_ITEM_CELLS = 16384       # default size of a fold work item (columns x N), in cells
+LIST_LDS_LIMIT = 163840       # LDS one workgroup may allocate on gfx950 (160 KiB)
+
+
+def lds_line(n):
+    """Length of one transform line in the kernels' padded LDS layout (a slot after every 32)."""
+    return n + n // 32
 ROW_ITEM_CELLS = 1            # default size of a row work item (rows x N/2): one chunk
 STAGE_LDS = 32768             # LDS per fold workgroup allowed for staging (5 workgroups per CU)
 
@@ -130,7 +136,7 @@ def work_lists(recs, groups, cols, rows, elem, bins=None, chunks=None, extra=0,
             raise ValueError("tiles must hold at least one column and one row")
         W = N // 2 + 1
         nk = int(recs['Nk'][first:first + count].max()) if count else 0
-        bufs = 2 * cpg * N * elem + extra
+        bufs = 2 * cpg * lds_line(N) * elem + extra
         staged = bufs + (N + 3 * (nk // 2) + 2) * elem
         nchunk = -(-W // cpg)
         if staged <= STAGE_LDS:
@@ -143,7 +149,7 @@ def work_lists(recs, groups, cols, rows, elem, bins=None, chunks=None, extra=0,
             lds_f = max(lds_f, bufs)
         nitem = -(-nchunk // k)
         per_item = -(-nchunk // nitem) * cpg            # columns per item, a multiple of cpg
-        rbufs = 2 * rpg * (N // 2) * elem
+        rbufs = 2 * rpg * lds_line(N // 2) * elem
         lds_r = max(lds_r, rbufs + N * elem if rbufs + N * elem <= 65536 else rbufs)
         kr = int(row_chunks.get(N, max(1, ROW_ITEM_CELLS // (rpg * (N // 2)))))
         if kr < 1:
@@ -156,9 +162,9 @@ def work_lists(recs, groups, cols, rows, elem, bins=None, chunks=None, extra=0,
             rows_item = -(-nrc // -(-nrc // kr)) * rpg     # rows per item, a multiple of rpg
             for r0 in range(0, ny, rows_item):
                 rw.append((s, r0, min(rows_item, ny - r0), rpg))
-    if max(lds_f, lds_r) > 65536:
-        raise _lib.GalSimAMDError("work-list tiles need %d bytes of LDS (limit 65536)"
-                                  % max(lds_f, lds_r))
+    if max(lds_f, lds_r) > LIST_LDS_LIMIT:
+        raise _lib.GalSimAMDError("work-list tiles need %d bytes of LDS (limit %d)"
+                                  % (max(lds_f, lds_r), LIST_LDS_LIMIT))
     return (np.array(fold, dtype=np.int32).reshape(-1, 4),
             np.array(rw, dtype=np.int32).reshape(-1, 4), lds_f, lds_r)
 
@@ -225,7 +231,7 @@ class BatchPlan:
         self._o_leaves = pk.add(ctx.leaves_array())
         tables = np.concatenate(ctx.tables) if ctx.tables else np.zeros(1)
         self._o_tab = pk.add(tables)
-        self._o_tab32 = pk.add(tables.astype(np.float32))
+        self._o_tab32 = pk.add(ctx.tables32_array())
         clip = np.concatenate(ctx.clip) if ctx.clip else np.zeros(2, dtype=np.int32)
         self._o_clip = pk.add(clip)
         self.recs = recs
@@ -265,8 +271,16 @@ class BatchPlan:
 
     def _fold_extra(self):
         """LDS bytes the specialised fold kernel of the batch's program shape keeps beside its
-        buffers: 4 decoded leaves of 16 values."""
-        return 4 * 16 * (4 if self.precision == 'f32' else 8) if self.program_shape else 0
+        buffers: 4 decoded leaves of 16 values, and in FP32 the cubic coefficients of the
+        largest Sersic table (16 bytes per knot interval)."""
+        if not self.program_shape:
+            return 0
+        extra = 4 * 16 * (4 if self.precision == 'f32' else 8)
+        if self.precision == 'f32':
+            sel = self.host_leaves['type'] == _lib.LEAF_SERSIC
+            if sel.any():
+                extra += 16 * int(self.host_leaves['ntab'][sel].max() - 1)
+        return extra
 
     def _work_lists(self, cols, rows, bins=None, chunks=None):
         """work_lists with the specialised kernel's LDS extra, or, if a tile leaves no room for
@@ -361,8 +375,8 @@ class BatchPlan:
             for f in factors:
                 c = int(self.default_cols(N) * f)
                 # the launch's LDS is the largest size's: keep tiles within the staging budget
-                if c < 1 or c > W or 2 * c * N * elem + self._fold_extra() > \
-                        (65536 if c == 1 else STAGE_LDS):
+                if c < 1 or c > W or 2 * c * lds_line(N) * elem + self._fold_extra() > \
+                        (LIST_LDS_LIMIT if c == 1 else STAGE_LDS):
                     continue
                 sched = self.make_schedule({N: c}, {}, bins=(N,))
                 t = self._time(lambda: self._launch_list(sched, 'fold', s), reps)
@@ -374,7 +388,7 @@ class BatchPlan:
             for f in factors:
                 r = int(self.default_rows(N) * f)
                 if r < 1 or r > int(self.recs['ny'][first:first + count].max()) * 2 or \
-                        2 * r * (N // 2) * elem > 65536:
+                        2 * r * lds_line(N // 2) * elem > LIST_LDS_LIMIT:
                     continue
                 sched = self.make_schedule({N: cols[N]}, {N: r}, bins=(N,))
                 self._launch_list(sched, 'fold', s)

@@ -29,6 +29,7 @@ template <>
 struct Tables<float> {
     const double* tab;
     const float* tab32;
+    const float4* coef;   // Sersic cubic coefficients (LeafI::cf indexes it); see sersic_f32
 };
 
 // TopHat's 2 J1(x) / x (x^2 >= 1e-4), out of line: the Bessel function's code would otherwise
@@ -60,6 +61,8 @@ struct LeafC {
 
 struct LeafI {
     int type, kind, sep, aux, ntab;   // aux: Sersic table offset, or Exponential clip-row offset
+    int cf;                           // Sersic (FP32): index of its first cubic coefficient
+                                      // quadruple in Tables<float>::coef, -1 if none
 };
 
 template <typename T>
@@ -88,6 +91,7 @@ __device__ inline LeafI leaf_ints(const gsmi_leaf& L)
     r.aux = (L.type == GSMI_LEAF_SERSIC || (L.type == GSMI_LEAF_MOFFAT && L.kind == 1))
                 ? L.table : L.cliprows;
     r.ntab = L.ntab;
+    r.cf = (L.type == GSMI_LEAF_SERSIC && L.cliprows >= 0) ? L.cliprows / 4 : -1;
     return r;
 }
 
@@ -246,6 +250,30 @@ __device__ inline float spline_eq32_d(const float* __restrict__ args, int n, flo
     return (aa * vals[i - 1] + bb * vals[i] - (1.f / 6.f) * aa * bb * cc) * ih;
 }
 
+// SersicInfo::kValue (src/SBSersic.cpp:363-376) in FP32 without branches: the high-k asymptote
+// (a + b / k) / k^2 for ksq >= ksq_max, the Taylor series 1 + ksq (d2 + ksq d4) below ksq_min,
+// and otherwise the table's natural cubic spline in ln k over ksq.  The spline is read as the
+// per-interval coefficients c of S = c0 + s (c1 + s (c2 + s c3)), s = (ln k - knot i-1) / spacing
+// in [0, 1] (the same cubic as spline_eq's form; the host derives them in FP64): one 16-byte
+// load per sample.  The interval is Table::lookup's, i = ceil((ln k - a0) / spacing) clamped to
+// [1, n-1] (src/Table.cpp:84-92), and the spline is 0 outside the knots; every index stays in
+// [0, n-2] whatever ksq is (NaN included).
+__device__ __forceinline__ float sersic_f32(const LeafC<float>& C, const float4* __restrict__ cf,
+                                            int ntab, float ksq)
+{
+    const float r = __builtin_amdgcn_rsqf(ksq);
+    const float asym = (C.p[2] + C.p[3] * r) * (r * r);
+    const float tay = 1.f + ksq * (C.p[4] + ksq * C.p[5]);
+    const float lk = 0.34657359027997264f * __builtin_amdgcn_logf(ksq);   // ln(ksq) / 2
+    const float u = fminf(fmaxf((lk - C.t0) * C.t2, 0.f), float(ntab - 1));
+    const int i = max(int(ceilf(u)), 1);
+    const float s = u - float(i - 1);
+    const float4 c = cf[i - 1];
+    const float S = (lk < C.t0 || lk > C.t1) ? 0.f : c.x + s * (c.y + s * (c.z + s * c.w));
+    const float spl = S * __builtin_amdgcn_rcpf(ksq);
+    return ksq >= C.p[1] ? asym : (ksq < C.p[0] ? tay : spl);
+}
+
 // Exponential clipping: FP64 uses the row ranges the host computed as GetKValueRange1d/2d
 // (src/SBProfile.cpp:508-588) for the stored k-image rows; FP32 clips at ksq > ksq_max.
 template <typename T>
@@ -264,6 +292,13 @@ __device__ inline bool exp_clip(const LeafC<T>& C, const LeafI& I, const Tables<
 template <typename T>
 __device__ inline void leaf_kprime(const LeafC<T>& C, const LeafI& I, T kx, T ky, T* kxp, T* kyp)
 {
+    if constexpr (sizeof(T) == 4) {
+        // a separable leaf's record holds exact zeros off the diagonal, so the general form
+        // gives the same products without a per-leaf select
+        *kxp = C.j0 * kx + C.j2 * ky;
+        *kyp = C.j1 * kx + C.j3 * ky;
+        return;
+    }
     if (I.sep) {
         *kxp = C.j0 * kx;
         *kyp = C.j3 * ky;
@@ -447,6 +482,7 @@ __device__ inline T eval_leafc(const LeafC<T>& C, const LeafI& I, const gsmi_lea
         // SersicInfo::kValue (src/SBSersic.cpp:363-376)
         T ksq = X * X + Y * Y;
         if constexpr (sizeof(T) == 4) {
+            if (I.cf >= 0) return flux * sersic_f32(C, tb.coef + I.cf, I.ntab, ksq);
             if (ksq >= C.p[1]) {
                 const float r = __builtin_amdgcn_rsqf(ksq);
                 return flux * (C.p[2] + C.p[3] * r) * (r * r);
@@ -601,7 +637,7 @@ __device__ inline bool decode_program(const gsmi_op* __restrict__ ops, int nops,
         d.opc[o] = (o < nops) ? (ops[o].code | (ops[o].arg << 8)) : 0;
 #pragma unroll
     for (int l = 0; l < kProgLeaves; ++l)
-        d.li[l] = (l < nl) ? leaf_ints(leaves[l]) : LeafI{0, 0, 0, -1, 0};
+        d.li[l] = (l < nl) ? leaf_ints(leaves[l]) : LeafI{0, 0, 0, -1, 0, -1};
     if (int(threadIdx.x) < nl) lc[threadIdx.x] = leaf_consts_t<T>(leaves[threadIdx.x], tb);
     return true;
 }
@@ -727,6 +763,15 @@ constexpr StaticProgram kStaticPrograms[] = {
 };
 constexpr int kNumStaticPrograms = 5;
 
+// The Sersic leaf slot of program shape SIG, or -1.
+template <int SIG>
+constexpr int sersic_slot()
+{
+    for (int l = 0; l < kStaticPrograms[SIG].nl; ++l)
+        if (kStaticPrograms[SIG].type[l] == GSMI_LEAF_SERSIC) return l;
+    return -1;
+}
+
 // Does the stamp's program (ops, leaves) have shape SIG?
 template <int SIG>
 __device__ inline bool matches_static(const gsmi_op* __restrict__ ops, int nops,
@@ -780,6 +825,16 @@ __device__ __forceinline__ T eval_static(const LeafC<T>* __restrict__ lc,
     for (int l = 0; l < P.nl; ++l) {
         LeafI I = li[l];
         I.type = P.type[l];
+        if constexpr (sizeof(T) == 4) {
+            if (P.type[l] == GSMI_LEAF_SERSIC) {
+                // specialised FP32 kernels stage the coefficients at tb.coef (LeafI::cf = 0)
+                T kxp, kyp;
+                leaf_kprime(lc[l], I, kx, ky, &kxp, &kyp);
+                const T X = kxp * lc[l].s, Y = kyp * lc[l].s;
+                v[l] = lc[l].flux * sersic_f32(lc[l], tb.coef, I.ntab, X * X + Y * Y);
+                continue;
+            }
+        }
         v[l] = eval_leafc<T>(lc[l], I, leaves + l, tb, kx, ky, i, j, Nk);
     }
     return static_ops<T, SIG>(v, -1, T(0));
@@ -980,10 +1035,37 @@ __device__ inline void bfly5(cplx<T>* v)
     v[3] = {a2.x + b2.y, a2.y - b2.x};
 }
 
-template <typename T, int R>
+// LDS layout of the padded transforms: element i of a line at lds_pad(i), one slot of padding
+// after every 32 elements (256 bytes of FP32 complex values, one sweep of the 64 four-byte
+// banks), lines lds_line(n) apart.  The Stockham writes of stride R (and the multi-line reads)
+// then fall on different banks instead of piling onto a few.
+__host__ __device__ constexpr int lds_pad(int i) { return i + (i >> 5); }
+__host__ __device__ constexpr int lds_line(int n) { return n + (n >> 5); }
+
+template <bool PAD>
+__device__ __forceinline__ int lpos(int i)
+{
+    if constexpr (PAD) return lds_pad(i);
+    return i;
+}
+
+template <int N>
+constexpr bool kPow2 = N > 0 && (N & (N - 1)) == 0;
+
+// Padded offset of element b + r S from element b, for powers of two S and b a multiple of
+// min(S, 32) plus less than S (so b + r S crosses 32-runs exactly where r S does):
+// r S + (r S) / 32.
+template <int S>
+__device__ __forceinline__ constexpr int pad_step(int r)
+{
+    return r * S + ((r * S) >> 5);
+}
+
+template <typename T, int R, bool PAD = false>
 __device__ __forceinline__ void stockham_pass(const cplx<T>* src, cplx<T>* dst, int n, int nseg, int Ns,
                                      const cplx<T>* tw, int twstride)
 {
+    const int line = PAD ? lds_line(n) : n;
     const int nb = n / R;
     const int tstep = (n / (Ns * R)) * twstride;
     const float inv_nb = 1.f / float(nb);
@@ -993,10 +1075,10 @@ __device__ __forceinline__ void stockham_pass(const cplx<T>* src, cplx<T>* dst, 
         const int tt = t - seg * nb;
         const int tq = fdiv(tt, inv_ns);
         const int jj = tt - tq * Ns;
-        const cplx<T>* s = src + seg * n + tt;
+        const cplx<T>* s = src + seg * line;
         cplx<T> v[R];
 #pragma unroll
-        for (int r = 0; r < R; ++r) v[r] = s[r * nb];
+        for (int r = 0; r < R; ++r) v[r] = s[lpos<PAD>(tt + r * nb)];
         if (Ns > 1) {
 #pragma unroll
             for (int r = 1; r < R; ++r) v[r] = cmul(v[r], tw[jj * r * tstep]);
@@ -1006,13 +1088,14 @@ __device__ __forceinline__ void stockham_pass(const cplx<T>* src, cplx<T>* dst, 
         if (R == 4) bfly4(v);
         if (R == 5) bfly5(v);
         if (R == 8) bfly8(v);
-        cplx<T>* d = dst + seg * n + tq * Ns * R + jj;
+        cplx<T>* d = dst + seg * line;
+        const int d0 = tq * Ns * R + jj;
 #pragma unroll
-        for (int r = 0; r < R; ++r) d[r * Ns] = v[r];
+        for (int r = 0; r < R; ++r) d[lpos<PAD>(d0 + r * Ns)] = v[r];
     }
 }
 
-template <typename T>
+template <typename T, bool PAD = false>
 __device__ __forceinline__ cplx<T>* fft_lds(cplx<T>* a, cplx<T>* b, int n, int nseg, const cplx<T>* tw,
                             int twstride)
 {
@@ -1022,13 +1105,13 @@ __device__ __forceinline__ cplx<T>* fft_lds(cplx<T>* a, cplx<T>* b, int n, int n
     cplx<T>* dst = b;
     while (rem > 1) {
         if (rem % 3 == 0) {
-            stockham_pass<T, 3>(src, dst, n, nseg, Ns, tw, twstride);
+            stockham_pass<T, 3, PAD>(src, dst, n, nseg, Ns, tw, twstride);
             Ns *= 3; rem /= 3;
         } else if (rem % 4 == 0) {
-            stockham_pass<T, 4>(src, dst, n, nseg, Ns, tw, twstride);
+            stockham_pass<T, 4, PAD>(src, dst, n, nseg, Ns, tw, twstride);
             Ns *= 4; rem /= 4;
         } else {
-            stockham_pass<T, 2>(src, dst, n, nseg, Ns, tw, twstride);
+            stockham_pass<T, 2, PAD>(src, dst, n, nseg, Ns, tw, twstride);
             Ns *= 2; rem /= 2;
         }
         __syncthreads();
@@ -1041,10 +1124,11 @@ __device__ __forceinline__ cplx<T>* fft_lds(cplx<T>* a, cplx<T>* b, int n, int n
 // each pass become multiplications by constants, the twiddles of a butterfly are loaded together
 // before they are used, and radix-8 passes (radix 3 first, then 8, 4, 2) cut the number of LDS
 // round trips (128 = 8 8 2, 192 = 3 8 8, 256 = 8 8 4, 512 = 8 8 8).
-template <typename T, int N, int R, int NS>
+template <typename T, int N, int R, int NS, bool PAD = false>
 __device__ __forceinline__ void stockham_pass_c(const cplx<T>* src, cplx<T>* dst, int nseg,
                                                 const cplx<T>* tw, int twstride)
 {
+    constexpr unsigned line = PAD ? lds_line(N) : N;
     constexpr unsigned nb = N / R;
     const int tstep = (N / (NS * R)) * twstride;
     const unsigned total = unsigned(nseg) * nb;
@@ -1053,10 +1137,18 @@ __device__ __forceinline__ void stockham_pass_c(const cplx<T>* src, cplx<T>* dst
         const unsigned tt = t - seg * nb;
         const unsigned tq = tt / NS;
         const unsigned jj = tt - tq * NS;
-        const cplx<T>* s = src + seg * N + tt;
+        const cplx<T>* s = src + seg * line;
         cplx<T> v[R];
+        if constexpr (PAD && kPow2<N>) {
+            // tt < nb and both powers of two: element tt + r nb sits at lds_pad(tt) plus a
+            // compile-time offset (a multiple of 32 when nb >= 32, within one 32-run otherwise)
+            const cplx<T>* s0 = s + lds_pad(int(tt));
 #pragma unroll
-        for (int r = 0; r < R; ++r) v[r] = s[r * nb];
+            for (int r = 0; r < R; ++r) v[r] = s0[pad_step<nb>(r)];
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[r] = s[lpos<PAD>(int(tt + r * nb))];
+        }
         if constexpr (NS > 1) {
             cplx<T> w[R];
 #pragma unroll
@@ -1068,13 +1160,22 @@ __device__ __forceinline__ void stockham_pass_c(const cplx<T>* src, cplx<T>* dst
         if constexpr (R == 3) bfly3(v);
         if constexpr (R == 4) bfly4(v);
         if constexpr (R == 8) bfly8(v);
-        cplx<T>* d = dst + seg * N + tq * (NS * R) + jj;
+        cplx<T>* d = dst + seg * line;
+        const unsigned d0 = tq * (NS * R) + jj;
+        if constexpr (PAD && kPow2<N>) {
+            // d0 = tq NS R + jj, jj < NS: element d0 + r NS sits at lds_pad(d0) plus
+            // pad_step<NS>(r) (the block tq NS R never straddles a 32-run differently per r)
+            cplx<T>* d1 = d + lds_pad(int(d0));
 #pragma unroll
-        for (int r = 0; r < R; ++r) d[r * NS] = v[r];
+            for (int r = 0; r < R; ++r) d1[pad_step<NS>(r)] = v[r];
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; ++r) d[lpos<PAD>(int(d0 + r * NS))] = v[r];
+        }
     }
 }
 
-template <typename T, int N, int NS>
+template <typename T, int N, int NS, bool PAD = false>
 __device__ __forceinline__ cplx<T>* fft_fixed(cplx<T>* src, cplx<T>* dst, int nseg,
                                               const cplx<T>* tw, int twstride)
 {
@@ -1083,27 +1184,28 @@ __device__ __forceinline__ cplx<T>* fft_fixed(cplx<T>* src, cplx<T>* dst, int ns
         return src;
     } else {
         constexpr int R = (rem % 3 == 0) ? 3 : (rem % 8 == 0) ? 8 : (rem % 4 == 0) ? 4 : 2;
-        stockham_pass_c<T, N, R, NS>(src, dst, nseg, tw, twstride);
+        stockham_pass_c<T, N, R, NS, PAD>(src, dst, nseg, tw, twstride);
         __syncthreads();
-        return fft_fixed<T, N, NS * R>(dst, src, nseg, tw, twstride);
+        return fft_fixed<T, N, NS * R, PAD>(dst, src, nseg, tw, twstride);
     }
 }
 
-// fft_lds with the common lengths compiled for their size.
-template <typename T>
+// fft_lds with the common lengths compiled for their size.  PAD: lines in the padded layout
+// (lds_pad / lds_line) instead of n apart.
+template <typename T, bool PAD = false>
 __device__ __forceinline__ cplx<T>* fft_lds_c(cplx<T>* a, cplx<T>* b, int n, int nseg,
                                               const cplx<T>* tw, int twstride)
 {
     switch (n) {
-      case 64: return fft_fixed<T, 64, 1>(a, b, nseg, tw, twstride);
-      case 96: return fft_fixed<T, 96, 1>(a, b, nseg, tw, twstride);
-      case 128: return fft_fixed<T, 128, 1>(a, b, nseg, tw, twstride);
-      case 192: return fft_fixed<T, 192, 1>(a, b, nseg, tw, twstride);
-      case 256: return fft_fixed<T, 256, 1>(a, b, nseg, tw, twstride);
-      case 384: return fft_fixed<T, 384, 1>(a, b, nseg, tw, twstride);
-      case 512: return fft_fixed<T, 512, 1>(a, b, nseg, tw, twstride);
-      case 768: return fft_fixed<T, 768, 1>(a, b, nseg, tw, twstride);
-      default: return fft_lds<T>(a, b, n, nseg, tw, twstride);
+      case 64: return fft_fixed<T, 64, 1, PAD>(a, b, nseg, tw, twstride);
+      case 96: return fft_fixed<T, 96, 1, PAD>(a, b, nseg, tw, twstride);
+      case 128: return fft_fixed<T, 128, 1, PAD>(a, b, nseg, tw, twstride);
+      case 192: return fft_fixed<T, 192, 1, PAD>(a, b, nseg, tw, twstride);
+      case 256: return fft_fixed<T, 256, 1, PAD>(a, b, nseg, tw, twstride);
+      case 384: return fft_fixed<T, 384, 1, PAD>(a, b, nseg, tw, twstride);
+      case 512: return fft_fixed<T, 512, 1, PAD>(a, b, nseg, tw, twstride);
+      case 768: return fft_fixed<T, 768, 1, PAD>(a, b, nseg, tw, twstride);
+      default: return fft_lds<T, PAD>(a, b, n, nseg, tw, twstride);
     }
 }
 

@@ -701,7 +701,7 @@ int gsmi_adjoint_f32(const gsmi_stamp* stamps, int nstamps, int N, int max_ny, i
         set_error("gsmi_adjoint_f32: null buffer or empty stamp");
         return GSMI_E_ARG;
     }
-    Tables<float> tb{tables, tables32};
+    Tables<float> tb{tables, tables32, reinterpret_cast<const float4*>(tables32)};
     return adjoint_impl<float>(stamps, nstamps, N, max_ny, max_leaves, flags, ops,
                                leaves, tb, resid,
                                scratch, partials, grad, stream);

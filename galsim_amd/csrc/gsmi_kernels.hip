@@ -48,9 +48,90 @@ axis_factors(const gsmi_stamp* __restrict__ stamps, const gsmi_op* __restrict__ 
     }
 }
 
-// Every target cell (c, r) of the group's columns: the sum over the k-image samples congruent to
-// it modulo N (the stored sample, conjugated where it stands for kx < 0) of program x axis
-// factors X(kx) Y(ky), written to bufa.
+// The wrap: target cell (c, r) of the wrapped N-grid (c = 0..N/2, r = 0..N-1) is the sum of
+// every k-image sample (kx, ky) congruent to it modulo N with |kx|, |ky| <= hk = Nk/2 (the stored
+// sample, conjugated where it stands for kx < 0) of program x axis factors X(kx) Y(ky).  Every
+// cell has the primary sample (c, kyp), kyp = r < N/2 ? r : r - N (Nk >= N); only the fringe has
+// others: columns c >= a and rows |kyp| >= a, a = N - hk (every cell when a <= 0).  So the cells
+// are filled in two passes: one primary sample per cell, straight-line, then the fringe cells,
+// listed compactly (every column of the chunk x the fringe rows, then the fringe columns x the
+// other rows), each adding its other samples.
+
+// Row of fringe-row index f (0 <= f < fringe_rows): kyp = a.. N/2-1, then -N/2 .. -a.
+__device__ __forceinline__ int fringe_row(int f, int a, int N)
+{
+    if (a <= 0) return f;
+    const int up = N / 2 - a;
+    const int kyp = f < up ? a + f : -N / 2 + (f - up);
+    return kyp < 0 ? kyp + N : kyp;
+}
+
+// Row of non-fringe-row index g (0 <= g < 2a - 1): kyp = -(a-1) .. a-1.
+__device__ __forceinline__ int inner_row(int g, int a, int N)
+{
+    const int kyp = g - (a - 1);
+    return kyp < 0 ? kyp + N : kyp;
+}
+
+// Fill cells of columns [c0, c0 + ncol) into bufa (column-major in the padded transform layout:
+// cell (cl, r) at cl * lds_line(N) + lds_pad(r)).
+// value(i, j, neg) is the contribution of the stored sample (i, j) = (|kx|, kx < 0 ? -ky : ky),
+// conjugated by the caller when neg.  Ends with a barrier.
+template <typename T, typename V>
+__device__ __forceinline__ void fold_cells_split(V&& value, int N, int Nk, int c0, int ncol,
+                                                 cplx<T>* bufa)
+{
+    const int hk = Nk / 2;
+    const int a = N - hk;
+    const int LN = lds_line(N);
+    const float inv_n = 1.f / float(N);
+    for (int e = threadIdx.x; e < ncol * N; e += blockDim.x) {
+        const int cl = fdiv(e, inv_n);
+        const int r = e - cl * N;
+        const int kyp = r < N / 2 ? r : r - N;
+        bufa[cl * LN + lds_pad(r)] = value(c0 + cl, kyp);
+    }
+    __syncthreads();
+    const int nfr = a <= 0 ? N : N - 2 * a + 1;
+    const int cfirst = max(c0, a);
+    const int ncf = (a <= 0) ? 0 : max(0, c0 + ncol - cfirst);
+    const int ninner = 2 * a - 1;
+    const int part1 = ncol * nfr;
+    const int total = part1 + ncf * ninner;
+    const float inv_nfr = 1.f / float(nfr);
+    const float inv_inner = 1.f / float(max(ninner, 1));
+    for (int e = threadIdx.x; e < total; e += blockDim.x) {
+        int cl, r;
+        if (e < part1) {
+            cl = fdiv(e, inv_nfr);
+            r = fringe_row(e - cl * nfr, a, N);
+        } else {
+            const int q = e - part1;
+            const int k = fdiv(q, inv_inner);
+            cl = cfirst - c0 + k;
+            r = inner_row(q - k * ninner, a, N);
+        }
+        const int c = c0 + cl;
+        const int kyp = r < N / 2 ? r : r - N;
+        const int kx0 = c - N * fdiv(c + hk, inv_n);
+        const int ky0 = r - N * fdiv(r + hk, inv_n);
+        cplx<T> acc = {T(0), T(0)};
+        for (int kx = kx0; kx <= hk; kx += N) {
+            for (int ky = ky0; ky <= hk; ky += N) {
+                if (kx == c && ky == kyp) continue;          // the primary, already in bufa
+                const bool neg = kx < 0;
+                const cplx<T> v = value(neg ? -kx : kx, neg ? -ky : ky);
+                acc.x += v.x;
+                acc.y += neg ? -v.y : v.y;
+            }
+        }
+        cplx<T>& cell = bufa[cl * LN + lds_pad(r)];
+        cell.x += acc.x;
+        cell.y += acc.y;
+    }
+    __syncthreads();
+}
+
 template <typename T>
 __device__ __forceinline__ void fold_cells(const gsmi_op* __restrict__ sops, int nops,
                                            const gsmi_leaf* __restrict__ sleaves,
@@ -58,29 +139,11 @@ __device__ __forceinline__ void fold_cells(const gsmi_op* __restrict__ sops, int
                                            const cplx<T>* __restrict__ Y, T dk, int N, int Nk,
                                            int c0, int ncol, cplx<T>* bufa)
 {
-    const int hk = Nk / 2;
-    const float inv_n = 1.f / float(N);
-    for (int e = threadIdx.x; e < ncol * N; e += blockDim.x) {
-        const int cl = fdiv(e, inv_n);
-        const int r = e - cl * N;
-        const int c = c0 + cl;
-        cplx<T> acc = {T(0), T(0)};
-        const int kx0 = c - N * fdiv(c + hk, inv_n);
-        const int ky0 = r - N * fdiv(r + hk, inv_n);
-        for (int kx = kx0; kx <= hk; kx += N) {
-            for (int ky = ky0; ky <= hk; ky += N) {
-                // the stored sample (kx >= 0); its conjugate stands for (kx, ky) when kx < 0
-                const bool neg = kx < 0;
-                const int i = neg ? -kx : kx;
-                const int j = neg ? -ky : ky;
-                cplx<T> v = eval_program<T>(sops, nops, sleaves, tb, dk, i, j, Nk);
-                v = cmul(v, cmul(X[i], Y[j]));
-                acc.x += v.x;
-                acc.y += neg ? -v.y : v.y;
-            }
-        }
-        bufa[cl * N + r] = acc;
-    }
+    auto value = [&](int i, int j) -> cplx<T> {
+        const cplx<T> v = eval_program<T>(sops, nops, sleaves, tb, dk, i, j, Nk);
+        return cmul(v, cmul(X[i], Y[j]));
+    };
+    fold_cells_split<T>(value, N, Nk, c0, ncol, bufa);
 }
 
 // fold_cells for a stamp whose program has the compile-time shape SIG (leaf constants in lc).
@@ -93,37 +156,20 @@ __device__ __forceinline__ void fold_cells_static(const LeafC<T>* __restrict__ l
                                                   const cplx<T>* __restrict__ Y, T dk, int N,
                                                   int Nk, int c0, int ncol, cplx<T>* bufa)
 {
-    const int hk = Nk / 2;
-    const float inv_n = 1.f / float(N);
     LeafC<T> lr[kProgLeaves];
 #pragma unroll
     for (int l = 0; l < kStaticPrograms[SIG].nl; ++l) lr[l] = uniform_leaf(lc[l]);
-    for (int e = threadIdx.x; e < ncol * N; e += blockDim.x) {
-        const int cl = fdiv(e, inv_n);
-        const int r = e - cl * N;
-        const int c = c0 + cl;
-        cplx<T> acc = {T(0), T(0)};
-        const int kx0 = c - N * fdiv(c + hk, inv_n);
-        const int ky0 = r - N * fdiv(r + hk, inv_n);
-        for (int kx = kx0; kx <= hk; kx += N) {
-            for (int ky = ky0; ky <= hk; ky += N) {
-                const bool neg = kx < 0;
-                const int i = neg ? -kx : kx;
-                const int j = neg ? -ky : ky;
-                const T P = eval_static<T, SIG>(lr, li, sleaves, tb, T(i) * dk, T(j) * dk, i, j,
-                                                Nk);
-                const cplx<T> xy = cmul(X[i], Y[j]);
-                acc.x += P * xy.x;
-                acc.y += neg ? -(P * xy.y) : P * xy.y;
-            }
-        }
-        bufa[cl * N + r] = acc;
-    }
+    auto value = [&](int i, int j) -> cplx<T> {
+        const T P = eval_static<T, SIG>(lr, li, sleaves, tb, T(i) * dk, T(j) * dk, i, j, Nk);
+        const cplx<T> xy = cmul(X[i], Y[j]);
+        return {P * xy.x, P * xy.y};
+    };
+    fold_cells_split<T>(value, N, Nk, c0, ncol, bufa);
 }
 
 // ---------------------------------------------------------------------------------------------
 // Kernel 1: fused k-space evaluation + product/sinc/phase + Hermitian wrap + column inverse DFT
-// grid: (ceil((N/2+1)/cpg), nstamps); dynamic LDS: 2 * cpg * N complex
+// grid: (ceil((N/2+1)/cpg), nstamps); dynamic LDS: 2 * cpg * lds_line(N) complex
 // ---------------------------------------------------------------------------------------------
 
 // Columns [first, first + count) of one stamp in chunks of at most cpg: each chunk's cells are
@@ -160,7 +206,6 @@ __device__ __forceinline__ void fold_chunks(const gsmi_stamp* __restrict__ st, i
     const int hk = Nk / 2;
     const T dk = T(st->dk);
     cplx<T>* out = scratch + st->scratch_offset;
-    const int W = N / 2 + 1;
     const int ny = st->ny;
     const int y0 = st->y0;
     const float inv_n = 1.f / float(N);
@@ -174,16 +219,18 @@ __device__ __forceinline__ void fold_chunks(const gsmi_stamp* __restrict__ st, i
                                  bufa);
         else
             fold_cells<T>(sops, nops, sleaves, tb, X, X + 2 * hk + 1, dk, N, Nk, c0, ncol, bufa);
-        __syncthreads();
-        cplx<T>* res = fft_lds_c<T>(bufa, bufb, N, ncol, twp, twstride);
-        // rows of the output window: position p <-> DFT index p mod N
-        const float inv_ncol = 1.f / float(ncol);
+        cplx<T>* res = fft_lds_c<T, true>(bufa, bufb, N, ncol, twp, twstride);
+        // rows of the output window (position p <-> DFT index p mod N), column-major: the
+        // chunk's columns are one contiguous run of ncol * ny values in scratch
+        const int LN = lds_line(N);
+        const float inv_ny = 1.f / float(ny);
+        cplx<T>* oc = out + size_t(c0) * ny;
         for (int e = threadIdx.x; e < ncol * ny; e += blockDim.x) {
-            const int row = fdiv(e, inv_ncol);
-            const int cl = e - row * ncol;
+            const int cl = fdiv(e, inv_ny);
+            const int row = e - cl * ny;
             const int p = y0 + row + N;               // >= 0: y0 >= -N/2
             const int y = p - N * fdiv(p, inv_n);
-            out[row * W + c0 + cl] = res[cl * N + y];
+            oc[e] = res[cl * LN + lds_pad(y)];
         }
         __syncthreads();                          // bufa / bufb are reused by the next chunk
     }
@@ -208,10 +255,11 @@ __device__ __forceinline__ void fold_item(const gsmi_stamp* __restrict__ st, int
     const gsmi_leaf* sleaves = leaves + st->leaf_begin;
     const int hk = st->Nk / 2;
     const cplx<T>* Xg = scratch + st->axis_offset;    // X[kx], kx in [0, hk]; then Y[ky]
+    const int LN = lds_line(N);
     cplx<T>* bufa = lds;
-    cplx<T>* bufb = lds + cpg * N;
+    cplx<T>* bufb = lds + cpg * LN;
     const int naxis = 3 * hk + 2;
-    size_t used = size_t(2) * cpg * N;
+    size_t used = size_t(2) * cpg * LN;
     LeafC<T>* lc = reinterpret_cast<LeafC<T>*>(lds + used);
     LeafI li[kProgLeaves];
     bool specialised = false;
@@ -227,6 +275,28 @@ __device__ __forceinline__ void fold_item(const gsmi_stamp* __restrict__ st, int
                 lc[threadIdx.x] = leaf_consts_t<T>(sleaves[threadIdx.x], tb);
         }
     }
+    // FP32 programs with a Sersic leaf read its cubic coefficients from LDS: staged here, and a
+    // stamp whose coefficients do not fit takes the interpreter
+    // (the specialised path's coefficient pointer is the LDS address whether or not a stamp
+    // uses it, so that its loads compile to LDS reads)
+    Tables<T> tbs = tb;
+    if constexpr (sizeof(T) == 4 && sersic_slot<SIG>() >= 0) {
+        float4* cfl = reinterpret_cast<float4*>(lds + used);
+        tbs.coef = cfl;
+        if (specialised) {
+            constexpr int ss = sersic_slot<SIG>();
+            const int ncf = sleaves[ss].ntab - 1;
+            const int cfo = sleaves[ss].cliprows;
+            if (cfo < 0 || ncf < 1 || used + 2 * size_t(ncf) > lds_elems) {
+                specialised = false;
+            } else {
+                const float4* src = tb.coef + cfo / 4;
+                for (int e = threadIdx.x; e < ncf; e += blockDim.x) cfl[e] = src[e];
+                li[ss].cf = 0;
+                used += 2 * size_t(ncf);
+            }
+        }
+    }
     cplx<T>* twl = lds + used;
     cplx<T>* axl = twl + N;
     if (used + N + naxis <= lds_elems) {
@@ -235,7 +305,7 @@ __device__ __forceinline__ void fold_item(const gsmi_stamp* __restrict__ st, int
         __syncthreads();
         if constexpr (SIG > 0)
             if (specialised) {
-                fold_chunks<T, SIG, true>(st, first, count, cpg, lc, li, sops, nops, sleaves, tb,
+                fold_chunks<T, SIG, true>(st, first, count, cpg, lc, li, sops, nops, sleaves, tbs,
                                           axl, twl, 1, scratch, bufa, bufb);
                 return;
             }
@@ -243,12 +313,12 @@ __device__ __forceinline__ void fold_item(const gsmi_stamp* __restrict__ st, int
                                    twl, 1, scratch, bufa, bufb);
         return;
     }
-    if (count <= cpg && naxis <= cpg * N) {
+    if (count <= cpg && naxis <= cpg * LN) {
         for (int e = threadIdx.x; e < naxis; e += blockDim.x) bufb[e] = Xg[e];
         __syncthreads();
         if constexpr (SIG > 0)
             if (specialised) {
-                fold_chunks<T, SIG, true>(st, first, count, cpg, lc, li, sops, nops, sleaves, tb,
+                fold_chunks<T, SIG, true>(st, first, count, cpg, lc, li, sops, nops, sleaves, tbs,
                                           bufb, tw, tws, scratch, bufa, bufb);
                 return;
             }
@@ -259,7 +329,7 @@ __device__ __forceinline__ void fold_item(const gsmi_stamp* __restrict__ st, int
     __syncthreads();
     if constexpr (SIG > 0)
         if (specialised) {
-            fold_chunks<T, SIG, true>(st, first, count, cpg, lc, li, sops, nops, sleaves, tb, Xg,
+            fold_chunks<T, SIG, true>(st, first, count, cpg, lc, li, sops, nops, sleaves, tbs, Xg,
                                       tw, tws, scratch, bufa, bufb);
             return;
         }
@@ -283,7 +353,7 @@ __device__ __forceinline__ void fold_colfft_body(const gsmi_stamp* __restrict__ 
     const int ncol = min(cpg, N / 2 + 1 - c0);
     if (ncol <= 0) return;
     fold_item<T>(st, c0, ncol, cpg, ops, leaves, tb, tw, 1, scratch,
-                 reinterpret_cast<cplx<T>*>(smem_raw), size_t(2) * cpg * N);
+                 reinterpret_cast<cplx<T>*>(smem_raw), size_t(2) * cpg * lds_line(N));
 }
 
 // LDS a work-list fold launch needs beyond its two transform buffers (the decoded leaf constants
@@ -313,7 +383,8 @@ __device__ __forceinline__ void fold_list_body(const gsmi_stamp* __restrict__ st
         const int N = st->N;
         if (it.count <= 0 || it.first < 0 || it.first + it.count > N / 2 + 1 ||
             it.per_group <= 0 ||
-            size_t(2) * it.per_group * N * sizeof(cplx<T>) + (SIG > 0 ? kFoldLdsExtra<T> : 0) >
+            size_t(2) * it.per_group * lds_line(N) * sizeof(cplx<T>) +
+                    (SIG > 0 ? kFoldLdsExtra<T> : 0) >
                 size_t(lds_bytes))
             continue;
         int stride;
@@ -385,29 +456,40 @@ struct FoldKernel<double> {
 // extension) is formed as v = IDFT_{N/2}(E + i O), x(2m) = Re v(m), x(2m+1) = Im v(m), with
 // E(c) = Z(c) + conj(Z(N/2-c)) and O(c) = (Z(c) - conj(Z(N/2-c))) e^{2 pi i c / N}.
 // ---------------------------------------------------------------------------------------------
+// Row rl, column c of the input is in[rl * rs + c * cs]: column-major scratch of the drawing
+// kernels (rs = 1, cs = the stamp's rows; threads take consecutive rows of a column, so a wave
+// reads whole cache lines), or row-major (cs = 1; threads take consecutive columns).
 template <typename T>
-__device__ __forceinline__ void row_c2r_body(cplx<T>* lds, int rpg, const cplx<T>* __restrict__ in, int nrow,
-                             int N, int x0, int nx, const cplx<T>* __restrict__ tw,
-                             T* __restrict__ out, int tws = 1)
+__device__ __forceinline__ void row_c2r_body(cplx<T>* lds, int rpg, const cplx<T>* __restrict__ in,
+                                             int rs, int cs, int nrow, int N, int x0, int nx,
+                                             const cplx<T>* __restrict__ tw,
+                                             T* __restrict__ out, int tws = 1)
 {
     const int n2 = N / 2;
-    const int W = n2 + 1;
+    const int L2 = lds_line(n2);
     cplx<T>* bufa = lds;
-    cplx<T>* bufb = lds + rpg * n2;
+    cplx<T>* bufb = lds + rpg * L2;
     const float inv_n2 = 1.f / float(n2);
+    const float inv_nrow = 1.f / float(nrow);
     for (int e = threadIdx.x; e < nrow * n2; e += blockDim.x) {
-        const int rl = fdiv(e, inv_n2);
-        const int c = e - rl * n2;
-        const cplx<T>* z = in + size_t(rl) * W;
-        cplx<T> zc = z[c];
-        cplx<T> zm = z[n2 - c];
+        int rl, c;
+        if (rs == 1) {
+            c = fdiv(e, inv_nrow);
+            rl = e - c * nrow;
+        } else {
+            rl = fdiv(e, inv_n2);
+            c = e - rl * n2;
+        }
+        const cplx<T>* z = in + size_t(rl) * rs;
+        cplx<T> zc = z[size_t(c) * cs];
+        cplx<T> zm = z[size_t(n2 - c) * cs];
         if (c == 0) { zc.y = T(0); zm.y = T(0); }    // DC and Nyquist are real
         cplx<T> E = cadd(zc, cconj(zm));
         cplx<T> O = cmul(csub(zc, cconj(zm)), tw[c * tws]);
-        bufa[rl * n2 + c] = {E.x - O.y, E.y + O.x};  // E + i O
+        bufa[rl * L2 + lds_pad(c)] = {E.x - O.y, E.y + O.x};  // E + i O
     }
     __syncthreads();
-    cplx<T>* res = fft_lds_c<T>(bufa, bufb, n2, nrow, tw, 2 * tws);
+    cplx<T>* res = fft_lds_c<T, true>(bufa, bufb, n2, nrow, tw, 2 * tws);
     const T norm = T(1) / (T(N) * T(N));
     const float inv_nx = 1.f / float(nx);
     const float inv_n = 1.f / float(N);
@@ -416,8 +498,175 @@ __device__ __forceinline__ void row_c2r_body(cplx<T>* lds, int rpg, const cplx<T
         const int col = e - rl * nx;
         const int p = x0 + col + N;                   // >= 0: x0 >= -N/2
         const int q = p - N * fdiv(p, inv_n);
-        const cplx<T> v = res[rl * n2 + (q >> 1)];
+        const cplx<T> v = res[rl * L2 + lds_pad(q >> 1)];
         out[size_t(rl) * nx + col] = ((q & 1) ? v.y : v.x) * norm;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// The row transform with N a compile-time constant (the common FFT sizes).  The half-length
+// transform of M = N/2 points runs as Stockham radix passes (3 first, then 8, 4, 2) over rows of
+// the chunk held in LDS lines M + 1 apart (an odd stride: consecutive rows fall on different
+// banks).  The first pass forms E + iO straight from the column-major scratch into registers
+// (threads take consecutive rows of a column: whole cache lines), the middle passes go LDS to
+// LDS (threads again take consecutive rows: no bank conflicts), and the last pass writes the
+// cropped, 1/N^2-scaled pixels straight to the output (threads take consecutive outputs of a
+// row: contiguous stores).  twl: exp(2 pi i k / N), k < N, in LDS.
+// ---------------------------------------------------------------------------------------------
+constexpr int row_radix(int rem)
+{
+    return (rem % 3 == 0) ? 3 : (rem % 8 == 0) ? 8 : (rem % 4 == 0) ? 4 : 2;
+}
+
+template <int R, typename T>
+__device__ __forceinline__ void bfly(cplx<T>* v)
+{
+    if constexpr (R == 2) bfly2(v);
+    if constexpr (R == 3) bfly3(v);
+    if constexpr (R == 4) bfly4(v);
+    if constexpr (R == 8) bfly8(v);
+}
+
+template <typename T, int N, int R>
+__device__ __forceinline__ void rows_first(const cplx<T>* __restrict__ in, int cs, int nrow,
+                                           float inv_nrow, const cplx<T>* twl, cplx<T>* dst)
+{
+    constexpr int M = N / 2, L = M + 1, NB = M / R;
+    const int total = nrow * NB;
+    for (int t = threadIdx.x; t < total; t += blockDim.x) {
+        const int b = fdiv(t, inv_nrow);
+        const int rl = t - b * nrow;
+        const cplx<T>* z = in + rl;
+        cplx<T> v[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int c = b + r * NB;
+            cplx<T> zc = z[size_t(c) * cs];
+            cplx<T> zm = z[size_t(M - c) * cs];
+            if (r == 0 && b == 0) { zc.y = T(0); zm.y = T(0); }   // DC and Nyquist are real
+            const cplx<T> E = cadd(zc, cconj(zm));
+            const cplx<T> O = cmul(csub(zc, cconj(zm)), twl[c]);
+            v[r] = {E.x - O.y, E.y + O.x};
+        }
+        bfly<R>(v);
+        cplx<T>* d = dst + rl * L + b * R;
+#pragma unroll
+        for (int r = 0; r < R; ++r) d[r] = v[r];
+    }
+}
+
+template <typename T, int N, int R, int NS>
+__device__ __forceinline__ void rows_mid(const cplx<T>* src, cplx<T>* dst, int nrow,
+                                         float inv_nrow, const cplx<T>* twl)
+{
+    constexpr int M = N / 2, L = M + 1, NB = M / R;
+    constexpr int TSTEP = 2 * (M / (NS * R));
+    const int total = nrow * NB;
+    for (int t = threadIdx.x; t < total; t += blockDim.x) {
+        const int b = fdiv(t, inv_nrow);
+        const int rl = t - b * nrow;
+        const int tq = b / NS;
+        const int jj = b - tq * NS;
+        const cplx<T>* sp = src + rl * L + b;
+        cplx<T> v[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = sp[r * NB];
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[r] = cmul(v[r], twl[jj * r * TSTEP]);
+        bfly<R>(v);
+        cplx<T>* d = dst + rl * L + tq * (NS * R) + jj;
+#pragma unroll
+        for (int r = 0; r < R; ++r) d[r * NS] = v[r];
+    }
+}
+
+template <typename T, int N, int R, int NS>
+__device__ __forceinline__ void rows_last(const cplx<T>* src, int nrow, const cplx<T>* twl,
+                                          int x0, int nx, T* __restrict__ out)
+{
+    constexpr int M = N / 2, L = M + 1, NB = M / R;          // NB == NS
+    const T norm = T(1) / (T(N) * T(N));
+    const int total = nrow * NB;
+    for (int t = threadIdx.x; t < total; t += blockDim.x) {
+        const int rl = t / NB;
+        const int jj = t - rl * NB;
+        const cplx<T>* sp = src + rl * L + jj;
+        cplx<T> v[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = sp[r * NB];
+        if constexpr (NS > 1) {
+#pragma unroll
+            for (int r = 1; r < R; ++r) v[r] = cmul(v[r], twl[2 * jj * r]);
+        }
+        bfly<R>(v);
+        T* o = out + size_t(rl) * nx;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            // output m = jj + r NS holds pixels q = 2m (real part) and 2m + 1 (imaginary part);
+            // pixel q is column (q - x0) mod N of the window
+            const int q = 2 * (jj + r * NS);
+            int col = q - x0;
+            col = col >= N ? col - N : (col < 0 ? col + N : col);
+            int col1 = col + 1;
+            col1 = col1 >= N ? col1 - N : col1;
+            if (col < nx) o[col] = v[r].x * norm;
+            if (col1 < nx) o[col1] = v[r].y * norm;
+        }
+    }
+}
+
+template <typename T, int N, int NS>
+__device__ __forceinline__ void rows_rest(cplx<T>* src, cplx<T>* dst, int nrow, float inv_nrow,
+                                          const cplx<T>* twl, int x0, int nx, T* out)
+{
+    constexpr int rem = (N / 2) / NS;
+    constexpr int R = row_radix(rem);
+    if constexpr (rem == R) {
+        rows_last<T, N, R, NS>(src, nrow, twl, x0, nx, out);
+    } else {
+        rows_mid<T, N, R, NS>(src, dst, nrow, inv_nrow, twl);
+        __syncthreads();
+        rows_rest<T, N, NS * R>(dst, src, nrow, inv_nrow, twl, x0, nx, out);
+    }
+}
+
+// nrow (<= rpg) rows of one stamp: in = the stamp's scratch + its first row (column stride cs),
+// out = its first output row; LDS: 2 rpg (N/2 + 1) complex values.
+template <typename T, int N>
+__device__ __forceinline__ void rows_fixed(cplx<T>* lds, int rpg, const cplx<T>* __restrict__ in,
+                                           int cs, int nrow, int x0, int nx, const cplx<T>* twl,
+                                           T* __restrict__ out)
+{
+    constexpr int M = N / 2;
+    constexpr int R1 = row_radix(M);
+    cplx<T>* bufa = lds;
+    cplx<T>* bufb = lds + rpg * (M + 1);
+    const float inv_nrow = 1.f / float(nrow);
+    rows_first<T, N, R1>(in, cs, nrow, inv_nrow, twl, bufa);
+    __syncthreads();
+    rows_rest<T, N, R1>(bufa, bufb, nrow, inv_nrow, twl, x0, nx, out);
+}
+
+// rows_fixed for the compiled sizes; false (nothing done) for any other N.
+template <typename T>
+__device__ __forceinline__ bool rows_fixed_any(int N, cplx<T>* lds, int rpg,
+                                               const cplx<T>* __restrict__ in, int cs, int nrow,
+                                               int x0, int nx, const cplx<T>* twl,
+                                               T* __restrict__ out)
+{
+    switch (N) {
+#define GSMI_ROWS_FIXED(NN)                                                                     \
+      case NN: rows_fixed<T, NN>(lds, rpg, in, cs, nrow, x0, nx, twl, out); return true;
+      GSMI_ROWS_FIXED(64)
+      GSMI_ROWS_FIXED(96)
+      GSMI_ROWS_FIXED(128)
+      GSMI_ROWS_FIXED(192)
+      GSMI_ROWS_FIXED(256)
+      GSMI_ROWS_FIXED(384)
+      GSMI_ROWS_FIXED(512)
+      GSMI_ROWS_FIXED(768)
+#undef GSMI_ROWS_FIXED
+      default: return false;
     }
 }
 
@@ -434,8 +683,7 @@ row_c2r(const gsmi_stamp* __restrict__ stamps, const cplx<T>* __restrict__ scrat
     const int r0 = blockIdx.x * rpg;
     const int nrow = min(rpg, st->ny - r0);
     if (nrow <= 0) return;
-    const int W = N / 2 + 1;
-    row_c2r_body<T>(lds, rpg, scratch + st->scratch_offset + size_t(r0) * W, nrow, N, st->x0,
+    row_c2r_body<T>(lds, rpg, scratch + st->scratch_offset + r0, 1, st->ny, nrow, N, st->x0,
                     st->nx, tw, out + st->out_offset + size_t(r0) * st->nx);
 }
 
@@ -455,27 +703,30 @@ rows_list(const gsmi_stamp* __restrict__ stamps, int nstamps,
         if (it.stamp < 0 || it.stamp >= nstamps) continue;
         const gsmi_stamp* st = stamps + it.stamp;
         const int N = st->N;
-        const size_t bufs = size_t(2) * it.per_group * (N / 2);
+        const size_t bufs = size_t(2) * it.per_group * lds_line(N / 2);
         if (it.count <= 0 || it.first < 0 || it.first + it.count > st->ny ||
             it.per_group <= 0 || bufs * sizeof(cplx<T>) > size_t(lds_bytes))
             continue;
         int stride;
         const cplx<T>* tw = tws.pick(N, &stride);
-        const int W = N / 2 + 1;
         const int end = it.first + it.count;
         if ((bufs + N) * sizeof(cplx<T>) <= size_t(lds_bytes)) {
             cplx<T>* twl = lds + bufs;
             for (int e = threadIdx.x; e < N; e += blockDim.x) twl[e] = tw[e * stride];
             __syncthreads();
             for (int r0 = it.first; r0 < end; r0 += it.per_group) {
-                row_c2r_body<T>(lds, it.per_group, scratch + st->scratch_offset + size_t(r0) * W,
-                                min(it.per_group, end - r0), N, st->x0, st->nx, twl,
-                                out + st->out_offset + size_t(r0) * st->nx, 1);
+                const int nrow = min(it.per_group, end - r0);
+                const cplx<T>* in = scratch + st->scratch_offset + r0;
+                T* o = out + st->out_offset + size_t(r0) * st->nx;
+                if (!rows_fixed_any<T>(N, lds, it.per_group, in, st->ny, nrow, st->x0, st->nx,
+                                       twl, o))
+                    row_c2r_body<T>(lds, it.per_group, in, 1, st->ny, nrow, N, st->x0, st->nx,
+                                    twl, o, 1);
                 __syncthreads();
             }
         } else {
             for (int r0 = it.first; r0 < end; r0 += it.per_group) {
-                row_c2r_body<T>(lds, it.per_group, scratch + st->scratch_offset + size_t(r0) * W,
+                row_c2r_body<T>(lds, it.per_group, scratch + st->scratch_offset + r0, 1, st->ny,
                                 min(it.per_group, end - r0), N, st->x0, st->nx, tw,
                                 out + st->out_offset + size_t(r0) * st->nx, stride);
                 __syncthreads();
@@ -496,7 +747,7 @@ irfft_rows_f64(const cplx<double>* __restrict__ scratch, int N, const cplx<doubl
     const int nrow = min(rpg, N - r0);
     if (nrow <= 0) return;
     const int W = N / 2 + 1;
-    row_c2r_body<double>(lds, rpg, scratch + size_t(r0) * W, nrow, N, -N / 2, N, tw,
+    row_c2r_body<double>(lds, rpg, scratch + size_t(r0) * W, W, 1, nrow, N, -N / 2, N, tw,
                          out + size_t(r0) * N, tws);
 }
 
@@ -616,13 +867,13 @@ const cplx<float>* twiddles<float>(int N) { return twf().get(N); }
 
 // 2^k or 3*2^k with both in-LDS transforms within `lds` bytes per workgroup (64 KiB: N <= 2048
 // in FP64, N <= 4096 in FP32; 160 KiB: N <= 4096 in FP64, N <= 8192 in FP32).
-bool fft_size_ok(int N, size_t elem, size_t lds)
+bool fft_size_ok(int N, size_t elem, size_t lds, bool padded)
 {
     if (N < 4 || N % 2) return false;
     int m = N;
     while (m % 2 == 0) m /= 2;
     if (m != 1 && m != 3) return false;
-    return 2 * size_t(cols_per_group(N)) * N * elem <= lds;
+    return 2 * size_t(cols_per_group(N)) * (padded ? lds_line(N) : N) * elem <= lds;
 }
 
 size_t lds_limit()
@@ -687,7 +938,7 @@ template <typename T>
 int fold_impl(const gsmi_stamp* stamps, int nstamps, int N, const gsmi_op* ops,
               const gsmi_leaf* leaves, Tables<T> tb, void* scratch, void* stream)
 {
-    if (!fft_size_ok(N, sizeof(cplx<T>), lds_limit())) {
+    if (!fft_size_ok(N, sizeof(cplx<T>), lds_limit(), true)) {
         set_error("unsupported FFT size " + std::to_string(N) +
                   " (need 2^k or 3*2^k, <= 4096 in FP64, <= 8192 in FP32)");
         return GSMI_E_SIZE;
@@ -702,7 +953,7 @@ int fold_impl(const gsmi_stamp* stamps, int nstamps, int N, const gsmi_op* ops,
     int rc = launch_axis_factors<T>(stamps, nstamps, ops, leaves, scratch, stream);
     if (rc) return rc;
     dim3 g1((N / 2 + 1 + cpg - 1) / cpg, nstamps);
-    size_t lds1 = 2 * size_t(cpg) * N * sizeof(cplx<T>);
+    size_t lds1 = 2 * size_t(cpg) * lds_line(N) * sizeof(cplx<T>);
     if ((rc = allow_lds(reinterpret_cast<const void*>(FoldKernel<T>::fn), lds1))) return rc;
     hipLaunchKernelGGL(FoldKernel<T>::fn, g1, dim3(kThreads), lds1,
                        reinterpret_cast<hipStream_t>(stream), stamps, ops, leaves, tb, tw,
@@ -714,7 +965,7 @@ template <typename T>
 int rows_impl(const gsmi_stamp* stamps, int nstamps, int N, int max_ny, const void* scratch,
               T* out, void* stream)
 {
-    if (!fft_size_ok(N, sizeof(cplx<T>), lds_limit())) {
+    if (!fft_size_ok(N, sizeof(cplx<T>), lds_limit(), true)) {
         set_error("unsupported FFT size " + std::to_string(N));
         return GSMI_E_SIZE;
     }
@@ -726,7 +977,7 @@ int rows_impl(const gsmi_stamp* stamps, int nstamps, int N, int max_ny, const vo
     }
     const int rpg = small_launch_tile(rows_per_group(N), max_ny, nstamps);
     dim3 g2((max_ny + rpg - 1) / rpg, nstamps);
-    size_t lds2 = 2 * size_t(rpg) * (N / 2) * sizeof(cplx<T>);
+    size_t lds2 = 2 * size_t(rpg) * lds_line(N / 2) * sizeof(cplx<T>);
     int rc = allow_lds(reinterpret_cast<const void*>(row_c2r<T>), lds2);
     if (rc) return rc;
     hipLaunchKernelGGL(row_c2r<T>, g2, dim3(kThreads), lds2, reinterpret_cast<hipStream_t>(stream),
@@ -749,7 +1000,8 @@ int sched_check(const gsmi_stamp* stamps, int nstamps, const gsmi_schedule* sc)
     if (!stamps || !sc || nstamps < 0 || sc->nfold < 0 || sc->nrows < 0 ||
         (sc->nfold > 0 && (!sc->fold || sc->fold_grid <= 0)) ||
         (sc->nrows > 0 && (!sc->rows || sc->rows_grid <= 0)) || sc->fold_lds < 0 ||
-        sc->rows_lds < 0 || sc->fold_lds > 65536 || sc->rows_lds > 65536) {
+        sc->rows_lds < 0 || size_t(sc->fold_lds) > lds_limit() ||
+        size_t(sc->rows_lds) > lds_limit()) {
         set_error("bad schedule");
         return GSMI_E_ARG;
     }
@@ -777,6 +1029,9 @@ int fold_list_impl(const gsmi_stamp* stamps, int nstamps, const gsmi_schedule* s
     switch (sc->program) {
 #define GSMI_FOLD_LIST(SIG)                                                                      \
       case SIG:                                                                                  \
+        if ((rc = allow_lds(reinterpret_cast<const void*>(FoldKernel<T>::template list<SIG>),    \
+                            sc->fold_lds)))                                                      \
+            return rc;                                                                           \
         hipLaunchKernelGGL(FoldKernel<T>::template list<SIG>, grid, dim3(kThreads), sc->fold_lds, \
                            s, stamps, nstamps, sc->fold, sc->nfold, sc->fold_lds, ops, leaves,  \
                            tb, tws, scr);                                                       \
@@ -806,6 +1061,7 @@ int rows_list_impl(const gsmi_stamp* stamps, int nstamps, const gsmi_schedule* s
         set_error("twiddle table allocation failed");
         return GSMI_E_HIP;
     }
+    if ((rc = allow_lds(reinterpret_cast<const void*>(rows_list<T>), sc->rows_lds))) return rc;
     hipLaunchKernelGGL(rows_list<T>, dim3(min(sc->rows_grid, sc->nrows)), dim3(kThreads),
                        sc->rows_lds, reinterpret_cast<hipStream_t>(stream), stamps, nstamps,
                        sc->rows, sc->nrows, sc->rows_lds,
@@ -829,7 +1085,7 @@ int irfft_launch(const double* kin, int N, double* out, cplx<double>* scratch,
     int rc = hip_check(hipGetLastError(), "irfft columns launch");
     if (rc) return rc;
     const int rpg = rows_per_group(N);
-    size_t lds2 = 2 * size_t(rpg) * (N / 2) * sizeof(cplx<double>);
+    size_t lds2 = 2 * size_t(rpg) * lds_line(N / 2) * sizeof(cplx<double>);
     hipLaunchKernelGGL(irfft_rows_f64, dim3((N + rpg - 1) / rpg), dim3(kThreads), lds2, s,
                        scratch, N, tw, tws, out);
     return hip_check(hipGetLastError(), "irfft rows launch");
@@ -934,7 +1190,7 @@ int gsmi_draw_fft_f32(const gsmi_stamp* stamps, int nstamps, int N, int max_ny,
         set_error("gsmi_draw_fft_f32: null buffer or empty stamp");
         return GSMI_E_ARG;
     }
-    Tables<float> tb{tables, tables32};
+    Tables<float> tb{tables, tables32, reinterpret_cast<const float4*>(tables32)};
     return draw_fft_impl<float>(stamps, nstamps, N, ops, leaves, tb, scratch, out, stream, max_ny);
 }
 
@@ -1022,7 +1278,7 @@ int gsmi_fold_colfft_f32(const gsmi_stamp* stamps, int nstamps, int N, const gsm
                          void* scratch, void* stream)
 {
     if (!stamps || !ops || !leaves || !scratch) { set_error("null buffer"); return GSMI_E_ARG; }
-    Tables<float> tb{tables, tables32};
+    Tables<float> tb{tables, tables32, reinterpret_cast<const float4*>(tables32)};
     return fold_impl<float>(stamps, nstamps, N, ops, leaves, tb, scratch, stream);
 }
 
@@ -1060,7 +1316,7 @@ int gsmi_fold_list_f32(const gsmi_stamp* stamps, int nstamps, const gsmi_schedul
                        const float* tables32, void* scratch, void* stream)
 {
     if (!ops || !leaves || !scratch) { set_error("null buffer"); return GSMI_E_ARG; }
-    Tables<float> tb{tables, tables32};
+    Tables<float> tb{tables, tables32, reinterpret_cast<const float4*>(tables32)};
     return fold_list_impl<float>(stamps, nstamps, sched, ops, leaves, tb, scratch, stream);
 }
 

@@ -18,8 +18,9 @@ void set_error(const std::string& msg);
 // GSMI_OK, or GSMI_E_HIP with the HIP error text recorded.
 int hip_check(hipError_t e, const char* what);
 
-// N is 2^k or 3*2^k and the two in-LDS transforms of a workgroup fit `lds` bytes.
-bool fft_size_ok(int N, size_t elem, size_t lds = 65536);
+// N is 2^k or 3*2^k and the two in-LDS transforms of a workgroup fit `lds` bytes (padded: lines
+// in the padded layout of lds_line).
+bool fft_size_ok(int N, size_t elem, size_t lds = 65536, bool padded = false);
 
 // LDS bytes one workgroup may allocate on the current device (160 KiB on gfx950; 64 KiB if the
 // query fails).  Launches above 64 KiB set the kernel's dynamic-LDS attribute first.

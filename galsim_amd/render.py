@@ -22,6 +22,23 @@ def _torch():
     return torch
 
 
+def spline_coefficients(args, vals, y2):
+    """Per-interval (c0, c1, c2, c3), FP64, of the natural cubic spline through (args, vals) with
+    second derivatives y2: on interval i (args[i]..args[i+1], width h) the spline is
+    c0 + s (c1 + s (c2 + s c3)), s = (x - args[i]) / h -- Table's spline (src/Table.cpp:560-577)
+    written as a polynomial in s."""
+    a = np.asarray(args, float)
+    v = np.asarray(vals, float)
+    y = np.asarray(y2, float)
+    hh = np.diff(a) ** 2
+    c = np.empty((len(a) - 1, 4))
+    c[:, 0] = v[:-1]
+    c[:, 1] = (v[1:] - v[:-1]) - hh * (2. * y[:-1] + y[1:]) / 6.
+    c[:, 2] = 0.5 * hh * y[:-1]
+    c[:, 3] = hh * (y[1:] - y[:-1]) / 6.
+    return c
+
+
 class Lowering:
     """Accumulates the k-space program of one or more stamps."""
 
@@ -36,6 +53,9 @@ class Lowering:
         self.ntab = 0
         self.clip = []
         self.nclip = 0
+        self.coefs = []
+        self.coef_off = {}
+        self.ncoef = 0
         self.kvalue_mode = False     # lowering for kValue: no k-image cuts
 
     def emit(self, code, arg=0, a=0., b=0., c=0.):
@@ -51,6 +71,36 @@ class Lowering:
             self.tables.append(np.concatenate([args, vals, y2]))
             self.ntab += 3 * len(args)
         return self.tab_off[key]
+
+    def add_spline_coefs(self, key, args, vals, y2):
+        """The natural cubic spline of a table on equally spaced knots as per-interval
+        coefficients for the FP32 kernels: S = c0 + s (c1 + s (c2 + s c3)) with s = (x - args[i])
+        / h in [0, 1] on interval i, derived in FP64 from the knots, values and second
+        derivatives (Table's spline, src/Table.cpp:560-577) and rounded to float.  Returns the
+        offset (floats) of the table's quadruples in the coefficient region of
+        ``tables32_array``; the leaf records hold it, made absolute by ``leaves_array``."""
+        if key not in self.coef_off:
+            c = spline_coefficients(args, vals, y2)
+            self.coef_off[key] = self.ncoef
+            self.coefs.append(c.astype(np.float32).ravel())
+            self.ncoef += c.size
+        return self.coef_off[key]
+
+    def _coef_base(self):
+        """Offset (floats) of the coefficient region in tables32_array: after the float copy of
+        the FP64 tables, rounded up to a 16-byte boundary."""
+        return -(-max(self.ntab, 1) // 4) * 4
+
+    def tables32_array(self):
+        """The FP32 kernels' table buffer: the FP64 tables rounded to float, then the spline
+        coefficient quadruples of add_spline_coefs."""
+        t = np.concatenate(self.tables) if self.tables else np.zeros(1)
+        base = self._coef_base()
+        out = np.zeros(base + self.ncoef, dtype=np.float32)
+        out[:len(t)] = t
+        if self.coefs:
+            out[base:] = np.concatenate(self.coefs)
+        return out
 
     def exponential_cliprows(self, r0, gsparams, jac):
         """Host row ranges of the reference's Exponential clipping (FP64 path only)."""
@@ -90,6 +140,10 @@ class Lowering:
         a = np.zeros(len(self.leaves), dtype=_lib.LEAF_DTYPE)
         for i, r in enumerate(self.leaves):
             a[i] = r
+        if self.precision == 'f32':
+            # Sersic leaves: coefficient offsets relative to the region -> absolute in tables32
+            sel = (a['type'] == _lib.LEAF_SERSIC) & (a['cliprows'] >= 0)
+            a['cliprows'][sel] += self._coef_base()
         return a
 
 
@@ -201,7 +255,7 @@ def draw_fft(prof, bounds, precision='f64', return_device=False):
     o_leaves = pk.add(ctx.leaves_array())
     tables = np.concatenate(ctx.tables) if ctx.tables else np.zeros(1)
     o_tab = pk.add(tables)
-    o_tab32 = pk.add(tables.astype(np.float32))
+    o_tab32 = pk.add(ctx.tables32_array())
     clip = np.concatenate(ctx.clip) if ctx.clip else np.zeros(2, dtype=np.int32)
     o_clip = pk.add(clip)
     dev = _device_buffer(torch, pk.host())

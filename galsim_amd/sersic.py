@@ -119,6 +119,10 @@ class Sersic(Leaf):
                                      vals, y2)
         rec['ntab'] = len(args)
         rec['p'][:] = [sc[4], sc[5], sc[6], sc[7], sc[8], sc[9]]
+        if ctx.precision == 'f32' and len(args) > 1:
+            # FP32 kernels read the spline as per-interval cubic coefficients
+            rec['cliprows'] = ctx.add_spline_coefs((self._n, self._trunc / self._r0,
+                                                    self.gsparams), args, vals, y2)
 
     def withFlux(self, flux):
         return Sersic(self._n, scale_radius=self._r0, trunc=self._trunc, flux=flux,

@@ -93,7 +93,11 @@ typedef struct gsmi_leaf {
     int32_t table;        /* Sersic, truncated Moffat: offset of its table in the table buffer
                              (doubles: knots, values, spline second derivatives) */
     int32_t ntab;         /* Sersic, truncated Moffat: number of knots */
-    int32_t cliprows;     /* Exponential (FP64): offset of its int2 row ranges, -1 if none */
+    int32_t cliprows;     /* Exponential (FP64): offset of its int2 row ranges, -1 if none;
+                             Sersic (FP32): offset (floats, a multiple of 4) in tables32 of its
+                             spline as per-interval cubic coefficients (c0, c1, c2, c3) of
+                             s = (ln k - knot) / spacing in [0, 1], ntab - 1 quadruples, or -1
+                             (then the kernels evaluate the knots' spline form) */
     double jac[4];        /* composite Jacobian (dudx, dudy, dvdx, dvdy) of the leaf       */
     double scale;         /* r0, sigma, rD, 1/k0, r0, or width/2pi                         */
     double scale2;        /* Box: height/2pi                                                */
@@ -116,7 +120,8 @@ typedef struct gsmi_stamp {
     int32_t axis_count;   /* separable factors stored after the program: Box leaves with a
                              diagonal Jacobian, PHASE and SCALE applying to the whole result  */
     int64_t out_offset;   /* element offset of the stamp in the output buffer                 */
-    int64_t scratch_offset; /* complex offset in scratch of the stamp's ny x (N/2+1) rows     */
+    int64_t scratch_offset; /* complex offset in scratch of the stamp's column-transformed
+                               values: N/2+1 columns of ny output rows, column-major          */
     int64_t axis_offset;  /* complex offset in scratch of its (Nk/2+1) + (Nk+1) axis factors  */
     double dk;            /* k spacing in pixel units: 2 pi / N                               */
 } gsmi_stamp;
@@ -249,7 +254,7 @@ int gsmi_sersic_kvalue(double n, double trunc, const double* gsp, const double* 
  * ------------------------------------------------------------------------------------- */
 
 /* drawImage(method='fft') in FP64 for a batch of stamps that share one FFT size N (max_ny =
- * tallest output stamp; scratch holds each stamp's ny * (N/2+1) rows and its
+ * tallest output stamp; scratch holds each stamp's ny * (N/2+1) values (column-major) and its
  * (Nk/2+1) + (Nk+1) axis factors at the offsets in its record, complex128) (the
  * reference: gsobject.py:1927-2045 drawFFT_makeKImage + _drawKImage + drawFFT_finish, with
  * SBProfile::drawK src/SBProfile.cpp:383-425, ApplyKImagePhases src/SBTransform.cpp:763-883,

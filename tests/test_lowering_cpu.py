@@ -115,3 +115,69 @@ def test_lowered_phases_match_closed_form(idx):
     ref = closed_form(terms, nx, ny, dk)
     err = float(np.max(np.abs(got - ref)) / np.max(np.abs(ref)))
     assert err < 1e-12, (name, err)
+
+
+def _sersic_f32_emulated(L, coef, ksq):
+    """The FP32 kernels' branch-free Sersic k-value (sersic_f32) restated in FP64 arithmetic."""
+    p = L['p']
+    n = int(L['ntab'])
+    with np.errstate(divide='ignore', invalid='ignore'):
+        lk = 0.5 * np.log(ksq)
+        asym = (p[2] + p[3] / np.sqrt(ksq)) / ksq
+        tay = 1. + ksq * (p[4] + ksq * p[5])
+        # the knot range as the kernel has it (leaf_consts_t): first and last knot
+        a0, a1 = L['_a0'], L['_a1']
+        u = np.clip((lk - a0) * (n - 1) / (a1 - a0), 0., n - 1.)
+        u = np.where(np.isnan(u), 0., u)
+        i = np.maximum(np.ceil(u), 1).astype(np.int64)
+        s = u - (i - 1)
+        c = coef[i - 1]
+        S = c[:, 0] + s * (c[:, 1] + s * (c[:, 2] + s * c[:, 3]))
+        S = np.where((lk < a0) | (lk > a1), 0., S)
+        spl = S / ksq
+    return float(L['flux']) * np.where(ksq >= p[1], asym, np.where(ksq < p[0], tay, spl))
+
+
+@pytest.mark.parametrize("n,trunc", [(1.5, 0.), (2.5, 0.), (4.0, 0.), (3.1, 0.), (2.0, 8.)])
+def test_fp32_sersic_spline_coefficients_against_compiled_reference(n, trunc):
+    # The FP32 kernels read a Sersic table as per-interval cubic coefficients.  In FP64 they must
+    # give the reference's own SersicInfo::kValue (compiled from its sources) to rounding; rounded
+    # to float they stay within FP32 accuracy.  Regimes: Taylor, spline, asymptote.
+    from oracle import ref
+    if not ref.available():
+        pytest.skip("reference numerics not built")
+    import galsim_amd as g
+    from galsim_amd import render, _lib
+    ctx = render.Lowering('f32')
+    prof = g.Sersic(n, scale_radius=1.0, trunc=trunc, flux=1.0)
+    prof._lower(ctx, None)
+    leaves = ctx.leaves_array()
+    L = leaves[0]
+    assert L['type'] == _lib.LEAF_SERSIC and L['cliprows'] % 4 == 0 and L['cliprows'] >= 0
+    t32 = ctx.tables32_array()
+    nt = int(L['ntab'])
+    coef32 = t32[L['cliprows']:L['cliprows'] + 4 * (nt - 1)].reshape(-1, 4).astype(float)
+    tab = np.concatenate(ctx.tables)[L['table']:L['table'] + 3 * nt]
+    args, vals, y2 = tab[:nt], tab[nt:2 * nt], tab[2 * nt:]
+    coef64 = render.spline_coefficients(args, vals, y2)
+    np.testing.assert_allclose(coef32, coef64.astype(np.float32).astype(float), rtol=0, atol=0)
+    rec = {'p': L['p'], 'ntab': nt, 'flux': 1.0, '_a0': args[0], '_a1': args[-1]}
+    ksq = np.concatenate([[0.], np.logspace(-14, 4, 4001)])
+    want = ref.sersic_kvalue(n, ksq, trunc)
+    got64 = _sersic_f32_emulated(rec, coef64, ksq)
+    got32 = _sersic_f32_emulated(rec, coef32, ksq)
+    peak = np.max(np.abs(want))
+    # truncated tables are built here by a finite-range Hankel transform that differs from the
+    # reference's integrator by ~3e-9; untruncated ones equal the reference's to rounding
+    assert np.max(np.abs(got64 - want)) < (1e-11 if trunc == 0. else 1e-8) * peak
+    assert np.max(np.abs(got32 - want)) < 1e-6 * peak
+    # against the same table's knot form (the FP64 kernels' spline_eq, restated in the program
+    # emulator) the coefficient form agrees to rounding in every case
+    from tests.program_emulator import _leaf
+    full = ctx.leaves_array()[0].copy()
+    knots = _leaf(full, np.concatenate(ctx.tables), None, np.sqrt(ksq), 0. * ksq, 0, 0, 0)
+    assert np.max(np.abs(got64 - knots)) < 1e-12 * peak
+    # a planted mistake in the coefficient form (c2 taken from the interval's other end) fails
+    bad = coef64.copy()
+    bad[:, 2] = 0.5 * np.diff(args) ** 2 * y2[1:]
+    assert np.max(np.abs(_sersic_f32_emulated(rec, bad, ksq) - want)) > 1e-9 * peak

@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 from galsim_amd import _lib
-from galsim_amd.batch import STAGE_LDS, BatchPlan, work_lists
+from galsim_amd.batch import STAGE_LDS, BatchPlan, lds_line, work_lists
 
 
 def _recs(spec):
@@ -53,10 +53,11 @@ def test_work_lists_cover_every_column_and_row_once(cols, rows, chunks):
     assert np.all(np.diff(Ns) <= 0)                       # largest FFT sizes first
     need = []
     for (_, _, _, per), N in zip(fold, recs['N'][fold[:, 0]]):
-        staged = (2 * per * N + N + 3 * N + 2) * 8
-        need.append(staged if staged <= STAGE_LDS else 2 * per * N * 8)
+        # transform lines are padded by one slot per 32 elements in LDS
+        staged = (2 * per * lds_line(N) + N + 3 * N + 2) * 8
+        need.append(staged if staged <= STAGE_LDS else 2 * per * lds_line(N) * 8)
     assert lds_f == max(need)
-    assert lds_r == max((2 * per * (int(recs['N'][s]) // 2) + int(recs['N'][s])) * 8
+    assert lds_r == max((2 * per * lds_line(int(recs['N'][s]) // 2) + int(recs['N'][s])) * 8
                         for s, _, _, per in rw)
 
 
