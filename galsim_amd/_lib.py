@@ -126,6 +126,7 @@ _SIGS = {
     'gsmi_draw_kimage_f64': (_I, [_P, _I, _P, _P, _P, _I, _I, _I, _I, _I, _D, _P, _P]),
     'gsmi_irfft_f64': (_I, [_P, _I, _P, _P, _P]),
     'gsmi_irfft_f64_noalloc': (_I, [_P, _I, _P, _P]),
+    'gsmi_apply_kimage_phases_f64': (_I, [_P, _I, _I, _I, _I, _D, _P, _D, _D, _D, _P]),
     'gsmi_wrap_hermx_f64': (_I, [_P, _I, _I, _P, _P]),
     'gsmi_cfft2_f64': (_I, [_P, _I, _I, _I, _I, _I, _P, _P, _P]),
     'gsmi_rfft2_f64': (_I, [_P, _I, _I, _I, _I, _P, _P, _P]),

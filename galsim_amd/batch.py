@@ -114,7 +114,8 @@ def work_lists(recs, groups, cols, rows, elem, bins=None, chunks=None, extra=0,
     tiles ``cols`` / ``rows`` (dict N -> columns / rows per workgroup chunk; defaults 1024/N and
     1024/(N/2)), largest FFT sizes first, and the LDS bytes of a fold and of a row workgroup
     (elem: bytes per complex value; extra: bytes the fold kernel keeps beside its buffers).
-    Every column 0..N/2 and every output row of every stamp is in exactly one item.
+    Every fold column 0..N/2-1 (column 0 carries kx = N/2 too) and every output row of every
+    stamp is in exactly one item.
 
     A fold item of a size whose two transform buffers, N twiddles and 3 Nk/2 + 2 axis factors
     fit in STAGE_LDS spans ``chunks[N]`` chunks of ``cols[N]`` columns (default: about
@@ -134,7 +135,7 @@ def work_lists(recs, groups, cols, rows, elem, bins=None, chunks=None, extra=0,
         rpg = int(rows.get(N, BatchPlan.default_rows(N)))
         if cpg < 1 or rpg < 1:
             raise ValueError("tiles must hold at least one column and one row")
-        W = N // 2 + 1
+        W = N // 2               # fold columns: kx = 0 and kx = N/2 travel as one
         nk = int(recs['Nk'][first:first + count].max()) if count else 0
         bufs = 2 * cpg * lds_line(N) * elem + extra
         staged = bufs + (N + 3 * (nk // 2) + 2) * elem
@@ -370,10 +371,14 @@ class BatchPlan:
         cols, rows, log = {}, {}, {}
         elem = self._elem()
         for N, first, count, _ in self.groups:
-            W = N // 2 + 1
+            W = N // 2               # fold columns (kx = 0 and kx = N/2 packed as one)
             best = None
-            for f in factors:
-                c = int(self.default_cols(N) * f)
+            # powers of two around the default, and widths that split the W columns into
+            # equal chunks
+            d = self.default_cols(N)
+            cands = {int(d * f) for f in factors}
+            cands |= {-(-W // m) for m in range(max(1, W // (2 * d)), 2 * (-(-W // d)) + 1)}
+            for c in sorted(cands):
                 # the launch's LDS is the largest size's: keep tiles within the staging budget
                 if c < 1 or c > W or 2 * c * lds_line(N) * elem + self._fold_extra() > \
                         (LIST_LDS_LIMIT if c == 1 else STAGE_LDS):

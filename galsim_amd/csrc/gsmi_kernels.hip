@@ -56,6 +56,13 @@ axis_factors(const gsmi_stamp* __restrict__ stamps, const gsmi_op* __restrict__ 
 // are filled in two passes: one primary sample per cell, straight-line, then the fringe cells,
 // listed compactly (every column of the chunk x the fringe rows, then the fringe columns x the
 // other rows), each adding its other samples.
+//
+// Columns kx = 0 and kx = N/2 are Hermitian along ky (F(-k) = conj F(k)), so their transforms
+// along ky are real -- the real inverse transform along x reads only their real parts.  They
+// travel as one complex column: column 0 holds cell(0, r) + i cell(N/2, r), and the column
+// transform gives column 0 in its real part and column N/2 in its imaginary part.  The fold
+// therefore transforms N/2 columns, c = 0..N/2-1 (column N/2's samples are gathered into
+// column 0 by the fringe pass).
 
 // Row of fringe-row index f (0 <= f < fringe_rows): kyp = a.. N/2-1, then -N/2 .. -a.
 __device__ __forceinline__ int fringe_row(int f, int a, int N)
@@ -73,10 +80,10 @@ __device__ __forceinline__ int inner_row(int g, int a, int N)
     return kyp < 0 ? kyp + N : kyp;
 }
 
-// Fill cells of columns [c0, c0 + ncol) into bufa (column-major in the padded transform layout:
-// cell (cl, r) at cl * lds_line(N) + lds_pad(r)).
-// value(i, j, neg) is the contribution of the stored sample (i, j) = (|kx|, kx < 0 ? -ky : ky),
-// conjugated by the caller when neg.  Ends with a barrier.
+// Fill cells of columns [c0, c0 + ncol) (within 0..N/2-1) into bufa (column-major in the padded
+// transform layout: cell (cl, r) at cl * lds_line(N) + lds_pad(r)).  value(i, j) is the
+// contribution of the stored sample (i, j) = (|kx|, kx < 0 ? -ky : ky), conjugated here when it
+// stands for kx < 0.  Ends with a barrier.
 template <typename T, typename V>
 __device__ __forceinline__ void fold_cells_split(V&& value, int N, int Nk, int c0, int ncol,
                                                  cplx<T>* bufa)
@@ -92,12 +99,30 @@ __device__ __forceinline__ void fold_cells_split(V&& value, int N, int Nk, int c
         bufa[cl * LN + lds_pad(r)] = value(c0 + cl, kyp);
     }
     __syncthreads();
+    // every sample congruent to (c, r) except (skip_c, skip_ky)
+    auto gather = [&](int c, int r, int skip_c, int skip_ky) -> cplx<T> {
+        const int kx0 = c - N * fdiv(c + hk, inv_n);
+        const int ky0 = r - N * fdiv(r + hk, inv_n);
+        cplx<T> acc = {T(0), T(0)};
+        for (int kx = kx0; kx <= hk; kx += N) {
+            for (int ky = ky0; ky <= hk; ky += N) {
+                if (kx == skip_c && ky == skip_ky) continue;
+                const bool neg = kx < 0;
+                const cplx<T> v = value(neg ? -kx : kx, neg ? -ky : ky);
+                acc.x += v.x;
+                acc.y += neg ? -v.y : v.y;
+            }
+        }
+        return acc;
+    };
     const int nfr = a <= 0 ? N : N - 2 * a + 1;
     const int cfirst = max(c0, a);
     const int ncf = (a <= 0) ? 0 : max(0, c0 + ncol - cfirst);
     const int ninner = 2 * a - 1;
     const int part1 = ncol * nfr;
-    const int total = part1 + ncf * ninner;
+    const int part2 = part1 + ncf * ninner;
+    // column 0's rows outside the fringe rows still take column N/2's samples
+    const int total = part2 + ((c0 == 0 && a > 0) ? ninner : 0);
     const float inv_nfr = 1.f / float(nfr);
     const float inv_inner = 1.f / float(max(ninner, 1));
     for (int e = threadIdx.x; e < total; e += blockDim.x) {
@@ -105,25 +130,25 @@ __device__ __forceinline__ void fold_cells_split(V&& value, int N, int Nk, int c
         if (e < part1) {
             cl = fdiv(e, inv_nfr);
             r = fringe_row(e - cl * nfr, a, N);
-        } else {
+        } else if (e < part2) {
             const int q = e - part1;
             const int k = fdiv(q, inv_inner);
             cl = cfirst - c0 + k;
             r = inner_row(q - k * ninner, a, N);
+        } else {
+            cl = 0;
+            r = inner_row(e - part2, a, N);
         }
         const int c = c0 + cl;
         const int kyp = r < N / 2 ? r : r - N;
-        const int kx0 = c - N * fdiv(c + hk, inv_n);
-        const int ky0 = r - N * fdiv(r + hk, inv_n);
-        cplx<T> acc = {T(0), T(0)};
-        for (int kx = kx0; kx <= hk; kx += N) {
-            for (int ky = ky0; ky <= hk; ky += N) {
-                if (kx == c && ky == kyp) continue;          // the primary, already in bufa
-                const bool neg = kx < 0;
-                const cplx<T> v = value(neg ? -kx : kx, neg ? -ky : ky);
-                acc.x += v.x;
-                acc.y += neg ? -v.y : v.y;
-            }
+        // the primary sample is already in bufa; a cell of column 0 outside the fringe has no
+        // other samples of its own, only column N/2's
+        const bool own = (e < part2);
+        cplx<T> acc = own ? gather(c, r, c, kyp) : cplx<T>{T(0), T(0)};
+        if (c == 0) {
+            const cplx<T> h = gather(N / 2, r, -1, 0);      // + i cell(N/2, r)
+            acc.x -= h.y;
+            acc.y += h.x;
         }
         cplx<T>& cell = bufa[cl * LN + lds_pad(r)];
         cell.x += acc.x;
@@ -169,7 +194,7 @@ __device__ __forceinline__ void fold_cells_static(const LeafC<T>* __restrict__ l
 
 // ---------------------------------------------------------------------------------------------
 // Kernel 1: fused k-space evaluation + product/sinc/phase + Hermitian wrap + column inverse DFT
-// grid: (ceil((N/2+1)/cpg), nstamps); dynamic LDS: 2 * cpg * lds_line(N) complex
+// grid: (ceil((N/2)/cpg), nstamps); dynamic LDS: 2 * cpg * lds_line(N) complex
 // ---------------------------------------------------------------------------------------------
 
 // Columns [first, first + count) of one stamp in chunks of at most cpg: each chunk's cells are
@@ -230,7 +255,14 @@ __device__ __forceinline__ void fold_chunks(const gsmi_stamp* __restrict__ st, i
             const int row = e - cl * ny;
             const int p = y0 + row + N;               // >= 0: y0 >= -N/2
             const int y = p - N * fdiv(p, inv_n);
-            oc[e] = res[cl * LN + lds_pad(y)];
+            const cplx<T> v = res[cl * LN + lds_pad(y)];
+            if (c0 + cl == 0) {
+                // the packed column: column 0 in the real part, column N/2 in the imaginary part
+                oc[e] = {v.x, T(0)};
+                out[size_t(N / 2) * ny + row] = {v.y, T(0)};
+            } else {
+                oc[e] = v;
+            }
         }
         __syncthreads();                          // bufa / bufb are reused by the next chunk
     }
@@ -350,7 +382,7 @@ __device__ __forceinline__ void fold_colfft_body(const gsmi_stamp* __restrict__ 
     const gsmi_stamp* st = stamps + blockIdx.y;
     const int N = st->N;
     const int c0 = blockIdx.x * cpg;
-    const int ncol = min(cpg, N / 2 + 1 - c0);
+    const int ncol = min(cpg, N / 2 - c0);
     if (ncol <= 0) return;
     fold_item<T>(st, c0, ncol, cpg, ops, leaves, tb, tw, 1, scratch,
                  reinterpret_cast<cplx<T>*>(smem_raw), size_t(2) * cpg * lds_line(N));
@@ -381,7 +413,7 @@ __device__ __forceinline__ void fold_list_body(const gsmi_stamp* __restrict__ st
         if (it.stamp < 0 || it.stamp >= nstamps) continue;
         const gsmi_stamp* st = stamps + it.stamp;
         const int N = st->N;
-        if (it.count <= 0 || it.first < 0 || it.first + it.count > N / 2 + 1 ||
+        if (it.count <= 0 || it.first < 0 || it.first + it.count > N / 2 ||
             it.per_group <= 0 ||
             size_t(2) * it.per_group * lds_line(N) * sizeof(cplx<T>) +
                     (SIG > 0 ? kFoldLdsExtra<T> : 0) >
@@ -766,6 +798,26 @@ draw_kimage_f64(const gsmi_op* __restrict__ ops, int nops, const gsmi_leaf* __re
     out[idx] = eval_program<double>(ops, nops, leaves, tb, dk, xmin + col, ymin + row, Nk);
 }
 
+// ApplyKImagePhases on a row-major complex128 k-image: pixel (x, y) = (xmin + i, ymin + j) at
+// k = (x, y) dk is multiplied by fs exp(-i (kx' cenx + ky' ceny)), (kx', ky') = (A kx + C ky,
+// B kx + D ky) for the Jacobian (A, B, C, D), the identity when none is given.
+__global__ void __launch_bounds__(kThreads)
+apply_kimage_phases_f64(cplx<double>* __restrict__ img, int xmin, int ymin, int nx, int ny,
+                        double dk, double A, double B, double C, double D, double cenx,
+                        double ceny, double fs)
+{
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nx * ny) return;
+    const int j = idx / nx;
+    const int i = idx - j * nx;
+    const double kx = double(xmin + i) * dk;
+    const double ky = double(ymin + j) * dk;
+    const double arg = (A * kx + C * ky) * cenx + (B * kx + D * ky) * ceny;
+    double sn, cs;
+    sincos(arg, &sn, &cs);
+    img[idx] = cmul(img[idx], cplx<double>{fs * cs, -fs * sn});
+}
+
 // wrapImage(hermx) of a stored (Nk/2+1) x (Nk+1) k-image onto (N/2+1) x N, rows ky=-N/2..N/2-1
 __global__ void __launch_bounds__(kThreads)
 wrap_hermx_f64(const cplx<double>* __restrict__ kin, int Nk, int N, cplx<double>* __restrict__ kout)
@@ -949,10 +1001,10 @@ int fold_impl(const gsmi_stamp* stamps, int nstamps, int N, const gsmi_op* ops,
         set_error("twiddle table allocation failed");
         return GSMI_E_HIP;
     }
-    const int cpg = small_launch_tile(cols_per_group(N), N / 2 + 1, nstamps);
+    const int cpg = small_launch_tile(cols_per_group(N), N / 2, nstamps);
     int rc = launch_axis_factors<T>(stamps, nstamps, ops, leaves, scratch, stream);
     if (rc) return rc;
-    dim3 g1((N / 2 + 1 + cpg - 1) / cpg, nstamps);
+    dim3 g1((N / 2 + cpg - 1) / cpg, nstamps);
     size_t lds1 = 2 * size_t(cpg) * lds_line(N) * sizeof(cplx<T>);
     if ((rc = allow_lds(reinterpret_cast<const void*>(FoldKernel<T>::fn), lds1))) return rc;
     hipLaunchKernelGGL(FoldKernel<T>::fn, g1, dim3(kThreads), lds1,
@@ -1208,6 +1260,24 @@ int gsmi_draw_kimage_f64(const gsmi_op* ops, int nops, const gsmi_leaf* leaves,
                        reinterpret_cast<hipStream_t>(stream), ops, nops, leaves, tb, xmin, ymin,
                        nx, ny, dk, Nk, reinterpret_cast<cplx<double>*>(out));
     return hip_check(hipGetLastError(), "draw_kimage launch");
+}
+
+int gsmi_apply_kimage_phases_f64(double* kimage, int xmin, int ymin, int nx, int ny, double dk,
+                                 const double* jac, double cenx, double ceny, double flux_scaling,
+                                 void* stream)
+{
+    if (!kimage || nx <= 0 || ny <= 0) {
+        set_error("gsmi_apply_kimage_phases_f64: bad arguments");
+        return GSMI_E_ARG;
+    }
+    const double A = jac ? jac[0] : 1., B = jac ? jac[1] : 0., C = jac ? jac[2] : 0.,
+                 D = jac ? jac[3] : 1.;
+    const int total = nx * ny;
+    hipLaunchKernelGGL(apply_kimage_phases_f64, dim3((total + kThreads - 1) / kThreads),
+                       dim3(kThreads), 0, reinterpret_cast<hipStream_t>(stream),
+                       reinterpret_cast<cplx<double>*>(kimage), xmin, ymin, nx, ny, dk, A, B, C,
+                       D, cenx, ceny, flux_scaling);
+    return hip_check(hipGetLastError(), "apply_kimage_phases launch");
 }
 
 int gsmi_wrap_hermx_f64(const double* kin, int Nk, int N, double* kout, void* stream)

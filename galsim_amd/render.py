@@ -326,11 +326,19 @@ def _draw_fft_unfused(prof, bounds, N, Nk, dk, return_device=False):
     return host, float(host.sum(dtype=float))
 
 
-def draw_kimage(prof, bounds, dk):
-    """k-image of ``prof`` on ``bounds`` with spacing dk (complex128)."""
+def draw_kimage(prof, bounds, dk, jac=None, Nk=None, return_device=False):
+    """k-image of ``prof`` on ``bounds`` with spacing dk (complex128): SBProfile::drawK
+    (src/SBProfile.cpp:383-425), k mapped by the 2x2 Jacobian ``jac`` if given.  With ``Nk`` the
+    bounds are drawFFT_makeKImage's (0..Nk/2, -Nk/2..Nk/2) and Exponential / Spergel leaves clip
+    rows exactly as the reference's GetKValueRange1d/2d does on that image; otherwise they clip
+    at k^2 > ksq_max.  Returns a host array, or with return_device a (ny, nx, 2) float64 device
+    tensor."""
     lib = _lib.lib()
-    ctx = Lowering('f64', dk=dk)
-    prof._lower(ctx, None)
+    if Nk is not None and not (bounds.xmin == 0 and bounds.xmax == Nk // 2 and
+                               bounds.ymin == -(Nk // 2) and bounds.ymax == Nk // 2):
+        raise _lib.GalSimAMDError("clip rows need the bounds (0..Nk/2, -Nk/2..Nk/2)")
+    ctx = Lowering('f64', Nk=Nk, dk=dk)
+    prof._lower(ctx, None if jac is None else np.asarray(jac, dtype=float).reshape(2, 2))
     check_stack(ctx.ops, precision='f64')
     ny, nx = bounds.numpyShape()
     torch = _torch()
@@ -339,13 +347,18 @@ def draw_kimage(prof, bounds, dk):
     o_leaves = pk.add(ctx.leaves_array())
     tables = np.concatenate(ctx.tables) if ctx.tables else np.zeros(1)
     o_tab = pk.add(tables)
+    clip = np.concatenate(ctx.clip) if ctx.clip else np.zeros(2, dtype=np.int32)
+    o_clip = pk.add(clip)
     dev = _device_buffer(torch, pk.host())
     base = dev.data_ptr()
     out = torch.empty((ny, nx, 2), dtype=torch.float64, device='cuda')
     rc = lib.gsmi_draw_kimage_f64(base + o_ops, len(ctx.ops), base + o_leaves, base + o_tab,
-                                  None, bounds.xmin, bounds.ymin, nx, ny, 0, float(dk),
-                                  out.data_ptr(), torch.cuda.current_stream().cuda_stream)
+                                  base + o_clip, bounds.xmin, bounds.ymin, nx, ny,
+                                  int(Nk or 0), float(dk), out.data_ptr(),
+                                  torch.cuda.current_stream().cuda_stream)
     _lib.check(rc, "gsmi_draw_kimage_f64")
+    if return_device:
+        return out
     h = out.cpu().numpy()
     return h[..., 0] + 1j * h[..., 1]
 

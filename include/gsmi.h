@@ -133,7 +133,8 @@ typedef struct gsmi_stamp {
  * constants, axis factors and twiddles, staged in LDS when fold_lds has room for them). */
 typedef struct gsmi_work {
     int32_t stamp;        /* index into the stamps array                                      */
-    int32_t first;        /* first column (c = 0..N/2) or first output row                    */
+    int32_t first;        /* first column (c = 0..N/2-1: column 0 also carries kx = N/2, whose
+                             transform along ky, like column 0's, is real) or first output row */
     int32_t count;        /* columns or rows in this item                                      */
     int32_t per_group;    /* columns or rows per chunk                                         */
 } gsmi_work;
@@ -328,6 +329,15 @@ int gsmi_irfft_f64(const double* kin, int N, double* out, void* scratch, void* s
  * array. */
 #define GSMI_NOALLOC_MAX_N 2048
 int gsmi_irfft_f64_noalloc(const double* kin, int N, double* out, void* stream);
+
+/* ApplyKImagePhases (src/SBTransform.cpp:763-883; pybind pysrc/SBTransform.cpp:34-46) on a
+ * device k-image: nx x ny complex128, row-major, pixel (xmin + i, ymin + j) at k = (x, y) dk,
+ * multiplied by flux_scaling exp(-i (kx' cenx + ky' ceny)) with (kx', ky') the k mapped by the
+ * host 2x2 Jacobian jac (dudx, dudy, dvdx, dvdy; null for none) as in SBProfile::drawK.  The
+ * phase is evaluated per pixel (the reference's renormalised recurrence drifts by ~N ulp). */
+int gsmi_apply_kimage_phases_f64(double* kimage, int xmin, int ymin, int nx, int ny, double dk,
+                                 const double* jac, double cenx, double ceny, double flux_scaling,
+                                 void* stream);
 
 /* wrapImage(hermx=true, hermy=false) (src/Image.cpp:577-726; pybind pysrc/Image.cpp:80):
  * folds the (Nk/2+1) x (Nk+1) k-image (rows ky=-Nk/2..Nk/2) onto (N/2+1) x N (rows

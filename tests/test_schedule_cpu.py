@@ -31,7 +31,7 @@ def test_work_lists_cover_every_column_and_row_once(cols, rows, chunks):
     for s in range(len(recs)):
         N, ny = int(recs['N'][s]), int(recs['ny'][s])
         f = fold[fold[:, 0] == s]
-        seen = np.zeros(N // 2 + 1, int)
+        seen = np.zeros(N // 2, int)            # kx = N/2 travels with column 0
         staged = (2 * int(f[0, 3]) * N + N + 3 * N + 2) * 8 <= STAGE_LDS
         k = (chunks or {}).get(N) if staged else 1
         for _, c0, cnt, per in f:
