@@ -35,6 +35,44 @@ def _image_profile(prof, nx, ny, scale, offset, use_true_center, method):
     return p, b.shift(PositionI(-c.x, -c.y)), flux_scale
 
 
+# per-stamp status bits (BatchPlan.status); STAMP_NONFINITE (1) is the kernels' own
+STAMP_BAD_PARAMS = 2        # a parameter or size was not finite: the stamp was not drawn
+STAMP_FFT_TOO_LARGE = 4     # Nk > gsparams.maximum_fft_size (drawn; the reference warns)
+STAMP_RERUN_FP64 = 8        # redrawn on the FP64 per-stamp path by rerun_flagged
+
+
+def _plan_stamp(ctx, prof, nx, ny, scale, offset, use_true_center, method, precision,
+                differentiable, moffat_generic):
+    """One stamp's sizes (drawFFT_makeKImage, galsim/gsobject.py:1927-1973) and lowering into
+    ctx.  Returns (record, N, warn) with warn set where Nk exceeds maximum_fft_size; raises
+    ValueError for non-finite parameters or sizes."""
+    p, b, _ = _image_profile(prof, nx, ny, scale, offset, use_true_center, method)
+    for v in (p.maxk, p.stepk, p.flux):
+        if not math.isfinite(v):
+            raise ValueError("non-finite profile size or flux")
+    N, Nk, dk = render.plan_sizes(p, b)
+    warn = Nk > p.gsparams.maximum_fft_size
+    ctx.Nk, ctx.dk = Nk, dk
+    op0, leaf0 = len(ctx.ops), len(ctx.leaves)
+    p._lower(ctx, None)
+    nops, naxis = render.hoist_axis_factors(ctx, op0)
+    for k in range(op0, len(ctx.ops)):
+        code, arg, a, bb, c = ctx.ops[k]
+        if code == _lib.OP_LEAF:
+            ctx.ops[k] = (code, arg - leaf0, a, bb, c)
+        if not (math.isfinite(a) and math.isfinite(bb) and math.isfinite(c)):
+            raise ValueError("non-finite program constant")
+    for rec in ctx.leaves[leaf0:]:
+        if not (np.all(np.isfinite(rec['jac'])) and math.isfinite(float(rec['scale'])) and
+                math.isfinite(float(rec['flux'])) and np.all(np.isfinite(rec['p']))):
+            raise ValueError("non-finite leaf constant")
+    render.check_stack(ctx.ops, op0, nops, precision)
+    if differentiable:
+        naxis += _prepare_differentiable(ctx, op0, nops, naxis, leaf0, moffat_generic)
+    rec = (N, Nk, nx, ny, b.xmin, b.ymin, op0, nops, leaf0, naxis, 0, 0, 0, dk, 0, 0)
+    return rec, N, warn
+
+
 def _special_knorm2(kind):
     """knorm / flux of the closed-form Moffat kinds (src/SBMoffat.cpp:497-549)."""
     return {15: 1., 2: 1., 25: 1., 3: 0.5, 35: 1. / 3., 4: 0.125}[kind]
@@ -175,10 +213,13 @@ class BatchPlan:
     current stream (no allocation, no host transfer: it can be captured in a HIP graph)."""
 
     def __init__(self, profiles, nx, ny, scale, offsets=None, precision='f32', method='auto',
-                 use_true_center=True, differentiable=False, moffat_generic=False):
+                 use_true_center=True, differentiable=False, moffat_generic=False,
+                 on_error='flag'):
         import torch
         if precision not in ('f32', 'f64'):
             raise ValueError("precision must be 'f32' or 'f64'")
+        if on_error not in ('flag', 'raise'):
+            raise ValueError("on_error must be 'flag' or 'raise'")
         n = len(profiles)
         nx = np.broadcast_to(np.asarray(nx, dtype=np.int64), (n,))
         ny = np.broadcast_to(np.asarray(ny, dtype=np.int64), (n,))
@@ -186,41 +227,53 @@ class BatchPlan:
             offsets = np.zeros((n, 2))
         self.precision = precision
         self.n = n
+        self.profiles = list(profiles)
+        self._draw_args = (nx, ny, float(scale), np.asarray(offsets, dtype=float), method,
+                           use_true_center)
         ctx = render.Lowering(precision)
         recs = np.zeros(n, dtype=_lib.STAMP_DTYPE)
         Ns = np.zeros(n, dtype=np.int64)
+        # per-stamp status (input order): a stamp whose planning fails is left out of the launch
+        # (drawn as zeros) and flagged instead of stopping the batch, unless on_error='raise'
+        self.host_status = np.zeros(n, dtype=np.int32)
+        keep = np.ones(n, dtype=bool)
         for i, prof in enumerate(profiles):
-            p, b, _ = _image_profile(prof, int(nx[i]), int(ny[i]), float(scale), offsets[i],
-                                     use_true_center, method)
-            N, Nk, dk = render.plan_sizes(p, b)
-            ctx.Nk, ctx.dk = Nk, dk
-            op0, leaf0 = len(ctx.ops), len(ctx.leaves)
-            p._lower(ctx, None)
-            nops, naxis = render.hoist_axis_factors(ctx, op0)
-            for k in range(op0, len(ctx.ops)):
-                code, arg, a, bb, c = ctx.ops[k]
-                if code == _lib.OP_LEAF:
-                    ctx.ops[k] = (code, arg - leaf0, a, bb, c)
-            render.check_stack(ctx.ops, op0, nops, precision)
-            if differentiable:
-                naxis += _prepare_differentiable(ctx, op0, nops, naxis, leaf0, moffat_generic)
-            recs[i] = (N, Nk, nx[i], ny[i], b.xmin, b.ymin, op0, nops, leaf0, naxis,
-                       0, 0, 0, dk)
+            nops0, nleaves0 = len(ctx.ops), len(ctx.leaves)
+            try:
+                rec, N, warn = _plan_stamp(ctx, prof, int(nx[i]), int(ny[i]), float(scale),
+                                           offsets[i], use_true_center, method, precision,
+                                           differentiable, moffat_generic)
+            except (ValueError, OverflowError, ZeroDivisionError, _lib.GalSimAMDError) as e:
+                if on_error == 'raise':
+                    raise
+                del ctx.ops[nops0:]
+                del ctx.leaves[nleaves0:]
+                self.host_status[i] |= STAMP_BAD_PARAMS
+                keep[i] = False
+                continue
+            recs[i] = rec
             Ns[i] = N
-        # bin by N; stamp records sorted so each bin is contiguous
-        order = np.argsort(Ns, kind='stable')
+            if warn:
+                self.host_status[i] |= STAMP_FFT_TOO_LARGE
+        self.dropped = np.nonzero(~keep)[0]
+        # bin by N; stamp records sorted so each bin is contiguous (planned stamps only)
+        kept = np.nonzero(keep)[0]
+        order = kept[np.argsort(Ns[kept], kind='stable')]
         recs = recs[order]
         self.order = order
-        out_off = np.zeros(n, dtype=np.int64)
+        self.nrec = len(recs)
+        def starts(sizes):
+            # exclusive prefix sum (offsets of consecutive regions)
+            return np.concatenate([[0], np.cumsum(sizes)[:-1]])[:len(sizes)].astype(np.int64)
         sizes = (recs['nx'].astype(np.int64) * recs['ny'].astype(np.int64))
-        out_off[1:] = np.cumsum(sizes)[:-1]
+        out_off = starts(sizes)
         recs['out_offset'] = out_off
         # scratch: every stamp's column-transform rows, then every stamp's axis factors, with
         # no region shared between stamps (a work-list launch runs all FFT sizes at once)
         rows = recs['ny'].astype(np.int64) * (recs['N'].astype(np.int64) // 2 + 1)
         axis = 3 * (recs['Nk'].astype(np.int64) // 2) + 2
-        recs['scratch_offset'] = np.concatenate([[0], np.cumsum(rows)[:-1]])
-        recs['axis_offset'] = int(rows.sum()) + np.concatenate([[0], np.cumsum(axis)[:-1]])
+        recs['scratch_offset'] = starts(rows)
+        recs['axis_offset'] = int(rows.sum()) + starts(axis)
         scratch_need = int(rows.sum() + axis.sum())
         self.groups = []
         for N in np.unique(recs['N']):
@@ -244,11 +297,15 @@ class BatchPlan:
         fdt = torch.float32 if precision == 'f32' else torch.float64
         self.scratch = torch.empty(2 * max(scratch_need, 1), dtype=fdt, device='cuda')
         self.out = torch.empty(int(sizes.sum()), dtype=fdt, device='cuda')
-        # per input stamp: (offset, ny, nx) in the flat output
-        inv = np.empty(n, dtype=np.int64)
-        inv[order] = np.arange(n)
-        self.stamp_offset = out_off[inv]
-        self.stamp_shape = np.stack([recs['ny'][inv], recs['nx'][inv]], axis=1)
+        # per input stamp: (offset, ny, nx) in the flat output (-1: a stamp left out)
+        inv = np.full(n, -1, dtype=np.int64)
+        inv[order] = np.arange(len(order))
+        self.rec_of = inv
+        self.stamp_offset = np.where(inv >= 0, out_off[np.maximum(inv, 0)] if len(order) else 0,
+                                     -1)
+        self.stamp_shape = np.stack([ny, nx], axis=1).astype(np.int64)
+        self._rerun = np.zeros(n, dtype=np.int32)
+        self._rerun_cleared = np.zeros(n, dtype=bool)
         self.nops = len(ctx.ops)
         self.nleaves = len(ctx.leaves)
         self.program_shape = _program_shape(recs, self.host_ops, self.host_leaves)
@@ -329,7 +386,8 @@ class BatchPlan:
         lib = _lib.lib()
         base = self.dev.data_ptr()
         sc = ctypes.addressof(sched[1])
-        args_fold = (base + self._o_stamp, self.n, sc, base + self._o_ops, base + self._o_leaves,
+        args_fold = (base + self._o_stamp, self.nrec, sc, base + self._o_ops,
+                     base + self._o_leaves,
                      base + self._o_tab)
         if part in ('fold', 'all'):
             if self.precision == 'f32':
@@ -341,7 +399,7 @@ class BatchPlan:
             _lib.check(rc, "fold work list")
         if part in ('rows', 'all'):
             f = lib.gsmi_rows_list_f32 if self.precision == 'f32' else lib.gsmi_rows_list_f64
-            rc = f(base + self._o_stamp, self.n, sc, self.scratch.data_ptr(),
+            rc = f(base + self._o_stamp, self.nrec, sc, self.scratch.data_ptr(),
                    self.out.data_ptr(), s)
             _lib.check(rc, "row work list")
 
@@ -441,8 +499,14 @@ class BatchPlan:
     def replay(self):
         if self._graph is None:
             raise _lib.GalSimAMDError("capture() the batch first")
+        self._reset_reruns()
         self._graph.replay()
         return self.out
+
+    def _reset_reruns(self):
+        # a new run overwrites the stamps rerun_flagged redrew
+        self._rerun[:] = 0
+        self._rerun_cleared[:] = False
 
     def run(self, stream=None, events=None):
         """Launch the batch: with a work-list schedule (use_schedule / autotune), the axis
@@ -452,6 +516,7 @@ class BatchPlan:
         and after the last ('end', 0)."""
         import torch
         lib = _lib.lib()
+        self._reset_reruns()
         base = self.dev.data_ptr()
         s = torch.cuda.current_stream().cuda_stream if stream is None else stream
         sc = self.scratch.data_ptr()
@@ -490,10 +555,52 @@ class BatchPlan:
         return self.out
 
     def stamp(self, i):
-        """Stamp i (input order) as a (ny, nx) device tensor view."""
-        off = int(self.stamp_offset[i])
+        """Stamp i (input order) as a (ny, nx) device tensor view (zeros for a stamp left out of
+        the launch, see ``status``)."""
         ny, nx = (int(v) for v in self.stamp_shape[i])
+        off = int(self.stamp_offset[i])
+        if off < 0:
+            import torch
+            return torch.zeros((ny, nx), dtype=self.out.dtype, device='cuda')
         return self.out[off:off + ny * nx].view(ny, nx)
+
+    def status(self):
+        """Per-stamp status (input order), after a run: STAMP_BAD_PARAMS for a stamp whose
+        parameters or sizes were not finite (not drawn), STAMP_FFT_TOO_LARGE where Nk exceeds
+        gsparams.maximum_fft_size (drawn, as the reference warns at galsim/gsobject.py:1965),
+        STAMP_NONFINITE for a stamp with a NaN or infinite pixel (the kernels' own check),
+        STAMP_RERUN_FP64 for one redrawn by ``rerun_flagged``.  Reads the device records."""
+        st = self.host_status.copy()
+        if self.nrec:
+            import torch
+            torch.cuda.synchronize()
+            d = self.dev[self._o_stamp:self._o_stamp + self.nrec * _lib.STAMP_DTYPE.itemsize]
+            recs = d.cpu().numpy().view(_lib.STAMP_DTYPE)
+            st[self.order] |= recs['status'].astype(np.int32)
+        st |= self._rerun
+        st[self._rerun_cleared] &= ~_lib.STAMP_NONFINITE
+        return st
+
+    def rerun_flagged(self):
+        """Redraw every stamp whose pixels came out non-finite on the FP64 per-stamp path (an
+        FP32 intermediate can overflow where the FP64 one does not) and copy it into the
+        output.  Returns the input indices redrawn; their status gains STAMP_RERUN_FP64, and
+        keeps STAMP_NONFINITE only if the FP64 pixels are not finite either."""
+        import torch
+        st = self.status()
+        redo = np.nonzero((st & _lib.STAMP_NONFINITE) != 0)[0]
+        nx, ny, scale, offsets, method, utc = self._draw_args
+        for i in redo:
+            im = self.profiles[i].drawImage(nx=int(nx[i]), ny=int(ny[i]), scale=scale,
+                                            offset=tuple(offsets[i]), method=method,
+                                            use_true_center=utc, dtype=np.float64,
+                                            precision='f64')
+            a = torch.from_numpy(np.ascontiguousarray(im.array)).to('cuda', self.out.dtype)
+            self.stamp(i).copy_(a)
+            self._rerun[i] |= STAMP_RERUN_FP64
+            if torch.isfinite(a).all():
+                self._rerun_cleared[i] = True
+        return redo
 
     def fft_sizes(self):
         return {N: count for N, _, count, _ in self.groups}

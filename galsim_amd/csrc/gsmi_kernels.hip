@@ -32,10 +32,11 @@ namespace gsmi {
 // ---------------------------------------------------------------------------------------------
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
-axis_factors(const gsmi_stamp* __restrict__ stamps, const gsmi_op* __restrict__ ops,
+axis_factors(gsmi_stamp* __restrict__ stamps, const gsmi_op* __restrict__ ops,
              const gsmi_leaf* __restrict__ leaves, cplx<T>* __restrict__ ws)
 {
-    const gsmi_stamp* st = stamps + blockIdx.x;
+    gsmi_stamp* st = stamps + blockIdx.x;
+    if (threadIdx.x == 0) st->status = 0;         // this launch's status starts clear
     const int hk = st->Nk / 2;
     const T dk = T(st->dk);
     const gsmi_op* aops = ops + st->op_begin + st->op_count;
@@ -495,7 +496,7 @@ template <typename T>
 __device__ __forceinline__ void row_c2r_body(cplx<T>* lds, int rpg, const cplx<T>* __restrict__ in,
                                              int rs, int cs, int nrow, int N, int x0, int nx,
                                              const cplx<T>* __restrict__ tw,
-                                             T* __restrict__ out, int tws = 1)
+                                             T* __restrict__ out, int tws, bool& bad)
 {
     const int n2 = N / 2;
     const int L2 = lds_line(n2);
@@ -531,7 +532,9 @@ __device__ __forceinline__ void row_c2r_body(cplx<T>* lds, int rpg, const cplx<T
         const int p = x0 + col + N;                   // >= 0: x0 >= -N/2
         const int q = p - N * fdiv(p, inv_n);
         const cplx<T> v = res[rl * L2 + lds_pad(q >> 1)];
-        out[size_t(rl) * nx + col] = ((q & 1) ? v.y : v.x) * norm;
+        const T px = (q & 1) ? v.y : v.x;
+        out[size_t(rl) * nx + col] = px * norm;
+        bad |= !isfinite(px);
     }
 }
 
@@ -614,7 +617,7 @@ __device__ __forceinline__ void rows_mid(const cplx<T>* src, cplx<T>* dst, int n
 
 template <typename T, int N, int R, int NS>
 __device__ __forceinline__ void rows_last(const cplx<T>* src, int nrow, const cplx<T>* twl,
-                                          int x0, int nx, T* __restrict__ out)
+                                          int x0, int nx, T* __restrict__ out, bool& bad)
 {
     constexpr int M = N / 2, L = M + 1, NB = M / R;          // NB == NS
     const T norm = T(1) / (T(N) * T(N));
@@ -641,24 +644,30 @@ __device__ __forceinline__ void rows_last(const cplx<T>* src, int nrow, const cp
             col = col >= N ? col - N : (col < 0 ? col + N : col);
             int col1 = col + 1;
             col1 = col1 >= N ? col1 - N : col1;
-            if (col < nx) o[col] = v[r].x * norm;
-            if (col1 < nx) o[col1] = v[r].y * norm;
+            if (col < nx) {
+                o[col] = v[r].x * norm;
+                bad |= !isfinite(v[r].x);
+            }
+            if (col1 < nx) {
+                o[col1] = v[r].y * norm;
+                bad |= !isfinite(v[r].y);
+            }
         }
     }
 }
 
 template <typename T, int N, int NS>
 __device__ __forceinline__ void rows_rest(cplx<T>* src, cplx<T>* dst, int nrow, float inv_nrow,
-                                          const cplx<T>* twl, int x0, int nx, T* out)
+                                          const cplx<T>* twl, int x0, int nx, T* out, bool& bad)
 {
     constexpr int rem = (N / 2) / NS;
     constexpr int R = row_radix(rem);
     if constexpr (rem == R) {
-        rows_last<T, N, R, NS>(src, nrow, twl, x0, nx, out);
+        rows_last<T, N, R, NS>(src, nrow, twl, x0, nx, out, bad);
     } else {
         rows_mid<T, N, R, NS>(src, dst, nrow, inv_nrow, twl);
         __syncthreads();
-        rows_rest<T, N, NS * R>(dst, src, nrow, inv_nrow, twl, x0, nx, out);
+        rows_rest<T, N, NS * R>(dst, src, nrow, inv_nrow, twl, x0, nx, out, bad);
     }
 }
 
@@ -667,7 +676,7 @@ __device__ __forceinline__ void rows_rest(cplx<T>* src, cplx<T>* dst, int nrow, 
 template <typename T, int N>
 __device__ __forceinline__ void rows_fixed(cplx<T>* lds, int rpg, const cplx<T>* __restrict__ in,
                                            int cs, int nrow, int x0, int nx, const cplx<T>* twl,
-                                           T* __restrict__ out)
+                                           T* __restrict__ out, bool& bad)
 {
     constexpr int M = N / 2;
     constexpr int R1 = row_radix(M);
@@ -676,7 +685,7 @@ __device__ __forceinline__ void rows_fixed(cplx<T>* lds, int rpg, const cplx<T>*
     const float inv_nrow = 1.f / float(nrow);
     rows_first<T, N, R1>(in, cs, nrow, inv_nrow, twl, bufa);
     __syncthreads();
-    rows_rest<T, N, R1>(bufa, bufb, nrow, inv_nrow, twl, x0, nx, out);
+    rows_rest<T, N, R1>(bufa, bufb, nrow, inv_nrow, twl, x0, nx, out, bad);
 }
 
 // rows_fixed for the compiled sizes; false (nothing done) for any other N.
@@ -684,11 +693,11 @@ template <typename T>
 __device__ __forceinline__ bool rows_fixed_any(int N, cplx<T>* lds, int rpg,
                                                const cplx<T>* __restrict__ in, int cs, int nrow,
                                                int x0, int nx, const cplx<T>* twl,
-                                               T* __restrict__ out)
+                                               T* __restrict__ out, bool& bad)
 {
     switch (N) {
 #define GSMI_ROWS_FIXED(NN)                                                                     \
-      case NN: rows_fixed<T, NN>(lds, rpg, in, cs, nrow, x0, nx, twl, out); return true;
+      case NN: rows_fixed<T, NN>(lds, rpg, in, cs, nrow, x0, nx, twl, out, bad); return true;
       GSMI_ROWS_FIXED(64)
       GSMI_ROWS_FIXED(96)
       GSMI_ROWS_FIXED(128)
@@ -703,20 +712,28 @@ __device__ __forceinline__ bool rows_fixed_any(int N, cplx<T>* lds, int rpg,
 }
 
 // grid: (ceil(ny/rpg), nstamps); dynamic LDS: 2 * rpg * N/2 complex
+// A workgroup's verdict on its pixels: any non-finite one sets the stamp's status bit.
+__device__ __forceinline__ void flag_nonfinite(gsmi_stamp* st, bool bad)
+{
+    if (__syncthreads_or(bad) && threadIdx.x == 0) atomicOr(&st->status, GSMI_STAMP_NONFINITE);
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
-row_c2r(const gsmi_stamp* __restrict__ stamps, const cplx<T>* __restrict__ scratch,
+row_c2r(gsmi_stamp* __restrict__ stamps, const cplx<T>* __restrict__ scratch,
         const cplx<T>* __restrict__ tw, T* __restrict__ out, int rpg)
 {
     extern __shared__ unsigned char smem_raw[];
     cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
-    const gsmi_stamp* st = stamps + blockIdx.y;
+    gsmi_stamp* st = stamps + blockIdx.y;
     const int N = st->N;
     const int r0 = blockIdx.x * rpg;
     const int nrow = min(rpg, st->ny - r0);
     if (nrow <= 0) return;
+    bool bad = false;
     row_c2r_body<T>(lds, rpg, scratch + st->scratch_offset + r0, 1, st->ny, nrow, N, st->x0,
-                    st->nx, tw, out + st->out_offset + size_t(r0) * st->nx);
+                    st->nx, tw, out + st->out_offset + size_t(r0) * st->nx, 1, bad);
+    flag_nonfinite(st, bad);
 }
 
 // Work-list form of row_c2r: items (stamp, first row, rows, rows per chunk).  An item may span
@@ -724,7 +741,7 @@ row_c2r(const gsmi_stamp* __restrict__ stamps, const cplx<T>* __restrict__ scrat
 // launch's LDS has room for them after the two transform buffers.
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
-rows_list(const gsmi_stamp* __restrict__ stamps, int nstamps,
+rows_list(gsmi_stamp* __restrict__ stamps, int nstamps,
           const gsmi_work* __restrict__ items, int nitems, int lds_bytes,
           const cplx<T>* __restrict__ scratch, TwiddleSet<T> tws, T* __restrict__ out)
 {
@@ -733,7 +750,7 @@ rows_list(const gsmi_stamp* __restrict__ stamps, int nstamps,
     for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
         const gsmi_work it = items[w];
         if (it.stamp < 0 || it.stamp >= nstamps) continue;
-        const gsmi_stamp* st = stamps + it.stamp;
+        gsmi_stamp* st = stamps + it.stamp;
         const int N = st->N;
         const size_t bufs = size_t(2) * it.per_group * lds_line(N / 2);
         if (it.count <= 0 || it.first < 0 || it.first + it.count > st->ny ||
@@ -742,6 +759,7 @@ rows_list(const gsmi_stamp* __restrict__ stamps, int nstamps,
         int stride;
         const cplx<T>* tw = tws.pick(N, &stride);
         const int end = it.first + it.count;
+        bool bad = false;
         if ((bufs + N) * sizeof(cplx<T>) <= size_t(lds_bytes)) {
             cplx<T>* twl = lds + bufs;
             for (int e = threadIdx.x; e < N; e += blockDim.x) twl[e] = tw[e * stride];
@@ -751,19 +769,20 @@ rows_list(const gsmi_stamp* __restrict__ stamps, int nstamps,
                 const cplx<T>* in = scratch + st->scratch_offset + r0;
                 T* o = out + st->out_offset + size_t(r0) * st->nx;
                 if (!rows_fixed_any<T>(N, lds, it.per_group, in, st->ny, nrow, st->x0, st->nx,
-                                       twl, o))
+                                       twl, o, bad))
                     row_c2r_body<T>(lds, it.per_group, in, 1, st->ny, nrow, N, st->x0, st->nx,
-                                    twl, o, 1);
+                                    twl, o, 1, bad);
                 __syncthreads();
             }
         } else {
             for (int r0 = it.first; r0 < end; r0 += it.per_group) {
                 row_c2r_body<T>(lds, it.per_group, scratch + st->scratch_offset + r0, 1, st->ny,
                                 min(it.per_group, end - r0), N, st->x0, st->nx, tw,
-                                out + st->out_offset + size_t(r0) * st->nx, stride);
+                                out + st->out_offset + size_t(r0) * st->nx, stride, bad);
                 __syncthreads();
             }
         }
+        flag_nonfinite(st, bad);
     }
 }
 
@@ -779,8 +798,9 @@ irfft_rows_f64(const cplx<double>* __restrict__ scratch, int N, const cplx<doubl
     const int nrow = min(rpg, N - r0);
     if (nrow <= 0) return;
     const int W = N / 2 + 1;
+    bool bad = false;
     row_c2r_body<double>(lds, rpg, scratch + size_t(r0) * W, W, 1, nrow, N, -N / 2, N, tw,
-                         out + size_t(r0) * N, tws);
+                         out + size_t(r0) * N, tws, bad);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -964,9 +984,10 @@ template <typename T>
 int launch_axis_factors(const gsmi_stamp* stamps, int nstamps, const gsmi_op* ops,
                         const gsmi_leaf* leaves, void* scratch, void* stream)
 {
+    // the kernel also clears each record's status word (the only field a kernel writes)
     hipLaunchKernelGGL(axis_factors<T>, dim3(nstamps), dim3(kThreads), 0,
-                       reinterpret_cast<hipStream_t>(stream), stamps, ops, leaves,
-                       reinterpret_cast<cplx<T>*>(scratch));
+                       reinterpret_cast<hipStream_t>(stream), const_cast<gsmi_stamp*>(stamps),
+                       ops, leaves, reinterpret_cast<cplx<T>*>(scratch));
     return hip_check(hipGetLastError(), "axis_factors launch");
 }
 template int launch_axis_factors<double>(const gsmi_stamp*, int, const gsmi_op*,
@@ -1033,7 +1054,8 @@ int rows_impl(const gsmi_stamp* stamps, int nstamps, int N, int max_ny, const vo
     int rc = allow_lds(reinterpret_cast<const void*>(row_c2r<T>), lds2);
     if (rc) return rc;
     hipLaunchKernelGGL(row_c2r<T>, g2, dim3(kThreads), lds2, reinterpret_cast<hipStream_t>(stream),
-                       stamps, reinterpret_cast<const cplx<T>*>(scratch), tw, out, rpg);
+                       const_cast<gsmi_stamp*>(stamps), reinterpret_cast<const cplx<T>*>(scratch),
+                       tw, out, rpg);
     return hip_check(hipGetLastError(), "row_c2r launch");
 }
 
@@ -1115,7 +1137,8 @@ int rows_list_impl(const gsmi_stamp* stamps, int nstamps, const gsmi_schedule* s
     }
     if ((rc = allow_lds(reinterpret_cast<const void*>(rows_list<T>), sc->rows_lds))) return rc;
     hipLaunchKernelGGL(rows_list<T>, dim3(min(sc->rows_grid, sc->nrows)), dim3(kThreads),
-                       sc->rows_lds, reinterpret_cast<hipStream_t>(stream), stamps, nstamps,
+                       sc->rows_lds, reinterpret_cast<hipStream_t>(stream),
+                       const_cast<gsmi_stamp*>(stamps), nstamps,
                        sc->rows, sc->nrows, sc->rows_lds,
                        reinterpret_cast<const cplx<T>*>(scratch), tws, out);
     return hip_check(hipGetLastError(), "rows list launch");

@@ -42,9 +42,11 @@ class DiffPlan:
     def __init__(self, profiles, nx, ny, scale, offsets=None, precision='f32',
                  use_true_center=True, moffat_generic=False, shape_gradients=False):
         import torch
+        # a differentiable batch maps fields to every stamp: a stamp that cannot be planned
+        # stops it (no stamps are left out)
         self.plan = BatchPlan(profiles, nx, ny, scale, offsets, precision, 'auto',
                               use_true_center, differentiable=True,
-                              moffat_generic=moffat_generic)
+                              moffat_generic=moffat_generic, on_error='raise')
         p = self.plan
         n = p.n
         self.n = n

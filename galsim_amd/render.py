@@ -247,7 +247,8 @@ def draw_fft(prof, bounds, precision='f64', return_device=False):
         raise _lib.GalSimAMDError("output bounds %r do not fit the N=%d grid" % (bounds, N))
     rows = ny * (N // 2 + 1)
     stamp = np.zeros(1, dtype=_lib.STAMP_DTYPE)
-    stamp[0] = (N, Nk, nx, ny, bounds.xmin, bounds.ymin, 0, nops, 0, naxis, 0, 0, rows, dk)
+    stamp[0] = (N, Nk, nx, ny, bounds.xmin, bounds.ymin, 0, nops, 0, naxis, 0, 0, rows, dk, 0,
+                0)
     torch = _torch()
     pk = _Packer()
     o_stamp = pk.add(stamp)

@@ -124,7 +124,14 @@ typedef struct gsmi_stamp {
                                values: N/2+1 columns of ny output rows, column-major          */
     int64_t axis_offset;  /* complex offset in scratch of its (Nk/2+1) + (Nk+1) axis factors  */
     double dk;            /* k spacing in pixel units: 2 pi / N                               */
+    int32_t status;       /* written by the kernels: cleared by each launch's axis prepass,
+                             GSMI_STAMP_NONFINITE set by the row transform when a pixel of the
+                             stamp is NaN or infinite                                          */
+    int32_t reserved;
 } gsmi_stamp;
+
+/* Per-stamp status bits of gsmi_stamp.status. */
+#define GSMI_STAMP_NONFINITE 1
 
 /* A work item of the work-list kernels: a group of columns (fold) or rows (row transform) of
  * one stamp.  per_group sizes the workgroup's LDS tile (columns or rows per chunk).  A row item

@@ -32,7 +32,8 @@ def test_library_exports_every_declared_symbol():
 def test_record_layouts():
     assert _lib.OP_DTYPE.itemsize == 32
     assert _lib.LEAF_DTYPE.itemsize == 128
-    assert _lib.STAMP_DTYPE.itemsize == 72
+    assert _lib.STAMP_DTYPE.itemsize == 80
+    assert _lib.STAMP_DTYPE.fields['status'][1] == 72
     assert _lib.STAMP_DTYPE.fields['axis_offset'][1] == 56
     assert _lib.LEAF_DTYPE.fields['jac'][1] == 24
     assert _lib.STAMP_DTYPE.fields['out_offset'][1] == 40

@@ -205,3 +205,42 @@ def test_batch_of_added_profiles_matches_drawimage():
             err = np.max(np.abs(b - im.array)) / np.max(np.abs(im.array))
             print(prec, i, err)
             assert err < tol, (prec, i, err)
+
+
+def test_bad_stamps_are_flagged_not_fatal_and_overflow_reruns_in_fp64():
+    # A batch with a NaN half-light radius (cannot be planned: flagged, drawn as zeros), an
+    # FP32 overflow (flux 3e38: the column transform's sums exceed the FP32 range, the kernels
+    # flag the stamp's non-finite pixels, rerun_flagged redraws it in FP64), and good stamps,
+    # which come out as they do in a batch of their own.
+    import torch
+    from galsim_amd import batch
+    from galsim_amd import _lib as L
+
+    def bd(m, hlr, flux):
+        gal = m.Exponential(half_light_radius=hlr, flux=flux).shear(g1=0.1, g2=0.2)
+        return m.Convolve([gal, m.Kolmogorov(fwhm=0.8)])
+    profs = [bd(g, 1.0, 1e3), bd(g, float('nan'), 1e3), bd(g, 0.4, 3e38), bd(g, 1.4, 2e3)]
+    print("status batch, 4 stamps of 48x48, FP32", flush=True)
+    plan = batch.BatchPlan(profs, 48, 48, 0.2, precision='f32')
+    plan.run()
+    st = plan.status()
+    print("status", st)
+    assert st[1] & batch.STAMP_BAD_PARAMS and not st[1] & L.STAMP_NONFINITE
+    assert st[2] & L.STAMP_NONFINITE
+    assert st[0] == 0 and st[3] == 0
+    assert torch.count_nonzero(plan.stamp(1)) == 0
+    ref = batch.BatchPlan([profs[0], profs[3]], 48, 48, 0.2, precision='f32')
+    ref.run()
+    assert torch.equal(plan.stamp(0), ref.stamp(0)) and torch.equal(plan.stamp(3), ref.stamp(1))
+    redo = plan.rerun_flagged()
+    assert list(redo) == [2]
+    st = plan.status()
+    assert st[2] & batch.STAMP_RERUN_FP64 and not st[2] & L.STAMP_NONFINITE
+    a = plan.stamp(2).double().cpu().numpy()
+    want = od.Convolve([od.Exponential(half_light_radius=0.4, flux=3e38).shear(g1=0.1, g2=0.2),
+                        od.Kolmogorov(fwhm=0.8)]).drawImage(48, 48, scale=0.2)
+    assert np.all(np.isfinite(a))
+    assert np.max(np.abs(a - want)) < 1e-6 * np.max(np.abs(want))
+    # a later run clears the device flags and the rerun marks
+    plan.run()
+    assert plan.status()[2] & L.STAMP_NONFINITE and not plan.status()[2] & batch.STAMP_RERUN_FP64

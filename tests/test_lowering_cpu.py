@@ -181,3 +181,27 @@ def test_fp32_sersic_spline_coefficients_against_compiled_reference(n, trunc):
     bad = coef64.copy()
     bad[:, 2] = 0.5 * np.diff(args) ** 2 * y2[1:]
     assert np.max(np.abs(_sersic_f32_emulated(rec, bad, ksq) - want)) > 1e-9 * peak
+
+
+def test_plan_stamp_flags_non_finite_parameters_and_large_ffts():
+    # the batched planner's per-stamp step: non-finite parameters raise (the batch flags the
+    # stamp instead of stopping), Nk beyond maximum_fft_size only warns (as the reference,
+    # galsim/gsobject.py:1965-1966)
+    from galsim_amd import batch, render
+    def bd(m, hlr, flux=1e3):
+        return m.Convolve([m.Exponential(half_light_radius=hlr, flux=flux),
+                           m.Kolmogorov(fwhm=0.8)])
+    ok = dict(nx=32, ny=32, scale=0.2, offset=(0., 0.), use_true_center=True, method='auto',
+              precision='f32', differentiable=False, moffat_generic=False)
+    for bad in (bd(g, float('nan')), bd(g, 1.0, flux=float('inf')), bd(g, 1.0, flux=float('nan'))):
+        ctx = render.Lowering('f32')
+        with pytest.raises(ValueError):
+            batch._plan_stamp(ctx, bad, **ok)
+    ctx = render.Lowering('f32')
+    rec, N, warn = batch._plan_stamp(ctx, bd(g, 1.0), **ok)
+    assert N == 128 and not warn and rec[0] == N
+    small = g.GSParams(maximum_fft_size=256)
+    big = g.Convolve([g.Exponential(half_light_radius=8.0, flux=1e3, gsparams=small),
+                      g.Kolmogorov(fwhm=0.8, gsparams=small)])
+    rec, N, warn = batch._plan_stamp(render.Lowering('f32'), big, **ok)
+    assert warn and rec[1] > 256
