@@ -23,7 +23,8 @@ class Leaf(GSObject):
         raise NotImplementedError
 
     def _lower(self, ctx, jac):
-        rec = np.zeros((), dtype=_lib.LEAF_DTYPE)
+        from .render import LeafRecord
+        rec = LeafRecord()
         sep, j = jac_record(jac)
         rec['type'] = self._leaf_type
         rec['separable'] = sep

@@ -47,9 +47,8 @@ def _plan_stamp(ctx, prof, nx, ny, scale, offset, use_true_center, method, preci
     ctx.  Returns (record, N, warn) with warn set where Nk exceeds maximum_fft_size; raises
     ValueError for non-finite parameters or sizes."""
     p, b, _ = _image_profile(prof, nx, ny, scale, offset, use_true_center, method)
-    for v in (p.maxk, p.stepk, p.flux):
-        if not math.isfinite(v):
-            raise ValueError("non-finite profile size or flux")
+    # a non-finite size stops here (int() of NaN or inf); non-finite constants are found by
+    # _nonfinite_stamps over the whole batch
     N, Nk, dk = render.plan_sizes(p, b)
     warn = Nk > p.gsparams.maximum_fft_size
     ctx.Nk, ctx.dk = Nk, dk
@@ -60,17 +59,33 @@ def _plan_stamp(ctx, prof, nx, ny, scale, offset, use_true_center, method, preci
         code, arg, a, bb, c = ctx.ops[k]
         if code == _lib.OP_LEAF:
             ctx.ops[k] = (code, arg - leaf0, a, bb, c)
-        if not (math.isfinite(a) and math.isfinite(bb) and math.isfinite(c)):
-            raise ValueError("non-finite program constant")
-    for rec in ctx.leaves[leaf0:]:
-        if not (np.all(np.isfinite(rec['jac'])) and math.isfinite(float(rec['scale'])) and
-                math.isfinite(float(rec['flux'])) and np.all(np.isfinite(rec['p']))):
-            raise ValueError("non-finite leaf constant")
     render.check_stack(ctx.ops, op0, nops, precision)
     if differentiable:
         naxis += _prepare_differentiable(ctx, op0, nops, naxis, leaf0, moffat_generic)
     rec = (N, Nk, nx, ny, b.xmin, b.ymin, op0, nops, leaf0, naxis, 0, 0, 0, dk, 0, 0)
     return rec, N, warn
+
+
+def _nonfinite_stamps(recs, ops, leaves):
+    """Indices of the records whose program constants or leaf constants are not finite."""
+    bad_op = ~(np.isfinite(ops['a']) & np.isfinite(ops['b']) & np.isfinite(ops['c']))
+    bad_leaf = ~(np.isfinite(leaves['jac']).all(axis=1) & np.isfinite(leaves['scale']) &
+                 np.isfinite(leaves['scale2']) & np.isfinite(leaves['flux']) &
+                 np.isfinite(leaves['p']).all(axis=1))
+    cop = np.concatenate([[0], np.cumsum(bad_op)])
+    clf = np.concatenate([[0], np.cumsum(bad_leaf)])
+    # each stamp's ops and leaves are contiguous blocks that do not interleave: a block runs to
+    # the next block's start
+    def ends(starts, total):
+        o = np.argsort(starts, kind='stable')
+        e = np.empty_like(starts)
+        e[o] = np.append(starts[o][1:], total)
+        return e
+    o0 = recs['op_begin'].astype(np.int64)
+    l0 = recs['leaf_begin'].astype(np.int64)
+    o1 = ends(o0, len(ops))
+    l1 = ends(l0, len(leaves))
+    return np.nonzero((cop[o1] - cop[o0] > 0) | (clf[l1] - clf[l0] > 0))[0]
 
 
 def _special_knorm2(kind):
@@ -208,58 +223,101 @@ def work_lists(recs, groups, cols, rows, elem, bins=None, chunks=None, extra=0,
             np.array(rw, dtype=np.int32).reshape(-1, 4), lds_f, lds_r)
 
 
+class HostPlan:
+    """A batch lowered on the host: stamp records in input order (``keep`` marks the planned
+    ones), the program, leaf, table and clip-row arrays of include/gsmi.h, per-stamp host
+    status, the drawing arguments, and ``profile(i)``, the profile object of stamp i (for the
+    FP64 re-run of flagged stamps)."""
+
+    def __init__(self, **kw):
+        self.__dict__.update(kw)
+
+
+def plan_host(profiles, nx, ny, scale, offsets=None, precision='f32', method='auto',
+              use_true_center=True, differentiable=False, moffat_generic=False,
+              on_error='flag'):
+    """The per-object planner: drawImage's steps for each profile (image-coordinate profile,
+    drawFFT_makeKImage sizes) and its lowering, stamp by stamp."""
+    if precision not in ('f32', 'f64'):
+        raise ValueError("precision must be 'f32' or 'f64'")
+    if on_error not in ('flag', 'raise'):
+        raise ValueError("on_error must be 'flag' or 'raise'")
+    n = len(profiles)
+    nx = np.broadcast_to(np.asarray(nx, dtype=np.int64), (n,)).copy()
+    ny = np.broadcast_to(np.asarray(ny, dtype=np.int64), (n,)).copy()
+    offsets = np.zeros((n, 2)) if offsets is None else np.asarray(offsets, dtype=float)
+    ctx = render.Lowering(precision)
+    recs = np.zeros(n, dtype=_lib.STAMP_DTYPE)
+    # per-stamp status (input order): a stamp whose planning fails is left out of the launch
+    # (drawn as zeros) and flagged instead of stopping the batch, unless on_error='raise'
+    host_status = np.zeros(n, dtype=np.int32)
+    keep = np.ones(n, dtype=bool)
+    for i, prof in enumerate(profiles):
+        nops0, nleaves0 = len(ctx.ops), len(ctx.leaves)
+        try:
+            rec, N, warn = _plan_stamp(ctx, prof, int(nx[i]), int(ny[i]), float(scale),
+                                       offsets[i], use_true_center, method, precision,
+                                       differentiable, moffat_generic)
+        except (ValueError, OverflowError, ZeroDivisionError, _lib.GalSimAMDError):
+            if on_error == 'raise':
+                raise
+            del ctx.ops[nops0:]
+            del ctx.leaves[nleaves0:]
+            host_status[i] |= STAMP_BAD_PARAMS
+            keep[i] = False
+            continue
+        recs[i] = rec
+        if warn:
+            host_status[i] |= STAMP_FFT_TOO_LARGE
+    # non-finite constants (a NaN or infinite flux, shear, offset...): flagged and dropped
+    ops, leaves = ctx.ops_array(), ctx.leaves_array()
+    cand = np.nonzero(keep)[0]
+    for i in cand[_nonfinite_stamps(recs[cand], ops, leaves)]:
+        if on_error == 'raise':
+            raise ValueError("stamp %d has a non-finite parameter" % i)
+        host_status[i] |= STAMP_BAD_PARAMS
+        keep[i] = False
+    profs = list(profiles)
+    return HostPlan(n=n, precision=precision, recs=recs, keep=keep, host_status=host_status,
+                    ops=ops, leaves=leaves,
+                    tables=np.concatenate(ctx.tables) if ctx.tables else np.zeros(1),
+                    tables32=ctx.tables32_array(),
+                    clip=np.concatenate(ctx.clip) if ctx.clip else np.zeros(2, dtype=np.int32),
+                    nx=nx, ny=ny, scale=float(scale), offsets=offsets, method=method,
+                    use_true_center=use_true_center, profile=lambda i: profs[i])
+
+
 class BatchPlan:
     """A lowered batch, resident on the device.  ``run()`` launches the kernels on torch's
     current stream (no allocation, no host transfer: it can be captured in a HIP graph)."""
 
     def __init__(self, profiles, nx, ny, scale, offsets=None, precision='f32', method='auto',
                  use_true_center=True, differentiable=False, moffat_generic=False,
-                 on_error='flag'):
+                 on_error='flag', lowered=None):
+        """Plan ``profiles`` (drawn on nx x ny stamps at ``scale`` with ``offsets``) with the
+        per-object planner, or take a ``HostPlan`` already lowered (``lowered``, e.g. from
+        galsim_amd.config), and set the batch up on the device."""
+        if lowered is None:
+            lowered = plan_host(profiles, nx, ny, scale, offsets, precision, method,
+                                use_true_center, differentiable, moffat_generic, on_error)
+        self._setup(lowered)
+
+    def _setup(self, hp):
         import torch
-        if precision not in ('f32', 'f64'):
-            raise ValueError("precision must be 'f32' or 'f64'")
-        if on_error not in ('flag', 'raise'):
-            raise ValueError("on_error must be 'flag' or 'raise'")
-        n = len(profiles)
-        nx = np.broadcast_to(np.asarray(nx, dtype=np.int64), (n,))
-        ny = np.broadcast_to(np.asarray(ny, dtype=np.int64), (n,))
-        if offsets is None:
-            offsets = np.zeros((n, 2))
+        n = hp.n
+        precision = hp.precision
         self.precision = precision
         self.n = n
-        self.profiles = list(profiles)
-        self._draw_args = (nx, ny, float(scale), np.asarray(offsets, dtype=float), method,
-                           use_true_center)
-        ctx = render.Lowering(precision)
-        recs = np.zeros(n, dtype=_lib.STAMP_DTYPE)
-        Ns = np.zeros(n, dtype=np.int64)
-        # per-stamp status (input order): a stamp whose planning fails is left out of the launch
-        # (drawn as zeros) and flagged instead of stopping the batch, unless on_error='raise'
-        self.host_status = np.zeros(n, dtype=np.int32)
-        keep = np.ones(n, dtype=bool)
-        for i, prof in enumerate(profiles):
-            nops0, nleaves0 = len(ctx.ops), len(ctx.leaves)
-            try:
-                rec, N, warn = _plan_stamp(ctx, prof, int(nx[i]), int(ny[i]), float(scale),
-                                           offsets[i], use_true_center, method, precision,
-                                           differentiable, moffat_generic)
-            except (ValueError, OverflowError, ZeroDivisionError, _lib.GalSimAMDError) as e:
-                if on_error == 'raise':
-                    raise
-                del ctx.ops[nops0:]
-                del ctx.leaves[nleaves0:]
-                self.host_status[i] |= STAMP_BAD_PARAMS
-                keep[i] = False
-                continue
-            recs[i] = rec
-            Ns[i] = N
-            if warn:
-                self.host_status[i] |= STAMP_FFT_TOO_LARGE
+        self.lowered = hp
+        self.host_status = hp.host_status.copy()
+        keep = hp.keep
+        Ns = hp.recs['N'].astype(np.int64)
+        host_ops, host_leaves = hp.ops, hp.leaves
         self.dropped = np.nonzero(~keep)[0]
         # bin by N; stamp records sorted so each bin is contiguous (planned stamps only)
         kept = np.nonzero(keep)[0]
         order = kept[np.argsort(Ns[kept], kind='stable')]
-        recs = recs[order]
+        recs = hp.recs[order]
         self.order = order
         self.nrec = len(recs)
         def starts(sizes):
@@ -281,16 +339,14 @@ class BatchPlan:
             self.groups.append((int(N), int(idx[0]), len(idx), int(recs['ny'][idx].max())))
         pk = render._Packer()
         self._o_stamp = pk.add(recs)
-        self._o_ops = pk.add(ctx.ops_array())
-        self._o_leaves = pk.add(ctx.leaves_array())
-        tables = np.concatenate(ctx.tables) if ctx.tables else np.zeros(1)
-        self._o_tab = pk.add(tables)
-        self._o_tab32 = pk.add(ctx.tables32_array())
-        clip = np.concatenate(ctx.clip) if ctx.clip else np.zeros(2, dtype=np.int32)
-        self._o_clip = pk.add(clip)
+        self._o_ops = pk.add(host_ops)
+        self._o_leaves = pk.add(host_leaves)
+        self._o_tab = pk.add(hp.tables)
+        self._o_tab32 = pk.add(hp.tables32)
+        self._o_clip = pk.add(hp.clip)
         self.recs = recs
-        self.host_ops = ctx.ops_array()
-        self.host_leaves = ctx.leaves_array()
+        self.host_ops = host_ops
+        self.host_leaves = host_leaves
         self.offsets = {'stamp': self._o_stamp, 'ops': self._o_ops, 'leaves': self._o_leaves,
                         'tables': self._o_tab, 'tables32': self._o_tab32, 'clip': self._o_clip}
         self.dev = torch.from_numpy(pk.host()).to('cuda')
@@ -303,11 +359,11 @@ class BatchPlan:
         self.rec_of = inv
         self.stamp_offset = np.where(inv >= 0, out_off[np.maximum(inv, 0)] if len(order) else 0,
                                      -1)
-        self.stamp_shape = np.stack([ny, nx], axis=1).astype(np.int64)
+        self.stamp_shape = np.stack([hp.ny, hp.nx], axis=1).astype(np.int64)
         self._rerun = np.zeros(n, dtype=np.int32)
         self._rerun_cleared = np.zeros(n, dtype=bool)
-        self.nops = len(ctx.ops)
-        self.nleaves = len(ctx.leaves)
+        self.nops = len(host_ops)
+        self.nleaves = len(host_leaves)
         self.program_shape = _program_shape(recs, self.host_ops, self.host_leaves)
         self._sched = None
         self._graph = None
@@ -589,12 +645,12 @@ class BatchPlan:
         import torch
         st = self.status()
         redo = np.nonzero((st & _lib.STAMP_NONFINITE) != 0)[0]
-        nx, ny, scale, offsets, method, utc = self._draw_args
+        hp = self.lowered
         for i in redo:
-            im = self.profiles[i].drawImage(nx=int(nx[i]), ny=int(ny[i]), scale=scale,
-                                            offset=tuple(offsets[i]), method=method,
-                                            use_true_center=utc, dtype=np.float64,
-                                            precision='f64')
+            im = hp.profile(i).drawImage(nx=int(hp.nx[i]), ny=int(hp.ny[i]), scale=hp.scale,
+                                         offset=tuple(hp.offsets[i]), method=hp.method,
+                                         use_true_center=hp.use_true_center, dtype=np.float64,
+                                         precision='f64')
             a = torch.from_numpy(np.ascontiguousarray(im.array)).to('cuda', self.out.dtype)
             self.stamp(i).copy_(a)
             self._rerun[i] |= STAMP_RERUN_FP64

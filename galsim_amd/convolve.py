@@ -156,7 +156,8 @@ class Deconvolution(GSObject):
             return                      # Deconvolution._kValue has no maxk cut
         if ctx.dk is None:
             raise _lib.GalSimAMDError("a Deconvolution needs the k-image spacing when lowered")
-        rec = np.zeros((), dtype=_lib.LEAF_DTYPE)
+        from .render import LeafRecord
+        rec = LeafRecord()
         sep, j = jac_record(jac)
         rec['type'] = _lib.LEAF_KCLIP
         rec['separable'] = sep

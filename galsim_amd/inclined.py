@@ -237,7 +237,8 @@ class InclinedSersic(GSObject):
         cosi, hs, info = self._info()
         j = np.eye(2) if jac is None else np.asarray(jac, dtype=float).reshape(2, 2)
         self._base._lower(ctx, j @ np.diag([1., cosi]))
-        rec = np.zeros((), dtype=_lib.LEAF_DTYPE)
+        from .render import LeafRecord
+        rec = LeafRecord()
         sep, jj = jac_record(jac)
         rec['type'] = _lib.LEAF_INCLINED_VERT
         rec['separable'] = sep

@@ -39,6 +39,38 @@ def spline_coefficients(args, vals, y2):
     return c
 
 
+class LeafRecord:
+    """A gsmi_leaf under construction: the same field names (rec['scale'] = ..., rec['p'][k] =
+    ...), held in Python values until ``Lowering.leaves_array`` packs the batch column by
+    column (a numpy structured scalar per leaf costs more than the rest of its lowering)."""
+    __slots__ = ('d',)
+    _names = ('type', 'separable', 'kind', 'table', 'ntab', 'cliprows', 'jac', 'scale', 'scale2',
+              'flux', 'p')
+
+    def __init__(self, d=None):
+        self.d = d if d is not None else {
+            'type': 0, 'separable': 0, 'kind': 0, 'table': 0, 'ntab': 0, 'cliprows': -1,
+            'jac': [1., 0., 0., 1.], 'scale': 0., 'scale2': 0., 'flux': 0., 'p': [0.] * 6}
+
+    def __getitem__(self, k):
+        return self.d[k]
+
+    def __setitem__(self, k, v):
+        if k not in self.d:
+            raise KeyError(k)
+        if k in ('jac', 'p'):
+            v = [float(x) for x in v]
+            if len(v) != len(self.d[k]):
+                raise ValueError("%s takes %d values" % (k, len(self.d[k])))
+        self.d[k] = v
+
+    def copy(self):
+        d = dict(self.d)
+        d['jac'] = list(d['jac'])
+        d['p'] = list(d['p'])
+        return LeafRecord(d)
+
+
 class Lowering:
     """Accumulates the k-space program of one or more stamps."""
 
@@ -138,8 +170,9 @@ class Lowering:
 
     def leaves_array(self):
         a = np.zeros(len(self.leaves), dtype=_lib.LEAF_DTYPE)
-        for i, r in enumerate(self.leaves):
-            a[i] = r
+        if self.leaves:
+            for name in LeafRecord._names:
+                a[name] = [r.d[name] for r in self.leaves]
         if self.precision == 'f32':
             # Sersic leaves: coefficient offsets relative to the region -> absolute in tables32
             sel = (a['type'] == _lib.LEAF_SERSIC) & (a['cliprows'] >= 0)

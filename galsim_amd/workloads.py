@@ -63,6 +63,20 @@ def bulge_disk_profile(m, p, i):
     return m.Convolve([gal, m.Kolmogorov(fwhm=float(p['psf_fwhm'][i]))])
 
 
+def bulge_disk_config(p, sizes):
+    """The same bulge + disk stamps as a config (galsim_amd.config): per-object values from the
+    parameter arrays of bulge_disk_params, stamp sides ``sizes``."""
+    F, f, dh = p['flux'], p['bfrac'], p['disk_hlr']
+    return {'gal': {'type': 'Sum', 'items': [
+                {'type': 'Exponential', 'half_light_radius': dh, 'flux': (1. - f) * F},
+                {'type': 'Sersic', 'n': p['n'], 'half_light_radius': 0.5 * dh,
+                 'flux': f * F}],
+                    'shear': {'type': 'G1G2', 'g1': p['g1'], 'g2': p['g2']}},
+            'psf': {'type': 'Kolmogorov', 'fwhm': p['psf_fwhm']},
+            'image': {'pixel_scale': 0.2},
+            'stamp': {'size': np.asarray(sizes, dtype=float)}}
+
+
 # -- large Sersic stamps (x) Kolmogorov: FFT grids of 512..4096 --------------------------------
 def large_sersic_params(n, seed=0):
     """Sersic n ~ U(3, 6) quantised to 0.05, hlr ~ U(2, 6)", Kolmogorov fwhm 0.8", flux 1e4."""

@@ -244,3 +244,52 @@ def test_bad_stamps_are_flagged_not_fatal_and_overflow_reruns_in_fp64():
     # a later run clears the device flags and the rerun marks
     plan.run()
     assert plan.status()[2] & L.STAMP_NONFINITE and not plan.status()[2] & batch.STAMP_RERUN_FP64
+
+
+def _to_oracle(obj):
+    """The oracle object of a galsim_amd profile tree (same parameters)."""
+    if isinstance(obj, g.Transformation):
+        return od.Transformation(_to_oracle(obj._original), obj._jac, (obj._dx, obj._dy),
+                                 obj._flux_ratio)
+    if isinstance(obj, g.Convolution):
+        return od.Convolve(*[_to_oracle(o) for o in obj.obj_list])
+    if isinstance(obj, g.Sum):
+        return od.Add(*[_to_oracle(o) for o in obj.obj_list])
+    if isinstance(obj, g.Exponential):
+        return od.Exponential(scale_radius=obj.scale_radius, flux=obj.flux)
+    if isinstance(obj, g.Sersic):
+        return od.Sersic(obj.n, scale_radius=obj.scale_radius, flux=obj.flux)
+    if isinstance(obj, g.Kolmogorov):
+        return od.Kolmogorov(lam_over_r0=obj.lam_over_r0, flux=obj.flux)
+    raise TypeError(type(obj))
+
+
+def test_config_batch_matches_per_object_batch_and_oracle():
+    # a config-driven batch (vectorised planning) draws what the per-object plan of the same
+    # config values draws, and a sample of its stamps matches the oracle
+    import torch
+    from galsim_amd import config
+    from tests.test_config_cpu import bulge_disk_config
+    n = 256
+    cfg = bulge_disk_config(n)
+    profs, nx, ny, scale, offs, method = config.build_profiles(cfg, n, seed=3)
+    for prec, tol_plan, tol_oracle in (('f64', 1e-12, 1e-10), ('f32', 1e-6, 1e-5)):
+        print("config batch", prec, n, "stamps", flush=True)
+        cb = config.build_batch(cfg, n, prec, seed=3)
+        cb.run()
+        pb = BatchPlan(profs, nx, ny, scale, offs, precision=prec)
+        pb.run()
+        torch.cuda.synchronize()
+        assert np.all(cb.status() == 0)
+        worst = max(float((cb.stamp(i) - pb.stamp(i)).abs().max() / pb.stamp(i).abs().max())
+                    for i in range(n))
+        print("config vs per-object plan, max peak-relative difference", worst)
+        assert worst < tol_plan
+        wo = 0.
+        for i in range(0, n, 37):
+            ref = _to_oracle(profs[i]).drawImage(int(nx[i]), int(ny[i]), scale=scale,
+                                                 offset=tuple(offs[i]))
+            a = cb.stamp(i).double().cpu().numpy()
+            wo = max(wo, float(np.max(np.abs(a - ref)) / np.max(np.abs(ref))))
+        print("config batch vs oracle, max peak-relative error", wo)
+        assert wo < tol_oracle

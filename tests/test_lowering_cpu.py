@@ -4,6 +4,7 @@ import numpy as np
 import pytest
 
 import galsim_amd as g
+from galsim_amd import _lib
 from galsim_amd.position import BoundsI, PositionI
 from galsim_amd.transform import _Transform
 from oracle import draw as od
@@ -193,13 +194,18 @@ def test_plan_stamp_flags_non_finite_parameters_and_large_ffts():
                            m.Kolmogorov(fwhm=0.8)])
     ok = dict(nx=32, ny=32, scale=0.2, offset=(0., 0.), use_true_center=True, method='auto',
               precision='f32', differentiable=False, moffat_generic=False)
-    for bad in (bd(g, float('nan')), bd(g, 1.0, flux=float('inf')), bd(g, 1.0, flux=float('nan'))):
-        ctx = render.Lowering('f32')
-        with pytest.raises(ValueError):
-            batch._plan_stamp(ctx, bad, **ok)
+    with pytest.raises(ValueError):                          # a NaN size: int(ceil(NaN))
+        batch._plan_stamp(render.Lowering('f32'), bd(g, float('nan')), **ok)
+    # non-finite constants are found over the whole batch
     ctx = render.Lowering('f32')
-    rec, N, warn = batch._plan_stamp(ctx, bd(g, 1.0), **ok)
-    assert N == 128 and not warn and rec[0] == N
+    recs = np.zeros(4, dtype=_lib.STAMP_DTYPE)
+    for k, prof in enumerate([bd(g, 1.0), bd(g, 1.0, flux=float('inf')),
+                              bd(g, 1.2, flux=float('nan')), bd(g, 0.9)]):
+        rec, N, warn = batch._plan_stamp(ctx, prof, **ok)
+        recs[k] = rec
+        assert N == 128 and not warn and rec[0] == N
+    bad = batch._nonfinite_stamps(recs, ctx.ops_array(), ctx.leaves_array())
+    assert list(bad) == [1, 2]
     small = g.GSParams(maximum_fft_size=256)
     big = g.Convolve([g.Exponential(half_light_radius=8.0, flux=1e3, gsparams=small),
                       g.Kolmogorov(fwhm=0.8, gsparams=small)])
