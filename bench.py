@@ -141,6 +141,23 @@ def fwd_bwd_bench(args, torch):
     scale = (g64.abs() * t_fit.abs().clamp(min=1.)).max(dim=1, keepdim=True).values
     err = ((g32 - g64).abs() * t_fit.abs().clamp(min=1.) / scale).max().item()
     log("FP32 vs FP64 gradient: max error / largest sensitivity = %.2e" % err)
+    # the adjoint alone: J^T v for one cotangent v in both precisions (the FP64 chi^2 cotangent at
+    # t_fit).  The chi^2 gradient above also carries each precision's own forward image through
+    # residuals 100x smaller than the peak, which magnifies the FP32 image error 100-fold.
+    with torch.no_grad():
+        img64 = m64.render(t_fit).double()
+        cot = 2. * (img64 - tg64) * is64 * is64
+
+    def vjp(model):
+        t = t_fit.detach().requires_grad_(True)
+        img = model.render(t)
+        (g,) = torch.autograd.grad(img, t, grad_outputs=cot.to(img.dtype))
+        return g
+
+    j32, j64 = vjp(m32), vjp(m64)
+    scale_j = (j64.abs() * t_fit.abs().clamp(min=1.)).max(dim=1, keepdim=True).values
+    err_adj = ((j32 - j64).abs() * t_fit.abs().clamp(min=1.) / scale_j).max().item()
+    log("FP32 vs FP64 adjoint (same cotangent): max error / largest sensitivity = %.2e" % err_adj)
     # plain per-stamp FP64 path
     mp = min(args.plain_sample, n, 64)
     singles = [setup('f64', np.array([i])) for i in range(mp)]
@@ -161,7 +178,12 @@ def fwd_bwd_bench(args, torch):
                         "dx, dy, psf_fwhm via the adjoint kernels" % n,
             "plain_fp64_gpu_fwd_bwd_stamps_per_s": round(plain, 1),
             "speedup_vs_plain_fp64_gpu": round(rate / plain, 2),
-            "grad_err_fp32_vs_fp64": err}
+            "grad_err_fp32_vs_fp64": err,
+            "adjoint_err_fp32_vs_fp64_same_cotangent": err_adj,
+            "grad_err_note": "errors are max |g32 - g64| |theta| over the stamp's largest "
+                             "|g64| |theta|; grad_err compares the chi^2 gradients (each "
+                             "precision with its own targets and forward image), "
+                             "adjoint_err feeds one FP64 chi^2 cotangent to both adjoints"}
 
 
 def fit_bench(args, torch):

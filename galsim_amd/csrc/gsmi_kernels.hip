@@ -634,6 +634,10 @@ __device__ __forceinline__ void rows_last(const cplx<T>* src, int nrow, const cp
             for (int r = 1; r < R; ++r) v[r] = cmul(v[r], twl[2 * jj * r]);
         }
         bfly<R>(v);
+        // a non-finite value anywhere in the row's spectrum reaches every output of its
+        // transform (each output sums every input with a unit-modulus weight, and no add or
+        // multiply turns inf or NaN finite), so one output per thread stands for the row
+        bad |= !isfinite(v[0].x + v[0].y);
         T* o = out + size_t(rl) * nx;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -644,14 +648,8 @@ __device__ __forceinline__ void rows_last(const cplx<T>* src, int nrow, const cp
             col = col >= N ? col - N : (col < 0 ? col + N : col);
             int col1 = col + 1;
             col1 = col1 >= N ? col1 - N : col1;
-            if (col < nx) {
-                o[col] = v[r].x * norm;
-                bad |= !isfinite(v[r].x);
-            }
-            if (col1 < nx) {
-                o[col1] = v[r].y * norm;
-                bad |= !isfinite(v[r].y);
-            }
+            if (col < nx) o[col] = v[r].x * norm;
+            if (col1 < nx) o[col1] = v[r].y * norm;
         }
     }
 }
