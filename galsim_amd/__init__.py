@@ -16,6 +16,8 @@ from .gsobject import GSObject
 from .gsparams import GSParams
 from .image import Image, ImageCD, ImageCF, ImageD, ImageF
 from .inclined import InclinedExponential, InclinedSersic
+from .interpolant import Cubic, Interpolant, Lanczos, Linear, Nearest, Quintic
+from .interpolatedimage import InterpolatedImage
 from .kolmogorov import Kolmogorov
 from .moffat import Moffat
 from .position import BoundsI, PositionD, PositionI
@@ -32,6 +34,7 @@ __all__ = ['Add', 'Airy', 'Angle', 'AngleUnit', 'AutoConvolution', 'AutoConvolve
            'Exponential',
            'GSObject', 'GSParams', 'GalSimAMDBoundsError', 'GalSimAMDError', 'GalSimAMDRangeError',
            'GalSimAMDUndefinedBoundsError', 'GalSimAMDValueError', 'GalSimError',
-           'Gaussian', 'Image', 'InclinedExponential', 'InclinedSersic', 'ImageCD', 'ImageCF', 'ImageD', 'ImageF', 'Kolmogorov', 'Moffat',
+           'Cubic', 'Gaussian', 'Image', 'InclinedExponential', 'InclinedSersic', 'Interpolant',
+           'InterpolatedImage', 'Lanczos', 'Linear', 'Nearest', 'Quintic', 'ImageCD', 'ImageCF', 'ImageD', 'ImageF', 'Kolmogorov', 'Moffat',
            'Pixel', 'PositionD', 'PositionI', 'Sersic', 'Shear', 'Spergel', 'Sum', 'TopHat', 'Transform',
            'Transformation', 'arcmin', 'arcsec', 'degrees', 'fft', 'hours', 'radians']

@@ -41,6 +41,15 @@ LEAF_INCLINED_EXP = 10
 LEAF_INCLINED_VERT = 11
 LEAF_TOPHAT = 12
 LEAF_DELTA = 13
+LEAF_INTERP_IMAGE = 14
+
+# GSMI_LEAF_INTERP_IMAGE parameter block (include/gsmi.h GSMI_INTERP_*)
+INTERP_N2, INTERP_KSCALE, INTERP_MAXK1, INTERP_MAXK1S, INTERP_KXR = range(5)
+INTERP_I1, INTERP_I2, INTERP_J1, INTERP_J2 = 5, 6, 7, 8
+INTERP_XCODE, INTERP_XN, INTERP_XUMAX, INTERP_XUTAB, INTERP_XUTABN = 9, 10, 11, 12, 13
+INTERP_KCODE, INTERP_KN, INTERP_KDC, INTERP_KK = 14, 15, 16, 17
+INTERP_KIMAGE, INTERP_XMIN, INTERP_YMIN, INTERP_BLOCK = 22, 23, 24, 25
+INTERP_MAX_TAPS = 17
 
 ERRORS = {-1: "invalid argument", -2: "parameter out of range", -3: "solve/table failure",
           -4: "HIP runtime failure", -5: "unsupported FFT size"}
@@ -119,6 +128,9 @@ _SIGS = {
     'gsmi_spergel_flux_radius': (_I, [_D, _D, _P]),
     'gsmi_spergel_integrated_flux': (_I, [_D, _D, _P]),
     'gsmi_sersic_kvalue': (_I, [_D, _D, _P, _P, _P, _I]),
+    'gsmi_sine_integral': (_I, [_P, _P, _I]),
+    'gsmi_lanczos_info': (_I, [_I, _I, _P, _P, _P]),
+    'gsmi_lanczos_utable': (_I, [_I, _I, _P, _P, _P, _P, _I]),
     'gsmi_draw_fft_f64': (_I, [_P, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P]),
     'gsmi_draw_fft_f32': (_I, [_P, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P]),
     'gsmi_fold_colfft_f64': (_I, [_P, _I, _I, _P, _P, _P, _P, _P, _P]),

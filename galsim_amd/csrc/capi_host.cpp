@@ -308,4 +308,48 @@ int gsmi_moffat_trunc_table(double beta, double scale_radius, double trunc, cons
     });
 }
 
+
+int gsmi_sine_integral(const double* x, double* out, int n)
+{
+    if ((!x || !out) && n > 0) {
+        set_error("gsmi_sine_integral: bad argument");
+        return GSMI_E_ARG;
+    }
+    for (int i = 0; i < n; ++i) out[i] = sine_integral(x[i]);
+    return GSMI_OK;
+}
+
+int gsmi_lanczos_info(int n, int conserve_dc, const double* gsp, double* out, int* ntab)
+{
+    if (!out || n < 1) {
+        set_error("gsmi_lanczos_info: bad argument");
+        return GSMI_E_ARG;
+    }
+    return guarded("gsmi_lanczos_info", [&] {
+        const LanczosInfo& L = lanczos_info(n, conserve_dc != 0, GSP::from(gsp));
+        for (int j = 0; j < 5; ++j) out[j] = L.K[j + 1];
+        out[5] = L.umax;
+        if (ntab) *ntab = int(L.utab.args.size());
+    });
+}
+
+int gsmi_lanczos_utable(int n, int conserve_dc, const double* gsp, double* args, double* vals,
+                        double* y2, int cap)
+{
+    if (!args || !vals || !y2 || n < 1) {
+        set_error("gsmi_lanczos_utable: bad argument");
+        return GSMI_E_ARG;
+    }
+    return guarded("gsmi_lanczos_utable", [&] {
+        const LanczosInfo& L = lanczos_info(n, conserve_dc != 0, GSP::from(gsp));
+        const int m = int(L.utab.args.size());
+        if (cap < m) throw std::runtime_error("table out of range of the buffer");
+        for (int i = 0; i < m; ++i) {
+            args[i] = L.utab.args[i];
+            vals[i] = L.utab.vals[i];
+            y2[i] = L.utab.y2[i];
+        }
+    });
+}
+
 }

@@ -570,6 +570,215 @@ __device__ inline T eval_leaf(const gsmi_leaf* __restrict__ Lp, const Tables<T>&
     return eval_leafc<T>(leaf_consts_t<T>(*Lp, tb), leaf_ints(*Lp), Lp, tb, kx, ky, i, j, Nk);
 }
 
+// ---------------------------------------------------------------------------------------------
+// InterpolatedImage (GSMI_LEAF_INTERP_IMAGE)
+// ---------------------------------------------------------------------------------------------
+template <typename T>
+__device__ inline const T* table_at(const Tables<T>& tb, int off);
+template <>
+__device__ inline const double* table_at<double>(const Tables<double>& tb, int off)
+{
+    return tb.tab + off;
+}
+template <>
+__device__ inline const float* table_at<float>(const Tables<float>& tb, int off)
+{
+    return tb.tab32 + off;
+}
+
+// math::sinc (src/math/Sinc.cpp:29-33)
+template <typename T>
+__device__ inline T sinc_pi(T x)
+{
+    if (fabs(x) < T(1e-4)) return T(1) - T(M_PI * M_PI / 6.) * x * x;
+    return ddiv(dsinpi(x), T(M_PI) * x);
+}
+
+// Interpolant::xval (src/Interpolant.cpp:171-176, 245-250, 284-290, 389-433, 563-912): code as
+// in the leaf's parameter block; n, dc and K(1..5) are Lanczos's.
+template <typename T>
+__device__ inline T interp_xval(int code, int n, bool dc, const double* __restrict__ K, T x)
+{
+    x = fabs(x);
+    switch (code) {
+      case 0: return x > T(0.5) ? T(0) : T(1);
+      case 1: return x > T(1) ? T(0) : T(1) - x;
+      case 2:
+        if (x < T(1)) return T(1) + x * x * (T(1.5) * x - T(2.5));
+        if (x < T(2)) return T(-0.5) * (x - T(1)) * (x - T(2)) * (x - T(2));
+        return T(0);
+      case 3:
+        if (x <= T(1))
+            return T(1) + x * x * x * (T(-95. / 12.) + x * (T(23. / 2.) + x * T(-55. / 12.)));
+        if (x <= T(2))
+            return (x - T(1)) * (x - T(2)) *
+                   (T(-23. / 4.) + x * (T(29. / 2.) + x * (T(-83. / 8.) + x * T(55. / 24.))));
+        if (x <= T(3))
+            return (x - T(2)) * (x - T(3)) * (x - T(3)) *
+                   (T(-9. / 4.) + x * (T(25. / 12.) + x * T(-11. / 24.)));
+        return T(0);
+      default: {
+        const T nd = T(n);
+        if (x >= nd) return T(0);
+        T res, s;
+        if (x > T(1e-4)) {
+            s = dsinpi(x);
+            res = ddiv(nd * T(1. / (M_PI * M_PI)) * s * dsinpi(ddiv(x, nd)), x * x);
+        } else {
+            const T pix = T(M_PI) * x;
+            const T temp = T(1. / 6.) * pix * pix;
+            s = pix * (T(1) - temp);
+            res = T(1) - temp * (T(1) + ddiv(T(1), nd * nd));
+        }
+        if (dc) {
+            const T ssq = s * s;
+            const T factor =
+                T(1) - T(4) * T(K[0]) * ssq - T(16) * T(K[1]) * ssq * (T(1) - ssq) -
+                T(4) * T(K[2]) * ssq * (T(9) - ssq * (T(24) - T(16) * ssq)) -
+                T(64) * T(K[3]) * ssq * (T(1) - ssq * (T(5) - ssq * (T(8) - T(4) * ssq))) -
+                T(4) * T(K[4]) * ssq *
+                    (T(25) - ssq * (T(200) - ssq * (T(560) - ssq * (T(640) - T(256) * ssq))));
+            res = ddiv(res, factor);
+        }
+        return res;
+      }
+    }
+}
+
+// Interpolant::uval (src/Interpolant.cpp:176, 251-255, 292-312, 435-462, 914-919); Lanczos reads
+// its spline table (knots, values, second derivatives; 0 beyond umax).
+template <typename T>
+__device__ inline T interp_uval(int code, const T* __restrict__ utab, int ntab, T umax, T u)
+{
+    u = fabs(u);
+    switch (code) {
+      case 0: return sinc_pi(u);
+      case 1: {
+        const T s = sinc_pi(u);
+        return s * s;
+      }
+      case 2: {
+        const T s = sinc_pi(u);
+        const T c = cos(T(M_PI) * u);
+        return s * s * s * (T(3) * s - T(2) * c);
+      }
+      case 3: {
+        const T s = sinc_pi(u);
+        const T piu = T(M_PI) * u;
+        const T c = cos(piu);
+        const T ssq = s * s;
+        const T piusq = piu * piu;
+        return s * ssq * ssq * (s * (T(55) - T(19) * piusq) + T(2) * c * (piusq - T(27)));
+      }
+      default:
+        return u > umax ? T(0) : spline_eq<T>(utab, ntab, u, nullptr);
+    }
+}
+
+// Taps of the k interpolation at scaled k: ceil(k - xr) .. floor(k + xr), or the one at an
+// integer k when the form checks for it (src/SBInterpolatedImage.cpp:415-427, 827-840).
+template <typename T>
+__device__ inline void interp_taps(T k, T xr, bool icheck, int* first, int* count)
+{
+    const T fl = floor(k + T(0.01));
+    if (icheck && fabs(k - fl) < T(10. * 2.220446049250313e-16) * (fabs(k) + T(1))) {
+        *first = int(fl);
+        *count = 1;
+        return;
+    }
+    *first = int(ceil(k - xr));
+    *count = min(int(floor(k + xr)) - *first + 1, GSMI_INTERP_MAX_TAPS);
+}
+
+// WrapKIndex (src/SBInterpolatedImage.cpp:217-222)
+__device__ inline int wrap_k_index(int k, int No2, int N)
+{
+    k = (k + No2) % N;
+    if (k < 0) k += N;
+    return k - No2;
+}
+
+// The leaf's complex value at k-image sample (i, j) (kx = i dk, ky = j dk): k in the image's
+// frame through the leaf's Jacobian, the cut of its form, then sum_q ywt(q) sum_p xwt(p) K(p, q)
+// over the wrapped taps as KValueInnerLoop walks them (p runs N/2 + 1 -> -N/2 + 1, q runs
+// N/2 -> -N/2; p < 0 reads the conjugate at (-p, -q), q = -N/2 its own row), times
+// uval(kx / 2 pi) uval(ky / 2 pi) of the x-interpolant.
+template <typename T>
+__device__ inline cplx<T> interp_image_value(const gsmi_leaf* __restrict__ L, const Tables<T>& tb,
+                                             T kx, T ky, int i, int j)
+{
+    const double* pb = tb.tab + L->table;
+    const bool kv = L->kind == 1;
+    const bool sep = L->separable != 0;
+    T kxp, kyp;
+    if (sep) {
+        kxp = T(L->jac[0]) * kx;
+        kyp = T(L->jac[3]) * ky;
+    } else {
+        kxp = T(L->jac[0]) * kx + T(L->jac[2]) * ky;
+        kyp = T(L->jac[1]) * kx + T(L->jac[3]) * ky;
+    }
+    const cplx<T> zero = {T(0), T(0)};
+    if (kv) {
+        const T m1 = T(pb[GSMI_INTERP_MAXK1]);
+        if (fabs(kxp) > m1 || fabs(kyp) > m1) return zero;
+    } else if (sep) {
+        const int ir = i - int(pb[GSMI_INTERP_XMIN]);
+        const int jr = j - int(pb[GSMI_INTERP_YMIN]);
+        if (ir < int(pb[GSMI_INTERP_I1]) || ir >= int(pb[GSMI_INTERP_I2]) ||
+            jr < int(pb[GSMI_INTERP_J1]) || jr >= int(pb[GSMI_INTERP_J2]))
+            return zero;
+    }
+    const int No2 = int(pb[GSMI_INTERP_N2]);
+    const int N = 2 * No2;
+    const T kscale = T(pb[GSMI_INTERP_KSCALE]);
+    const T kxs = kxp * kscale;
+    const T kys = kyp * kscale;
+    if (!kv && !sep) {
+        const T m1s = T(pb[GSMI_INTERP_MAXK1S]);
+        if (fabs(kxs) > m1s || fabs(kys) > m1s) return zero;
+    }
+    const bool icheck = kv || sep;
+    const T xr = T(pb[GSMI_INTERP_KXR]);
+    int p1, np, q1, nq;
+    interp_taps(kxs, xr, icheck, &p1, &np);
+    interp_taps(kys, xr, icheck, &q1, &nq);
+    const int kcode = int(pb[GSMI_INTERP_KCODE]);
+    const int kn = int(pb[GSMI_INTERP_KN]);
+    const bool kdc = pb[GSMI_INTERP_KDC] != 0.;
+    const double* KK = pb + GSMI_INTERP_KK;
+    const T* K = table_at<T>(tb, int(pb[GSMI_INTERP_KIMAGE]));
+    const int W = No2 + 1;
+    const int pw0 = wrap_k_index(p1, No2, N);
+    int qw = wrap_k_index(q1, No2, N);
+    cplx<T> sum = zero;
+    for (int t = 0; t < nq; ++t, ++qw) {
+        if (qw == No2) qw -= N;
+        const T ywt = interp_xval<T>(kcode, kn, kdc, KK, T(q1 + t) - kys);
+        const int mq = qw == -No2 ? qw : -qw;
+        cplx<T> xs = zero;
+        int pw = pw0;
+        for (int s = 0; s < np; ++s, ++pw) {
+            if (pw == No2 + 1) pw -= N;
+            const T xwt = interp_xval<T>(kcode, kn, kdc, KK, T(p1 + s) - kxs);
+            const bool neg = pw < 0;
+            const T* e = K + 2 * (size_t((neg ? mq : qw) + No2) * W + (neg ? -pw : pw));
+            xs.x += xwt * e[0];
+            xs.y += xwt * (neg ? -e[1] : e[1]);
+        }
+        sum.x += ywt * xs.x;
+        sum.y += ywt * xs.y;
+    }
+    const int xcode = int(pb[GSMI_INTERP_XCODE]);
+    const int xt = int(pb[GSMI_INTERP_XUTAB]);
+    const T* utab = xt >= 0 ? table_at<T>(tb, xt) : nullptr;
+    const int un = int(pb[GSMI_INTERP_XUTABN]);
+    const T umax = T(pb[GSMI_INTERP_XUMAX]);
+    const T f = interp_uval<T>(xcode, utab, un, umax, kxp * T(1. / (2. * M_PI))) *
+                interp_uval<T>(xcode, utab, un, umax, kyp * T(1. / (2. * M_PI)));
+    return {f * sum.x, f * sum.y};
+}
+
 // A small stack held in registers: push/pop shift every slot, so all indices are static.
 template <typename T, int S>
 struct RegStack {
@@ -865,7 +1074,12 @@ __device__ inline cplx<T> eval_program(const gsmi_op* __restrict__ ops, int nops
         const gsmi_op* op = ops + o;
         switch (op->code) {
           case GSMI_OP_LEAF: {
-            T v = eval_leaf<T>(leaves + op->arg, tb, kx, ky, i, j, Nk);
+            const gsmi_leaf* L = leaves + op->arg;
+            if (L->type == GSMI_LEAF_INTERP_IMAGE) {
+                st.push(interp_image_value<T>(L, tb, kx, ky, i, j));
+                break;
+            }
+            T v = eval_leaf<T>(L, tb, kx, ky, i, j, Nk);
             st.push({v, T(0)});
             break;
           }

@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <complex>
 #include <cstring>
 #include <deque>
 #include <limits>
@@ -1238,6 +1239,116 @@ InclinedSersicInfo inclined_sersic_info(double n, double trunc, double cosi, dou
     const double clipk = cs.bisect() + cs.xtol;
     info.ksq_max = clipk * clipk;
     return info;
+}
+
+
+// ---------------------------------------------------------------------------------------------
+// Sine integral and the Lanczos interpolant's transform (src/Interpolant.cpp:719-899)
+// ---------------------------------------------------------------------------------------------
+double sine_integral(double x)
+{
+    const double ax = std::abs(x);
+    if (ax <= 4.) {
+        // sum_k (-1)^k x^(2k+1) / ((2k+1) (2k+1)!)
+        const double x2 = x * x;
+        double term = x;       // (-1)^k x^(2k+1) / (2k+1)!
+        double sum = x;
+        for (int k = 1; k < 40; ++k) {
+            term *= -x2 / ((2. * k) * (2. * k + 1.));
+            const double t = term / (2. * k + 1.);
+            sum += t;
+            if (std::abs(t) < 1e-18 * std::abs(sum)) break;
+        }
+        return sum;
+    }
+    // E1(z) = e^-z / (z + 1 - 1 / (z + 3 - 4 / (z + 5 - ...))), z = i ax; Si = pi/2 + Im E1(z)
+    using C = std::complex<double>;
+    const C z(0., ax);
+    const double tiny = 1e-300;
+    C b = z + 1.;
+    C f = 1. / b;            // the fraction 1 / (b0 - a1 / (b1 - ...)) by modified Lentz
+    C c = 1. / tiny;
+    C d = 1. / b;
+    for (int i = 1; i < 1000; ++i) {
+        const double a = -double(i) * double(i);
+        b += 2.;
+        d = 1. / (a * d + b);
+        c = b + a / c;
+        const C del = c * d;
+        f *= del;
+        if (std::abs(del - 1.) < 1e-17) break;
+    }
+    const C e1 = f * std::exp(-z);
+    const double si = M_PI / 2. + e1.imag();
+    return x < 0. ? -si : si;
+}
+
+namespace {
+struct LanczosKey {
+    int n;
+    bool dc;
+    double tol, spacing;
+    bool operator<(const LanczosKey& o) const
+    {
+        if (n != o.n) return n < o.n;
+        if (dc != o.dc) return dc < o.dc;
+        if (tol != o.tol) return tol < o.tol;
+        return spacing < o.spacing;
+    }
+};
+
+double lanczos_ucalc_raw(double nd, double u)
+{
+    const double vp = nd * (2. * u + 1.);
+    const double vm = nd * (2. * u - 1.);
+    const double r = (vm - 1.) * sine_integral(M_PI * (vm - 1.)) -
+                     (vm + 1.) * sine_integral(M_PI * (vm + 1.)) -
+                     (vp - 1.) * sine_integral(M_PI * (vp - 1.)) +
+                     (vp + 1.) * sine_integral(M_PI * (vp + 1.));
+    return r / (2. * M_PI);
+}
+}  // namespace
+
+const LanczosInfo& lanczos_info(int n, bool conserve_dc, const GSP& gsp)
+{
+    static std::map<LanczosKey, LanczosInfo> cache;
+    static std::mutex mu;
+    std::lock_guard<std::mutex> lock(mu);
+    const LanczosKey key{n, conserve_dc, gsp.kvalue_accuracy, gsp.table_spacing};
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    LanczosInfo info;
+    const double nd = n;
+    double* K = info.K;
+    for (int j = 1; j <= 5; ++j) K[j] = lanczos_ucalc_raw(nd, double(j));
+    double C[6];
+    C[0] = 1. + 2. * (K[1] * (1. + 3. * K[1] + K[2] + K[3]) + K[2] + K[3] + K[4] + K[5]);
+    C[1] = -K[1] * (1. + 4. * K[1] + K[2] + 2. * K[3]);
+    C[2] = K[1] * (K[1] - 2. * K[2] + K[3]) - K[2];
+    C[3] = K[1] * (K[1] - 2. * K[3]) - K[3];
+    C[4] = K[1] * K[3] - K[4];
+    C[5] = -K[5];
+    auto ucalc = [&](double u) {
+        double r = lanczos_ucalc_raw(nd, u);
+        if (conserve_dc) {
+            r *= C[0];
+            for (int j = 1; j <= 5; ++j)
+                r += C[j] * (lanczos_ucalc_raw(nd, u + j) + lanczos_ucalc_raw(nd, u - j));
+        }
+        return r;
+    };
+    const double tol = gsp.kvalue_accuracy;
+    const double ustep = gsp.table_spacing * std::pow(tol / 200., 0.25) / nd;
+    double umax = 0.;
+    for (double u = 0.; u - umax < 1. / nd || u < 1.1; u += ustep) {
+        const double v = ucalc(u);
+        info.utab.args.push_back(u);
+        info.utab.vals.push_back(v);
+        if (std::abs(v) > tol) umax = u;
+    }
+    info.utab.build();
+    info.umax = umax;
+    return cache.emplace(key, std::move(info)).first->second;
 }
 
 }  // namespace gsmi

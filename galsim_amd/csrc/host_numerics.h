@@ -110,4 +110,18 @@ double sersic_kvalue(const SersicInfo& s, double ksq);
 InclinedSersicInfo inclined_sersic_info(double n, double trunc, double cosi, double hs,
                                         const GSP& gsp);
 
+// Sine integral Si(x) = int_0^x sin(t)/t dt: its power series for |x| <= 4, else
+// pi/2 + Im E1(i x) with E1's continued fraction (Abramowitz & Stegun 5.1.22, 5.2.21-23)
+// evaluated by the modified Lentz method.
+double sine_integral(double x);
+
+// Lanczos(n, conserve_dc) (src/Interpolant.cpp:719-899): K(1..5), the transform's cut umax and
+// its spline table on u = 0, du, 2 du, ... as the reference builds it.
+struct LanczosInfo {
+    double K[6] = {0., 0., 0., 0., 0., 0.};
+    double umax = 0.;
+    Spline utab;
+};
+const LanczosInfo& lanczos_info(int n, bool conserve_dc, const GSP& gsp);
+
 }  // namespace gsmi

@@ -75,6 +75,10 @@ class DiffPlan:
                 raise _lib.GalSimAMDError(
                     "differentiable rendering supports at most %d leaves and %d ops per profile "
                     "(got %d and %d)" % (MAXLEAF, MAXOPS, nleaf, cnt))
+            lb = int(r['leaf_begin'])
+            if np.any(leaves['type'][lb:lb + nleaf] == _lib.LEAF_INTERP_IMAGE):
+                raise NotImplementedError("differentiable rendering of an InterpolatedImage is "
+                                          "not part of galsim_amd")
             nl[i] = nleaf
             cplx[i] = bool(np.any(prog['code'] == _lib.OP_PHASE))
             for l in range(nleaf):

@@ -89,6 +89,8 @@ class Lowering:
         self.coef_off = {}
         self.ncoef = 0
         self.kvalue_mode = False     # lowering for kValue: no k-image cuts
+        self.kbounds = None          # (xmin, ymin, columns, rows) of a k-image not drawFFT's
+        self.owners = []
 
     def emit(self, code, arg=0, a=0., b=0., c=0.):
         self.ops.append((code, arg, a, b, c))
@@ -103,6 +105,21 @@ class Lowering:
             self.tables.append(np.concatenate([args, vals, y2]))
             self.ntab += 3 * len(args)
         return self.tab_off[key]
+
+    def add_block(self, key, arr, owner=None):
+        """Append a block of doubles to the table buffer, once per ``key`` (None: always);
+        returns its offset.  ``owner`` is kept alive with the lowering (keys built from id())."""
+        if key is not None and key in self.tab_off:
+            return self.tab_off[key]
+        a = np.ascontiguousarray(arr, dtype=np.float64).ravel()
+        off = self.ntab
+        self.tables.append(a)
+        self.ntab += a.size
+        if key is not None:
+            self.tab_off[key] = off
+        if owner is not None:
+            self.owners.append(owner)
+        return off
 
     def add_spline_coefs(self, key, args, vals, y2):
         """The natural cubic spline of a table on equally spaced knots as per-interval
@@ -372,6 +389,8 @@ def draw_kimage(prof, bounds, dk, jac=None, Nk=None, return_device=False):
                                bounds.ymin == -(Nk // 2) and bounds.ymax == Nk // 2):
         raise _lib.GalSimAMDError("clip rows need the bounds (0..Nk/2, -Nk/2..Nk/2)")
     ctx = Lowering('f64', Nk=Nk, dk=dk)
+    ny, nx = bounds.numpyShape()
+    ctx.kbounds = (bounds.xmin, bounds.ymin, nx, ny)
     prof._lower(ctx, None if jac is None else np.asarray(jac, dtype=float).reshape(2, 2))
     check_stack(ctx.ops, precision='f64')
     ny, nx = bounds.numpyShape()

@@ -76,7 +76,19 @@ enum {
                              below ksq_min), x = hs ky r0; same fields as INCLINED_EXP */
     GSMI_LEAF_TOPHAT = 12, /* flux 2 J1(k r) / (k r), Taylor below (k r)^2 = 1e-4
                              (SBTopHatImpl::kValue2, src/SBBox.cpp:289-299): scale = radius */
-    GSMI_LEAF_DELTA = 13  /* flux everywhere (DeltaFunction, galsim/deltafunction.py) */
+    GSMI_LEAF_DELTA = 13, /* flux everywhere (DeltaFunction, galsim/deltafunction.py) */
+    GSMI_LEAF_INTERP_IMAGE = 14 /* SBInterpolatedImage's k value (src/SBInterpolatedImage.cpp:
+                             385-448, 767-1009), complex: the k-interpolant's weighted sum of
+                             the padded image's stored rfft half-plane at the taps around
+                             k N / (2 pi), times the x-interpolant's transform at k / (2 pi).
+                             table = offset (doubles) of a GSMI_INTERP_BLOCK-double parameter
+                             block (GSMI_INTERP_* indices); ntab = N/2 of the padded image;
+                             kind = 1: kValue (no cut but |k| <= maxk1, one tap on an integer
+                             k); 0: fillKImage, separable (rows and columns outside the
+                             reference's [i1, i2) x [j1, j2) are 0, one tap on an integer k)
+                             or, separable = 0, the form with cross terms (0 where |k N /
+                             2 pi| > maxk1 N / 2 pi).  Interpolant codes: 0 Nearest,
+                             1 Linear, 2 Cubic, 3 Quintic, 4 Lanczos. */
 };
 
 typedef struct gsmi_op {
@@ -84,6 +96,34 @@ typedef struct gsmi_op {
     int32_t arg;          /* leaf index for LEAF */
     double a, b, c;       /* SCALE: a; PHASE: ux, uy, fs; INV: threshold, value below it */
 } gsmi_op;
+
+/* Parameter block of a GSMI_LEAF_INTERP_IMAGE leaf (doubles in the table buffer). */
+#define GSMI_INTERP_N2 0        /* N/2 of the padded image */
+#define GSMI_INTERP_KSCALE 1    /* N / (2 pi): k -> k-image index */
+#define GSMI_INTERP_MAXK1 2     /* x-interpolant urange * 2 pi */
+#define GSMI_INTERP_MAXK1S 3    /* maxk1 * N / (2 pi) */
+#define GSMI_INTERP_KXR 4       /* k-interpolant xrange */
+#define GSMI_INTERP_I1 5        /* separable fill: first nonzero column (counted from the
+                                   k-image's first column) */
+#define GSMI_INTERP_I2 6        /* ... one past the last */
+#define GSMI_INTERP_J1 7        /* first nonzero row (counted from the k-image's first row) */
+#define GSMI_INTERP_J2 8        /* ... one past the last */
+#define GSMI_INTERP_XCODE 9     /* x-interpolant code */
+#define GSMI_INTERP_XN 10       /* x-interpolant Lanczos order */
+#define GSMI_INTERP_XUMAX 11    /* x-interpolant: transform taken as 0 beyond (Lanczos) */
+#define GSMI_INTERP_XUTAB 12    /* Lanczos: offset of its transform's spline table (knots,
+                                   values, second derivatives), -1 if none */
+#define GSMI_INTERP_XUTABN 13   /* number of knots */
+#define GSMI_INTERP_KCODE 14    /* k-interpolant code */
+#define GSMI_INTERP_KN 15       /* k-interpolant Lanczos order */
+#define GSMI_INTERP_KDC 16      /* k-interpolant Lanczos conserve_dc */
+#define GSMI_INTERP_KK 17       /* 5 values: its K(1..5) flux-correction coefficients */
+#define GSMI_INTERP_KIMAGE 22   /* offset of the k-image: (N rows ky = -N/2..N/2-1) x
+                                   (N/2 + 1 columns kx = 0..N/2) complex, interleaved */
+#define GSMI_INTERP_XMIN 23     /* the k-image's first column index (kx = index dk) */
+#define GSMI_INTERP_YMIN 24     /* ... and first row index */
+#define GSMI_INTERP_BLOCK 25
+#define GSMI_INTERP_MAX_TAPS 17 /* taps per axis the kernels take (Lanczos order <= 8) */
 
 typedef struct gsmi_leaf {
     int32_t type;         /* GSMI_LEAF_* */
@@ -255,6 +295,17 @@ int gsmi_spergel_integrated_flux(double nu, double u, double* out);
 /* Host evaluation of a Sersic leaf k-value (SersicInfo::kValue, src/SBSersic.cpp:363-376). */
 int gsmi_sersic_kvalue(double n, double trunc, const double* gsp, const double* ksq, double* out,
                        int count);
+
+/* Sine integral Si(x) of n values (math::Si, src/math/Sinc.cpp:101-170; here its power series
+ * below |x| = 4 and E1's continued fraction above). */
+int gsmi_sine_integral(const double* x, double* out, int n);
+/* Lanczos(n, conserve_dc) interpolant (Lanczos::Lanczos, src/Interpolant.cpp:787-895):
+ * out = {K(1), .., K(5), umax}, *ntab = knots of its transform's spline table; the table itself
+ * (knots u = 0, du, ..., values uCalc(u), spline second derivatives) by gsmi_lanczos_utable into
+ * buffers of cap values. */
+int gsmi_lanczos_info(int n, int conserve_dc, const double* gsp, double* out, int* ntab);
+int gsmi_lanczos_utable(int n, int conserve_dc, const double* gsp, double* args, double* vals,
+                        double* y2, int cap);
 
 /* ---------------------------------------------------------------------------------------
  * Device rendering.  All buffers are device pointers; stamps/ops/leaves/tables are device

@@ -20,6 +20,8 @@
 #include "math/Gamma.h"
 #include "math/Hankel.h"
 #include "fmath/fmath.hpp"
+#include "Interpolant.h"
+#include "math/Sinc.h"
 
 using namespace galsim;
 
@@ -127,6 +129,44 @@ void gsref_kvalue_range_2d(int m, double kmax, double ksqmax, double kx0, double
                            double dky, int* out)
 {
     GetKValueRange2d(out[0], out[1], m, kmax, ksqmax, kx0, dkx, ky0, dky);
+}
+
+
+// math::Si (src/math/Sinc.cpp:101-170)
+double gsref_si(double x) { return math::Si(x); }
+
+// Interpolant xval / uval / ranges (src/Interpolant.cpp): code 0 Nearest, 1 Linear, 2 Cubic,
+// 3 Quintic, 4 Lanczos(n, conserve_dc); out = {xrange, urange, ixrange}
+namespace {
+std::unique_ptr<Interpolant> make_interp(int code, int n, int dc, const double* gsp)
+{
+    GSParams g = *make_gsp(gsp);
+    switch (code) {
+      case 0: return std::unique_ptr<Interpolant>(new Nearest(g));
+      case 1: return std::unique_ptr<Interpolant>(new Linear(g));
+      case 2: return std::unique_ptr<Interpolant>(new Cubic(g));
+      case 3: return std::unique_ptr<Interpolant>(new Quintic(g));
+      default: return std::unique_ptr<Interpolant>(new Lanczos(n, dc != 0, g));
+    }
+}
+}
+
+int gsref_interp_eval(int code, int n, int dc, const double* gsp, const double* x, int count,
+                      double* xval, double* uval, double* ranges)
+{
+    try {
+        std::unique_ptr<Interpolant> I = make_interp(code, n, dc, gsp);
+        for (int i = 0; i < count; ++i) {
+            xval[i] = I->xval(x[i]);
+            uval[i] = I->uval(x[i]);
+        }
+        ranges[0] = I->xrange();
+        ranges[1] = I->urange();
+        ranges[2] = I->ixrange();
+        return 0;
+    } catch (...) {
+        return 1;
+    }
 }
 
 }
