@@ -23,6 +23,13 @@ Layout:
   profiles): SBProfile::draw's set-up and each leaf's fillXImage with the reference's
   accumulated coordinates, quadrant fill and per-component sum (src/SBProfile.cpp:304-488,
   src/SB*.cpp fillXImage, galsim/gsobject.py:1846-1898).
+* ``interpimage`` -- InterpolatedImage: the interpolants, SBInterpolatedImage's k-image, kValue
+  and both fillKImage forms, calculateMaxK, CalculateSizeContainingFlux, and the Python set-up
+  (padding, stepk / maxk, the SBTransform it draws through) (src/SBInterpolatedImage.cpp,
+  src/Interpolant.cpp, galsim/interpolatedimage.py).
+* ``cpu_draw.cpp`` / ``cpu_port`` -- the bulge + disk drawing restated in C++ (one stamp per
+  thread) for the CPU baseline, built into ``oracle/libcpudraw.so``; planned by the numpy oracle
+  and pinned against it.
 * ``ref``       -- ctypes loader for ``oracle/_ref/libgsref.so``, the reference's own profile
   numerics compiled from its sources by ``oracle/Makefile`` (used to pin this restatement).
 
