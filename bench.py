@@ -739,8 +739,8 @@ def main():
         t0 = time.perf_counter()
         cb = cpu_port.Batch(oprofs, nx, ny, 0.2, offs)       # sizes, leaf constants, tables
         t_cplan = time.perf_counter() - t0
-        cores = int(os.environ.get('OMP_NUM_THREADS') or 0) or (os.cpu_count() or 1)
-        cores = max(1, min(cores, os.cpu_count() or 1))
+        # the cores this process may run on (the GPU box gives one GPU's job 16)
+        cores = max(1, min(16, len(os.sched_getaffinity(0))))
         cb.draw(cores)                                          # warm-up
         reps = []
         for _ in range(5):
