@@ -38,7 +38,9 @@ __all__ = ['GSParams', 'BoundsI', 'PositionD', 'SBProfile', 'SBExponential', 'SB
            'SBConvolve', 'SBAutoConvolve', 'SBAutoCorrelate', 'SBAdd', 'SBDeconvolve',
            'ImageViewD', 'ImageViewF', 'ImageViewCD', 'ImageViewCF', 'ApplyKImagePhases',
            'wrapImage', 'irfft', 'goodFFTSize', 'SersicHLR', 'SersicIntegratedFlux',
-           'SersicTruncatedScale', 'MoffatCalculateSRFromHLR', 'lower_to_gsmi']
+           'SersicTruncatedScale', 'MoffatCalculateSRFromHLR', 'lower_to_gsmi',
+           'Interpolant', 'Nearest', 'Linear', 'Cubic', 'Quintic', 'Lanczos', 'Delta',
+           'SincInterpolant', 'SBInterpolatedImage', 'CalculateSizeContainingFlux']
 
 
 # -- small value types (pysrc/Bounds.cpp, pysrc/SBProfile.cpp:54-57) ------------------------------
@@ -256,6 +258,95 @@ class SBDeconvolve(SBProfile):
     def __init__(self, sbp, gsp):
         SBProfile.__init__(self, _amd().Deconvolution(sbp._obj, gsparams=_gsp(gsp),
                                                       propagate_gsparams=False))
+
+
+# -- interpolants and SBInterpolatedImage (pysrc/Interpolant.cpp:37-70,
+#    pysrc/SBInterpolatedImage.cpp:25-37) ------------------------------------------------------
+class Interpolant:
+    """An interpolant: the galsim_amd object it stands for, in ``_i``."""
+
+    def xval(self, x):
+        return float(self._i.xval(x))
+
+    def uval(self, u):
+        return float(self._i.uval(u))
+
+    def xvalMany(self, addr, n):
+        """In place on n doubles at address ``addr``."""
+        a = np.frombuffer((ctypes.c_double * int(n)).from_address(int(addr)), dtype=np.float64)
+        a[:] = self._i.xval(a)
+
+    def uvalMany(self, addr, n):
+        a = np.frombuffer((ctypes.c_double * int(n)).from_address(int(addr)), dtype=np.float64)
+        a[:] = self._i.uval(a)
+
+    def urange(self):
+        return self._i.urange
+
+
+def _interp(cls_name, make):
+    def init(self, *args):
+        self._i = make(*args)
+    return type(cls_name, (Interpolant,), {'__init__': init})
+
+
+def _ia():
+    from . import interpolant
+    return interpolant
+
+
+Nearest = _interp('Nearest', lambda gsp: _ia().Nearest(gsparams=_gsp(gsp)))
+Linear = _interp('Linear', lambda gsp: _ia().Linear(gsparams=_gsp(gsp)))
+Cubic = _interp('Cubic', lambda gsp: _ia().Cubic(gsparams=_gsp(gsp)))
+Quintic = _interp('Quintic', lambda gsp: _ia().Quintic(gsparams=_gsp(gsp)))
+Lanczos = _interp('Lanczos', lambda n, conserve_dc, gsp: _ia().Lanczos(
+    n, conserve_dc, gsparams=_gsp(gsp)))
+
+
+def _no_interp(name):
+    def init(self, *args):
+        raise NotImplementedError("the %s interpolant is not part of galsim_amd's "
+                                  "InterpolatedImage" % name)
+    return type(name, (Interpolant,), {'__init__': init})
+
+
+Delta = _no_interp('Delta')
+SincInterpolant = _no_interp('SincInterpolant')
+
+
+class SBInterpolatedImage(SBProfile):
+    """SBInterpolatedImage(image, init_bounds, nonzero_bounds, xInterp, kInterp, stepk, maxk,
+    gsp): ``image`` a double view of the padded image on (-N/2..N/2-1)^2 (copied here), the
+    bounds of the original image and of its nonzero part; stepk / maxk <= 0 are computed as the
+    reference's constructor does.  Its k-image (the padded image's rfft) is taken on the GPU."""
+
+    def __init__(self, image, init_bounds, nonzero_bounds, xInterp, kInterp, stepk, maxk, gsp):
+        from .interpolatedimage import SBInterpolatedImageProfile
+        b = image._bounds
+        N = b.xmax - b.xmin + 1
+        if b.xmin != -(N // 2) or b.ymin != b.xmin or b.ymax != b.xmax or N % 2:
+            raise _lib.GalSimAMDError("SBInterpolatedImage needs the padded image on "
+                                      "(-N/2..N/2-1)^2")
+        ib, nb = init_bounds, nonzero_bounds
+        SBProfile.__init__(self, SBInterpolatedImageProfile(
+            np.array(image._array(), dtype=np.float64), (ib.xmin, ib.xmax, ib.ymin, ib.ymax),
+            (nb.xmin, nb.xmax, nb.ymin, nb.ymax), xInterp._i, kInterp._i, float(stepk),
+            float(maxk), _gsp(gsp)))
+
+    def calculateMaxK(self, max_maxk=0.):
+        self._obj.calculateMaxK(float(max_maxk))
+
+    def calculateStepK(self, max_stepk=0.):
+        self._obj.calculateStepK(float(max_stepk))
+
+
+def CalculateSizeContainingFlux(image, target_flux):
+    """CalculateSizeContainingFlux (src/SBInterpolatedImage.cpp:1084-1110) of a double view whose
+    bounds contain (0, 0)."""
+    from .interpolatedimage import _size_containing_flux
+    b = image._bounds
+    return _size_containing_flux(np.asarray(image._array(), dtype=np.float64), b.xmin, b.ymin,
+                                 float(target_flux))
 
 
 # -- free functions (pysrc/SBTransform.cpp:34-46, pysrc/Image.cpp:58-86) ----------------------

@@ -23,6 +23,144 @@ from .interpolant import Quintic, convert_interpolant
 from .position import PositionD
 
 
+class SBInterpolatedImageProfile(GSObject):
+    """SBInterpolatedImage itself (src/SBInterpolatedImage.cpp:95-152), in the image's pixel
+    coordinates with no transformation: the padded image ``xim`` (array on (-N/2..N/2-1)^2),
+    the original image's bounds ``init_bounds`` and the nonzero region ``nonzero_bounds`` (each
+    (xmin, xmax, ymin, ymax)), the two interpolants, and stepk / maxk (<= 0: computed as the
+    constructor does -- the image box in quadrature with the x-interpolant's range, and the
+    x-interpolant's own maxk).  ``kimage`` (optional) is the padded image's rfft, if already
+    taken."""
+    _is_analytic_x = True
+
+    def __init__(self, xim, init_bounds, nonzero_bounds, x_interpolant, k_interpolant, stepk,
+                 maxk, gsparams=None, kimage=None):
+        self._gsparams = GSParams.check(gsparams)
+        self._xim = np.ascontiguousarray(xim, dtype=np.float64)
+        if self._xim.ndim != 2 or self._xim.shape[0] != self._xim.shape[1] or \
+                self._xim.shape[0] % 2:
+            raise ValueError("the padded image must be square with an even side")
+        self._init_bounds = tuple(int(v) for v in init_bounds)
+        self._nonzero_bounds = tuple(int(v) for v in nonzero_bounds)
+        self._x_interpolant = x_interpolant
+        self._k_interpolant = k_interpolant
+        self._maxk1 = x_interpolant.urange * 2. * math.pi
+        if stepk <= 0.:
+            xmin, xmax, ymin, ymax = self._init_bounds
+            R = max((xmax - xmin) / 2., (ymax - ymin) / 2.)
+            R2 = x_interpolant.xrange
+            stepk = math.pi / math.sqrt(R * R + R2 * R2)
+        self._stepk = float(stepk)
+        self._maxk = float(maxk) if maxk > 0. else self._maxk1
+        self._kimage = kimage
+
+    def _sub(self, b):
+        h = self._xim.shape[0] // 2
+        xmin, xmax, ymin, ymax = b
+        return self._xim[ymin + h: ymax + h + 1, xmin + h: xmax + h + 1]
+
+    @property
+    def _flux(self):
+        """getFlux (:1064-1078): the sum over the nonzero image."""
+        return float(np.sum(self._sub(self._nonzero_bounds), dtype=np.float64))
+
+    @property
+    def kimage(self):
+        """rfft(shift_in, shift_out) of the padded image on (0..N/2, -N/2..N/2-1), array
+        [ky + N/2, kx] (checkK, :450-464), taken on the GPU once."""
+        if self._kimage is None:
+            from .fft import rfft2
+            self._kimage = np.ascontiguousarray(rfft2(self._xim, shift_in=True, shift_out=True),
+                                                dtype=np.complex128)
+        return self._kimage
+
+    def calculateMaxK(self, max_maxk=0.):
+        """calculateMaxK (:1146-1217); sets and returns maxk."""
+        if max_maxk == 0.:
+            max_maxk = self._maxk
+        self._maxk = _calculate_maxk(self.kimage, self._flux, self._gsparams.maxk_threshold,
+                                     max_maxk)
+        return self._maxk
+
+    def calculateStepK(self, max_stepk=0.):
+        """calculateStepK (:1117-1144): the box holding 1 - folding_threshold of the flux, in
+        quadrature with the x-interpolant's range; sets and returns stepk."""
+        b = self._init_bounds
+        thresh = (1. - self._gsparams.folding_threshold) * self._flux
+        R = _size_containing_flux(self._sub(b), b[0], b[2], thresh)
+        R2 = self._x_interpolant.xrange
+        self._stepk = math.pi / math.sqrt(R * R + R2 * R2)
+        return self._stepk
+
+    def _lower(self, ctx, jac):
+        self._lower_leaf(ctx, jac, None)
+
+    def _lower_leaf(self, ctx, jac, J):
+        """The leaf as SBInterpolatedImage::fillKImage gets it from SBProfile::drawK with the
+        Jacobian ``jac``, through an SBTransform of diagonal Jacobian ``J`` if given
+        (src/SBProfile.cpp:383-425, src/SBTransform.cpp:885-969): the separable fill with its
+        column / row range while every Jacobian is diagonal, else the fill with cross terms; in
+        a kValue lowering, kValue's form."""
+        from .render import LeafRecord
+        A = D = 1.
+        if J is not None:
+            A, D = J[0, 0], J[1, 1]
+        jac1 = J if jac is None else jac if J is None else jac.dot(J)
+        j1 = np.eye(2) if jac1 is None else jac1
+        sep = jac is None or (jac[0, 1] == 0. and jac[1, 0] == 0.)
+        kv = bool(getattr(ctx, 'kvalue_mode', False))
+        rec = LeafRecord()
+        rec['type'] = _lib.LEAF_INTERP_IMAGE
+        rec['separable'] = 1 if sep else 0
+        rec['kind'] = 1 if kv else 0
+        rec['jac'] = (j1[0, 0], j1[0, 1], j1[1, 0], j1[1, 1])
+        rec['flux'] = 1.
+        K = self.kimage
+        No2 = K.shape[1] - 1
+        kscale = No2 / math.pi
+        xi, ki = self._x_interpolant, self._k_interpolant
+        maxk1 = self._maxk1
+        blk = np.zeros(_lib.INTERP_BLOCK)
+        blk[_lib.INTERP_N2] = No2
+        blk[_lib.INTERP_KSCALE] = kscale
+        blk[_lib.INTERP_MAXK1] = maxk1
+        blk[_lib.INTERP_MAXK1S] = maxk1 * kscale
+        blk[_lib.INTERP_KXR] = ki.xrange
+        if sep and not kv:
+            xmin, ymin, m, n = _kgrid(ctx)
+            dk = float(ctx.dk)
+            # SBProfile::drawK (src/SBProfile.cpp:392-410), then SBTransform's fwdT
+            x0, y0 = xmin * dk, ymin * dk
+            if jac is None:
+                kx0, dkx, ky0, dky = x0, dk, y0, dk
+            else:
+                kx0, dkx, ky0, dky = x0 * jac[0, 0], dk * jac[0, 0], y0 * jac[1, 1], \
+                    dk * jac[1, 1]
+            kx0, dkx, ky0, dky = A * kx0, A * dkx, D * ky0, D * dky
+            i1, i2, j1_, j2 = _fill_range(maxk1, kx0, dkx, ky0, dky, m, n)
+            blk[_lib.INTERP_I1], blk[_lib.INTERP_I2] = i1, i2
+            blk[_lib.INTERP_J1], blk[_lib.INTERP_J2] = j1_, j2
+            blk[_lib.INTERP_XMIN], blk[_lib.INTERP_YMIN] = xmin, ymin
+        xcode, xn, _, _, xumax, xtab = xi._device()
+        blk[_lib.INTERP_XCODE] = xcode
+        blk[_lib.INTERP_XN] = xn
+        blk[_lib.INTERP_XUMAX] = xumax
+        blk[_lib.INTERP_XUTAB] = -1
+        if xtab is not None:
+            blk[_lib.INTERP_XUTAB] = ctx.add_block(('lanczos_u', xi._key()), xtab)
+            blk[_lib.INTERP_XUTABN] = len(xtab) // 3
+        kcode, kn, kdc, kK, _, _ = ki._device()
+        blk[_lib.INTERP_KCODE] = kcode
+        blk[_lib.INTERP_KN] = kn
+        blk[_lib.INTERP_KDC] = 1. if kdc else 0.
+        blk[_lib.INTERP_KK: _lib.INTERP_KK + 5] = kK
+        blk[_lib.INTERP_KIMAGE] = ctx.add_block(('interp_kimage', id(K)), K.view(np.float64),
+                                                owner=K)
+        rec['table'] = ctx.add_block(None, blk)
+        rec['ntab'] = No2
+        ctx.emit(_lib.OP_LEAF, ctx.add_leaf(rec))
+
+
 class InterpolatedImage(GSObject):
     """galsim.InterpolatedImage(image, x_interpolant=None, k_interpolant=None,
     normalization='flux', scale=None, flux=None, pad_factor=4., calculate_stepk=True,
@@ -183,72 +321,20 @@ class InterpolatedImage(GSObject):
         return J, (cx, cy), abs(det) * (inv * inv * ratio)
 
     # -- lowering ------------------------------------------------------------------------------
+    def _sb(self):
+        """The SBInterpolatedImage it draws through (stepk and maxk in pixel units)."""
+        b = (self._bx0, self._bx0 + self._nx - 1, self._by0, self._by0 + self._ny - 1)
+        return SBInterpolatedImageProfile(self._xim, b, b, self._x_interpolant,
+                                          self._k_interpolant, self._stepk * self._scale,
+                                          self._maxk * self._scale, self._gsparams,
+                                          kimage=self.kimage)
+
     def _lower(self, ctx, jac):
         """InterpolatedImage._drawKImage (galsim/interpolatedimage.py:733-735): _sbp.drawK ->
         SBTransform::fillKImage (src/SBTransform.cpp:885-969): the SBInterpolatedImage's fill
-        with the k grid the transform hands it (the separable form, with its column / row range,
-        while every Jacobian on the way is diagonal), then the shift's phases or the flux
-        scaling."""
-        from .render import LeafRecord
+        with the k grid the transform hands it, then the shift's phases or the flux scaling."""
         J, (cx, cy), fs = self._world()
-        A = D = 1.
-        if J is not None:
-            A, D = J[0, 0], J[1, 1]
-        jac1 = J if jac is None else jac if J is None else jac.dot(J)
-        j1 = np.eye(2) if jac1 is None else jac1
-        diag_outer = jac is None or (jac[0, 1] == 0. and jac[1, 0] == 0.)
-        sep = diag_outer            # the pixel-scale Jacobian is diagonal
-        kv = bool(getattr(ctx, 'kvalue_mode', False))
-        rec = LeafRecord()
-        rec['type'] = _lib.LEAF_INTERP_IMAGE
-        rec['separable'] = 1 if sep else 0
-        rec['kind'] = 1 if kv else 0
-        rec['jac'] = (j1[0, 0], j1[0, 1], j1[1, 0], j1[1, 1])
-        rec['flux'] = 1.
-        K = self.kimage
-        No2 = K.shape[1] - 1
-        kscale = No2 / math.pi
-        xi, ki = self._x_interpolant, self._k_interpolant
-        maxk1 = xi.urange * 2. * math.pi
-        blk = np.zeros(_lib.INTERP_BLOCK)
-        blk[_lib.INTERP_N2] = No2
-        blk[_lib.INTERP_KSCALE] = kscale
-        blk[_lib.INTERP_MAXK1] = maxk1
-        blk[_lib.INTERP_MAXK1S] = maxk1 * kscale
-        blk[_lib.INTERP_KXR] = ki.xrange
-        if sep and not kv:
-            xmin, ymin, m, n = _kgrid(ctx)
-            dk = float(ctx.dk)
-            # SBProfile::drawK (src/SBProfile.cpp:392-410), then SBTransform's fwdT
-            x0, y0 = xmin * dk, ymin * dk
-            if jac is None:
-                kx0, dkx, ky0, dky = x0, dk, y0, dk
-            else:
-                kx0, dkx, ky0, dky = x0 * jac[0, 0], dk * jac[0, 0], y0 * jac[1, 1], \
-                    dk * jac[1, 1]
-            kx0, dkx, ky0, dky = A * kx0, A * dkx, D * ky0, D * dky
-            i1, i2, j1_, j2 = _fill_range(maxk1, kx0, dkx, ky0, dky, m, n)
-            blk[_lib.INTERP_I1], blk[_lib.INTERP_I2] = i1, i2
-            blk[_lib.INTERP_J1], blk[_lib.INTERP_J2] = j1_, j2
-            blk[_lib.INTERP_XMIN], blk[_lib.INTERP_YMIN] = xmin, ymin
-        xcode, xn, _, _, xumax, xtab = xi._device()
-        blk[_lib.INTERP_XCODE] = xcode
-        blk[_lib.INTERP_XN] = xn
-        blk[_lib.INTERP_XUMAX] = xumax
-        blk[_lib.INTERP_XUTAB] = -1
-        if xtab is not None:
-            blk[_lib.INTERP_XUTAB] = ctx.add_block(('lanczos_u', xi._key()), xtab)
-            blk[_lib.INTERP_XUTABN] = len(xtab) // 3
-        kcode, kn, kdc, kK, _, _ = ki._device()
-        blk[_lib.INTERP_KCODE] = kcode
-        blk[_lib.INTERP_KN] = kn
-        blk[_lib.INTERP_KDC] = 1. if kdc else 0.
-        blk[_lib.INTERP_KK: _lib.INTERP_KK + 5] = kK
-        blk[_lib.INTERP_KIMAGE] = ctx.add_block(('interp_kimage', id(self)), K.view(np.float64),
-                                                owner=self)
-        rec['table'] = ctx.add_block(None, blk)
-        rec['ntab'] = No2
-        ctx.emit(_lib.OP_LEAF, ctx.add_leaf(rec))
+        self._sb()._lower_leaf(ctx, jac, J)
         if cx != 0. or cy != 0.:
             if jac is None:
                 ux, uy = cx, cy

@@ -120,7 +120,36 @@ def to_galsim_style(obj):
         return Leaf(G.SBKolmogorov(obj.lam_over_r0, obj.flux, GSP))
     if isinstance(obj, g.Box):
         return Leaf(G.SBBox(obj.width, obj.height, obj.flux, GSP))
+    if isinstance(obj, g.InterpolatedImage):
+        return interpolated_image_style(obj)
     raise TypeError(type(obj))
+
+
+def _galsim_interp(i):
+    if isinstance(i, g.Lanczos):
+        return G.Lanczos(i.n, i.conserve_dc, GSP)
+    return getattr(G, type(i).__name__)(GSP)
+
+
+def interpolated_image_style(obj):
+    """InterpolatedImage._sbp as galsim builds it (galsim/interpolatedimage.py:398-427): an
+    SBInterpolatedImage of the padded image with stepk / maxk in pixel units, shifted by
+    -offset, then the pixel scale's transform with the flux ratio (galsim/wcs.py:1538-1548)."""
+    xim = np.ascontiguousarray(obj._xim)
+    N = xim.shape[0]
+    view = G.ImageViewD(xim.ctypes.data, 1, N, G.BoundsI(-(N // 2), N // 2 - 1, -(N // 2),
+                                                        N // 2 - 1))
+    b = G.BoundsI(obj._bx0, obj._bx0 + obj._nx - 1, obj._by0, obj._by0 + obj._ny - 1)
+    s = obj._scale
+    sbii = G.SBInterpolatedImage(view, b, b, _galsim_interp(obj.x_interpolant),
+                                 _galsim_interp(obj.k_interpolant), obj.stepk * s,
+                                 obj.maxk * s, GSP)
+    leaf = Leaf(sbii)
+    leaf._keep = xim
+    ratio = obj.flux / obj._image_flux if obj.flux != obj._image_flux else 1.
+    shifted = Transformation(leaf, None, (-obj._offset[0], -obj._offset[1]), 1.)
+    return Transformation(shifted, np.array([[s, 0.], [0., s]]), (0., 0.),
+                          (1. / s) ** 2 * ratio)
 
 
 def draw_fft_via_galsim_boundary(prof, nx, ny, scale, offset):
@@ -134,6 +163,7 @@ def draw_fft_via_galsim_boundary(prof, nx, ny, scale, offset):
     obj._drawKImage(kimage)
     k_direct = KImage(0, Nk // 2, -(Nk // 2), Nk // 2, dk)
     obj._sbp.drawK(k_direct._image, dk, 0)                  # the C++-side tree fill
+    filled = kimage.copy()                                  # wrapImage folds kimage in place
     # drawFFT_finish
     G.wrapImage(kimage._image, G.BoundsI(0, N // 2, -(N // 2), N // 2 - 1), True, False)
     r0 = Nk // 2 - N // 2
@@ -145,7 +175,7 @@ def draw_fft_via_galsim_boundary(prof, nx, ny, scale, offset):
                                 G.BoundsI(-(N // 2), N // 2 + 1, -(N // 2), N // 2 - 1)),
             True, True)
     img = real[b.ymin + N // 2:b.ymax + N // 2 + 1, b.xmin + N // 2:b.xmax + N // 2 + 1]
-    return img, kimage, k_direct, (N, Nk)
+    return img, filled, k_direct, (N, Nk)
 
 
 def _peak_rel(a, b):
@@ -210,3 +240,39 @@ def test_apply_kimage_phases_closed_form():
     kyp = jac[0, 1] * x + jac[1, 1] * y
     want = before * 2.5 * np.exp(-1j * (kxp * 0.4 + kyp * -0.7))
     np.testing.assert_allclose(im.array, want, rtol=0, atol=1e-13)
+
+
+def test_interpolated_image_through_galsim_boundary():
+    """An InterpolatedImage built as galsim builds it on _galsim (SBInterpolatedImage, its
+    calculateMaxK, the shift and pixel-scale transforms), drawn by drawFFT through the boundary,
+    against the oracle's InterpolatedImage; CalculateSizeContainingFlux and calculateMaxK against
+    the set-up of galsim_amd.InterpolatedImage."""
+    from oracle import interpimage as ii
+    from tests.test_interpimage_cpu import ref_image
+    arr = ref_image()[36:92, 40:90]
+    for xi in ('quintic', 'lanczos3'):
+        prod = g.InterpolatedImage(g.Image(arr), scale=0.3, x_interpolant=xi, offset=(0.2, -0.1))
+        orc = ii.InterpolatedImage(arr, 0.3, x_interpolant=xi, offset=(0.2, -0.1))
+        # the set-up helpers of the boundary give the product's numbers
+        xim = np.ascontiguousarray(prod._xim)
+        N = xim.shape[0]
+        b = G.BoundsI(prod._bx0, prod._bx0 + prod._nx - 1, prod._by0, prod._by0 + prod._ny - 1)
+        im_view = G.ImageViewD(xim[b.ymin + N // 2:, b.xmin + N // 2:].ctypes.data, 1, N, b)
+        R = G.CalculateSizeContainingFlux(im_view, (1. - 5e-3) * prod._image_flux)
+        assert math.pi / (math.hypot(R, prod.x_interpolant.xrange) * 0.3) == prod.stepk
+        view = G.ImageViewD(xim.ctypes.data, 1, N, G.BoundsI(-(N // 2), N // 2 - 1,
+                                                            -(N // 2), N // 2 - 1))
+        sb = G.SBInterpolatedImage(view, b, b, _galsim_interp(prod.x_interpolant),
+                                   _galsim_interp(prod.k_interpolant), prod.stepk * 0.3, 0., GSP)
+        sb.calculateMaxK(0.)
+        assert sb.maxK() / 0.3 == prod.maxk
+        # drawFFT through the boundary against the oracle
+        print("case InterpolatedImage", xi, flush=True)
+        img, kimage, k_direct, (N2, Nk) = draw_fft_via_galsim_boundary(prod, 48, 48, 0.3,
+                                                                        (0.1, 0.05))
+        want = orc.drawImage(48, 48, scale=0.3, offset=(0.1, 0.05))
+        err = _peak_rel(img, want)
+        kerr = float(np.max(np.abs(kimage.array - k_direct.array)) / np.max(np.abs(
+            k_direct.array)))
+        print("N", N2, "Nk", Nk, "peak-rel vs oracle", err, "k-image recursion vs tree", kerr)
+        assert err < F64_TOL and kerr < 1e-12
