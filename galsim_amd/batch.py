@@ -516,23 +516,31 @@ class BatchPlan:
                 if best is None or t < best[0]:
                     best = (t, r)
             rows[N] = best[1]
-        # fold item size (cells per item, the same for every FFT size), on the whole launch
-        best = None
+        # fold item size (cells per item, the same for every FFT size) x grid, on the whole
+        # launch; the best few combinations are then timed again, interleaved over several
+        # rounds, and the one with the lowest median wins (single short timings of launches
+        # this size vary by a few percent from run to run)
+        combos = []
         for cells in item_cells:
             ck = {N: max(1, cells // (cols[N] * N)) for N, _, _, _ in self.groups}
-            sched = self.make_schedule(cols, rows, chunks=ck)
-            t = self._time(lambda: self._launch_list(sched, 'fold', s), reps)
-            log[('item_cells', cells)] = t
-            if best is None or t < best[0]:
-                best = (t, ck)
-        chunks = best[1]
-        best = None
-        for g in grids:
-            sched = self.make_schedule(cols, rows, g, g, chunks=chunks)
-            t = self._time(lambda: self._launch_list(sched, 'all', s), reps)
-            log[('grid', g)] = t
-            if best is None or t < best[0]:
-                best = (t, g)
+            for g in grids:
+                sched = self.make_schedule(cols, rows, g, g, chunks=ck)
+                t = self._time(lambda: self._launch_list(sched, 'all', s), reps)
+                log[('item_cells', cells, 'grid', g)] = t
+                combos.append((t, cells, g, ck))
+        combos.sort(key=lambda c: c[0])
+        finalists = combos[:4]
+        scheds = [self.make_schedule(cols, rows, g, g, chunks=ck) for _, _, g, ck in finalists]
+        rounds = [[] for _ in finalists]
+        for _ in range(3):
+            for k, sched in enumerate(scheds):
+                rounds[k].append(self._time(lambda: self._launch_list(sched, 'all', s),
+                                            2 * reps))
+        med = [float(np.median(r)) for r in rounds]
+        k = int(np.argmin(med))
+        _, cells, g, chunks = finalists[k]
+        best = (med[k], g)
+        log[('final', cells, g)] = med[k]
         self.use_schedule(cols, rows, best[1], best[1], chunks)
         self.tuning = {'cols': cols, 'rows': rows, 'chunks': chunks, 'grid': best[1],
                        'ms': best[0],

@@ -562,17 +562,24 @@ __device__ __forceinline__ void bfly(cplx<T>* v)
     if constexpr (R == 8) bfly8(v);
 }
 
+// The first pass reads each column pair once: the inputs of output c, X[c] and X[N/2 - c], give
+// both v(c) = E + iO and v(N/2 - c) = conj(E) + i conj(O) (E(N/2 - c) = conj E(c) and, as
+// e^{2 pi i (N/2 - c) / N} = -conj e^{2 pi i c / N}, O(N/2 - c) = conj O(c)).  Butterfly b
+// (columns b + r NB) therefore runs with its partner NB - b (columns N/2 - b - r NB), whose
+// input r is this one's R - 1 - r; b = 0 and b = NB / 2 are their own partners.
 template <typename T, int N, int R>
 __device__ __forceinline__ void rows_first(const cplx<T>* __restrict__ in, int cs, int nrow,
                                            float inv_nrow, const cplx<T>* twl, cplx<T>* dst)
 {
-    constexpr int M = N / 2, L = M + 1, NB = M / R;
-    const int total = nrow * NB;
+    constexpr int M = N / 2, L = M + 1, NB = M / R, NP = NB / 2 + 1;
+    const int total = nrow * NP;
     for (int t = threadIdx.x; t < total; t += blockDim.x) {
         const int b = fdiv(t, inv_nrow);
         const int rl = t - b * nrow;
+        const int b2 = NB - b;
+        const bool pair = b != 0 && b2 != b;
         const cplx<T>* z = in + rl;
-        cplx<T> v[R];
+        cplx<T> v[R], vp[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int c = b + r * NB;
@@ -582,11 +589,18 @@ __device__ __forceinline__ void rows_first(const cplx<T>* __restrict__ in, int c
             const cplx<T> E = cadd(zc, cconj(zm));
             const cplx<T> O = cmul(csub(zc, cconj(zm)), twl[c]);
             v[r] = {E.x - O.y, E.y + O.x};
+            vp[R - 1 - r] = {E.x + O.y, -E.y + O.x};             // conj(E) + i conj(O)
         }
         bfly<R>(v);
         cplx<T>* d = dst + rl * L + b * R;
 #pragma unroll
         for (int r = 0; r < R; ++r) d[r] = v[r];
+        if (pair) {
+            bfly<R>(vp);
+            cplx<T>* d2 = dst + rl * L + b2 * R;
+#pragma unroll
+            for (int r = 0; r < R; ++r) d2[r] = vp[r];
+        }
     }
 }
 
