@@ -170,8 +170,23 @@ class SBProfile:
         image._array()[...] = arr.astype(image._dtype)
 
     def draw(self, image, dx, jac, xoff, yoff, flux_ratio):
-        raise NotImplementedError("real-space drawing through _galsim is not provided here; "
-                                  "use galsim_amd's drawImage(method='no_pixel')")
+        """SBProfile::draw (src/SBProfile.cpp:304-381): the profile at the centres of the
+        view's pixels (coordinates x dx - xoff, mapped by the Jacobian at address ``jac``),
+        times the pixel area and ``flux_ratio``, on the GPU (analytic leaves, their transforms
+        and sums; real-space convolution raises)."""
+        from . import real
+        if image._step != 1:
+            raise _lib.GalSimAMDError("draw needs a view with step 1 (as the reference)")
+        arr = real.draw_real(self._obj, image._bounds._amd(), float(dx), jac=_jac(jac),
+                             offset=(float(xoff), float(yoff)), flux_scaling=float(flux_ratio))
+        image._array()[...] = arr.astype(image._dtype)
+
+    def xValue(self, pos):
+        """The surface brightness at ``pos`` (SBProfile::xValue), from a one-pixel draw."""
+        from . import real
+        from .position import BoundsI as B
+        arr = real.draw_real(self._obj, B(0, 0, 0, 0), 1., offset=(-pos.x, -pos.y))
+        return float(arr[0, 0])
 
     def shoot(self, *args):
         raise NotImplementedError("photon shooting is outside this library's path")

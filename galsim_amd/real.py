@@ -228,10 +228,12 @@ def _leaf_component(leaf, m, n, setup, tables):
     return rec
 
 
-def components(prof, bounds, imscale=1., tables=None):
+def components(prof, bounds, imscale=1., tables=None, jac=None, offset=(0., 0.),
+               flux_scaling=1.):
     """gsmi_xcomp records (in drawing order) of ``prof`` drawn into an image with ``bounds``
-    (its profile origin at (0, 0)) and pixel scale ``imscale``; the Kolmogorov radial tables the
-    records index are appended to the list ``tables``."""
+    (its profile origin at (0, 0)) and pixel scale ``imscale``, under the Jacobian ``jac``, the
+    offset and the flux ratio SBProfile::draw takes (none by default); the Kolmogorov radial
+    tables the records index are appended to the list ``tables``."""
     if tables is None:
         tables = []
     from .convolve import Convolution
@@ -268,19 +270,23 @@ def components(prof, bounds, imscale=1., tables=None):
                                 float(offset[1]), float(flux_scaling))
             out.append(_leaf_component(obj, m, n, setup, tables))
 
-    walk(prof, None, (0., 0.), 1.)
+    walk(prof, None if jac is None else np.asarray(jac, dtype=float).reshape(2, 2),
+         (float(offset[0]), float(offset[1])), float(flux_scaling))
     return np.array(out, dtype=XCOMP_DTYPE) if out else np.zeros(0, dtype=XCOMP_DTYPE)
 
 
-def draw_real(prof, bounds, imscale=1., device_out=False):
+def draw_real(prof, bounds, imscale=1., device_out=False, jac=None, offset=(0., 0.),
+              flux_scaling=1.):
     """The FP64 image of ``prof`` sampled at the pixel centres of ``bounds`` (times the pixel
-    area, as SBProfile::draw), as a numpy array (or the device tensor with device_out)."""
+    area, as SBProfile::draw, with its Jacobian, offset and flux ratio if given), as a numpy
+    array (or the device tensor with device_out)."""
     import torch
     if not torch.cuda.is_available():
         raise _lib.GalSimAMDError("galsim_amd drawReal runs on the GPU; no HIP device is "
                                   "available")
     tables = []
-    recs = np.ascontiguousarray(components(prof, bounds, imscale, tables))
+    recs = np.ascontiguousarray(components(prof, bounds, imscale, tables, jac, offset,
+                                           flux_scaling))
     m, n = bounds.xmax - bounds.xmin + 1, bounds.ymax - bounds.ymin + 1
     out = torch.empty((n, m), dtype=torch.float64, device='cuda')
     dtab = (torch.from_numpy(np.concatenate(tables)).to('cuda') if tables else None)

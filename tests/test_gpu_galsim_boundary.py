@@ -276,3 +276,31 @@ def test_interpolated_image_through_galsim_boundary():
             k_direct.array)))
         print("N", N2, "Nk", Nk, "peak-rel vs oracle", err, "k-image recursion vs tree", kerr)
         assert err < F64_TOL and kerr < 1e-12
+
+
+def test_real_space_draw_and_xvalue_through_galsim_boundary():
+    """SBProfile.draw (src/SBProfile.cpp:304-381) on a view, as galsim's _drawReal calls it
+    on a leaf (Jacobian, offset, flux ratio), against the oracle's real-space restatement; and
+    xValue against the closed form of a sheared, shifted Gaussian."""
+    import oracle.xdraw as ox
+    sb = G.SBExponential(0.7, 2.5, GSP)
+    arr = np.zeros((21, 24))
+    view = G.ImageViewD(arr.ctypes.data, 1, 24, G.BoundsI(-12, 11, -10, 10))
+    jac = np.ascontiguousarray([[1.1, 0.2], [-0.1, 0.9]])
+    sb.draw(view, 0.2, _addr(jac), 0.13, -0.07, 1.7)
+    want = ox.draw_real(od.Transformation(od.Exponential(scale_radius=0.7, flux=2.5), jac,
+                                          (0.13, -0.07), 1.7 / abs(np.linalg.det(jac))),
+                        -12, 11, -10, 10, 0.2)
+    err = _peak_rel(arr, want)
+    print("draw peak-rel vs oracle", err)
+    assert err < 1e-12
+    gs = G.SBTransform(G.SBGaussian(0.8, 3.0, GSP), _addr(np.ascontiguousarray(
+        [[1.2, 0.1], [0.1, 0.9]])), 0.2, -0.1, 1.0, GSP)
+    x, y = 0.35, -0.4
+    J = np.array([[1.2, 0.1], [0.1, 0.9]])
+    u = np.linalg.solve(J, np.array([x - 0.2, y + 0.1]))
+    # SBTransform's amplitude scaling multiplies the surface brightness (its flux by |det J|)
+    val = 3.0 / (2. * np.pi * 0.8 ** 2) * np.exp(-0.5 * (u @ u) / 0.8 ** 2)
+    got = gs.xValue(G.PositionD(x, y))
+    print("xValue", got, val)
+    assert abs(got - val) < 1e-13 * val
