@@ -455,9 +455,9 @@ def fft_module_bench(args, torch):
     res = {}
     for N in (1174, 2048, 4096):
         x = torch.randn((N, N), dtype=torch.float64, device='cuda')
-        for _ in range(2):
+        for _ in range(args.fft_warmup):
             gfft.irfft2(gfft.rfft2(x, True, True), True, True)
-        k = 10
+        k = args.fft_pairs
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
         e0.record()
@@ -569,6 +569,11 @@ def main():
     ap.add_argument('--large-oracle', type=int, default=4,
                     help='stamps of the large-stamp workload checked against the numpy oracle')
     ap.add_argument('--no-fft-module', action='store_true')
+    ap.add_argument('--fft-module-only', action='store_true',
+                    help='only run the FFT-module leg (rfft2 + irfft2 of 1174^2, 2048^2 and '
+                         '4096^2 float64 images) and exit')
+    ap.add_argument('--fft-pairs', type=int, default=10)
+    ap.add_argument('--fft-warmup', type=int, default=2)
     ap.add_argument('--no-real-space', action='store_true')
     ap.add_argument('--no-config0', action='store_true')
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='MIB',
@@ -591,6 +596,9 @@ def main():
             y.copy_(x)
         torch.cuda.synchronize()
         log("copied %d MiB three times" % args.copy_calibration)
+        return
+    if args.fft_module_only:
+        fft_module_bench(args, torch)
         return
     import galsim_amd as g
     from galsim_amd import workloads

@@ -9,9 +9,11 @@
 //   * the real transforms use the half-length complex transform of z(m) = x(2m) + i x(2m+1) and
 //     split its result into the even / odd halves (forward), or build that sequence from the
 //     Hermitian half-plane (inverse);
-//   * a line length whose prime factors are 2, 3 and 5 runs as a mixed-radix Stockham transform
-//     (radix 5, 3, 8, 4, 2); any other length L as Bluestein's chirp-z convolution, in LDS, with a
-//     5-smooth transform length S >= 2L - 1;
+//   * a power-of-two line length from 128 to 4096 runs register-resident (radix-8 passes in
+//     registers, one LDS line for the exchanges between them, compile-time indexing);
+//   * any other length whose prime factors are 2, 3 and 5 runs as a mixed-radix Stockham
+//     transform in LDS (radix 5, 3, 8, 4, 2); any other length L as Bluestein's chirp-z
+//     convolution, in LDS, with a 5-smooth transform length S >= 2L - 1;
 //   * fftshift of an even axis is a rotation of its index by half its length: an input shift is
 //     applied where the line is read, an output shift where it is written (the reference
 //     multiplies by the equivalent alternating signs).
@@ -21,6 +23,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -258,6 +261,282 @@ c2r_rows(const cplx<T>* __restrict__ src, T* __restrict__ dst, int M, int N, int
     }
 }
 
+// ---------------------------------------------------------------------------------------------
+// Power-of-two lines of 128..4096 points, register-resident.  The T = L/8 threads of a line hold
+// its values eight to a thread (thread t: elements t + s T, s < 8).  A radix pass (8, and a last
+// 4 or 2 where L is not a power of 8) transforms a thread's own values in registers; between
+// passes the values go through one padded LDS line (written in Stockham order, read back in
+// thread order), and the first pass reads and the last pass writes global memory straight from
+// registers: two to three LDS round trips per line instead of one per pass plus the staging, one
+// LDS line per line instead of two, and every index a compile-time shift or mask.
+// ---------------------------------------------------------------------------------------------
+constexpr int kRegE = 8;
+
+// LDS slot of element p: one slot of padding after every 8 values (128 bytes), so the stride-8
+// scatter of the first pass falls on distinct banks.
+__device__ __forceinline__ int rpad(int p) { return p + (p >> 3); }
+
+// Slots per LDS line: odd, so neighbouring lines of a workgroup start on different banks.
+template <int L>
+constexpr int kRegLine = L + L / 8 + 1;
+
+// 16-byte loads and stores of complex values (the kernels below are launched only on 16-byte
+// aligned buffers).
+__device__ __forceinline__ cplx<double> ld2(const cplx<double>* p)
+{
+    const double2 x = *reinterpret_cast<const double2*>(p);
+    return {x.x, x.y};
+}
+__device__ __forceinline__ void st2(cplx<double>* p, cplx<double> v)
+{
+    *reinterpret_cast<double2*>(p) = double2{v.x, v.y};
+}
+
+template <int R, typename T>
+__device__ __forceinline__ void bfly_reg(cplx<T>* v)
+{
+    if constexpr (R == 2) bfly2(v);
+    if constexpr (R == 4) bfly4(v);
+    if constexpr (R == 8) bfly8(v);
+}
+
+// exp(+2 pi i m / L), m < L <= 4096, from the two 64-entry LDS tables tws[i] = exp(+2 pi i i / L)
+// and tws[64 + i] = exp(+2 pi i 64 i / L): one complex product (the passes then wait on no global
+// load, so a prefetch of the next lines stays in flight through them).
+template <int L>
+__device__ __forceinline__ cplx<double> reg_twiddle(const cplx<double>* tws, int m)
+{
+    if constexpr (L <= 64) return ld2(tws + m);
+    else return cmul(ld2(tws + 64 + (m >> 6)), ld2(tws + (m & 63)));
+}
+
+// Fill the twiddle tables of reg_twiddle from tw[m] = exp(+2 pi i m / L); ends with a barrier.
+template <int L>
+__device__ __forceinline__ void reg_twiddle_stage(const cplx<double>* __restrict__ tw,
+                                                  cplx<double>* tws)
+{
+    for (int e = threadIdx.x; e < 128; e += blockDim.x) {
+        const int m = e < 64 ? e : (e - 64) * 64;
+        st2(tws + e, m < L ? tw[m] : cplx<double>{1., 0.});
+    }
+    __syncthreads();
+}
+
+// Radix-R Stockham pass after passes whose radices multiply to NS, over thread t's values
+// (v[s] = element t + s T).  Butterfly j = t + b T (b < 8/R) takes elements j + r L/R, i.e.
+// v[b + r 8/R], and its output r goes to (j / NS) NS R + j % NS + r NS: to the LDS line, or, in the
+// last pass (NS R = L, so j / NS = 0), to position t + (b + r 8/R) T, back in v.
+// tws: the twiddle tables of reg_twiddle.
+template <int L, int R, int NS>
+__device__ __forceinline__ void reg_pass(cplx<double> (&v)[kRegE], int t,
+                                         const cplx<double>* tws, cplx<double>* line)
+{
+    constexpr int T = L / kRegE, B = kRegE / R;
+    cplx<double> out[kRegE];
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        const int j = t + b * T;
+        cplx<double> u[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) u[r] = v[b + r * B];
+        if constexpr (NS > 1) {
+            const int k = j & (NS - 1);
+            cplx<double> w[R];
+#pragma unroll
+            for (int r = 1; r < R; ++r) w[r] = reg_twiddle<L>(tws, (L / (NS * R)) * r * k);
+#pragma unroll
+            for (int r = 1; r < R; ++r) u[r] = cmul(u[r], w[r]);
+        }
+        bfly_reg<R>(u);
+        if constexpr (NS * R == L) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) out[b + r * B] = u[r];
+        } else {
+            const int base = (j - (j & (NS - 1))) * R + (j & (NS - 1));
+#pragma unroll
+            for (int r = 0; r < R; ++r) st2(line + rpad(base + r * NS), u[r]);
+        }
+    }
+    if constexpr (NS * R == L) {
+#pragma unroll
+        for (int s = 0; s < kRegE; ++s) v[s] = out[s];
+    }
+}
+
+// The e^{+2 pi i} transform of the line held in v (thread order in, thread order out: v[s] =
+// Y(t + s T)).  Every thread of the workgroup calls it (it holds barriers); `line` is this
+// thread's line in LDS, free on entry and on return.
+template <int L, int NS = 1>
+__device__ __forceinline__ void reg_fft(cplx<double> (&v)[kRegE], int t, const cplx<double>* tws,
+                                        cplx<double>* line)
+{
+    constexpr int rem = L / NS;
+    constexpr int R = rem >= kRegE ? kRegE : rem;
+    reg_pass<L, R, NS>(v, t, tws, line);
+    if constexpr (NS * R < L) {
+        constexpr int T = L / kRegE;
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < kRegE; ++s) v[s] = ld2(line + rpad(t + s * T));
+        __syncthreads();
+        reg_fft<L, NS * R>(v, t, tws, line);
+    }
+}
+
+// c2c_lines for plain line maps (line l at l ls, element n at n es) and power-of-two L: nl lines
+// per workgroup of nl L/8 threads.  COLS (es != 1): lanes take adjacent lines first, so each
+// access of a wave covers nl adjacent values of a row, and neighbouring workgroups go to one XCD;
+// else lanes take adjacent elements of a line.
+template <int L, bool COLS>
+__global__ void __launch_bounds__(1024)
+c2c_lines_reg(const cplx<double>* __restrict__ src, cplx<double>* __restrict__ dst, int nlines,
+              int nl, long long ls, long long es, int lrot, int erot, int krot, int conj_in,
+              int conj_out, double scale, const cplx<double>* __restrict__ tw)
+{
+    extern __shared__ unsigned char smem_raw[];
+    constexpr int T = L / kRegE;
+    const int li = COLS ? int(threadIdx.x) % nl : int(threadIdx.x) / T;
+    const int t = COLS ? int(threadIdx.x) / nl : int(threadIdx.x) % T;
+    const int l = (COLS ? xcd_contiguous(blockIdx.x, gridDim.x) : int(blockIdx.x)) * nl + li;
+    const bool live = l < nlines;
+    cplx<double>* lds = reinterpret_cast<cplx<double>*>(smem_raw);
+    cplx<double>* line = lds + li * kRegLine<L>;
+    cplx<double>* tws = lds + nl * kRegLine<L>;
+    reg_twiddle_stage<L>(tw, tws);
+    cplx<double> v[kRegE];
+    if (live) {
+        int sl = l + lrot;
+        if (sl >= nlines) sl -= nlines;
+        const cplx<double>* s = src + sl * ls;
+#pragma unroll
+        for (int q = 0; q < kRegE; ++q) {
+            cplx<double> x = ld2(s + ((t + q * T + erot) & (L - 1)) * es);
+            if (conj_in) x.y = -x.y;
+            v[q] = x;
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < kRegE; ++q) v[q] = {0., 0.};
+    }
+    reg_fft<L>(v, t, tws, line);
+    if (!live) return;
+    cplx<double>* d = dst + l * ls;
+    const double sy = conj_out ? -scale : scale;
+#pragma unroll
+    for (int q = 0; q < kRegE; ++q) {
+        // output position k holds Y((k + krot) mod L)
+        const int k = (t + q * T - krot) & (L - 1);
+        st2(d + k * es, {v[q].x * scale, v[q].y * sy});
+    }
+}
+
+// r2c_rows for N = 2L (L a power of two) and an even column rotation erot: the row's pairs
+// x(2m), x(2m+1) are one 16-byte load.
+template <int L>
+__global__ void __launch_bounds__(1024)
+r2c_rows_reg(const double* __restrict__ src, cplx<double>* __restrict__ dst, int M, int nl,
+             int lrot, int erot, const cplx<double>* __restrict__ tw,
+             const cplx<double>* __restrict__ twn)
+{
+    extern __shared__ unsigned char smem_raw[];
+    constexpr int T = L / kRegE, N = 2 * L, h = L, W = L + 1;
+    const int li = int(threadIdx.x) / T;
+    const int t = int(threadIdx.x) % T;
+    const int l = int(blockIdx.x) * nl + li;
+    const bool live = l < M;
+    cplx<double>* lds = reinterpret_cast<cplx<double>*>(smem_raw);
+    cplx<double>* line = lds + li * kRegLine<L>;
+    cplx<double>* tws = lds + nl * kRegLine<L>;
+    reg_twiddle_stage<L>(tw, tws);
+    cplx<double> v[kRegE];
+    if (live) {
+        int sl = l + lrot;
+        if (sl >= M) sl -= M;
+        const double* row = src + size_t(sl) * N;
+#pragma unroll
+        for (int q = 0; q < kRegE; ++q) {
+            const double2 x =
+                *reinterpret_cast<const double2*>(row + ((2 * (t + q * T) + erot) & (N - 1)));
+            v[q] = {x.x, -x.y};                  // conj(z): forward transform
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < kRegE; ++q) v[q] = {0., 0.};
+    }
+    reg_fft<L>(v, t, tws, line);                // conj(Z)
+#pragma unroll
+    for (int q = 0; q < kRegE; ++q) st2(line + rpad(t + q * T), v[q]);
+    __syncthreads();
+    if (!live) return;
+    cplx<double>* d = dst + size_t(l) * W;
+#pragma unroll
+    for (int q = 0; q < kRegE; ++q) {
+        const int k = t + q * T;
+        const cplx<double> zk = cconj(v[q]);
+        const cplx<double> zm = cconj(ld2(line + rpad(k == 0 ? 0 : h - k)));
+        const cplx<double> E = cadd(zk, cconj(zm));
+        const cplx<double> O = csub(zk, cconj(zm));
+        const cplx<double> u = cmul(cconj(twn[k]), O);
+        st2(d + k, {0.5 * (E.x + u.y), 0.5 * (E.y - u.x)});
+    }
+    if (t == 0) {
+        // k = h: Z(h) = Z(0)
+        const cplx<double> z0 = cconj(v[0]);
+        const cplx<double> E = cadd(z0, cconj(z0));
+        const cplx<double> O = csub(z0, cconj(z0));
+        const cplx<double> u = cmul(cconj(twn[h]), O);
+        st2(d + h, {0.5 * (E.x + u.y), 0.5 * (E.y - u.x)});
+    }
+}
+
+// c2r_rows for N = 2L (L a power of two) and an even column rotation krot: pixels x(2m), x(2m+1)
+// are one 16-byte store.
+template <int L>
+__global__ void __launch_bounds__(1024)
+c2r_rows_reg(const cplx<double>* __restrict__ src, double* __restrict__ dst, int M, int nl,
+             int lrot, int krot, double scale, const cplx<double>* __restrict__ tw,
+             const cplx<double>* __restrict__ twn)
+{
+    extern __shared__ unsigned char smem_raw[];
+    constexpr int T = L / kRegE, N = 2 * L, h = L, W = L + 1;
+    const int li = int(threadIdx.x) / T;
+    const int t = int(threadIdx.x) % T;
+    const int l = int(blockIdx.x) * nl + li;
+    const bool live = l < M;
+    cplx<double>* lds = reinterpret_cast<cplx<double>*>(smem_raw);
+    cplx<double>* line = lds + li * kRegLine<L>;
+    cplx<double>* tws = lds + nl * kRegLine<L>;
+    reg_twiddle_stage<L>(tw, tws);
+    cplx<double> v[kRegE];
+    if (live) {
+        int sl = l + lrot;
+        if (sl >= M) sl -= M;
+        const cplx<double>* z = src + size_t(sl) * W;
+#pragma unroll
+        for (int q = 0; q < kRegE; ++q) {
+            const int c = t + q * T;
+            cplx<double> zc = ld2(z + c);
+            cplx<double> zm = ld2(z + (h - c));
+            if (c == 0) { zc.y = 0.; zm.y = 0.; }
+            const cplx<double> E = cadd(zc, cconj(zm));
+            const cplx<double> O = cmul(csub(zc, cconj(zm)), twn[c]);
+            v[q] = {E.x - O.y, E.y + O.x};
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < kRegE; ++q) v[q] = {0., 0.};
+    }
+    reg_fft<L>(v, t, tws, line);
+    if (!live) return;
+    double* o = dst + size_t(l) * N;
+#pragma unroll
+    for (int q = 0; q < kRegE; ++q) {
+        // x(2m) and x(2m+1) of m = t + q T land at columns 2m - krot and 2m + 1 - krot (mod N)
+        const int j = (2 * (t + q * T) - krot) & (N - 1);
+        *reinterpret_cast<double2*>(o + j) = double2{v[q].x * scale, v[q].y * scale};
+    }
+}
+
 // The full M x N transform of a real image from its M x (N/2 + 1) half (galsim/fft.py fft2's
 // Hermitian fill): column j >= N/2 (or j < N/2 with shift_out, whose columns are rotated by N/2)
 // is conj of the half-plane at row (M - i) mod M, column N - j (N/2 - j with shift_out).
@@ -409,6 +688,120 @@ int lines_per_group(int S, int nlines, size_t lds)
     return nl;
 }
 
+// The register-resident kernels: power-of-two lengths 128..4096 on 16-byte aligned buffers,
+// 4096 / L lines (512 threads) per workgroup, two LDS lines (about 74 KiB) and two workgroups
+// per CU.
+bool reg_len(int L) { return L >= 128 && L <= 4096 && (L & (L - 1)) == 0; }
+
+bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
+
+int reg_lines(int L, int nlines) { return std::max(1, std::min(nlines, 4096 / L)); }
+
+template <int L>
+int launch_c2c_reg_t(const cplx<double>* src, cplx<double>* dst, int nlines, long long ls,
+                     long long es, int lrot, int erot, int krot, int conj_io, double scale,
+                     hipStream_t s)
+{
+    const cplx<double>* tw = twiddles<double>(L);
+    if (!tw) {
+        set_error("twiddle table allocation failed");
+        return GSMI_E_HIP;
+    }
+    // columns of 4096: two per workgroup (1024 threads), so a wave's row accesses are 32 bytes
+    const int nl = (es != 1 && L == 4096) ? std::min(nlines, 2) : reg_lines(L, nlines);
+    const size_t bytes = (size_t(nl) * kRegLine<L> + 128) * sizeof(cplx<double>);
+    const dim3 grid((nlines + nl - 1) / nl), block(nl * (L / kRegE));
+    int rc;
+    if (es == 1) {
+        if ((rc = allow_lds(reinterpret_cast<const void*>(c2c_lines_reg<L, false>), bytes)))
+            return rc;
+        hipLaunchKernelGGL((c2c_lines_reg<L, false>), grid, block, bytes, s, src, dst, nlines, nl,
+                           ls, es, lrot, erot, krot, conj_io, conj_io, scale, tw);
+    } else {
+        if ((rc = allow_lds(reinterpret_cast<const void*>(c2c_lines_reg<L, true>), bytes)))
+            return rc;
+        hipLaunchKernelGGL((c2c_lines_reg<L, true>), grid, block, bytes, s, src, dst, nlines, nl,
+                           ls, es, lrot, erot, krot, conj_io, conj_io, scale, tw);
+    }
+    return hip_check(hipGetLastError(), "c2c_lines_reg launch");
+}
+
+template <int L>
+int launch_r2c_reg_t(const double* src, cplx<double>* dst, int M, int lrot, int erot,
+                     hipStream_t s)
+{
+    const cplx<double>* tw = twiddles<double>(L);
+    const cplx<double>* twn = twiddles<double>(2 * L);
+    if (!tw || !twn) {
+        set_error("twiddle table allocation failed");
+        return GSMI_E_HIP;
+    }
+    const int nl = reg_lines(L, M);
+    const size_t bytes = (size_t(nl) * kRegLine<L> + 128) * sizeof(cplx<double>);
+    int rc = allow_lds(reinterpret_cast<const void*>(r2c_rows_reg<L>), bytes);
+    if (rc) return rc;
+    hipLaunchKernelGGL(r2c_rows_reg<L>, dim3((M + nl - 1) / nl), dim3(nl * (L / kRegE)), bytes, s,
+                       src, dst, M, nl, lrot, erot, tw, twn);
+    return hip_check(hipGetLastError(), "r2c_rows_reg launch");
+}
+
+template <int L>
+int launch_c2r_reg_t(const cplx<double>* src, double* dst, int M, int lrot, int krot,
+                     double scale, hipStream_t s)
+{
+    const cplx<double>* tw = twiddles<double>(L);
+    const cplx<double>* twn = twiddles<double>(2 * L);
+    if (!tw || !twn) {
+        set_error("twiddle table allocation failed");
+        return GSMI_E_HIP;
+    }
+    const int nl = reg_lines(L, M);
+    const size_t bytes = (size_t(nl) * kRegLine<L> + 128) * sizeof(cplx<double>);
+    int rc = allow_lds(reinterpret_cast<const void*>(c2r_rows_reg<L>), bytes);
+    if (rc) return rc;
+    hipLaunchKernelGGL(c2r_rows_reg<L>, dim3((M + nl - 1) / nl), dim3(nl * (L / kRegE)), bytes, s,
+                       src, dst, M, nl, lrot, krot, scale, tw, twn);
+    return hip_check(hipGetLastError(), "c2r_rows_reg launch");
+}
+
+#define GSMI_REG_SWITCH(L, CALL)                                                                \
+    switch (L) {                                                                                \
+      case 128: return CALL(128);                                                               \
+      case 256: return CALL(256);                                                               \
+      case 512: return CALL(512);                                                               \
+      case 1024: return CALL(1024);                                                             \
+      case 2048: return CALL(2048);                                                             \
+      case 4096: return CALL(4096);                                                             \
+      default: set_error("no register-resident transform for this length"); return GSMI_E_SIZE; \
+    }
+
+int launch_c2c_reg(int L, const cplx<double>* src, cplx<double>* dst, int nlines, long long ls,
+                   long long es, int lrot, int erot, int krot, int conj_io, double scale,
+                   hipStream_t s)
+{
+#define GSMI_C2C(NN) launch_c2c_reg_t<NN>(src, dst, nlines, ls, es, lrot, erot, krot, conj_io, \
+                                          scale, s)
+    GSMI_REG_SWITCH(L, GSMI_C2C)
+#undef GSMI_C2C
+}
+
+int launch_r2c_reg(int L, const double* src, cplx<double>* dst, int M, int lrot, int erot,
+                   hipStream_t s)
+{
+#define GSMI_R2C(NN) launch_r2c_reg_t<NN>(src, dst, M, lrot, erot, s)
+    GSMI_REG_SWITCH(L, GSMI_R2C)
+#undef GSMI_R2C
+}
+
+int launch_c2r_reg(int L, const cplx<double>* src, double* dst, int M, int lrot, int krot,
+                   double scale, hipStream_t s)
+{
+#define GSMI_C2R(NN) launch_c2r_reg_t<NN>(src, dst, M, lrot, krot, scale, s)
+    GSMI_REG_SWITCH(L, GSMI_C2R)
+#undef GSMI_C2R
+}
+#undef GSMI_REG_SWITCH
+
 int launch_lines(const cplx<double>* src, cplx<double>* dst, int nlines, LineMap in, int lrot,
                  int erot, LineMap out, int krot, int conj_in, int conj_out, double scale,
                  const LinePlan<double>& p, PassTwiddle<double> ptw, size_t lds, hipStream_t s)
@@ -428,6 +821,8 @@ int launch_c2c(const cplx<double>* src, cplx<double>* dst, int nlines, long long
                int lrot, int erot, int krot, int conj_io, double scale, const LinePlan<double>& p,
                size_t lds, hipStream_t s)
 {
+    if (reg_len(p.L) && !p.chirp && aligned16(src) && aligned16(dst))
+        return launch_c2c_reg(p.L, src, dst, nlines, ls, es, lrot, erot, krot, conj_io, scale, s);
     const LineMap m{nlines, 0, ls, es};
     return launch_lines(src, dst, nlines, m, lrot, erot, m, krot, conj_io, conj_io, scale, p,
                         PassTwiddle<double>{nullptr, 1, 1}, lds, s);
@@ -578,12 +973,17 @@ int gsmi_rfft2_f64(const double* in, int M, int N, int shift_in, int shift_out, 
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     auto* tmp = reinterpret_cast<cplx<double>*>(scratch);
     const int W = N / 2 + 1;
+    if (reg_len(N / 2) && aligned16(in) && aligned16(tmp)) {
+        rc = launch_r2c_reg(N / 2, in, tmp, M, shift_in ? M / 2 : 0, shift_in ? N / 2 : 0, s);
+    } else {
     const int nl = lines_per_group(px.S, M, lds);
     const size_t bytes = 2 * size_t(nl) * px.S * sizeof(cplx<double>);
     if ((rc = allow_lds(reinterpret_cast<const void*>(r2c_rows<double>), bytes))) return rc;
     hipLaunchKernelGGL(r2c_rows<double>, dim3((M + nl - 1) / nl), dim3(kThreads), bytes, s, in,
                        tmp, M, N, nl, shift_in ? M / 2 : 0, shift_in ? N / 2 : 0, px, twn);
-    if ((rc = hip_check(hipGetLastError(), "r2c_rows launch"))) return rc;
+    rc = hip_check(hipGetLastError(), "r2c_rows launch");
+    }
+    if (rc) return rc;
     if (ly)
         return launch_columns_long(tmp, tmp, reinterpret_cast<cplx<double>*>(out), W, lpy, 0,
                                    shift_out, 1, 1., lds, s);
@@ -625,6 +1025,9 @@ int gsmi_irfft2_f64(const double* in, int M, int N, int shift_in, int shift_out,
                         shift_in ? M / 2 : 0, 0, 0, 1., py, lds, s);
     }
     if (rc) return rc;
+    if (reg_len(N / 2) && aligned16(tmp) && aligned16(out))
+        return launch_c2r_reg(N / 2, tmp, out, M, shift_out ? M / 2 : 0, shift_out ? N / 2 : 0,
+                              1. / (double(M) * double(N)), s);
     const int nl = lines_per_group(px.S, M, lds);
     const size_t bytes = 2 * size_t(nl) * px.S * sizeof(cplx<double>);
     if ((rc = allow_lds(reinterpret_cast<const void*>(c2r_rows<double>), bytes))) return rc;

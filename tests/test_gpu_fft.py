@@ -58,7 +58,10 @@ def test_matches_numpy_on_reference_inputs(si, so):
 # direct mixed-radix lengths (2, 3, 5 factors) and Bluestein lengths (prime factors 7, 11, 23,
 # 131, 587)
 SIZES = [(64, 96), (128, 128), (250, 384), (14, 22), (46, 262), (1174, 1174), (2048, 1536),
-         (300, 2048), (6144, 40), (8192, 64)]
+         (300, 2048), (6144, 40), (8192, 64),
+         # every power-of-two line length of the register-resident transforms (128..4096) along
+         # rows and columns, line counts below and off a multiple of a workgroup's lines
+         (256, 512), (1024, 1024), (4096, 256), (512, 4096), (6, 256), (4096, 4), (2048, 2048)]
 
 
 @pytest.mark.parametrize("shape", SIZES)
@@ -73,6 +76,17 @@ def test_sizes_against_numpy(shape):
         rk = g.fft.rfft2(x, si, so)
         _close(rk, of.rfft2(x, si, so))
         _close(g.fft.fft2(x, si, so), of.fft2(x, si, so))
+        _close(g.fft.irfft2(rk, so, si), x)
+
+
+def test_real_transform_rows_of_8192():
+    # rfft2 / irfft2 rows of 8192 run as complex transforms of 4096 (complex rows of 8192 exceed
+    # one workgroup's LDS and raise)
+    rng = np.random.default_rng(8192)
+    x = rng.normal(size=(64, 8192))
+    for si, so in ((False, False), (True, True)):
+        rk = g.fft.rfft2(x, si, so)
+        _close(rk, of.rfft2(x, si, so))
         _close(g.fft.irfft2(rk, so, si), x)
 
 
