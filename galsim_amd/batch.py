@@ -385,11 +385,11 @@ class BatchPlan:
 
     def _fold_extra(self):
         """LDS bytes the specialised fold kernel of the batch's program shape keeps beside its
-        buffers: 4 decoded leaves of 16 values, and in FP32 the cubic coefficients of the
+        buffers: 4 decoded leaves of 20 values, and in FP32 the cubic coefficients of the
         largest Sersic table (16 bytes per knot interval)."""
         if not self.program_shape:
             return 0
-        extra = 4 * 16 * (4 if self.precision == 'f32' else 8)
+        extra = 4 * 20 * (4 if self.precision == 'f32' else 8)
         if self.precision == 'f32':
             sel = self.host_leaves['type'] == _lib.LEAF_SERSIC
             if sel.any():

@@ -57,6 +57,7 @@ struct LeafC {
     T s, s2, flux, t0;    // scale, Box height scale, flux (Moffat: knorm); t0..t2: Sersic
     T p[6];               // type-specific constants
     T t1, t2;             // Sersic: t0, t1 = first and last log-k knot, t2 = 1 / knot spacing
+    T q0, q1, q2, q3;     // |k'|^2 s^2 = q0 kx^2 + q1 kx ky + q2 ky^2 (q3: padding)
 };
 
 struct LeafI {
@@ -79,6 +80,12 @@ __device__ inline LeafC<T> leaf_consts(const gsmi_leaf& L)
     c.t0 = c.t1 = c.t2 = T(0);
 #pragma unroll
     for (int k = 0; k < 6; ++k) c.p[k] = T(L.p[k]);
+    // the quadratic form of k' = J^T k scaled by s, in FP64 from the record
+    const double s2 = L.scale * L.scale;
+    c.q0 = T(s2 * (L.jac[0] * L.jac[0] + L.jac[1] * L.jac[1]));
+    c.q1 = T(2. * s2 * (L.jac[0] * L.jac[2] + L.jac[1] * L.jac[3]));
+    c.q2 = T(s2 * (L.jac[2] * L.jac[2] + L.jac[3] * L.jac[3]));
+    c.q3 = T(0);
     return c;
 }
 
@@ -165,6 +172,8 @@ __device__ __forceinline__ LeafC<T> uniform_leaf(const LeafC<T>& in)
 #pragma unroll
     for (int k = 0; k < 6; ++k) out.p[k] = uniform(in.p[k]);
     out.t1 = uniform(in.t1); out.t2 = uniform(in.t2);
+    out.q0 = uniform(in.q0); out.q1 = uniform(in.q1); out.q2 = uniform(in.q2);
+    out.q3 = T(0);
     return out;
 }
 
@@ -287,6 +296,24 @@ __device__ inline bool exp_clip(const LeafC<T>& C, const LeafI& I, const Tables<
         }
     }
     return ksq > C.p[0];
+}
+
+// FP32 values of the radial leaves from ksq = |k'|^2 s^2: Exponential (clipped at ksq > p0, as
+// exp_clip does in FP32), Kolmogorov, and Sersic read from its cubic coefficients (I.cf >= 0).
+template <typename T>
+__device__ __forceinline__ float radial_f32(const LeafC<T>& C, const LeafI& I,
+                                            const Tables<T>& tb, float ksq)
+{
+    if (I.type == GSMI_LEAF_EXPONENTIAL) {
+        if (ksq > C.p[0]) return 0.f;
+        const float r = __builtin_amdgcn_rsqf(1.f + ksq);      // (1 + ksq)^(-3/2) = r^3
+        return C.flux * r * r * r;
+    }
+    if (I.type == GSMI_LEAF_KOLMOGOROV) {
+        const float t = __builtin_amdgcn_exp2f((5.f / 6.f) * __builtin_amdgcn_logf(ksq));
+        return C.flux * __builtin_amdgcn_exp2f(-1.44269504088896341f * t);
+    }
+    return C.flux * sersic_f32(C, tb.coef + I.cf, I.ntab, ksq);
 }
 
 template <typename T>
@@ -415,6 +442,13 @@ template <typename T>
 __device__ inline T eval_leafc(const LeafC<T>& C, const LeafI& I, const gsmi_leaf* rec,
                                const Tables<T>& tb, T kx, T ky, int i, int j, int Nk)
 {
+    if constexpr (sizeof(T) == 4) {
+        // the radial leaves need only |k'|^2 s^2, the leaf's quadratic form in (kx, ky): three
+        // multiply-adds per leaf, with kx^2, kx ky and ky^2 shared by a program's leaves
+        if (I.type == GSMI_LEAF_EXPONENTIAL || I.type == GSMI_LEAF_KOLMOGOROV ||
+            (I.type == GSMI_LEAF_SERSIC && I.cf >= 0))
+            return radial_f32(C, I, tb, C.q0 * (kx * kx) + C.q1 * (kx * ky) + C.q2 * (ky * ky));
+    }
     T kxp, kyp;
     leaf_kprime(C, I, kx, ky, &kxp, &kyp);
     const T X = kxp * C.s;
@@ -1037,10 +1071,8 @@ __device__ __forceinline__ T eval_static(const LeafC<T>* __restrict__ lc,
         if constexpr (sizeof(T) == 4) {
             if (P.type[l] == GSMI_LEAF_SERSIC) {
                 // specialised FP32 kernels stage the coefficients at tb.coef (LeafI::cf = 0)
-                T kxp, kyp;
-                leaf_kprime(lc[l], I, kx, ky, &kxp, &kyp);
-                const T X = kxp * lc[l].s, Y = kyp * lc[l].s;
-                v[l] = lc[l].flux * sersic_f32(lc[l], tb.coef, I.ntab, X * X + Y * Y);
+                const T ksq = lc[l].q0 * (kx * kx) + lc[l].q1 * (kx * ky) + lc[l].q2 * (ky * ky);
+                v[l] = lc[l].flux * sersic_f32(lc[l], tb.coef, I.ntab, ksq);
                 continue;
             }
         }

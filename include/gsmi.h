@@ -203,8 +203,8 @@ typedef struct gsmi_schedule {
                              3 Sersic x Kolmogorov, 4 Exponential x Kolmogorov (leaf slots in
                              that order).  Stamps of another shape still draw correctly (the
                              kernel checks every stamp) at the interpreter's speed.  With a
-                             nonzero program fold_lds must also hold 4 decoded leaves (64 B
-                             each in FP32, 128 B in FP64). */
+                             nonzero program fold_lds must also hold 4 decoded leaves (80 B
+                             each in FP32, 160 B in FP64). */
 } gsmi_schedule;
 
 const char* gsmi_last_error(void);
