@@ -21,7 +21,7 @@ STAMP_DTYPE = np.dtype([('N', np.int32), ('Nk', np.int32), ('nx', np.int32), ('n
                         ('op_count', np.int32), ('leaf_begin', np.int32),
                         ('axis_count', np.int32), ('out_offset', np.int64),
                         ('scratch_offset', np.int64), ('axis_offset', np.int64),
-                        ('dk', np.float64), ('status', np.int32), ('reserved', np.int32)],
+                        ('dk', np.float64), ('status', np.int32), ('col_stride', np.int32)],
                        align=True)
 assert OP_DTYPE.itemsize == 32 and LEAF_DTYPE.itemsize == 128 and STAMP_DTYPE.itemsize == 80
 STAMP_NONFINITE = 1      # device: a non-finite output pixel (gsmi_stamp.status)

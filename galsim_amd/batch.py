@@ -327,8 +327,11 @@ class BatchPlan:
         out_off = starts(sizes)
         recs['out_offset'] = out_off
         # scratch: every stamp's column-transform rows, then every stamp's axis factors, with
-        # no region shared between stamps (a work-list launch runs all FFT sizes at once)
-        rows = recs['ny'].astype(np.int64) * (recs['N'].astype(np.int64) // 2 + 1)
+        # no region shared between stamps (a work-list launch runs all FFT sizes at once); the
+        # columns are padded to a multiple of 16 values so each starts on a 128-byte line (the
+        # row transform reads chunks of rows of every column)
+        recs['col_stride'] = (recs['ny'] + 15) // 16 * 16
+        rows = recs['col_stride'].astype(np.int64) * (recs['N'].astype(np.int64) // 2 + 1)
         axis = 3 * (recs['Nk'].astype(np.int64) // 2) + 2
         recs['scratch_offset'] = starts(rows)
         recs['axis_offset'] = int(rows.sum()) + starts(axis)

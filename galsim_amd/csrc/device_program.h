@@ -1455,6 +1455,12 @@ __device__ __forceinline__ cplx<T>* fft_lds_c(cplx<T>* a, cplx<T>* b, int n, int
     }
 }
 
+// Complex values between the columns of a stamp's column-transformed scratch (gsmi_stamp).
+__device__ __forceinline__ int col_stride(const gsmi_stamp* st)
+{
+    return st->col_stride > st->ny ? st->col_stride : st->ny;
+}
+
 // Workgroups of a launch are dispatched round-robin over the 8 XCDs, each with its own L2.  This
 // bijection of [0, G) gives each XCD one contiguous range of logical indices, so neighbouring
 // tiles (adjacent columns, adjacent column chunks of one stamp) share an L2 and whole cache lines.

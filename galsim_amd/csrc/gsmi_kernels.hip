@@ -233,6 +233,7 @@ __device__ __forceinline__ void fold_chunks(const gsmi_stamp* __restrict__ st, i
     const T dk = T(st->dk);
     cplx<T>* out = scratch + st->scratch_offset;
     const int ny = st->ny;
+    const size_t cs = col_stride(st);
     const int y0 = st->y0;
     const float inv_n = 1.f / float(N);
     for (int c0 = first; c0 < first + count; c0 += cpg) {
@@ -246,11 +247,11 @@ __device__ __forceinline__ void fold_chunks(const gsmi_stamp* __restrict__ st, i
         else
             fold_cells<T>(sops, nops, sleaves, tb, X, X + 2 * hk + 1, dk, N, Nk, c0, ncol, bufa);
         cplx<T>* res = fft_lds_c<T, true>(bufa, bufb, N, ncol, twp, twstride);
-        // rows of the output window (position p <-> DFT index p mod N), column-major: the
-        // chunk's columns are one contiguous run of ncol * ny values in scratch
+        // rows of the output window (position p <-> DFT index p mod N), column-major, columns
+        // cs apart
         const int LN = lds_line(N);
         const float inv_ny = 1.f / float(ny);
-        cplx<T>* oc = out + size_t(c0) * ny;
+        cplx<T>* oc = out + size_t(c0) * cs;
         for (int e = threadIdx.x; e < ncol * ny; e += blockDim.x) {
             const int cl = fdiv(e, inv_ny);
             const int row = e - cl * ny;
@@ -259,10 +260,10 @@ __device__ __forceinline__ void fold_chunks(const gsmi_stamp* __restrict__ st, i
             const cplx<T> v = res[cl * LN + lds_pad(y)];
             if (c0 + cl == 0) {
                 // the packed column: column 0 in the real part, column N/2 in the imaginary part
-                oc[e] = {v.x, T(0)};
-                out[size_t(N / 2) * ny + row] = {v.y, T(0)};
+                oc[row] = {v.x, T(0)};
+                out[size_t(N / 2) * cs + row] = {v.y, T(0)};
             } else {
-                oc[e] = v;
+                oc[cl * cs + row] = v;
             }
         }
         __syncthreads();                          // bufa / bufb are reused by the next chunk
@@ -743,8 +744,8 @@ row_c2r(gsmi_stamp* __restrict__ stamps, const cplx<T>* __restrict__ scratch,
     const int nrow = min(rpg, st->ny - r0);
     if (nrow <= 0) return;
     bool bad = false;
-    row_c2r_body<T>(lds, rpg, scratch + st->scratch_offset + r0, 1, st->ny, nrow, N, st->x0,
-                    st->nx, tw, out + st->out_offset + size_t(r0) * st->nx, 1, bad);
+    row_c2r_body<T>(lds, rpg, scratch + st->scratch_offset + r0, 1, col_stride(st), nrow, N,
+                    st->x0, st->nx, tw, out + st->out_offset + size_t(r0) * st->nx, 1, bad);
     flag_nonfinite(st, bad);
 }
 
@@ -780,16 +781,16 @@ rows_list(gsmi_stamp* __restrict__ stamps, int nstamps,
                 const int nrow = min(it.per_group, end - r0);
                 const cplx<T>* in = scratch + st->scratch_offset + r0;
                 T* o = out + st->out_offset + size_t(r0) * st->nx;
-                if (!rows_fixed_any<T>(N, lds, it.per_group, in, st->ny, nrow, st->x0, st->nx,
-                                       twl, o, bad))
-                    row_c2r_body<T>(lds, it.per_group, in, 1, st->ny, nrow, N, st->x0, st->nx,
-                                    twl, o, 1, bad);
+                if (!rows_fixed_any<T>(N, lds, it.per_group, in, col_stride(st), nrow, st->x0,
+                                       st->nx, twl, o, bad))
+                    row_c2r_body<T>(lds, it.per_group, in, 1, col_stride(st), nrow, N, st->x0,
+                                    st->nx, twl, o, 1, bad);
                 __syncthreads();
             }
         } else {
             for (int r0 = it.first; r0 < end; r0 += it.per_group) {
-                row_c2r_body<T>(lds, it.per_group, scratch + st->scratch_offset + r0, 1, st->ny,
-                                min(it.per_group, end - r0), N, st->x0, st->nx, tw,
+                row_c2r_body<T>(lds, it.per_group, scratch + st->scratch_offset + r0, 1,
+                                col_stride(st), min(it.per_group, end - r0), N, st->x0, st->nx, tw,
                                 out + st->out_offset + size_t(r0) * st->nx, stride, bad);
                 __syncthreads();
             }

@@ -161,13 +161,16 @@ typedef struct gsmi_stamp {
                              diagonal Jacobian, PHASE and SCALE applying to the whole result  */
     int64_t out_offset;   /* element offset of the stamp in the output buffer                 */
     int64_t scratch_offset; /* complex offset in scratch of the stamp's column-transformed
-                               values: N/2+1 columns of ny output rows, column-major          */
+                               values: N/2+1 columns of ny output rows, column-major, columns
+                               col_stride apart                                              */
     int64_t axis_offset;  /* complex offset in scratch of its (Nk/2+1) + (Nk+1) axis factors  */
     double dk;            /* k spacing in pixel units: 2 pi / N                               */
     int32_t status;       /* written by the kernels: cleared by each launch's axis prepass,
                              GSMI_STAMP_NONFINITE set by the row transform when a pixel of the
                              stamp is NaN or infinite                                          */
-    int32_t reserved;
+    int32_t col_stride;   /* complex values between the scratch columns: >= ny (0, or less than
+                             ny, means ny); a multiple of 16 keeps every column on 128-byte
+                             lines for the row transform's reads                               */
 } gsmi_stamp;
 
 /* Per-stamp status bits of gsmi_stamp.status. */
@@ -309,11 +312,11 @@ int gsmi_lanczos_utable(int n, int conserve_dc, const double* gsp, double* args,
 
 /* ---------------------------------------------------------------------------------------
  * Device rendering.  All buffers are device pointers; stamps/ops/leaves/tables are device
- * copies of the lowered description; scratch holds sum(ny * (N/2+1)) complex values.
+ * copies of the lowered description; scratch holds sum(col_stride * (N/2+1)) complex values.
  * ------------------------------------------------------------------------------------- */
 
 /* drawImage(method='fft') in FP64 for a batch of stamps that share one FFT size N (max_ny =
- * tallest output stamp; scratch holds each stamp's ny * (N/2+1) values (column-major) and its
+ * tallest output stamp; scratch holds each stamp's col_stride * (N/2+1) values (column-major) and its
  * (Nk/2+1) + (Nk+1) axis factors at the offsets in its record, complex128) (the
  * reference: gsobject.py:1927-2045 drawFFT_makeKImage + _drawKImage + drawFFT_finish, with
  * SBProfile::drawK src/SBProfile.cpp:383-425, ApplyKImagePhases src/SBTransform.cpp:763-883,
