@@ -1436,7 +1436,8 @@ __device__ __forceinline__ cplx<T>* fft_fixed(cplx<T>* src, cplx<T>* dst, int ns
     }
 }
 
-// fft_lds with the common lengths compiled for their size.  PAD: lines in the padded layout
+// fft_lds with the common lengths (64 to 3072, the sizes goodFFTSize gives) compiled for their
+// size.  PAD: lines in the padded layout
 // (lds_pad / lds_line) instead of n apart.
 template <typename T, bool PAD = false>
 __device__ __forceinline__ cplx<T>* fft_lds_c(cplx<T>* a, cplx<T>* b, int n, int nseg,
@@ -1451,6 +1452,10 @@ __device__ __forceinline__ cplx<T>* fft_lds_c(cplx<T>* a, cplx<T>* b, int n, int
       case 384: return fft_fixed<T, 384, 1, PAD>(a, b, nseg, tw, twstride);
       case 512: return fft_fixed<T, 512, 1, PAD>(a, b, nseg, tw, twstride);
       case 768: return fft_fixed<T, 768, 1, PAD>(a, b, nseg, tw, twstride);
+      case 1024: return fft_fixed<T, 1024, 1, PAD>(a, b, nseg, tw, twstride);
+      case 1536: return fft_fixed<T, 1536, 1, PAD>(a, b, nseg, tw, twstride);
+      case 2048: return fft_fixed<T, 2048, 1, PAD>(a, b, nseg, tw, twstride);
+      case 3072: return fft_fixed<T, 3072, 1, PAD>(a, b, nseg, tw, twstride);
       default: return fft_lds<T, PAD>(a, b, n, nseg, tw, twstride);
     }
 }

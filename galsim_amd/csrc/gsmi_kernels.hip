@@ -719,6 +719,10 @@ __device__ __forceinline__ bool rows_fixed_any(int N, cplx<T>* lds, int rpg,
       GSMI_ROWS_FIXED(384)
       GSMI_ROWS_FIXED(512)
       GSMI_ROWS_FIXED(768)
+      GSMI_ROWS_FIXED(1024)
+      GSMI_ROWS_FIXED(1536)
+      GSMI_ROWS_FIXED(2048)
+      GSMI_ROWS_FIXED(3072)
 #undef GSMI_ROWS_FIXED
       default: return false;
     }
