@@ -165,7 +165,8 @@ def work_lists(recs, groups, cols, rows, elem, bins=None, chunks=None, extra=0,
     """Fold and row work items (gsmi_work rows: stamp, first, count, per_group) of a batch whose
     stamp records ``recs`` are sorted into FFT-size ``groups`` (N, first, count, max_ny), for
     tiles ``cols`` / ``rows`` (dict N -> columns / rows per workgroup chunk; defaults 1024/N and
-    1024/(N/2)), largest FFT sizes first, and the LDS bytes of a fold and of a row workgroup
+    1024/(N/2)), fold items largest FFT sizes first and row items in the reverse order, and the
+    LDS bytes of a fold and of a row workgroup
     (elem: bytes per complex value; extra: bytes the fold kernel keeps beside its buffers).
     Every fold column 0..N/2-1 (column 0 carries kx = N/2 too) and every output row of every
     stamp is in exactly one item.
@@ -219,6 +220,9 @@ def work_lists(recs, groups, cols, rows, elem, bins=None, chunks=None, extra=0,
     if max(lds_f, lds_r) > LIST_LDS_LIMIT:
         raise _lib.GalSimAMDError("work-list tiles need %d bytes of LDS (limit %d)"
                                   % (max(lds_f, lds_r), LIST_LDS_LIMIT))
+    # the row launch takes its items in the reverse of the fold's order: it starts on the stamps
+    # the fold finished last, whose column-transform rows are still in the caches
+    rw.reverse()
     return (np.array(fold, dtype=np.int32).reshape(-1, 4),
             np.array(rw, dtype=np.int32).reshape(-1, 4), lds_f, lds_r)
 

@@ -48,9 +48,21 @@ def test_work_lists_cover_every_column_and_row_once(cols, rows, chunks):
             assert cnt >= 1 and per >= 1
             seen[r0:r0 + cnt] += 1
         assert np.all(seen == 1)
+        r = r[np.argsort(r[:, 1])]
         assert all(cnt % per == 0 for _, _, cnt, per in r[:-1])
     Ns = recs['N'][fold[:, 0]]
     assert np.all(np.diff(Ns) <= 0)                       # largest FFT sizes first
+    # the row items run in the reverse order of the fold's: the stamps folded last come first
+    assert np.array_equal(np.unique(rw[:, 0], return_index=True)[0], np.unique(fold[:, 0]))
+    first_row = {}
+    for k, s in enumerate(rw[:, 0]):
+        first_row.setdefault(int(s), k)
+    last_fold = {}
+    for k, s in enumerate(fold[:, 0]):
+        last_fold[int(s)] = k
+    order_rows = sorted(first_row, key=first_row.get)
+    assert order_rows == sorted(last_fold, key=last_fold.get, reverse=True)
+    assert np.all(np.diff(recs['N'][rw[:, 0]]) >= 0)
     need = []
     for (_, _, _, per), N in zip(fold, recs['N'][fold[:, 0]]):
         # transform lines are padded by one slot per 32 elements in LDS
