@@ -90,6 +90,34 @@ def test_real_transform_rows_of_8192():
         _close(g.fft.irfft2(rk, so, si), x)
 
 
+def test_unaligned_buffers_take_the_lds_kernels():
+    # the register-resident transforms need 16-byte aligned buffers: a float64 buffer starting
+    # 8 bytes into an allocation takes the LDS kernels, with the same results
+    import torch
+    from galsim_amd import _lib
+    L = _lib.lib()
+    M, N = 256, 512
+    W = N // 2 + 1
+    rng = np.random.default_rng(16)
+    x = rng.normal(size=(M, N))
+    stream = torch.cuda.current_stream().cuda_stream
+    xin = torch.zeros(M * N + 1, dtype=torch.float64, device='cuda')
+    xin[1:] = torch.from_numpy(x.ravel()).cuda()
+    assert xin.data_ptr() % 16 == 0
+    k = torch.empty((M, W), dtype=torch.complex128, device='cuda')
+    tmp = torch.empty((M, W), dtype=torch.complex128, device='cuda')
+    for si, so in ((False, False), (True, True)):
+        print("unaligned rfft2/irfft2", M, N, si, so, flush=True)
+        _lib.check(L.gsmi_rfft2_f64(xin.data_ptr() + 8, M, N, int(si), int(so), k.data_ptr(),
+                                    tmp.data_ptr(), stream), "gsmi_rfft2_f64")
+        _close(k.cpu().numpy(), of.rfft2(x, si, so))
+        xout = torch.zeros(M * N + 1, dtype=torch.float64, device='cuda')
+        _lib.check(L.gsmi_irfft2_f64(k.data_ptr(), M, N, int(so), int(si),
+                                     xout.data_ptr() + 8, tmp.data_ptr(), stream),
+                   "gsmi_irfft2_f64")
+        _close(xout[1:].reshape(M, N).cpu().numpy(), x)
+
+
 def test_column_passes_query():
     from galsim_amd import _lib
     L = _lib.lib()
