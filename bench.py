@@ -123,9 +123,7 @@ def fwd_bwd_bench(args, torch):
     graph = torch.cuda.CUDAGraph()
     with torch.cuda.graph(graph):
         body()
-    for _ in range(2):
-        graph.replay()
-    torch.cuda.synchronize()
+    settle(graph.replay, torch)
     t0 = time.perf_counter()
     for _ in range(k):
         graph.replay()
@@ -260,9 +258,7 @@ def large_bench(args, torch):
     plan.autotune(reps=2)
     plan.capture()
     t_setup = time.perf_counter() - t0
-    for _ in range(2):
-        plan.replay()
-    torch.cuda.synchronize()
+    settle(plan.replay, torch)
     k = 5
     t0 = time.perf_counter()
     for _ in range(k):
@@ -309,8 +305,7 @@ def large_bench(args, torch):
     plan_p = BatchPlan(profs, sides, sides, 0.2, offs, precision='f32')
     plan_p.autotune(reps=1)
     plan_p.capture()
-    plan_p.replay()
-    torch.cuda.synchronize()
+    settle(plan_p.replay, torch)
     t0 = time.perf_counter()
     for _ in range(3):
         plan_p.replay()
@@ -402,9 +397,7 @@ def config0_bench(args, torch):
     plan = BatchPlan(profs, nx, ny, 0.2, offs, precision='f64')
     plan.autotune(reps=2)
     plan.capture()
-    for _ in range(2):
-        plan.replay()
-    torch.cuda.synchronize()
+    settle(plan.replay, torch)
     k = 20
     t0 = time.perf_counter()
     for _ in range(k):
@@ -446,6 +439,19 @@ def config0_bench(args, torch):
     return out
 
 
+def settle(fn, torch, ms=30.):
+    """Run fn (at least twice) until about ``ms`` ms of wall time have passed: after host-side
+    set-up the GPU's clocks take ~25 ms of work to settle, and a short timed region right after
+    an idle gap reads several % slow."""
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    n = 0
+    while n < 2 or (time.perf_counter() - t0) * 1e3 < ms:
+        fn()
+        torch.cuda.synchronize()
+        n += 1
+
+
 def fft_module_bench(args, torch):
     """galsim_amd.fft on device-resident float64 images: rfft2 then irfft2 (shift_in and
     shift_out, as Image.calculate_fft / calculate_inverse_fft use them) for square sizes, timed
@@ -457,6 +463,7 @@ def fft_module_bench(args, torch):
         x = torch.randn((N, N), dtype=torch.float64, device='cuda')
         for _ in range(args.fft_warmup):
             gfft.irfft2(gfft.rfft2(x, True, True), True, True)
+        settle(lambda: gfft.irfft2(gfft.rfft2(x, True, True), True, True), torch)
         k = args.fft_pairs
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
@@ -504,8 +511,7 @@ def real_space_bench(args, torch):
     def fill():
         _lib.check(L.gsmi_draw_real_f64(recs.ctypes.data, len(recs), None, N, N, 0,
                                         out.data_ptr(), stream), "gsmi_draw_real_f64")
-    for _ in range(3):
-        fill()
+    settle(fill, torch)
     k = 20
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
