@@ -537,6 +537,153 @@ c2r_rows_reg(const cplx<double>* __restrict__ src, double* __restrict__ dst, int
     }
 }
 
+// Bluestein's algorithm (L. I. Bluestein, 1970) for a line of any length L with 2L - 1 <= S, S a
+// power of two, register-resident: v holds x(n) in thread order (zero for n >= L); on return
+// v[q] = Y(k) = sum_n x(n) e^{+2 pi i n k / L} for k = t + q S/8 < L (zero beyond).  With
+// 2 n k = n^2 + k^2 - (k - n)^2 and c(m) = e^{+i pi m^2 / L}: Y = c . (conj(G(conj(G(x c) . G(b) /
+// S))), G the length-S e^{+2 pi i} transform, b(m mod S) = conj c(|m|); chirp = c(0..L-1),
+// bh = G(b) / S.
+template <int S>
+__device__ __forceinline__ void reg_bluestein(cplx<double> (&v)[kRegE], int t, int L,
+                                              const cplx<double>* __restrict__ chirp,
+                                              const cplx<double>* __restrict__ bh,
+                                              const cplx<double>* tws, cplx<double>* line)
+{
+    constexpr int T = S / kRegE;
+#pragma unroll
+    for (int q = 0; q < kRegE; ++q) {
+        const int n = t + q * T;
+        v[q] = n < L ? cmul(v[q], chirp[n]) : cplx<double>{0., 0.};
+    }
+    reg_fft<S>(v, t, tws, line);
+#pragma unroll
+    for (int q = 0; q < kRegE; ++q) v[q] = cconj(cmul(v[q], bh[t + q * T]));
+    reg_fft<S>(v, t, tws, line);
+#pragma unroll
+    for (int q = 0; q < kRegE; ++q) {
+        const int k = t + q * T;
+        v[q] = k < L ? cmul(cconj(v[q]), chirp[k]) : cplx<double>{0., 0.};
+    }
+}
+
+// r2c_rows for a half-length L = N/2 that takes Bluestein's algorithm (2L - 1 <= S), any column
+// rotation erot: nl rows per workgroup of nl S/8 threads.
+template <int S>
+__global__ void __launch_bounds__(1024)
+r2c_rows_blue(const double* __restrict__ src, cplx<double>* __restrict__ dst, int M, int N, int nl,
+              int lrot, int erot, const cplx<double>* __restrict__ tw,
+              const cplx<double>* __restrict__ twn, const cplx<double>* __restrict__ chirp,
+              const cplx<double>* __restrict__ bh)
+{
+    extern __shared__ unsigned char smem_raw[];
+    constexpr int T = S / kRegE;
+    const int h = N / 2, L = h, W = h + 1;
+    const int li = int(threadIdx.x) / T;
+    const int t = int(threadIdx.x) % T;
+    const int l = int(blockIdx.x) * nl + li;
+    const bool live = l < M;
+    cplx<double>* lds = reinterpret_cast<cplx<double>*>(smem_raw);
+    cplx<double>* line = lds + li * kRegLine<S>;
+    cplx<double>* tws = lds + nl * kRegLine<S>;
+    reg_twiddle_stage<S>(tw, tws);
+    cplx<double> v[kRegE];
+    int sl = l + lrot;
+    if (sl >= M) sl -= M;
+    const double* row = src + size_t(live ? sl : 0) * N;
+#pragma unroll
+    for (int q = 0; q < kRegE; ++q) {
+        const int m = t + q * T;
+        if (live && m < L) {
+            int q0 = 2 * m + erot;               // x(2m) and x(2m+1) of the rotated row
+            if (q0 >= N) q0 -= N;
+            int q1 = q0 + 1;
+            if (q1 >= N) q1 -= N;
+            v[q] = {row[q0], -row[q1]};          // conj(z): forward transform
+        } else {
+            v[q] = {0., 0.};
+        }
+    }
+    reg_bluestein<S>(v, t, L, chirp, bh, tws, line);   // conj(Z)
+#pragma unroll
+    for (int q = 0; q < kRegE; ++q) line[rpad(t + q * T)] = v[q];
+    __syncthreads();
+    if (!live) return;
+    cplx<double>* d = dst + size_t(l) * W;
+#pragma unroll
+    for (int q = 0; q < kRegE; ++q) {
+        const int k = t + q * T;
+        if (k >= L) continue;
+        const cplx<double> zk = cconj(v[q]);
+        const cplx<double> zm = cconj(line[rpad(k == 0 ? 0 : h - k)]);
+        const cplx<double> E = cadd(zk, cconj(zm));
+        const cplx<double> O = csub(zk, cconj(zm));
+        const cplx<double> u = cmul(cconj(twn[k]), O);
+        d[k] = {0.5 * (E.x + u.y), 0.5 * (E.y - u.x)};
+    }
+    if (t == 0) {
+        const cplx<double> z0 = cconj(v[0]);             // k = h: Z(h) = Z(0)
+        const cplx<double> E = cadd(z0, cconj(z0));
+        const cplx<double> O = csub(z0, cconj(z0));
+        const cplx<double> u = cmul(cconj(twn[h]), O);
+        d[h] = {0.5 * (E.x + u.y), 0.5 * (E.y - u.x)};
+    }
+}
+
+// c2r_rows for a half-length L = N/2 that takes Bluestein's algorithm (2L - 1 <= S), any column
+// rotation krot.
+template <int S>
+__global__ void __launch_bounds__(1024)
+c2r_rows_blue(const cplx<double>* __restrict__ src, double* __restrict__ dst, int M, int N, int nl,
+              int lrot, int krot, double scale, const cplx<double>* __restrict__ tw,
+              const cplx<double>* __restrict__ twn, const cplx<double>* __restrict__ chirp,
+              const cplx<double>* __restrict__ bh)
+{
+    extern __shared__ unsigned char smem_raw[];
+    constexpr int T = S / kRegE;
+    const int h = N / 2, L = h, W = h + 1;
+    const int li = int(threadIdx.x) / T;
+    const int t = int(threadIdx.x) % T;
+    const int l = int(blockIdx.x) * nl + li;
+    const bool live = l < M;
+    cplx<double>* lds = reinterpret_cast<cplx<double>*>(smem_raw);
+    cplx<double>* line = lds + li * kRegLine<S>;
+    cplx<double>* tws = lds + nl * kRegLine<S>;
+    reg_twiddle_stage<S>(tw, tws);
+    cplx<double> v[kRegE];
+    int sl = l + lrot;
+    if (sl >= M) sl -= M;
+    const cplx<double>* z = src + size_t(live ? sl : 0) * W;
+#pragma unroll
+    for (int q = 0; q < kRegE; ++q) {
+        const int c = t + q * T;
+        if (live && c < L) {
+            cplx<double> zc = z[c];
+            cplx<double> zm = z[h - c];
+            if (c == 0) { zc.y = 0.; zm.y = 0.; }
+            const cplx<double> E = cadd(zc, cconj(zm));
+            const cplx<double> O = cmul(csub(zc, cconj(zm)), twn[c]);
+            v[q] = {E.x - O.y, E.y + O.x};
+        } else {
+            v[q] = {0., 0.};
+        }
+    }
+    reg_bluestein<S>(v, t, L, chirp, bh, tws, line);
+    if (!live) return;
+    double* o = dst + size_t(l) * N;
+#pragma unroll
+    for (int q = 0; q < kRegE; ++q) {
+        const int m = t + q * T;
+        if (m >= L) continue;
+        // x(2m) and x(2m+1) land at columns 2m - krot and 2m + 1 - krot (mod N)
+        int j0 = 2 * m - krot;
+        if (j0 < 0) j0 += N;
+        int j1 = j0 + 1;
+        if (j1 >= N) j1 -= N;
+        o[j0] = v[q].x * scale;
+        o[j1] = v[q].y * scale;
+    }
+}
+
 // The full M x N transform of a real image from its M x (N/2 + 1) half (galsim/fft.py fft2's
 // Hermitian fill): column j >= N/2 (or j < N/2 with shift_out, whose columns are rotated by N/2)
 // is conj of the half-plane at row (M - i) mod M, column N - j (N/2 - j with shift_out).
@@ -764,6 +911,66 @@ int launch_c2r_reg_t(const cplx<double>* src, double* dst, int M, int lrot, int 
     return hip_check(hipGetLastError(), "c2r_rows_reg launch");
 }
 
+// Bluestein rows (r2c / c2r) for a half-length L that is not 5-smooth, 2L - 1 <= 4096: the
+// smallest power-of-two S >= 2L - 1 (at least 128), or 0 if there is none.
+int blue_len(int L)
+{
+    int S = 128;
+    while (S < 2 * L - 1) S *= 2;
+    return S <= 4096 ? S : 0;
+}
+
+template <int S>
+int launch_blue_rows_t(bool r2c, const void* src, void* dst, int M, int N, int lrot, int rot,
+                       double scale, hipStream_t s)
+{
+    const int L = N / 2;
+    const cplx<double>* tw = twiddles<double>(S);
+    const cplx<double>* twn = twiddles<double>(N);
+    const cplx<double>* chirp = nullptr;
+    const cplx<double>* bh = nullptr;
+    if (!tw || !twn) {
+        set_error("twiddle table allocation failed");
+        return GSMI_E_HIP;
+    }
+    if (!chirps().get(L, S, &chirp, &bh)) {
+        set_error("Bluestein table allocation failed");
+        return GSMI_E_HIP;
+    }
+    const int nl = reg_lines(S, M);
+    const size_t bytes = (size_t(nl) * kRegLine<S> + 128) * sizeof(cplx<double>);
+    const dim3 grid((M + nl - 1) / nl), block(nl * (S / kRegE));
+    int rc;
+    if (r2c) {
+        if ((rc = allow_lds(reinterpret_cast<const void*>(r2c_rows_blue<S>), bytes))) return rc;
+        hipLaunchKernelGGL(r2c_rows_blue<S>, grid, block, bytes, s,
+                           reinterpret_cast<const double*>(src),
+                           reinterpret_cast<cplx<double>*>(dst), M, N, nl, lrot, rot, tw, twn,
+                           chirp, bh);
+    } else {
+        if ((rc = allow_lds(reinterpret_cast<const void*>(c2r_rows_blue<S>), bytes))) return rc;
+        hipLaunchKernelGGL(c2r_rows_blue<S>, grid, block, bytes, s,
+                           reinterpret_cast<const cplx<double>*>(src),
+                           reinterpret_cast<double*>(dst), M, N, nl, lrot, rot, scale, tw, twn,
+                           chirp, bh);
+    }
+    return hip_check(hipGetLastError(), "Bluestein row launch");
+}
+
+int launch_blue_rows(bool r2c, const void* src, void* dst, int M, int N, int lrot, int rot,
+                     double scale, hipStream_t s)
+{
+    switch (blue_len(N / 2)) {
+      case 128: return launch_blue_rows_t<128>(r2c, src, dst, M, N, lrot, rot, scale, s);
+      case 256: return launch_blue_rows_t<256>(r2c, src, dst, M, N, lrot, rot, scale, s);
+      case 512: return launch_blue_rows_t<512>(r2c, src, dst, M, N, lrot, rot, scale, s);
+      case 1024: return launch_blue_rows_t<1024>(r2c, src, dst, M, N, lrot, rot, scale, s);
+      case 2048: return launch_blue_rows_t<2048>(r2c, src, dst, M, N, lrot, rot, scale, s);
+      case 4096: return launch_blue_rows_t<4096>(r2c, src, dst, M, N, lrot, rot, scale, s);
+      default: set_error("no register-resident Bluestein length"); return GSMI_E_SIZE;
+    }
+}
+
 #define GSMI_REG_SWITCH(L, CALL)                                                                \
     switch (L) {                                                                                \
       case 128: return CALL(128);                                                               \
@@ -975,13 +1182,17 @@ int gsmi_rfft2_f64(const double* in, int M, int N, int shift_in, int shift_out, 
     const int W = N / 2 + 1;
     if (reg_len(N / 2) && aligned16(in) && aligned16(tmp)) {
         rc = launch_r2c_reg(N / 2, in, tmp, M, shift_in ? M / 2 : 0, shift_in ? N / 2 : 0, s);
+    } else if (px.chirp && blue_len(N / 2)) {
+        rc = launch_blue_rows(true, in, tmp, M, N, shift_in ? M / 2 : 0, shift_in ? N / 2 : 0, 1.,
+                              s);
     } else {
-    const int nl = lines_per_group(px.S, M, lds);
-    const size_t bytes = 2 * size_t(nl) * px.S * sizeof(cplx<double>);
-    if ((rc = allow_lds(reinterpret_cast<const void*>(r2c_rows<double>), bytes))) return rc;
-    hipLaunchKernelGGL(r2c_rows<double>, dim3((M + nl - 1) / nl), dim3(kThreads), bytes, s, in,
-                       tmp, M, N, nl, shift_in ? M / 2 : 0, shift_in ? N / 2 : 0, px, twn);
-    rc = hip_check(hipGetLastError(), "r2c_rows launch");
+        const int nl = lines_per_group(px.S, M, lds);
+        const size_t bytes = 2 * size_t(nl) * px.S * sizeof(cplx<double>);
+        if ((rc = allow_lds(reinterpret_cast<const void*>(r2c_rows<double>), bytes))) return rc;
+        hipLaunchKernelGGL(r2c_rows<double>, dim3((M + nl - 1) / nl), dim3(kThreads), bytes, s,
+                           in, tmp, M, N, nl, shift_in ? M / 2 : 0, shift_in ? N / 2 : 0, px,
+                           twn);
+        rc = hip_check(hipGetLastError(), "r2c_rows launch");
     }
     if (rc) return rc;
     if (ly)
@@ -1028,6 +1239,9 @@ int gsmi_irfft2_f64(const double* in, int M, int N, int shift_in, int shift_out,
     if (reg_len(N / 2) && aligned16(tmp) && aligned16(out))
         return launch_c2r_reg(N / 2, tmp, out, M, shift_out ? M / 2 : 0, shift_out ? N / 2 : 0,
                               1. / (double(M) * double(N)), s);
+    if (px.chirp && blue_len(N / 2))
+        return launch_blue_rows(false, tmp, out, M, N, shift_out ? M / 2 : 0,
+                                shift_out ? N / 2 : 0, 1. / (double(M) * double(N)), s);
     const int nl = lines_per_group(px.S, M, lds);
     const size_t bytes = 2 * size_t(nl) * px.S * sizeof(cplx<double>);
     if ((rc = allow_lds(reinterpret_cast<const void*>(c2r_rows<double>), bytes))) return rc;
