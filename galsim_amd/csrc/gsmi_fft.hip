@@ -566,6 +566,57 @@ __device__ __forceinline__ void reg_bluestein(cplx<double> (&v)[kRegE], int t, i
     }
 }
 
+// c2c_lines for plain line maps and a line length L that takes Bluestein's algorithm (2L - 1 <=
+// S): as c2c_lines_reg (nl lines per workgroup of nl S/8 threads; COLS: lanes take adjacent lines
+// first), with any rotations.
+template <int S, bool COLS>
+__global__ void __launch_bounds__(1024)
+c2c_lines_blue(const cplx<double>* __restrict__ src, cplx<double>* __restrict__ dst, int nlines,
+               int nl, int L, long long ls, long long es, int lrot, int erot, int krot,
+               int conj_in, int conj_out, double scale, const cplx<double>* __restrict__ tw,
+               const cplx<double>* __restrict__ chirp, const cplx<double>* __restrict__ bh)
+{
+    extern __shared__ unsigned char smem_raw[];
+    constexpr int T = S / kRegE;
+    const int li = COLS ? int(threadIdx.x) % nl : int(threadIdx.x) / T;
+    const int t = COLS ? int(threadIdx.x) / nl : int(threadIdx.x) % T;
+    const int l = (COLS ? xcd_contiguous(blockIdx.x, gridDim.x) : int(blockIdx.x)) * nl + li;
+    const bool live = l < nlines;
+    cplx<double>* lds = reinterpret_cast<cplx<double>*>(smem_raw);
+    cplx<double>* line = lds + li * kRegLine<S>;
+    cplx<double>* tws = lds + nl * kRegLine<S>;
+    reg_twiddle_stage<S>(tw, tws);
+    int sl = l + lrot;
+    if (sl >= nlines) sl -= nlines;
+    const cplx<double>* sp = src + (live ? sl : 0) * ls;
+    cplx<double> v[kRegE];
+#pragma unroll
+    for (int q = 0; q < kRegE; ++q) {
+        const int n = t + q * T;
+        if (live && n < L) {
+            int sn = n + erot;
+            if (sn >= L) sn -= L;
+            cplx<double> x = sp[sn * es];
+            if (conj_in) x.y = -x.y;
+            v[q] = x;
+        } else {
+            v[q] = {0., 0.};
+        }
+    }
+    reg_bluestein<S>(v, t, L, chirp, bh, tws, line);
+    if (!live) return;
+    cplx<double>* d = dst + l * ls;
+    const double sy = conj_out ? -scale : scale;
+#pragma unroll
+    for (int q = 0; q < kRegE; ++q) {
+        const int kk = t + q * T;            // holds Y(kk), written at position (kk - krot) mod L
+        if (kk >= L) continue;
+        int k = kk - krot;
+        if (k < 0) k += L;
+        d[k * es] = {v[q].x * scale, v[q].y * sy};
+    }
+}
+
 // r2c_rows for a half-length L = N/2 that takes Bluestein's algorithm (2L - 1 <= S), any column
 // rotation erot: nl rows per workgroup of nl S/8 threads.
 template <int S>
@@ -957,6 +1008,60 @@ int launch_blue_rows_t(bool r2c, const void* src, void* dst, int M, int N, int l
     return hip_check(hipGetLastError(), "Bluestein row launch");
 }
 
+template <int S>
+int launch_c2c_blue_t(const cplx<double>* src, cplx<double>* dst, int nlines, int L, long long ls,
+                      long long es, int lrot, int erot, int krot, int conj_io, double scale,
+                      hipStream_t s)
+{
+    const cplx<double>* tw = twiddles<double>(S);
+    const cplx<double>* chirp = nullptr;
+    const cplx<double>* bh = nullptr;
+    if (!tw) {
+        set_error("twiddle table allocation failed");
+        return GSMI_E_HIP;
+    }
+    if (!chirps().get(L, S, &chirp, &bh)) {
+        set_error("Bluestein table allocation failed");
+        return GSMI_E_HIP;
+    }
+    const int nl = reg_lines(S, nlines);
+    const size_t bytes = (size_t(nl) * kRegLine<S> + 128) * sizeof(cplx<double>);
+    const dim3 grid((nlines + nl - 1) / nl), block(nl * (S / kRegE));
+    int rc;
+    if (es == 1) {
+        if ((rc = allow_lds(reinterpret_cast<const void*>(c2c_lines_blue<S, false>), bytes)))
+            return rc;
+        hipLaunchKernelGGL((c2c_lines_blue<S, false>), grid, block, bytes, s, src, dst, nlines,
+                           nl, L, ls, es, lrot, erot, krot, conj_io, conj_io, scale, tw, chirp,
+                           bh);
+    } else {
+        if ((rc = allow_lds(reinterpret_cast<const void*>(c2c_lines_blue<S, true>), bytes)))
+            return rc;
+        hipLaunchKernelGGL((c2c_lines_blue<S, true>), grid, block, bytes, s, src, dst, nlines,
+                           nl, L, ls, es, lrot, erot, krot, conj_io, conj_io, scale, tw, chirp,
+                           bh);
+    }
+    return hip_check(hipGetLastError(), "Bluestein line launch");
+}
+
+int launch_c2c_blue(const cplx<double>* src, cplx<double>* dst, int nlines, int L, long long ls,
+                    long long es, int lrot, int erot, int krot, int conj_io, double scale,
+                    hipStream_t s)
+{
+#define GSMI_C2C_BLUE(SS) launch_c2c_blue_t<SS>(src, dst, nlines, L, ls, es, lrot, erot, krot, \
+                                                conj_io, scale, s)
+    switch (blue_len(L)) {
+      case 128: return GSMI_C2C_BLUE(128);
+      case 256: return GSMI_C2C_BLUE(256);
+      case 512: return GSMI_C2C_BLUE(512);
+      case 1024: return GSMI_C2C_BLUE(1024);
+      case 2048: return GSMI_C2C_BLUE(2048);
+      case 4096: return GSMI_C2C_BLUE(4096);
+      default: set_error("no register-resident Bluestein length"); return GSMI_E_SIZE;
+    }
+#undef GSMI_C2C_BLUE
+}
+
 int launch_blue_rows(bool r2c, const void* src, void* dst, int M, int N, int lrot, int rot,
                      double scale, hipStream_t s)
 {
@@ -1030,6 +1135,9 @@ int launch_c2c(const cplx<double>* src, cplx<double>* dst, int nlines, long long
 {
     if (reg_len(p.L) && !p.chirp && aligned16(src) && aligned16(dst))
         return launch_c2c_reg(p.L, src, dst, nlines, ls, es, lrot, erot, krot, conj_io, scale, s);
+    if (p.chirp && blue_len(p.L))
+        return launch_c2c_blue(src, dst, nlines, p.L, ls, es, lrot, erot, krot, conj_io, scale,
+                               s);
     const LineMap m{nlines, 0, ls, es};
     return launch_lines(src, dst, nlines, m, lrot, erot, m, krot, conj_io, conj_io, scale, p,
                         PassTwiddle<double>{nullptr, 1, 1}, lds, s);
