@@ -1281,6 +1281,37 @@ __device__ inline void bfly5(cplx<T>* v)
     v[3] = {a2.x + b2.y, a2.y - b2.x};
 }
 
+// Radix 12 (e^{+2 pi i}) as radix 3 over n1 and radix 4 over n2 with n = 4 n1 + n2, k = k1 + 3 k2:
+// w12^{nk} = w3^{n1 k1} w12^{n2 k1} w4^{n2 k2}.
+template <typename T>
+__device__ inline void bfly12(cplx<T>* v)
+{
+    const T h = T(0.86602540378443864676);   // sqrt(3)/2
+    cplx<T> a[4][3];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) {
+        cplx<T> u[3] = {v[n2], v[n2 + 4], v[n2 + 8]};
+        bfly3(u);
+        a[n2][0] = u[0];
+        a[n2][1] = u[1];
+        a[n2][2] = u[2];
+    }
+    // w12^m, m = n2 k1: w^1 = (h, 1/2), w^2 = (1/2, h), w^3 = i, w^4 = (-1/2, h), w^6 = -1
+    a[1][1] = cmul(a[1][1], cplx<T>{h, T(0.5)});
+    a[1][2] = cmul(a[1][2], cplx<T>{T(0.5), h});
+    a[2][1] = cmul(a[2][1], cplx<T>{T(0.5), h});
+    a[2][2] = cmul(a[2][2], cplx<T>{T(-0.5), h});
+    a[3][1] = {-a[3][1].y, a[3][1].x};
+    a[3][2] = {-a[3][2].x, -a[3][2].y};
+#pragma unroll
+    for (int k1 = 0; k1 < 3; ++k1) {
+        cplx<T> u[4] = {a[0][k1], a[1][k1], a[2][k1], a[3][k1]};
+        bfly4(u);
+#pragma unroll
+        for (int k2 = 0; k2 < 4; ++k2) v[k1 + 3 * k2] = u[k2];
+    }
+}
+
 // LDS layout of the padded transforms: element i of a line at lds_pad(i), one slot of padding
 // after every 32 elements (256 bytes of FP32 complex values, one sweep of the 64 four-byte
 // banks), lines lds_line(n) apart.  The Stockham writes of stride R (and the multi-line reads)

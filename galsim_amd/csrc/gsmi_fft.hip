@@ -270,7 +270,9 @@ c2r_rows(const cplx<T>* __restrict__ src, T* __restrict__ dst, int M, int N, int
 // registers: two to three LDS round trips per line instead of one per pass plus the staging, one
 // LDS line per line instead of two, and every index a compile-time shift or mask.
 // ---------------------------------------------------------------------------------------------
-constexpr int kRegE = 8;
+// values per thread: 8 for a power-of-two length, 12 for 3 x 2^k (radix 12, then 4 and 2)
+template <int L>
+constexpr int kRegE = (L % 3 == 0) ? 12 : 8;
 
 // LDS slot of element p: one slot of padding after every 8 values (128 bytes), so the stride-8
 // scatter of the first pass falls on distinct banks.
@@ -279,6 +281,9 @@ __device__ __forceinline__ int rpad(int p) { return p + (p >> 3); }
 // Slots per LDS line: odd, so neighbouring lines of a workgroup start on different banks.
 template <int L>
 constexpr int kRegLine = L + L / 8 + 1;
+
+// x mod n for 0 <= x < 2n.
+__device__ __forceinline__ int wrap_once(int x, int n) { return x >= n ? x - n : x; }
 
 // 16-byte loads and stores of complex values (the kernels below are launched only on 16-byte
 // aligned buffers).
@@ -298,6 +303,16 @@ __device__ __forceinline__ void bfly_reg(cplx<T>* v)
     if constexpr (R == 2) bfly2(v);
     if constexpr (R == 4) bfly4(v);
     if constexpr (R == 8) bfly8(v);
+    if constexpr (R == 12) bfly12(v);
+}
+
+// The radix of the pass after passes whose radices multiply to NS: 8 while at least 8 points
+// remain, then the remaining 4 or 2 (E = 8); 12 first, then 4, and a last 2 (E = 12).
+template <int L, int NS>
+constexpr int reg_radix()
+{
+    return (kRegE<L> == 12) ? (NS == 1 ? 12 : (L / NS >= 4 ? 4 : 2))
+                            : (L / NS >= 8 ? 8 : L / NS);
 }
 
 // exp(+2 pi i m / L), m < L <= 4096, from the two 64-entry LDS tables tws[i] = exp(+2 pi i i / L)
@@ -328,11 +343,11 @@ __device__ __forceinline__ void reg_twiddle_stage(const cplx<double>* __restrict
 // last pass (NS R = L, so j / NS = 0), to position t + (b + r 8/R) T, back in v.
 // tws: the twiddle tables of reg_twiddle.
 template <int L, int R, int NS>
-__device__ __forceinline__ void reg_pass(cplx<double> (&v)[kRegE], int t,
+__device__ __forceinline__ void reg_pass(cplx<double> (&v)[kRegE<L>], int t,
                                          const cplx<double>* tws, cplx<double>* line)
 {
-    constexpr int T = L / kRegE, B = kRegE / R;
-    cplx<double> out[kRegE];
+    constexpr int T = L / kRegE<L>, B = kRegE<L> / R;
+    cplx<double> out[kRegE<L>];
 #pragma unroll
     for (int b = 0; b < B; ++b) {
         const int j = t + b * T;
@@ -340,7 +355,7 @@ __device__ __forceinline__ void reg_pass(cplx<double> (&v)[kRegE], int t,
 #pragma unroll
         for (int r = 0; r < R; ++r) u[r] = v[b + r * B];
         if constexpr (NS > 1) {
-            const int k = j & (NS - 1);
+            const int k = (NS & (NS - 1)) == 0 ? (j & (NS - 1)) : j % NS;
             cplx<double> w[R];
 #pragma unroll
             for (int r = 1; r < R; ++r) w[r] = reg_twiddle<L>(tws, (L / (NS * R)) * r * k);
@@ -352,14 +367,15 @@ __device__ __forceinline__ void reg_pass(cplx<double> (&v)[kRegE], int t,
 #pragma unroll
             for (int r = 0; r < R; ++r) out[b + r * B] = u[r];
         } else {
-            const int base = (j - (j & (NS - 1))) * R + (j & (NS - 1));
+            const int jm = (NS & (NS - 1)) == 0 ? (j & (NS - 1)) : j % NS;
+            const int base = (j - jm) * R + jm;
 #pragma unroll
             for (int r = 0; r < R; ++r) st2(line + rpad(base + r * NS), u[r]);
         }
     }
     if constexpr (NS * R == L) {
 #pragma unroll
-        for (int s = 0; s < kRegE; ++s) v[s] = out[s];
+        for (int s = 0; s < kRegE<L>; ++s) v[s] = out[s];
     }
 }
 
@@ -367,34 +383,35 @@ __device__ __forceinline__ void reg_pass(cplx<double> (&v)[kRegE], int t,
 // Y(t + s T)).  Every thread of the workgroup calls it (it holds barriers); `line` is this
 // thread's line in LDS, free on entry and on return.
 template <int L, int NS = 1>
-__device__ __forceinline__ void reg_fft(cplx<double> (&v)[kRegE], int t, const cplx<double>* tws,
+__device__ __forceinline__ void reg_fft(cplx<double> (&v)[kRegE<L>], int t, const cplx<double>* tws,
                                         cplx<double>* line)
 {
-    constexpr int rem = L / NS;
-    constexpr int R = rem >= kRegE ? kRegE : rem;
+    constexpr int R = reg_radix<L, NS>();
     reg_pass<L, R, NS>(v, t, tws, line);
     if constexpr (NS * R < L) {
-        constexpr int T = L / kRegE;
+        constexpr int T = L / kRegE<L>;
         __syncthreads();
 #pragma unroll
-        for (int s = 0; s < kRegE; ++s) v[s] = ld2(line + rpad(t + s * T));
+        for (int s = 0; s < kRegE<L>; ++s) v[s] = ld2(line + rpad(t + s * T));
         __syncthreads();
         reg_fft<L, NS * R>(v, t, tws, line);
     }
 }
 
-// c2c_lines for plain line maps (line l at l ls, element n at n es) and power-of-two L: nl lines
-// per workgroup of nl L/8 threads.  COLS (es != 1): lanes take adjacent lines first, so each
-// access of a wave covers nl adjacent values of a row, and neighbouring workgroups go to one XCD;
-// else lanes take adjacent elements of a line.
+// c2c_lines for plain line maps (line l at l ls, element n at n es) and a register-resident length
+// L: nl lines per workgroup of nl L/E threads (at most 512, and 1024 for two columns of 4096; the
+// twelve-value forms are bounded to 512 threads, which leaves them the VGPRs their passes need).
+// COLS (es != 1): lanes take adjacent lines first, so each access of a wave covers nl adjacent
+// values of a row, and neighbouring workgroups go to one XCD; else lanes take adjacent elements
+// of a line.
 template <int L, bool COLS>
-__global__ void __launch_bounds__(1024)
+__global__ void __launch_bounds__(kRegE<L> == 12 ? 512 : 1024)
 c2c_lines_reg(const cplx<double>* __restrict__ src, cplx<double>* __restrict__ dst, int nlines,
               int nl, long long ls, long long es, int lrot, int erot, int krot, int conj_in,
               int conj_out, double scale, const cplx<double>* __restrict__ tw)
 {
     extern __shared__ unsigned char smem_raw[];
-    constexpr int T = L / kRegE;
+    constexpr int T = L / kRegE<L>;
     const int li = COLS ? int(threadIdx.x) % nl : int(threadIdx.x) / T;
     const int t = COLS ? int(threadIdx.x) / nl : int(threadIdx.x) % T;
     const int l = (COLS ? xcd_contiguous(blockIdx.x, gridDim.x) : int(blockIdx.x)) * nl + li;
@@ -403,29 +420,32 @@ c2c_lines_reg(const cplx<double>* __restrict__ src, cplx<double>* __restrict__ d
     cplx<double>* line = lds + li * kRegLine<L>;
     cplx<double>* tws = lds + nl * kRegLine<L>;
     reg_twiddle_stage<L>(tw, tws);
-    cplx<double> v[kRegE];
+    cplx<double> v[kRegE<L>];
     if (live) {
         int sl = l + lrot;
         if (sl >= nlines) sl -= nlines;
         const cplx<double>* s = src + sl * ls;
 #pragma unroll
-        for (int q = 0; q < kRegE; ++q) {
-            cplx<double> x = ld2(s + ((t + q * T + erot) & (L - 1)) * es);
+        for (int q = 0; q < kRegE<L>; ++q) {
+            int n = t + q * T + erot;            // erot < L
+            if (n >= L) n -= L;
+            cplx<double> x = ld2(s + n * es);
             if (conj_in) x.y = -x.y;
             v[q] = x;
         }
     } else {
 #pragma unroll
-        for (int q = 0; q < kRegE; ++q) v[q] = {0., 0.};
+        for (int q = 0; q < kRegE<L>; ++q) v[q] = {0., 0.};
     }
     reg_fft<L>(v, t, tws, line);
     if (!live) return;
     cplx<double>* d = dst + l * ls;
     const double sy = conj_out ? -scale : scale;
 #pragma unroll
-    for (int q = 0; q < kRegE; ++q) {
+    for (int q = 0; q < kRegE<L>; ++q) {
         // output position k holds Y((k + krot) mod L)
-        const int k = (t + q * T - krot) & (L - 1);
+        int k = t + q * T - krot;               // krot < L
+        if (k < 0) k += L;
         st2(d + k * es, {v[q].x * scale, v[q].y * sy});
     }
 }
@@ -433,13 +453,13 @@ c2c_lines_reg(const cplx<double>* __restrict__ src, cplx<double>* __restrict__ d
 // r2c_rows for N = 2L (L a power of two) and an even column rotation erot: the row's pairs
 // x(2m), x(2m+1) are one 16-byte load.
 template <int L>
-__global__ void __launch_bounds__(1024)
+__global__ void __launch_bounds__(kRegE<L> == 12 ? 512 : 1024)
 r2c_rows_reg(const double* __restrict__ src, cplx<double>* __restrict__ dst, int M, int nl,
              int lrot, int erot, const cplx<double>* __restrict__ tw,
              const cplx<double>* __restrict__ twn)
 {
     extern __shared__ unsigned char smem_raw[];
-    constexpr int T = L / kRegE, N = 2 * L, h = L, W = L + 1;
+    constexpr int T = L / kRegE<L>, N = 2 * L, h = L, W = L + 1;
     const int li = int(threadIdx.x) / T;
     const int t = int(threadIdx.x) % T;
     const int l = int(blockIdx.x) * nl + li;
@@ -448,29 +468,29 @@ r2c_rows_reg(const double* __restrict__ src, cplx<double>* __restrict__ dst, int
     cplx<double>* line = lds + li * kRegLine<L>;
     cplx<double>* tws = lds + nl * kRegLine<L>;
     reg_twiddle_stage<L>(tw, tws);
-    cplx<double> v[kRegE];
+    cplx<double> v[kRegE<L>];
     if (live) {
         int sl = l + lrot;
         if (sl >= M) sl -= M;
         const double* row = src + size_t(sl) * N;
 #pragma unroll
-        for (int q = 0; q < kRegE; ++q) {
+        for (int q = 0; q < kRegE<L>; ++q) {
             const double2 x =
-                *reinterpret_cast<const double2*>(row + ((2 * (t + q * T) + erot) & (N - 1)));
+                *reinterpret_cast<const double2*>(row + wrap_once(2 * (t + q * T) + erot, N));
             v[q] = {x.x, -x.y};                  // conj(z): forward transform
         }
     } else {
 #pragma unroll
-        for (int q = 0; q < kRegE; ++q) v[q] = {0., 0.};
+        for (int q = 0; q < kRegE<L>; ++q) v[q] = {0., 0.};
     }
     reg_fft<L>(v, t, tws, line);                // conj(Z)
 #pragma unroll
-    for (int q = 0; q < kRegE; ++q) st2(line + rpad(t + q * T), v[q]);
+    for (int q = 0; q < kRegE<L>; ++q) st2(line + rpad(t + q * T), v[q]);
     __syncthreads();
     if (!live) return;
     cplx<double>* d = dst + size_t(l) * W;
 #pragma unroll
-    for (int q = 0; q < kRegE; ++q) {
+    for (int q = 0; q < kRegE<L>; ++q) {
         const int k = t + q * T;
         const cplx<double> zk = cconj(v[q]);
         const cplx<double> zm = cconj(ld2(line + rpad(k == 0 ? 0 : h - k)));
@@ -492,13 +512,13 @@ r2c_rows_reg(const double* __restrict__ src, cplx<double>* __restrict__ dst, int
 // c2r_rows for N = 2L (L a power of two) and an even column rotation krot: pixels x(2m), x(2m+1)
 // are one 16-byte store.
 template <int L>
-__global__ void __launch_bounds__(1024)
+__global__ void __launch_bounds__(kRegE<L> == 12 ? 512 : 1024)
 c2r_rows_reg(const cplx<double>* __restrict__ src, double* __restrict__ dst, int M, int nl,
              int lrot, int krot, double scale, const cplx<double>* __restrict__ tw,
              const cplx<double>* __restrict__ twn)
 {
     extern __shared__ unsigned char smem_raw[];
-    constexpr int T = L / kRegE, N = 2 * L, h = L, W = L + 1;
+    constexpr int T = L / kRegE<L>, N = 2 * L, h = L, W = L + 1;
     const int li = int(threadIdx.x) / T;
     const int t = int(threadIdx.x) % T;
     const int l = int(blockIdx.x) * nl + li;
@@ -507,13 +527,13 @@ c2r_rows_reg(const cplx<double>* __restrict__ src, double* __restrict__ dst, int
     cplx<double>* line = lds + li * kRegLine<L>;
     cplx<double>* tws = lds + nl * kRegLine<L>;
     reg_twiddle_stage<L>(tw, tws);
-    cplx<double> v[kRegE];
+    cplx<double> v[kRegE<L>];
     if (live) {
         int sl = l + lrot;
         if (sl >= M) sl -= M;
         const cplx<double>* z = src + size_t(sl) * W;
 #pragma unroll
-        for (int q = 0; q < kRegE; ++q) {
+        for (int q = 0; q < kRegE<L>; ++q) {
             const int c = t + q * T;
             cplx<double> zc = ld2(z + c);
             cplx<double> zm = ld2(z + (h - c));
@@ -524,15 +544,15 @@ c2r_rows_reg(const cplx<double>* __restrict__ src, double* __restrict__ dst, int
         }
     } else {
 #pragma unroll
-        for (int q = 0; q < kRegE; ++q) v[q] = {0., 0.};
+        for (int q = 0; q < kRegE<L>; ++q) v[q] = {0., 0.};
     }
     reg_fft<L>(v, t, tws, line);
     if (!live) return;
     double* o = dst + size_t(l) * N;
 #pragma unroll
-    for (int q = 0; q < kRegE; ++q) {
+    for (int q = 0; q < kRegE<L>; ++q) {
         // x(2m) and x(2m+1) of m = t + q T land at columns 2m - krot and 2m + 1 - krot (mod N)
-        const int j = (2 * (t + q * T) - krot) & (N - 1);
+        const int j = wrap_once(2 * (t + q * T) - krot + N, N);
         *reinterpret_cast<double2*>(o + j) = double2{v[q].x * scale, v[q].y * scale};
     }
 }
@@ -544,23 +564,23 @@ c2r_rows_reg(const cplx<double>* __restrict__ src, double* __restrict__ dst, int
 // S))), G the length-S e^{+2 pi i} transform, b(m mod S) = conj c(|m|); chirp = c(0..L-1),
 // bh = G(b) / S.
 template <int S>
-__device__ __forceinline__ void reg_bluestein(cplx<double> (&v)[kRegE], int t, int L,
+__device__ __forceinline__ void reg_bluestein(cplx<double> (&v)[kRegE<S>], int t, int L,
                                               const cplx<double>* __restrict__ chirp,
                                               const cplx<double>* __restrict__ bh,
                                               const cplx<double>* tws, cplx<double>* line)
 {
-    constexpr int T = S / kRegE;
+    constexpr int T = S / kRegE<S>;
 #pragma unroll
-    for (int q = 0; q < kRegE; ++q) {
+    for (int q = 0; q < kRegE<S>; ++q) {
         const int n = t + q * T;
         v[q] = n < L ? cmul(v[q], chirp[n]) : cplx<double>{0., 0.};
     }
     reg_fft<S>(v, t, tws, line);
 #pragma unroll
-    for (int q = 0; q < kRegE; ++q) v[q] = cconj(cmul(v[q], bh[t + q * T]));
+    for (int q = 0; q < kRegE<S>; ++q) v[q] = cconj(cmul(v[q], bh[t + q * T]));
     reg_fft<S>(v, t, tws, line);
 #pragma unroll
-    for (int q = 0; q < kRegE; ++q) {
+    for (int q = 0; q < kRegE<S>; ++q) {
         const int k = t + q * T;
         v[q] = k < L ? cmul(cconj(v[q]), chirp[k]) : cplx<double>{0., 0.};
     }
@@ -577,7 +597,7 @@ c2c_lines_blue(const cplx<double>* __restrict__ src, cplx<double>* __restrict__ 
                const cplx<double>* __restrict__ chirp, const cplx<double>* __restrict__ bh)
 {
     extern __shared__ unsigned char smem_raw[];
-    constexpr int T = S / kRegE;
+    constexpr int T = S / kRegE<S>;
     const int li = COLS ? int(threadIdx.x) % nl : int(threadIdx.x) / T;
     const int t = COLS ? int(threadIdx.x) / nl : int(threadIdx.x) % T;
     const int l = (COLS ? xcd_contiguous(blockIdx.x, gridDim.x) : int(blockIdx.x)) * nl + li;
@@ -589,9 +609,9 @@ c2c_lines_blue(const cplx<double>* __restrict__ src, cplx<double>* __restrict__ 
     int sl = l + lrot;
     if (sl >= nlines) sl -= nlines;
     const cplx<double>* sp = src + (live ? sl : 0) * ls;
-    cplx<double> v[kRegE];
+    cplx<double> v[kRegE<S>];
 #pragma unroll
-    for (int q = 0; q < kRegE; ++q) {
+    for (int q = 0; q < kRegE<S>; ++q) {
         const int n = t + q * T;
         if (live && n < L) {
             int sn = n + erot;
@@ -608,7 +628,7 @@ c2c_lines_blue(const cplx<double>* __restrict__ src, cplx<double>* __restrict__ 
     cplx<double>* d = dst + l * ls;
     const double sy = conj_out ? -scale : scale;
 #pragma unroll
-    for (int q = 0; q < kRegE; ++q) {
+    for (int q = 0; q < kRegE<S>; ++q) {
         const int kk = t + q * T;            // holds Y(kk), written at position (kk - krot) mod L
         if (kk >= L) continue;
         int k = kk - krot;
@@ -627,7 +647,7 @@ r2c_rows_blue(const double* __restrict__ src, cplx<double>* __restrict__ dst, in
               const cplx<double>* __restrict__ bh)
 {
     extern __shared__ unsigned char smem_raw[];
-    constexpr int T = S / kRegE;
+    constexpr int T = S / kRegE<S>;
     const int h = N / 2, L = h, W = h + 1;
     const int li = int(threadIdx.x) / T;
     const int t = int(threadIdx.x) % T;
@@ -637,12 +657,12 @@ r2c_rows_blue(const double* __restrict__ src, cplx<double>* __restrict__ dst, in
     cplx<double>* line = lds + li * kRegLine<S>;
     cplx<double>* tws = lds + nl * kRegLine<S>;
     reg_twiddle_stage<S>(tw, tws);
-    cplx<double> v[kRegE];
+    cplx<double> v[kRegE<S>];
     int sl = l + lrot;
     if (sl >= M) sl -= M;
     const double* row = src + size_t(live ? sl : 0) * N;
 #pragma unroll
-    for (int q = 0; q < kRegE; ++q) {
+    for (int q = 0; q < kRegE<S>; ++q) {
         const int m = t + q * T;
         if (live && m < L) {
             int q0 = 2 * m + erot;               // x(2m) and x(2m+1) of the rotated row
@@ -656,12 +676,12 @@ r2c_rows_blue(const double* __restrict__ src, cplx<double>* __restrict__ dst, in
     }
     reg_bluestein<S>(v, t, L, chirp, bh, tws, line);   // conj(Z)
 #pragma unroll
-    for (int q = 0; q < kRegE; ++q) line[rpad(t + q * T)] = v[q];
+    for (int q = 0; q < kRegE<S>; ++q) line[rpad(t + q * T)] = v[q];
     __syncthreads();
     if (!live) return;
     cplx<double>* d = dst + size_t(l) * W;
 #pragma unroll
-    for (int q = 0; q < kRegE; ++q) {
+    for (int q = 0; q < kRegE<S>; ++q) {
         const int k = t + q * T;
         if (k >= L) continue;
         const cplx<double> zk = cconj(v[q]);
@@ -690,7 +710,7 @@ c2r_rows_blue(const cplx<double>* __restrict__ src, double* __restrict__ dst, in
               const cplx<double>* __restrict__ bh)
 {
     extern __shared__ unsigned char smem_raw[];
-    constexpr int T = S / kRegE;
+    constexpr int T = S / kRegE<S>;
     const int h = N / 2, L = h, W = h + 1;
     const int li = int(threadIdx.x) / T;
     const int t = int(threadIdx.x) % T;
@@ -700,12 +720,12 @@ c2r_rows_blue(const cplx<double>* __restrict__ src, double* __restrict__ dst, in
     cplx<double>* line = lds + li * kRegLine<S>;
     cplx<double>* tws = lds + nl * kRegLine<S>;
     reg_twiddle_stage<S>(tw, tws);
-    cplx<double> v[kRegE];
+    cplx<double> v[kRegE<S>];
     int sl = l + lrot;
     if (sl >= M) sl -= M;
     const cplx<double>* z = src + size_t(live ? sl : 0) * W;
 #pragma unroll
-    for (int q = 0; q < kRegE; ++q) {
+    for (int q = 0; q < kRegE<S>; ++q) {
         const int c = t + q * T;
         if (live && c < L) {
             cplx<double> zc = z[c];
@@ -722,7 +742,7 @@ c2r_rows_blue(const cplx<double>* __restrict__ src, double* __restrict__ dst, in
     if (!live) return;
     double* o = dst + size_t(l) * N;
 #pragma unroll
-    for (int q = 0; q < kRegE; ++q) {
+    for (int q = 0; q < kRegE<S>; ++q) {
         const int m = t + q * T;
         if (m >= L) continue;
         // x(2m) and x(2m+1) land at columns 2m - krot and 2m + 1 - krot (mod N)
@@ -886,10 +906,15 @@ int lines_per_group(int S, int nlines, size_t lds)
     return nl;
 }
 
-// The register-resident kernels: power-of-two lengths 128..4096 on 16-byte aligned buffers,
+// The register-resident kernels: power-of-two lengths 128..4096 and 3 x 2^k lengths 192..3072 on
+// 16-byte aligned buffers,
 // 4096 / L lines (512 threads) per workgroup, two LDS lines (about 74 KiB) and two workgroups
 // per CU.
-bool reg_len(int L) { return L >= 128 && L <= 4096 && (L & (L - 1)) == 0; }
+bool reg_len(int L)
+{
+    const int p = (L % 3 == 0) ? L / 3 : L;
+    return L >= 128 && L <= 4096 && (p & (p - 1)) == 0;
+}
 
 bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
 
@@ -908,7 +933,7 @@ int launch_c2c_reg_t(const cplx<double>* src, cplx<double>* dst, int nlines, lon
     // columns of 4096: two per workgroup (1024 threads), so a wave's row accesses are 32 bytes
     const int nl = (es != 1 && L == 4096) ? std::min(nlines, 2) : reg_lines(L, nlines);
     const size_t bytes = (size_t(nl) * kRegLine<L> + 128) * sizeof(cplx<double>);
-    const dim3 grid((nlines + nl - 1) / nl), block(nl * (L / kRegE));
+    const dim3 grid((nlines + nl - 1) / nl), block(nl * (L / kRegE<L>));
     int rc;
     if (es == 1) {
         if ((rc = allow_lds(reinterpret_cast<const void*>(c2c_lines_reg<L, false>), bytes)))
@@ -938,7 +963,7 @@ int launch_r2c_reg_t(const double* src, cplx<double>* dst, int M, int lrot, int 
     const size_t bytes = (size_t(nl) * kRegLine<L> + 128) * sizeof(cplx<double>);
     int rc = allow_lds(reinterpret_cast<const void*>(r2c_rows_reg<L>), bytes);
     if (rc) return rc;
-    hipLaunchKernelGGL(r2c_rows_reg<L>, dim3((M + nl - 1) / nl), dim3(nl * (L / kRegE)), bytes, s,
+    hipLaunchKernelGGL(r2c_rows_reg<L>, dim3((M + nl - 1) / nl), dim3(nl * (L / kRegE<L>)), bytes, s,
                        src, dst, M, nl, lrot, erot, tw, twn);
     return hip_check(hipGetLastError(), "r2c_rows_reg launch");
 }
@@ -957,13 +982,14 @@ int launch_c2r_reg_t(const cplx<double>* src, double* dst, int M, int lrot, int 
     const size_t bytes = (size_t(nl) * kRegLine<L> + 128) * sizeof(cplx<double>);
     int rc = allow_lds(reinterpret_cast<const void*>(c2r_rows_reg<L>), bytes);
     if (rc) return rc;
-    hipLaunchKernelGGL(c2r_rows_reg<L>, dim3((M + nl - 1) / nl), dim3(nl * (L / kRegE)), bytes, s,
+    hipLaunchKernelGGL(c2r_rows_reg<L>, dim3((M + nl - 1) / nl), dim3(nl * (L / kRegE<L>)), bytes, s,
                        src, dst, M, nl, lrot, krot, scale, tw, twn);
     return hip_check(hipGetLastError(), "c2r_rows_reg launch");
 }
 
-// Bluestein rows (r2c / c2r) for a half-length L that is not 5-smooth, 2L - 1 <= 4096: the
-// smallest power-of-two S >= 2L - 1 (at least 128), or 0 if there is none.
+// The register-resident Bluestein length for a line length L that is not 5-smooth: the smallest
+// power of two in 128..4096 that is >= 2L - 1, or 0 if there is none (a 3 x 2^k length, though
+// shorter, ran slower: its twelve-value passes hold more registers).
 int blue_len(int L)
 {
     int S = 128;
@@ -990,7 +1016,7 @@ int launch_blue_rows_t(bool r2c, const void* src, void* dst, int M, int N, int l
     }
     const int nl = reg_lines(S, M);
     const size_t bytes = (size_t(nl) * kRegLine<S> + 128) * sizeof(cplx<double>);
-    const dim3 grid((M + nl - 1) / nl), block(nl * (S / kRegE));
+    const dim3 grid((M + nl - 1) / nl), block(nl * (S / kRegE<S>));
     int rc;
     if (r2c) {
         if ((rc = allow_lds(reinterpret_cast<const void*>(r2c_rows_blue<S>), bytes))) return rc;
@@ -1026,7 +1052,7 @@ int launch_c2c_blue_t(const cplx<double>* src, cplx<double>* dst, int nlines, in
     }
     const int nl = reg_lines(S, nlines);
     const size_t bytes = (size_t(nl) * kRegLine<S> + 128) * sizeof(cplx<double>);
-    const dim3 grid((nlines + nl - 1) / nl), block(nl * (S / kRegE));
+    const dim3 grid((nlines + nl - 1) / nl), block(nl * (S / kRegE<S>));
     int rc;
     if (es == 1) {
         if ((rc = allow_lds(reinterpret_cast<const void*>(c2c_lines_blue<S, false>), bytes)))
@@ -1084,6 +1110,11 @@ int launch_blue_rows(bool r2c, const void* src, void* dst, int M, int N, int lro
       case 1024: return CALL(1024);                                                             \
       case 2048: return CALL(2048);                                                             \
       case 4096: return CALL(4096);                                                             \
+      case 192: return CALL(192);                                                               \
+      case 384: return CALL(384);                                                               \
+      case 768: return CALL(768);                                                               \
+      case 1536: return CALL(1536);                                                             \
+      case 3072: return CALL(3072);                                                             \
       default: set_error("no register-resident transform for this length"); return GSMI_E_SIZE; \
     }
 

@@ -61,7 +61,9 @@ SIZES = [(64, 96), (128, 128), (250, 384), (14, 22), (46, 262), (1174, 1174), (2
          (300, 2048), (6144, 40), (8192, 64),
          # every power-of-two line length of the register-resident transforms (128..4096) along
          # rows and columns, line counts below and off a multiple of a workgroup's lines
-         (256, 512), (1024, 1024), (4096, 256), (512, 4096), (6, 256), (4096, 4), (2048, 2048)]
+         (256, 512), (1024, 1024), (4096, 256), (512, 4096), (6, 256), (4096, 4), (2048, 2048),
+         # and the 3 x 2^k lengths (192..3072), twelve values per thread
+         (1536, 768), (384, 3072), (192, 1536)]
 
 
 @pytest.mark.parametrize("shape", SIZES)
