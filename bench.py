@@ -459,7 +459,7 @@ def fft_module_bench(args, torch):
     writes 16 M (N/2 + 1); irfft2 reads 16 M (N/2 + 1) and writes 8 M N."""
     from galsim_amd import fft as gfft
     res = {}
-    for N in (1174, 2048, 4096):
+    for N in (1174, 1536, 2048, 3072, 4096):
         x = torch.randn((N, N), dtype=torch.float64, device='cuda')
         for _ in range(args.fft_warmup):
             gfft.irfft2(gfft.rfft2(x, True, True), True, True)
@@ -578,8 +578,8 @@ def main():
                     help='stamps of the large-stamp workload checked against the numpy oracle')
     ap.add_argument('--no-fft-module', action='store_true')
     ap.add_argument('--fft-module-only', action='store_true',
-                    help='only run the FFT-module leg (rfft2 + irfft2 of 1174^2, 2048^2 and '
-                         '4096^2 float64 images) and exit')
+                    help='only run the FFT-module leg (rfft2 + irfft2 of 1174^2, 1536^2, '
+                         '2048^2, 3072^2 and 4096^2 float64 images) and exit')
     ap.add_argument('--fft-pairs', type=int, default=10)
     ap.add_argument('--fft-warmup', type=int, default=2)
     ap.add_argument('--no-real-space', action='store_true')
