@@ -63,7 +63,7 @@ SIZES = [(64, 96), (128, 128), (250, 384), (14, 22), (46, 262), (1174, 1174), (2
          # rows and columns, line counts below and off a multiple of a workgroup's lines
          (256, 512), (1024, 1024), (4096, 256), (512, 4096), (6, 256), (4096, 4), (2048, 2048),
          # and the 3 x 2^k lengths (192..3072), twelve values per thread
-         (1536, 768), (384, 3072), (192, 1536)]
+         (1536, 768), (384, 3072), (192, 1536), (3072, 384), (768, 3072)]
 
 
 @pytest.mark.parametrize("shape", SIZES)
@@ -81,11 +81,12 @@ def test_sizes_against_numpy(shape):
         _close(g.fft.irfft2(rk, so, si), x)
 
 
-def test_real_transform_rows_of_8192():
-    # rfft2 / irfft2 rows of 8192 run as complex transforms of 4096 (complex rows of 8192 exceed
-    # one workgroup's LDS and raise)
-    rng = np.random.default_rng(8192)
-    x = rng.normal(size=(64, 8192))
+@pytest.mark.parametrize("n", [8192, 6144])
+def test_long_real_rows(n):
+    # rfft2 / irfft2 rows of 8192 (6144) run as complex transforms of 4096 (3072); complex rows
+    # that long exceed one workgroup's LDS and raise
+    rng = np.random.default_rng(n)
+    x = rng.normal(size=(64, n))
     for si, so in ((False, False), (True, True)):
         rk = g.fft.rfft2(x, si, so)
         _close(rk, of.rfft2(x, si, so))
